@@ -1,0 +1,153 @@
+/*
+ * sodabayes.h -- C ABI of the MI355X (gfx950) SODA Gibbs sampler.
+ *
+ * Drop-in boundary for the SODA branch of the reference program:
+ *   SODABayes.f90:139  call setup_soda   -> sb_create + sb_build_aug
+ *   SODABayes.f90:142  call soda_bayes   -> sb_mcmc_init + sb_mcmc_run + sb_get
+ * The Fortran host keeps option parsing, file IO and the output writers; it
+ * binds these functions with `interface ... bind(C)` (iso_c_binding).
+ *
+ * Conventions: all arrays are plain host pointers in C order unless stated;
+ * SNP ids and positions are 0-based; every function returns SB_OK (0) or an
+ * error code, and sb_last_error() gives the message of the last failure on
+ * the calling thread.  A handle is not re-entrant: calls on one handle must
+ * not overlap.  One handle owns one HIP stream on the current device.
+ */
+#ifndef SODABAYES_H
+#define SODABAYES_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SB_OK 0
+#define SB_ERR_ARG (-1)      /* invalid argument or size */
+#define SB_ERR_HIP (-2)      /* HIP runtime error (message in sb_last_error) */
+#define SB_ERR_NOMEM (-3)    /* device allocation failed */
+#define SB_ERR_STATE (-4)    /* call out of order (e.g. run before init) */
+#define SB_ERR_NOT_PD (-5)   /* dI - X_s'X_s not positive definite: see sb_build_aug */
+
+/* Bayesian methods, globals.f90:22 */
+enum { SB_BAYES_A = 1, SB_BAYES_B = 2, SB_BAYES_C = 3, SB_BAYES_CPI = 4,
+       SB_BAYES_R = 5, SB_BAYES_RC = 6 };
+
+/* Random-number modes.
+ * REPLAY : every draw comes from the reference's xoroshiro128+ stream 0, in
+ *          the order of a -nthreads 1 run (parallelrng.f90, models.f90).
+ * COUNTER: Philox4x32-10 keyed by (seed, chain), counter (iteration, SNP,
+ *          draw): every SNP draws independently, in parallel. */
+enum { SB_RNG_REPLAY = 0, SB_RNG_COUNTER = 1 };
+
+/* Arithmetic of the genotype products X_s' r.
+ * FP64 : FP64 products and sums.
+ * MIXED: exact int8 MFMA on fixed-point residual limbs with int32 sums,
+ *        combined in FP64 (norm-wise error stated in sb_get_info). */
+enum { SB_PREC_FP64 = 0, SB_PREC_MIXED = 1 };
+
+typedef struct {
+    int32_t method;             /* SB_BAYES_* */
+    int32_t nind, nloci;        /* n individuals, p SNPs */
+    int32_t nmix, ngroup;       /* mixture classes (1 for BayesA), SNP groups */
+    int32_t nblocks, blocksize, blockt; /* resolved partition, soda.f90:407-417 */
+    int32_t nchains;            /* independent chains on the same genotypes */
+    int32_t rng_mode;           /* SB_RNG_* (REPLAY needs nchains == 1) */
+    int32_t precision;          /* SB_PREC_* */
+    int32_t bed_on_device;      /* 1: `bed` in sb_create is a device pointer */
+} sb_dims;
+
+/* Priors and chain settings (useroptions.f90:488-552, already widened from
+ * the reference's float32 option parsing by the caller). */
+typedef struct {
+    double pizero, scale_va, scale_ve, df_va, df_ve;
+    double vara0;               /* module value of vara before the job */
+    double mix[20];             /* class variances as a fraction of Va */
+    double alpha[20];           /* Dirichlet prior */
+    int32_t numit, burnin, thin;
+    int32_t pad0;
+    int64_t seed;               /* COUNTER mode key (REPLAY uses sb_rng_set) */
+} sb_priors;
+
+typedef struct sb_handle sb_handle;
+
+/* ---- host-side RNG of the reference (parallelrng.f90), for the host ---- */
+
+/* stream 0 of set_random_seed(seed) (utilities.f90:504-517): state[2] */
+int sb_rng_seed(int64_t seed, int32_t stream, uint64_t state[2]);
+/* permutate(p, n, s) (utilities.f90:588-607): permvec 0-based, advances state */
+int sb_permutate(int32_t n, int32_t s, uint64_t state[2], int32_t *permvec);
+
+/* ---- setup (replaces setup_soda, soda.f90:403-443) ---- */
+
+/* bed: nloci x ceil(nind/4) SNP-major PLINK payload (no magic bytes), in
+ * SNP-id order; freq: allele frequency per SNP id (utilities.f90:527);
+ * permvec: SNP id at each block position (soda.f90:429-435); snpgrp: 0-based
+ * group per SNP id (BayesRc) or NULL. */
+int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq,
+              const int32_t *permvec, const int32_t *snpgrp, sb_handle **out);
+int sb_destroy(sb_handle *h);
+
+/* REPLAY mode: load / read back the stream-0 xoroshiro128+ state. */
+int sb_rng_set(sb_handle *h, const uint64_t state[2]);
+int sb_rng_get(sb_handle *h, uint64_t state[2]);
+
+/* Builds the augmenting matrices (get_aug_matrix, soda.f90:445-518):
+ * int8-MFMA Gram X_s'X_s (exact int32) with the FP64 standardisation of
+ * xxproduct (soda.f90:520-556), power-method bound d = lambda + 0.001
+ * (soda.f90:378-401, start vectors drawn from the replay stream), and the
+ * FP64 Cholesky of dI - X_s'X_s (soda.f90:8-47).  xxd: nblocks values.
+ * Returns SB_ERR_NOT_PD with *fail_block / *fail_col (1-based) set when a
+ * block is not positive definite (the reference then switches to pivoted
+ * Cholesky, soda.f90:492-510). */
+int sb_build_aug(sb_handle *h, double *xxd, int32_t *fail_block, int32_t *fail_col);
+
+/* ---- MCMC (replaces soda_bayes, models.f90:9-605) ---- */
+
+/* Initial state, models.f90:56-128: why has nind values. */
+int sb_mcmc_init(sb_handle *h, const sb_priors *pr, const double *why);
+/* Runs iterations rep = first .. first+count-1 (models.f90:129-574): the
+ * sweep, hyperparameter draws, posterior accumulation and the residual
+ * resync every 1000 iterations.  hyp (optional, host) receives one record
+ * per iteration with mod(rep, thin) == 0 for chain 0:
+ *   [rep, included, mu, vara, vare, snpindist(ngroup*nmix), varindist(...)]
+ * i.e. 5 + 2*ngroup*nmix doubles per record. */
+int sb_mcmc_run(sb_handle *h, int32_t first, int32_t count, double *hyp);
+/* doubles per hyp record */
+int32_t sb_hyp_width(const sb_handle *h);
+
+/* Downloads one quantity of one chain.  Per-SNP arrays are in SNP-id order. */
+enum {
+    SB_Q_G = 1,          /* effects g (nloci) */
+    SB_Q_CLASS,          /* snptracker(:,2) as double, 1-based (nloci) */
+    SB_Q_VARA_S,         /* per-SNP variances, BayesA/B (nloci) */
+    SB_Q_YADJ,           /* residuals (nind) */
+    SB_Q_YAADJ,          /* augmented residuals (blocksize) */
+    SB_Q_P,              /* mixture proportions (ngroup*nmix, column-major) */
+    SB_Q_SCALARS,        /* mu, vare, vara, included (4) */
+    SB_Q_GSTORE,         /* sum of g over kept samples (nloci) */
+    SB_Q_INDISTSTORE,    /* class counts (nloci*nmix, column-major nloci x nmix) */
+    SB_Q_VARA_SSTORE,    /* sum of vara_s (nloci) */
+    SB_Q_STORES,         /* counter, mu, vare, vara, included sums (5) */
+    SB_Q_SNPSTORE,       /* (ngroup*nmix) */
+    SB_Q_VARSTORE,       /* (ngroup*nmix) */
+    SB_Q_PSTORE,         /* (ngroup*nmix) */
+    SB_Q_XA,             /* augmenting matrices, blocksize x nloci column-major */
+    SB_Q_XXD,            /* d per block (nblocks) */
+    SB_Q_GRAM,           /* standardised Gram of block `chain` (n_b x n_b), recomputed */
+    SB_Q_SNPINDIST,      /* last iteration's SNPs per (group, class) (ngroup*nmix) */
+    SB_Q_VARINDIST       /* last iteration's sum of g^2 per (group, class) (ngroup*nmix) */
+};
+int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count);
+
+/* Diagnostics: fills up to n doubles: [setup seconds, iterations run,
+ * last sweep milliseconds, device bytes allocated]. */
+int sb_get_info(sb_handle *h, double *out, int32_t n);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *sb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,449 @@
+// C ABI of the SODA sampler (include/sodabayes.h): handle lifetime, host-side
+// reference RNG helpers, argument checks and state download.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sb_internal.h"
+
+namespace {
+thread_local std::string g_err;
+
+constexpr double DEPS = 2.220446049250313080847e-16;  // epsilon(1.0d0)
+
+// xoroshiro128+ of parallelrng.f90:100-149, host copy for the caller's
+// setup draws (seeding, block permutation).
+struct HostXoro {
+    uint64_t s1, s2;
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next()
+    {
+        uint64_t a = s1, b = s2, r = a + b;
+        b ^= a;
+        s1 = rotl(a, 55) ^ b ^ (b << 14);
+        s2 = rotl(b, 36);
+        return r;
+    }
+    double uniform()
+    {
+        uint64_t b = 0x3FF0000000000000ULL | (next() >> 12);
+        double d;
+        std::memcpy(&d, &b, sizeof d);
+        return d - 1.0;
+    }
+    void jump()
+    {
+        static const uint64_t J[2] = {0xbeac0467eba5facbULL, 0xd86b048b86aa9922ULL};
+        uint64_t a = 0, b = 0;
+        for (int i = 0; i < 2; i++)
+            for (int bit = 0; bit < 64; bit++) {
+                if (J[i] & (1ULL << bit)) {
+                    a ^= s1;
+                    b ^= s2;
+                }
+                next();
+            }
+        s1 = a;
+        s2 = b;
+    }
+};
+}  // namespace
+
+namespace sb {
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return SB_OK;
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return SB_ERR_HIP;
+}
+
+int dev_alloc(sb_handle *h, void **p, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        g_err = "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e);
+        *p = nullptr;
+        return SB_ERR_NOMEM;
+    }
+    h->allocs.push_back(*p);
+    h->bytes += bytes;
+    return SB_OK;
+}
+
+}  // namespace sb
+
+using namespace sb;
+
+extern "C" {
+
+const char *sb_last_error(void) { return g_err.c_str(); }
+
+int sb_rng_seed(int64_t seed, int32_t stream, uint64_t state[2])
+{
+    if (!state || stream < 0) return fail(SB_ERR_ARG, "sb_rng_seed: bad argument");
+    uint64_t a = (uint64_t)(seed < 0 ? -seed : seed);
+    HostXoro s{a, a + 1};
+    for (int k = 0; k <= stream; k++) s.jump();
+    state[0] = s.s1;
+    state[1] = s.s2;
+    return SB_OK;
+}
+
+int sb_permutate(int32_t n, int32_t s, uint64_t state[2], int32_t *p)
+{
+    if (!state || !p || n <= 0 || s <= 0) return fail(SB_ERR_ARG, "sb_permutate: bad argument");
+    HostXoro x{state[0], state[1]};
+    for (int i = 0; i < n; i++) p[i] = i;
+    std::vector<double> u(s);
+    for (int i = 1; i <= n; i += s) {
+        int m = (n - i + 1) < s ? (n - i + 1) : s;
+        for (int j = 0; j < s; j++) u[j] = x.uniform();
+        for (int j = 1; j <= m; j++) {
+            int ipj = i + j - 1;
+            int k = (int)(u[j - 1] * (n - ipj + 1)) + ipj;
+            int t = p[ipj - 1];
+            p[ipj - 1] = p[k - 1];
+            p[k - 1] = t;
+        }
+    }
+    state[0] = x.s1;
+    state[1] = x.s2;
+    return SB_OK;
+}
+
+int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const int32_t *permvec,
+              const int32_t *snpgrp, sb_handle **out)
+{
+    if (!dims || !bed || !freq || !permvec || !out) return fail(SB_ERR_ARG, "sb_create: null argument");
+    *out = nullptr;
+    const sb_dims &d = *dims;
+    if (d.method < SB_BAYES_A || d.method > SB_BAYES_RC) return fail(SB_ERR_ARG, "unknown method");
+    if (d.nind <= 0 || d.nloci <= 0) return fail(SB_ERR_ARG, "nind and nloci must be positive");
+    int want_nmix = d.method == SB_BAYES_A ? 1 : (d.method <= SB_BAYES_CPI ? 2 : d.nmix);
+    if (d.nmix != want_nmix || d.nmix < 1 || d.nmix > MAXMIX) return fail(SB_ERR_ARG, "bad nmix for method");
+    int want_ngroup = d.method == SB_BAYES_RC ? d.ngroup : 1;
+    if (d.ngroup != want_ngroup || d.ngroup < 1 || d.ngroup > MAXGRP) return fail(SB_ERR_ARG, "bad ngroup");
+    if (d.method == SB_BAYES_RC && !snpgrp) return fail(SB_ERR_ARG, "BayesRc needs snpgrp");
+    if (d.blocksize < 2 || d.blocksize > 4096) return fail(SB_ERR_ARG, "blocksize must be in [2, 4096]");
+    if (d.nblocks < 1 || d.blockt < 0 || d.blockt > d.nblocks) return fail(SB_ERR_ARG, "bad block partition");
+    if (d.nchains != 1) return fail(SB_ERR_ARG, "this build runs one chain per handle");
+    if (d.rng_mode != SB_RNG_REPLAY && d.rng_mode != SB_RNG_COUNTER) return fail(SB_ERR_ARG, "bad rng_mode");
+    if (d.precision != SB_PREC_FP64) return fail(SB_ERR_ARG, "precision mode not available in this build");
+
+    // block bounds, soda.f90:463-469
+    std::vector<int> ls(d.nblocks + 1);
+    int pos = 0;
+    for (int b = 0; b < d.nblocks; b++) {
+        int s0, sz;
+        if (b < d.blockt) {
+            s0 = b * d.blocksize;
+            sz = std::min(d.blocksize, d.nloci - s0);
+        } else {
+            s0 = d.blocksize * d.blockt + (b - d.blockt) * (d.blocksize - 1);
+            sz = std::min(d.blocksize - 1, d.nloci - s0);
+        }
+        if (s0 != pos || sz < 1) return fail(SB_ERR_ARG, "block partition does not tile the SNPs");
+        ls[b] = s0;
+        pos = s0 + sz;
+    }
+    ls[d.nblocks] = pos;
+    if (pos != d.nloci) return fail(SB_ERR_ARG, "block partition does not cover nloci");
+
+    sb_handle *h = new sb_handle();
+    h->d = d;
+    hipGetDevice(&h->device);
+    if (hip_check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "stream")) {
+        delete h;
+        return SB_ERR_HIP;
+    }
+    Par &P = h->par;
+    std::memset(&P, 0, sizeof P);
+    P.method = d.method;
+    P.n = d.nind;
+    P.p = d.nloci;
+    P.nmix = d.nmix;
+    P.ngroup = d.ngroup;
+    P.nbins = d.nmix * d.ngroup;
+    P.nblocks = d.nblocks;
+    P.blocksize = d.blocksize;
+    P.blockt = d.blockt;
+    P.naind = d.blocksize;  // naind - naless with every block positive definite
+    int words = (d.nind + 15) / 16;
+    P.ldw = ((words + CHUNK_WORDS - 1) / CHUNK_WORDS) * CHUNK_WORDS;
+    P.nchunk = P.ldw / CHUNK_WORDS;
+    P.rng_mode = d.rng_mode;
+    P.nchains = d.nchains;
+    h->ls = ls;
+    h->tri_off.assign(d.nblocks + 1, 0);
+    for (int b = 0; b < d.nblocks; b++) {
+        int64_t nb = ls[b + 1] - ls[b];
+        h->tri_off[b + 1] = h->tri_off[b] + nb * (nb + 1) / 2;
+    }
+    h->permvec.assign(permvec, permvec + d.nloci);
+    h->pos_of.assign(d.nloci, -1);
+    for (int k = 0; k < d.nloci; k++) {
+        int s = permvec[k];
+        if (s < 0 || s >= d.nloci || h->pos_of[s] >= 0) {
+            delete h;
+            return fail(SB_ERR_ARG, "permvec is not a permutation of 0..nloci-1");
+        }
+        h->pos_of[s] = k;
+    }
+
+    const int64_t p = d.nloci, bs = d.blocksize, ldr = (int64_t)P.ldw * 16;
+    const int bin_chunk = 8192, nwg_bins = (int)((p + bin_chunk - 1) / bin_chunk);
+    int rc = 0;
+    auto A = [&](void **ptr, size_t bytes) {
+        if (!rc) rc = dev_alloc(h, ptr, bytes);
+    };
+    A((void **)&h->geno, sizeof(uint32_t) * (size_t)(p * P.ldw));
+    A((void **)&h->f2, sizeof(double) * 2 * p);
+    A((void **)&h->sdv, sizeof(double) * p);
+    A((void **)&h->zlut, sizeof(double) * 4 * p);
+    A((void **)&h->grp, sizeof(int32_t) * p);
+    A((void **)&h->d_pos_by_id, sizeof(int32_t) * 2 * p);
+    A((void **)&h->xxd, sizeof(double) * d.nblocks);
+    A((void **)&h->U, sizeof(double) * (size_t)h->tri_off[d.nblocks]);
+    A((void **)&h->d_tri_off, sizeof(int64_t) * (d.nblocks + 1));
+    A((void **)&h->d_ls, sizeof(int32_t) * (d.nblocks + 1));
+    A((void **)&h->r, sizeof(double) * ldr);
+    A((void **)&h->ra, sizeof(double) * bs);
+    A((void **)&h->why, sizeof(double) * ldr);
+    A((void **)&h->beta, sizeof(double) * p);
+    A((void **)&h->cls, sizeof(int32_t) * p);
+    A((void **)&h->vara_s, sizeof(double) * p);
+    A((void **)&h->gstore, sizeof(double) * p);
+    A((void **)&h->vara_sstore, sizeof(double) * p);
+    A((void **)&h->indiststore, sizeof(double) * p * d.nmix);
+    A((void **)&h->scal, sizeof(Scal));
+    A((void **)&h->sc.part, sizeof(double) * P.nchunk * bs);
+    A((void **)&h->sc.partA, sizeof(double) * bs);
+    A((void **)&h->sc.partR, sizeof(double) * P.nchunk);
+    A((void **)&h->sc.rhs, sizeof(double) * bs);
+    A((void **)&h->sc.stemp, sizeof(double) * bs * d.nmix);
+    A((void **)&h->sc.uu, sizeof(double) * 2 * bs);
+    A((void **)&h->sc.chg, sizeof(int32_t) * bs);
+    A((void **)&h->sc.flag, sizeof(uint8_t) * p + 64);
+    A((void **)&h->sc.delta, sizeof(double) * p);
+    A((void **)&h->sc.red, sizeof(double) * (4096 + 2 * (size_t)P.nbins * nwg_bins));
+    A((void **)&h->resync_list, sizeof(int32_t) * p);
+    A((void **)&h->d_count, sizeof(int32_t) * 4);
+    if (rc) {
+        sb_destroy(h);
+        return rc;
+    }
+    // per-position constants, computed with the reference's expressions
+    // (xscale, utilities.f90:526-533)
+    std::vector<double> f2(2 * p), sdv(p), zl(4 * p);
+    std::vector<int32_t> g(p, 0), pb(2 * p);
+    for (int64_t k = 0; k < p; k++) {
+        int s = permvec[k];
+        double f = freq[s];
+        f2[k] = 2.0 * f;
+        f2[p + k] = f;
+        if (f < DEPS || (1.0 - f) < DEPS) {
+            sdv[k] = 0.0;
+            zl[4 * k] = zl[4 * k + 1] = zl[4 * k + 2] = 0.0;
+        } else {
+            double sd = std::sqrt(2.0 * f * (1.0 - f));
+            sdv[k] = sd;
+            for (int x = 0; x < 3; x++) zl[4 * k + x] = ((double)x - 2.0 * f) / sd;
+        }
+        zl[4 * k + 3] = 0.0;
+        if (snpgrp) g[k] = snpgrp[s];
+        pb[p + k] = s;
+    }
+    for (int64_t s = 0; s < p; s++) pb[s] = h->pos_of[s];
+    for (int64_t k = 0; k < p; k++)
+        if (g[k] < 0 || g[k] >= d.ngroup) {
+            sb_destroy(h);
+            return fail(SB_ERR_ARG, "snpgrp out of range");
+        }
+    hipStream_t st = h->stream;
+    hipMemcpyAsync(h->f2, f2.data(), sizeof(double) * 2 * p, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(h->sdv, sdv.data(), sizeof(double) * p, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(h->zlut, zl.data(), sizeof(double) * 4 * p, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(h->grp, g.data(), sizeof(int32_t) * p, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(h->d_pos_by_id, pb.data(), sizeof(int32_t) * 2 * p, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(h->d_tri_off, h->tri_off.data(), sizeof(int64_t) * (d.nblocks + 1), hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(h->d_ls, ls.data(), sizeof(int32_t) * (d.nblocks + 1), hipMemcpyHostToDevice, st);
+    hipMemsetAsync(h->scal, 0, sizeof(Scal), st);
+    hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
+    rc = setup_pack(h, bed, d.bed_on_device != 0);
+    if (rc) {
+        sb_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return SB_OK;
+}
+
+int sb_destroy(sb_handle *h)
+{
+    if (!h) return SB_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) hipFree(p);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return SB_OK;
+}
+
+int sb_rng_set(sb_handle *h, const uint64_t state[2])
+{
+    if (!h || !state) return fail(SB_ERR_ARG, "sb_rng_set: null argument");
+    if (hip_check(hipMemcpyAsync(&h->scal->rng, state, 16, hipMemcpyHostToDevice, h->stream), "rng set"))
+        return SB_ERR_HIP;
+    return hip_check(hipStreamSynchronize(h->stream), "rng set");
+}
+
+int sb_rng_get(sb_handle *h, uint64_t state[2])
+{
+    if (!h || !state) return fail(SB_ERR_ARG, "sb_rng_get: null argument");
+    if (hip_check(hipMemcpyAsync(state, &h->scal->rng, 16, hipMemcpyDeviceToHost, h->stream), "rng get"))
+        return SB_ERR_HIP;
+    return hip_check(hipStreamSynchronize(h->stream), "rng get");
+}
+
+int sb_build_aug(sb_handle *h, double *xxd, int32_t *fail_block, int32_t *fail_col)
+{
+    if (!h) return fail(SB_ERR_ARG, "sb_build_aug: null handle");
+    return setup_build_aug(h, xxd, fail_block, fail_col);
+}
+
+int sb_mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
+{
+    if (!h || !pr || !why) return fail(SB_ERR_ARG, "sb_mcmc_init: null argument");
+    return mcmc_init(h, pr, why);
+}
+
+int sb_mcmc_run(sb_handle *h, int32_t first, int32_t count, double *hyp)
+{
+    if (!h) return fail(SB_ERR_ARG, "sb_mcmc_run: null handle");
+    return mcmc_run(h, first, count, hyp);
+}
+
+int32_t sb_hyp_width(const sb_handle *h) { return h ? 5 + 2 * h->par.nbins : 0; }
+
+int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count)
+{
+    if (!h || !out) return fail(SB_ERR_ARG, "sb_get: null argument");
+    const Par &P = h->par;
+    const int64_t p = P.p;
+    hipStream_t st = h->stream;
+    auto need = [&](int64_t n) { return count >= n ? SB_OK : fail(SB_ERR_ARG, "sb_get: output too small"); };
+    auto by_id = [&](const void *dsrc, bool is_int, int64_t ncol) -> int {
+        int rc = need(p * ncol);
+        if (rc) return rc;
+        std::vector<double> tmp(p * ncol);
+        if (is_int) {
+            std::vector<int32_t> ti(p);
+            if (hip_check(hipMemcpy(ti.data(), dsrc, sizeof(int32_t) * p, hipMemcpyDeviceToHost), "get"))
+                return SB_ERR_HIP;
+            for (int64_t k = 0; k < p; k++) tmp[k] = ti[k];
+        } else if (hip_check(hipMemcpy(tmp.data(), dsrc, sizeof(double) * p * ncol, hipMemcpyDeviceToHost), "get")) {
+            return SB_ERR_HIP;
+        }
+        for (int64_t c = 0; c < ncol; c++)
+            for (int64_t k = 0; k < p; k++) out[c * p + h->permvec[k]] = tmp[c * p + k];
+        return SB_OK;
+    };
+    hipStreamSynchronize(st);
+    Scal sc;
+    bool need_scal = what == SB_Q_P || what == SB_Q_SCALARS || what == SB_Q_STORES || what == SB_Q_SNPSTORE ||
+                     what == SB_Q_VARSTORE || what == SB_Q_PSTORE || what == SB_Q_SNPINDIST ||
+                     what == SB_Q_VARINDIST;
+    if (need_scal && hip_check(hipMemcpy(&sc, h->scal, sizeof sc, hipMemcpyDeviceToHost), "get scal"))
+        return SB_ERR_HIP;
+    switch (what) {
+    case SB_Q_G: return by_id(h->beta, false, 1);
+    case SB_Q_CLASS: return by_id(h->cls, true, 1);
+    case SB_Q_VARA_S: return by_id(h->vara_s, false, 1);
+    case SB_Q_GSTORE: return by_id(h->gstore, false, 1);
+    case SB_Q_VARA_SSTORE: return by_id(h->vara_sstore, false, 1);
+    case SB_Q_INDISTSTORE: return by_id(h->indiststore, false, P.nmix);
+    case SB_Q_YADJ:
+        if (need(P.n)) return SB_ERR_ARG;
+        return hip_check(hipMemcpy(out, h->r, sizeof(double) * P.n, hipMemcpyDeviceToHost), "get r");
+    case SB_Q_YAADJ:
+        if (need(P.blocksize)) return SB_ERR_ARG;
+        return hip_check(hipMemcpy(out, h->ra, sizeof(double) * P.blocksize, hipMemcpyDeviceToHost), "get ra");
+    case SB_Q_XXD:
+        if (need(P.nblocks)) return SB_ERR_ARG;
+        return hip_check(hipMemcpy(out, h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost), "get xxd");
+    case SB_Q_P:
+        if (need(P.nbins)) return SB_ERR_ARG;
+        std::memcpy(out, sc.P, sizeof(double) * P.nbins);
+        return SB_OK;
+    case SB_Q_SNPSTORE:
+    case SB_Q_VARSTORE:
+    case SB_Q_PSTORE:
+    case SB_Q_SNPINDIST:
+    case SB_Q_VARINDIST: {
+        if (need(P.nbins)) return SB_ERR_ARG;
+        const double *src = what == SB_Q_SNPSTORE   ? sc.snpstore
+                            : what == SB_Q_VARSTORE ? sc.varstore
+                            : what == SB_Q_PSTORE   ? sc.pstore
+                            : what == SB_Q_SNPINDIST ? sc.snpindist
+                                                     : sc.varindist;
+        std::memcpy(out, src, sizeof(double) * P.nbins);
+        return SB_OK;
+    }
+    case SB_Q_SCALARS:
+        if (need(4)) return SB_ERR_ARG;
+        out[0] = sc.mu;
+        out[1] = sc.vare;
+        out[2] = sc.vara;
+        out[3] = sc.included;
+        return SB_OK;
+    case SB_Q_STORES:
+        if (need(5)) return SB_ERR_ARG;
+        out[0] = sc.counter;
+        out[1] = sc.mu_store;
+        out[2] = sc.vare_store;
+        out[3] = sc.vara_store;
+        out[4] = sc.included_store;
+        return SB_OK;
+    case SB_Q_XA: {
+        if (need((int64_t)P.blocksize * p)) return SB_ERR_ARG;
+        std::vector<double> U(h->tri_off[P.nblocks]);
+        if (hip_check(hipMemcpy(U.data(), h->U, sizeof(double) * U.size(), hipMemcpyDeviceToHost), "get U"))
+            return SB_ERR_HIP;
+        std::memset(out, 0, sizeof(double) * (size_t)P.blocksize * p);
+        for (int b = 0; b < P.nblocks; b++) {
+            int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+            for (int j = 0; j < nb; j++)
+                for (int i = 0; i <= j; i++)
+                    out[(int64_t)(p0 + j) * P.blocksize + i] = U[h->tri_off[b] + (int64_t)j * (j + 1) / 2 + i];
+        }
+        return SB_OK;
+    }
+    case SB_Q_GRAM: {
+        if (chain < 0 || chain >= P.nblocks) return fail(SB_ERR_ARG, "block out of range");
+        int64_t nb = h->ls[chain + 1] - h->ls[chain];
+        if (need(nb * nb)) return SB_ERR_ARG;
+        return setup_gram_block(h, chain, out);
+    }
+    default: return fail(SB_ERR_ARG, "sb_get: unknown quantity");
+    }
+}
+
+int sb_get_info(sb_handle *h, double *out, int32_t n)
+{
+    if (!h || !out) return fail(SB_ERR_ARG, "sb_get_info: null argument");
+    double v[4] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes};
+    for (int i = 0; i < n && i < 4; i++) out[i] = v[i];
+    return SB_OK;
+}
+
+}  // extern "C"

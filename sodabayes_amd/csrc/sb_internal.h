@@ -1,0 +1,120 @@
+// Internal state of a SODA sampler handle (device buffers and launchers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "sodabayes.h"
+
+namespace sb {
+
+constexpr int MAXMIX = 20;          // globals.f90:8 maxdist
+constexpr int MAXGRP = 100;         // globals.f90:34 maxgrp
+constexpr int MAXBIN = MAXMIX * MAXGRP;
+constexpr int CHUNK_WORDS = 128;    // 2048 individuals per genotype chunk (16 per word)
+constexpr int CHUNK_IND = CHUNK_WORDS * 16;
+
+// Read-only run parameters, passed by value to kernels.
+struct Par {
+    int method, n, p, nmix, ngroup, nbins;
+    int nblocks, blocksize, blockt, naind;
+    int ldw;           // 32-bit genotype words per SNP column (multiple of CHUNK_WORDS)
+    int nchunk;        // ldw / CHUNK_WORDS
+    int rng_mode, nchains;
+    int numit, burnin, thin;
+    double pizero, scale_va0, scale_ve, df_va, df_ve, vara0;
+    double mix[MAXMIX], alpha[MAXMIX];
+    uint64_t seed;
+};
+
+// Mutable per-chain scalars, kept on the device (one struct per chain).
+struct Scal {
+    double mu, vare, vara, scale_va, included;
+    double sum_r, rtr;                         // reduction results
+    double counter, mu_store, vare_store, vara_store, included_store;
+    double gp[MAXMIX], vare_gp[MAXMIX];
+    double P[MAXBIN], log_p[MAXBIN];           // (group, class), group fastest
+    double snpindist[MAXBIN], varindist[MAXBIN];
+    double snpstore[MAXBIN], varstore[MAXBIN], pstore[MAXBIN];
+    uint64_t rng[2];                           // replay stream-0 state
+    int32_t nchg;                              // changed SNPs of the current block
+    int32_t pad;
+};
+
+// Per-block replay / draw scratch (one block at a time).
+struct Scratch {
+    double *part;      // nchunk * blocksize partial X's products
+    double *partA;     // blocksize augmented products
+    double *partR;     // nchunk partial sums of r
+    double *rhs;       // blocksize
+    double *stemp;     // blocksize * nmix class probabilities
+    double *uu;        // blocksize * 2 recorded uniforms (replay)
+    int32_t *flag;     // blocksize drawn class
+    int32_t *chg;      // blocksize changed positions (ordered)
+    double *delta;     // blocksize beta_old - beta_new of changed positions
+    double *red;       // generic reduction partials
+};
+
+}  // namespace sb
+
+struct sb_handle {
+    sb_dims d;
+    sb::Par par;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int> ls;          // block start position (0-based), nblocks+1
+    std::vector<int64_t> tri_off; // packed-triangle offset per block, nblocks+1
+    std::vector<int32_t> permvec; // SNP id per position
+    std::vector<int32_t> pos_of;  // position per SNP id
+    size_t bytes = 0;
+    bool aug_built = false, initialised = false;
+    double setup_seconds = 0.0, last_sweep_ms = 0.0;
+    int64_t iterations = 0;
+
+    // genotype data
+    uint32_t *geno = nullptr;     // nloci x ldw, position order, 2-bit counts
+    double *f2 = nullptr;         // 2 f per position
+    double *sdv = nullptr;        // sqrt(2 f (1-f)) per position (0: monomorphic)
+    double *zlut = nullptr;       // 4 per position: z(0), z(1), z(2), 0
+    int32_t *grp = nullptr;       // 0-based group per position
+    int32_t *d_pos_by_id = nullptr; // position of SNP id (resync order)
+    // augmenting matrices
+    double *xxd = nullptr;        // nblocks
+    double *U = nullptr;          // packed upper triangles
+    int64_t *d_tri_off = nullptr;
+    int32_t *d_ls = nullptr;
+    // chain state (nchains copies)
+    double *r = nullptr;          // ldw*16 per chain
+    double *ra = nullptr;         // blocksize per chain
+    double *why = nullptr;        // ldw*16
+    double *beta = nullptr;       // nloci per chain (position order)
+    int32_t *cls = nullptr;       // nloci per chain
+    double *vara_s = nullptr;     // nloci per chain
+    double *gstore = nullptr, *vara_sstore = nullptr, *indiststore = nullptr;
+    sb::Scal *scal = nullptr;     // per chain
+    double *hyp = nullptr;        // device hyp ring
+    int hyp_cap = 0;
+    sb::Scratch sc{};
+    int32_t *resync_list = nullptr; // positions with beta != 0 in SNP-id order
+    int32_t *d_count = nullptr;
+
+    std::vector<void *> allocs;
+};
+
+namespace sb {
+// error helpers (sb_api.hip)
+int fail(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+int dev_alloc(sb_handle *h, void **p, size_t bytes);
+
+// setup (sb_setup.hip)
+int setup_pack(sb_handle *h, const uint8_t *bed, bool on_device);
+int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t *fail_col);
+int setup_gram_block(sb_handle *h, int block, double *G_host);
+
+// MCMC (sb_sweep.hip)
+int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why);
+int mcmc_run(sb_handle *h, int first, int count, double *hyp);
+}  // namespace sb

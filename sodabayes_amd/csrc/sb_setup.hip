@@ -1,0 +1,622 @@
+// Setup of the SODA augmenting matrices on gfx950 (replaces setup_soda and
+// get_aug_matrix, soda.f90:403-518):
+//   k_pack        .bed bytes -> 2-bit allele counts, block-position order
+//   k_gram        X_s'X_s on int8 MFMA (exact int32) + xxproduct's FP64
+//                 standardisation (soda.f90:520-556), all blocks in one launch
+//   k_pm_*        batched power method (soda.f90:378-401)
+//   k_chol_*      batched blocked Cholesky of dI - X_s'X_s (soda.f90:8-47)
+//   k_pack_tri    upper factors -> packed column-major triangles
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "sb_device.h"
+#include "sb_internal.h"
+
+namespace sb {
+namespace {
+
+constexpr double DEPS = 2.220446049250313080847e-16;    // epsilon(1.0d0)
+constexpr double POWER_TOL = (double)1.0e-14f;          // soda.f90:385 (f32 literal)
+constexpr double D_EPS_ADD = (double)0.001f;            // soda.f90:485 (f32 literal)
+constexpr int TILE = 64;                                // Cholesky tile
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ pack --
+
+// One 32-bit word = 16 genotypes (4 .bed bytes, little-endian); the code ->
+// count transform w - ((w >> 1) & 0x55..) is utilities.f90:489.  Bytes past
+// the SNP's ceil(n/4) are zero, as in load_geno_bits' zero-filled tail.
+__global__ void k_pack(const uint8_t *__restrict__ bed, int64_t nbytes, const int32_t *permvec,
+                       int p, int ldw, uint32_t *__restrict__ geno)
+{
+    int pos = blockIdx.x;
+    int snp = permvec[pos];
+    const uint8_t *col = bed + (int64_t)snp * nbytes;
+    for (int w = blockIdx.y * blockDim.x + threadIdx.x; w < ldw; w += gridDim.y * blockDim.x) {
+        uint32_t v = 0;
+        int64_t b0 = (int64_t)w * 4;
+        if (b0 + 3 < nbytes) {
+            v = (uint32_t)col[b0] | ((uint32_t)col[b0 + 1] << 8) | ((uint32_t)col[b0 + 2] << 16) |
+                ((uint32_t)col[b0 + 3] << 24);
+        } else {
+            for (int b = 0; b < 4; b++)
+                if (b0 + b < nbytes) v |= (uint32_t)col[b0 + b] << (8 * b);
+        }
+        geno[(int64_t)pos * ldw + w] = v - ((v >> 1) & 0x55555555u);
+    }
+}
+
+// ------------------------------------------------------------------ gram --
+
+// 8 two-bit counts -> 8 int8 lanes of a 64-bit MFMA operand.
+__device__ __forceinline__ uint32_t spread8(uint32_t b)
+{
+    uint32_t t = (b | (b << 12)) & 0x000F000Fu;
+    return (t | (t << 6)) & 0x03030303u;
+}
+__device__ __forceinline__ long spread16(uint32_t x)
+{
+    return (long)(((uint64_t)spread8((x >> 8) & 0xFFu) << 32) | spread8(x & 0xFFu));
+}
+
+// xxproduct's FP64 epilogue (soda.f90:527-554) for the pair (lo, hi) of block
+// positions, lo <= hi: the reference calls xxproduct(jj = hi, kk = lo).
+__device__ __forceinline__ double std_gram(int S, double fhi, double flo, int nind)
+{
+    double pj = fhi, pk = flo, qj = 1.0 - pj, qk = 1.0 - pk;
+    if (pj <= DEPS || qj <= DEPS || pk <= DEPS || qk <= DEPS) return 0.0;
+    pj = pj * 2.0;
+    pk = pk * 2.0;
+    double res = (double)S;
+    return (res - (double)nind * pj * pk) / sqrt(pj * pk * qj * qk);
+}
+
+// One workgroup = 64x64 tile of one block's Gram (4 waves of 32x32, each
+// 2x2 MFMA 16x16x32 i8 accumulators).  Lane group g = lane/16 streams 32
+// consecutive individuals per 128-individual step; A and B use the same
+// individual -> k mapping, so the int32 sums are exact and order-free.
+__global__ __launch_bounds__(256) void k_gram(const uint32_t *__restrict__ geno, int ldw,
+                                              const int32_t *__restrict__ ls,
+                                              const int64_t *__restrict__ goff,
+                                              const double *__restrict__ freqp, int nind,
+                                              double *__restrict__ G, int only_block)
+{
+    const int b = only_block >= 0 ? only_block : (int)blockIdx.y;
+    const int p0 = ls[b], nb = ls[b + 1] - ls[b];
+    const int T = (nb + 63) / 64;
+    int t = blockIdx.x;
+    if (t >= T * (T + 1) / 2) return;
+    int ti = 0;
+    while (t >= T - ti) {
+        t -= T - ti;
+        ti++;
+    }
+    const int tj = ti + t;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = ti * 64 + (wave >> 1) * 32, c0 = tj * 64 + (wave & 1) * 32;
+    const int g = lane >> 4, l16 = lane & 15;
+    const uint32_t *base = geno + (int64_t)p0 * ldw;
+    const uint32_t *pa[2], *pb[2];
+    bool va[2], vb[2];
+#pragma unroll
+    for (int f = 0; f < 2; f++) {
+        int ra = r0 + 16 * f + l16, cb = c0 + 16 * f + l16;
+        va[f] = ra < nb;
+        vb[f] = cb < nb;
+        pa[f] = base + (int64_t)(va[f] ? ra : 0) * ldw + g * 2;
+        pb[f] = base + (int64_t)(vb[f] ? cb : 0) * ldw + g * 2;
+    }
+    v4i acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) acc[a][c] = v4i{0, 0, 0, 0};
+    const int nsteps = ldw / 8;
+    for (int ks = 0; ks < nsteps; ks++) {
+        uint64_t wa[2], wb[2];
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            wa[f] = va[f] ? *(const uint64_t *)(pa[f] + ks * 8) : 0ull;
+            wb[f] = vb[f] ? *(const uint64_t *)(pb[f] + ks * 8) : 0ull;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            long a8[2], b8[2];
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                a8[f] = spread16((uint32_t)(wa[f] >> (16 * c)) & 0xFFFFu);
+                b8[f] = spread16((uint32_t)(wb[f] >> (16 * c)) & 0xFFFFu);
+            }
+#pragma unroll
+            for (int fa = 0; fa < 2; fa++)
+#pragma unroll
+                for (int fb = 0; fb < 2; fb++)
+                    acc[fa][fb] =
+                        __builtin_amdgcn_mfma_i32_16x16x32_i8(a8[fa], b8[fb], acc[fa][fb], 0, 0, 0);
+        }
+    }
+    double *Gb = G + goff[b];
+#pragma unroll
+    for (int fa = 0; fa < 2; fa++)
+#pragma unroll
+        for (int fb = 0; fb < 2; fb++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int row = r0 + 16 * fa + 4 * g + e;
+                int col = c0 + 16 * fb + l16;
+                if (row >= nb || col >= nb) continue;
+                int lo = row < col ? row : col, hi = row < col ? col : row;
+                double v = std_gram(acc[fa][fb][e], freqp[p0 + hi], freqp[p0 + lo], nind);
+                Gb[(int64_t)col * nb + row] = v;
+                Gb[(int64_t)row * nb + col] = v;
+            }
+}
+
+// ---------------------------------------------------------- power method --
+
+// Start vectors: p uniforms of the replay stream in block order (each
+// block's power_method draws n_b uniforms from stream 0, soda.f90:390), or
+// Philox uniforms in COUNTER mode.
+__global__ void k_start_replay(Scal *scal, int p, double *y)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    Xoro s{scal->rng[0], scal->rng[1]};
+    for (int i = 0; i < p; i++) y[i] = xoro_uniform(s);
+    scal->rng[0] = s.s1;
+    scal->rng[1] = s.s2;
+}
+
+__global__ void k_start_counter(uint64_t key, int p, double *y)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p) return;
+    PhiloxSrc src(key, 0xFFFFFFFFu, (uint32_t)i, 7u);
+    y[i] = src.next();
+}
+
+// v = y / sqrt(sum(y**2)) per block (soda.f90:392)
+__global__ __launch_bounds__(256) void k_pm_normalize(const int32_t *ls, const double *y, double *v)
+{
+    __shared__ double lds[8];
+    int b = blockIdx.x, p0 = ls[b], nb = ls[b + 1] - p0;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += y[p0 + i] * y[p0 + i];
+    s = block_sum<256>(s, lds);
+    __shared__ double nrm;
+    if (threadIdx.x == 0) nrm = sqrt(s);
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += 256) v[p0 + i] = y[p0 + i] / nrm;
+}
+
+// y = A v for active blocks; one wave per output row (A symmetric, so row
+// i of A is its contiguous column i).
+__global__ __launch_bounds__(256) void k_pm_gemv(const double *__restrict__ G,
+                                                 const int64_t *__restrict__ goff,
+                                                 const int32_t *__restrict__ ls,
+                                                 const int32_t *__restrict__ active,
+                                                 const double *__restrict__ v, double *__restrict__ y)
+{
+    int b = blockIdx.y;
+    if (!active[b]) return;
+    int p0 = ls[b], nb = ls[b + 1] - p0;
+    int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= nb) return;
+    const double *col = G + goff[b] + (int64_t)i * nb;
+    const double *vb = v + p0;
+    double acc = 0.0;
+    for (int j = lane; j < nb; j += 64) acc = fma(col[j], vb[j], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) y[p0 + i] = acc;
+}
+
+// lambda = y.v, delta = sum((y - lambda v)**2)/|lambda|, stop at
+// delta <= tol (f32 1e-14) or maxit = 1000 n; else v = y/|y| (soda.f90:391-398)
+__global__ __launch_bounds__(256) void k_pm_post(const int32_t *ls, int32_t *active, int32_t *iters,
+                                                 double *lam_out, const double *y, double *v)
+{
+    __shared__ double lds[8];
+    __shared__ double lam_s, nrm_s;
+    __shared__ int stop_s;
+    int b = blockIdx.x;
+    if (!active[b]) return;
+    int p0 = ls[b], nb = ls[b + 1] - p0;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += y[p0 + i] * v[p0 + i];
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) lam_s = s;
+    __syncthreads();
+    double lam = lam_s, dl = 0.0, ss = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        double t = y[p0 + i] - lam * v[p0 + i];
+        dl += t * t;
+        ss += y[p0 + i] * y[p0 + i];
+    }
+    dl = block_sum<256>(dl, lds);
+    ss = block_sum<256>(ss, lds);
+    if (threadIdx.x == 0) {
+        int it = iters[b] + 1;
+        iters[b] = it;
+        double delta = dl / fabs(lam);
+        stop_s = (delta <= POWER_TOL) || (it >= 1000 * nb);
+        if (stop_s) {
+            active[b] = 0;
+            lam_out[b] = fabs(lam) + D_EPS_ADD;  // xxd(b) = |lambda| + 0.001
+        }
+        nrm_s = sqrt(ss);
+    }
+    __syncthreads();
+    if (stop_s) return;
+    for (int i = threadIdx.x; i < nb; i += 256) v[p0 + i] = y[p0 + i] / nrm_s;
+}
+
+// ------------------------------------------------------------- cholesky --
+
+// M = dI - G in place (soda.f90:487-490: xxtemp = -xxtemp; diag = d + diag)
+__global__ void k_form_m(double *G, const int64_t *goff, const int32_t *ls, const double *xxd)
+{
+    int b = blockIdx.y, nb = ls[b + 1] - ls[b];
+    double d = xxd[b];
+    double *A = G + goff[b];
+    int64_t n2 = (int64_t)nb * nb;
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n2;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        double m = -A[k];
+        if (k % (nb + 1) == 0) m = d + m;
+        A[k] = m;
+    }
+}
+
+// Factor diagonal tile k of every block in LDS (dot-product form of
+// choleskyPD, soda.f90:33-44).  fail[b] = first failing 1-based column.
+__global__ __launch_bounds__(256) void k_chol_diag(double *G, const int64_t *goff, const int32_t *ls,
+                                                   int k, int32_t *fail)
+{
+    __shared__ double A[TILE][TILE + 1];
+    __shared__ double piv;
+    __shared__ int bad;
+    int b = blockIdx.x, nb = ls[b + 1] - ls[b];
+    if (fail[b] != 0) return;
+    int o = k * TILE;
+    if (o >= nb) return;
+    int m = nb - o < TILE ? nb - o : TILE;
+    double *Ab = G + goff[b];
+    for (int e = threadIdx.x; e < TILE * TILE; e += 256) {
+        int r = e % TILE, c = e / TILE;
+        A[r][c] = (r < m && c < m && r <= c) ? Ab[(int64_t)(o + c) * nb + (o + r)] : 0.0;
+    }
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int i = 0; i < m; i++) {
+        // A[i][i] currently holds M_ii - sum of previous tiles' updates
+        if (threadIdx.x < 64) {
+            double s = 0.0;
+            for (int q = threadIdx.x; q < i; q += 64) s = fma(A[q][i], A[q][i], s);
+            s = wave_sum(s);
+            if (threadIdx.x == 0) {
+                double tmp = A[i][i] - s;
+                if (tmp <= 0.0 || isnan(tmp)) {
+                    bad = o + i + 1;
+                } else {
+                    piv = sqrt(tmp);
+                    A[i][i] = piv;
+                }
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        for (int j = i + 1 + threadIdx.x; j < m; j += 256) {
+            double s = 0.0;
+            for (int q = 0; q < i; q++) s = fma(A[q][j], A[q][i], s);
+            A[i][j] = (A[i][j] - s) / piv;
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (threadIdx.x == 0) fail[b] = bad;
+        return;
+    }
+    for (int e = threadIdx.x; e < TILE * TILE; e += 256) {
+        int r = e % TILE, c = e / TILE;
+        if (r < m && c < m && r <= c) Ab[(int64_t)(o + c) * nb + (o + r)] = A[r][c];
+    }
+}
+
+// Row panel: U_kj = U_kk^{-T} A_kj for tiles j > k (forward substitution).
+__global__ __launch_bounds__(256) void k_chol_panel(double *G, const int64_t *goff, const int32_t *ls,
+                                                    int k, const int32_t *fail)
+{
+    __shared__ double Ukk[TILE][TILE + 1];
+    __shared__ double X[TILE][TILE + 1];
+    int b = blockIdx.y, nb = ls[b + 1] - ls[b];
+    if (fail[b] != 0) return;
+    int o = k * TILE, j0 = (k + 1 + blockIdx.x) * TILE;
+    if (j0 >= nb) return;
+    int m = nb - o < TILE ? nb - o : TILE;
+    int w = nb - j0 < TILE ? nb - j0 : TILE;
+    double *Ab = G + goff[b];
+    for (int e = threadIdx.x; e < TILE * TILE; e += 256) {
+        int r = e % TILE, c = e / TILE;
+        Ukk[r][c] = (r < m && c < m && r <= c) ? Ab[(int64_t)(o + c) * nb + (o + r)] : 0.0;
+        X[r][c] = (r < m && c < w) ? Ab[(int64_t)(j0 + c) * nb + (o + r)] : 0.0;
+    }
+    __syncthreads();
+    // 4 threads per column: column c, row-slices interleaved; rows solved in order
+    int c = threadIdx.x & 63, part = threadIdx.x >> 6;
+    for (int i = 0; i < m; i++) {
+        if (part == 0 && c < w) {
+            double s = 0.0;
+            for (int q = 0; q < i; q++) s = fma(Ukk[q][i], X[q][c], s);
+            X[i][c] = (X[i][c] - s) / Ukk[i][i];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < TILE * TILE; e += 256) {
+        int r = e % TILE, cc = e / TILE;
+        if (r < m && cc < w) Ab[(int64_t)(j0 + cc) * nb + (o + r)] = X[r][cc];
+    }
+}
+
+// Trailing update A_ij -= U_ki' U_kj for k < i <= j (upper tiles only).
+__global__ __launch_bounds__(256) void k_chol_update(double *G, const int64_t *goff, const int32_t *ls,
+                                                     int k, const int32_t *fail)
+{
+    __shared__ double Ui[TILE][TILE + 1];
+    __shared__ double Uj[TILE][TILE + 1];
+    int b = blockIdx.y, nb = ls[b + 1] - ls[b];
+    if (fail[b] != 0) return;
+    int T = (nb + TILE - 1) / TILE;
+    int R = T - k - 1;  // trailing tiles
+    int t = blockIdx.x;
+    if (R <= 0 || t >= R * (R + 1) / 2) return;
+    int ti = 0;
+    while (t >= R - ti) {
+        t -= R - ti;
+        ti++;
+    }
+    int ii = k + 1 + ti, jj = ii + t;
+    int o = k * TILE, i0 = ii * TILE, j0 = jj * TILE;
+    int m = nb - o < TILE ? nb - o : TILE;
+    int wi = nb - i0 < TILE ? nb - i0 : TILE;
+    int wj = nb - j0 < TILE ? nb - j0 : TILE;
+    double *Ab = G + goff[b];
+    for (int e = threadIdx.x; e < TILE * TILE; e += 256) {
+        int r = e % TILE, c = e / TILE;
+        Ui[r][c] = (r < m && c < wi) ? Ab[(int64_t)(i0 + c) * nb + (o + r)] : 0.0;
+        Uj[r][c] = (r < m && c < wj) ? Ab[(int64_t)(j0 + c) * nb + (o + r)] : 0.0;
+    }
+    __syncthreads();
+    // thread owns a 4x4 micro-tile: rows ri..ri+3 (of tile i), cols cj..cj+3
+    int tr = threadIdx.x & 15, tc = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[a][c] = 0.0;
+    for (int q = 0; q < m; q++) {
+        double ui[4], uj[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            ui[a] = Ui[q][tr + 16 * a];
+            uj[a] = Uj[q][tc + 16 * a];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[a][c] = fma(ui[a], uj[c], acc[a][c]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            int r = tr + 16 * a, cc = tc + 16 * c;
+            if (r < wi && cc < wj && (ii < jj || r <= cc)) {
+                double *dst = Ab + (int64_t)(j0 + cc) * nb + (i0 + r);
+                *dst = *dst - acc[a][c];
+            }
+        }
+}
+
+// Dense upper factor (ld n_b) -> packed column-major triangle.
+__global__ void k_pack_tri(const double *G, const int64_t *goff, const int32_t *ls,
+                           const int64_t *tri_off, double *U)
+{
+    int b = blockIdx.y, nb = ls[b + 1] - ls[b];
+    int j = blockIdx.x;
+    if (j >= nb) return;
+    const double *src = G + goff[b] + (int64_t)j * nb;
+    double *dst = U + tri_off[b] + (int64_t)j * (j + 1) / 2;
+    for (int i = threadIdx.x; i <= j; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- host side --
+
+int setup_pack(sb_handle *h, const uint8_t *bed, bool on_device)
+{
+    const Par &P = h->par;
+    int64_t nbytes = ((int64_t)P.n + 3) / 4;
+    const uint8_t *dbed = bed;
+    uint8_t *tmp = nullptr;
+    if (!on_device) {
+        if (hip_check(hipMalloc(&tmp, (size_t)(nbytes * P.p)), "hipMalloc(bed)")) return SB_ERR_NOMEM;
+        if (hip_check(hipMemcpyAsync(tmp, bed, (size_t)(nbytes * P.p), hipMemcpyHostToDevice, h->stream),
+                      "upload bed"))
+            return SB_ERR_HIP;
+        dbed = tmp;
+    }
+    int32_t *dperm = nullptr;
+    int rc = dev_alloc(h, (void **)&dperm, sizeof(int32_t) * P.p);
+    if (rc) return rc;
+    hipMemcpyAsync(dperm, h->permvec.data(), sizeof(int32_t) * P.p, hipMemcpyHostToDevice, h->stream);
+    dim3 grid(P.p, std::min(64, (P.ldw + 255) / 256));
+    k_pack<<<grid, 256, 0, h->stream>>>(dbed, nbytes, dperm, P.p, P.ldw, h->geno);
+    if (hip_check(hipGetLastError(), "k_pack")) return SB_ERR_HIP;
+    if (tmp) {
+        hipStreamSynchronize(h->stream);
+        hipFree(tmp);
+    }
+    return hip_check(hipStreamSynchronize(h->stream), "pack");
+}
+
+namespace {
+struct GramBuf {
+    double *G = nullptr;
+    int64_t *goff = nullptr;
+    std::vector<int64_t> hoff;
+};
+
+int gram_alloc(sb_handle *h, GramBuf &gb, int only_block)
+{
+    const Par &P = h->par;
+    int nb = P.nblocks;
+    gb.hoff.assign(nb + 1, 0);
+    for (int b = 0; b < nb; b++) {
+        int64_t n = h->ls[b + 1] - h->ls[b];
+        int64_t sz = (only_block < 0 || only_block == b) ? n * n : 0;
+        gb.hoff[b + 1] = gb.hoff[b] + sz;
+    }
+    if (only_block >= 0) {  // single block at offset 0
+        for (int b = 0; b <= nb; b++) gb.hoff[b] = 0;
+    }
+    int64_t total = only_block >= 0 ? (int64_t)(h->ls[only_block + 1] - h->ls[only_block]) *
+                                          (h->ls[only_block + 1] - h->ls[only_block])
+                                    : gb.hoff[nb];
+    if (hip_check(hipMalloc(&gb.G, sizeof(double) * (size_t)total), "hipMalloc(gram)"))
+        return SB_ERR_NOMEM;
+    if (hip_check(hipMalloc(&gb.goff, sizeof(int64_t) * (nb + 1)), "hipMalloc(goff)")) return SB_ERR_NOMEM;
+    hipMemcpyAsync(gb.goff, gb.hoff.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, h->stream);
+    return SB_OK;
+}
+
+void gram_free(GramBuf &gb)
+{
+    if (gb.G) hipFree(gb.G);
+    if (gb.goff) hipFree(gb.goff);
+    gb.G = nullptr;
+    gb.goff = nullptr;
+}
+
+int launch_gram(sb_handle *h, GramBuf &gb, const double *freqp, int only_block)
+{
+    const Par &P = h->par;
+    int T = (P.blocksize + 63) / 64;
+    dim3 grid(T * (T + 1) / 2, only_block >= 0 ? 1 : P.nblocks);
+    k_gram<<<grid, 256, 0, h->stream>>>(h->geno, P.ldw, h->d_ls, gb.goff, freqp, P.n, gb.G, only_block);
+    return hip_check(hipGetLastError(), "k_gram");
+}
+}  // namespace
+
+// freq per position is kept in the handle's zlut slot 3 (see sb_api.hip).
+static const double *freq_pos(sb_handle *h) { return h->f2 + h->par.p; }
+
+int setup_gram_block(sb_handle *h, int block, double *G_host)
+{
+    if (block < 0 || block >= h->par.nblocks) return fail(SB_ERR_ARG, "block out of range");
+    GramBuf gb;
+    int rc = gram_alloc(h, gb, block);
+    if (rc) return rc;
+    rc = launch_gram(h, gb, freq_pos(h), block);
+    if (!rc) {
+        int64_t n = h->ls[block + 1] - h->ls[block];
+        rc = hip_check(hipMemcpyAsync(G_host, gb.G, sizeof(double) * n * n, hipMemcpyDeviceToHost, h->stream),
+                       "download gram");
+        if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "gram sync");
+    }
+    gram_free(gb);
+    return rc;
+}
+
+int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t *fail_col)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    const Par &P = h->par;
+    const int nb = P.nblocks;
+    GramBuf gb;
+    int rc = gram_alloc(h, gb, -1);
+    if (rc) return rc;
+    rc = launch_gram(h, gb, freq_pos(h), -1);
+    if (rc) {
+        gram_free(gb);
+        return rc;
+    }
+    // power method
+    double *y = nullptr, *v = nullptr;
+    int32_t *active = nullptr, *iters = nullptr, *dfail = nullptr;
+    hipMalloc(&y, sizeof(double) * P.p);
+    hipMalloc(&v, sizeof(double) * P.p);
+    hipMalloc(&active, sizeof(int32_t) * nb);
+    hipMalloc(&iters, sizeof(int32_t) * nb);
+    hipMalloc(&dfail, sizeof(int32_t) * nb);
+    std::vector<int32_t> ones(nb, 1);
+    hipMemcpyAsync(active, ones.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, h->stream);
+    hipMemsetAsync(iters, 0, sizeof(int32_t) * nb, h->stream);
+    hipMemsetAsync(dfail, 0, sizeof(int32_t) * nb, h->stream);
+    if (P.rng_mode == SB_RNG_REPLAY)
+        k_start_replay<<<1, 1, 0, h->stream>>>(h->scal, P.p, y);
+    else
+        k_start_counter<<<(P.p + 255) / 256, 256, 0, h->stream>>>(P.seed, P.p, y);
+    k_pm_normalize<<<nb, 256, 0, h->stream>>>(h->d_ls, y, v);
+    dim3 ggrid((P.blocksize + 3) / 4, nb);
+    std::vector<int32_t> hact(nb);
+    const int64_t maxit = 1000LL * P.blocksize;
+    for (int64_t it = 0; it < maxit; it += 16) {
+        for (int q = 0; q < 16; q++) {
+            k_pm_gemv<<<ggrid, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, active, v, y);
+            k_pm_post<<<nb, 256, 0, h->stream>>>(h->d_ls, active, iters, h->xxd, y, v);
+        }
+        if (hip_check(hipMemcpyAsync(hact.data(), active, sizeof(int32_t) * nb, hipMemcpyDeviceToHost,
+                                     h->stream),
+                      "power method"))
+            break;
+        hipStreamSynchronize(h->stream);
+        bool any = false;
+        for (int b = 0; b < nb; b++) any |= hact[b] != 0;
+        if (!any) break;
+    }
+    // Cholesky of dI - G
+    dim3 fgrid(64, nb);
+    k_form_m<<<fgrid, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->xxd);
+    int T = (P.blocksize + TILE - 1) / TILE;
+    for (int k = 0; k < T; k++) {
+        k_chol_diag<<<nb, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
+        if (T - k - 1 > 0) {
+            k_chol_panel<<<dim3(T - k - 1, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
+            int R = T - k - 1;
+            k_chol_update<<<dim3(R * (R + 1) / 2, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
+        }
+    }
+    k_pack_tri<<<dim3(P.blocksize, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->d_tri_off, h->U);
+    rc = hip_check(hipGetLastError(), "cholesky");
+    std::vector<int32_t> hfail(nb);
+    if (!rc) {
+        hipMemcpyAsync(hfail.data(), dfail, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
+        if (xxd_out) hipMemcpyAsync(xxd_out, h->xxd, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream);
+        rc = hip_check(hipStreamSynchronize(h->stream), "build_aug");
+    }
+    hipFree(y);
+    hipFree(v);
+    hipFree(active);
+    hipFree(iters);
+    hipFree(dfail);
+    gram_free(gb);
+    if (rc) return rc;
+    if (fail_block) *fail_block = 0;
+    if (fail_col) *fail_col = 0;
+    for (int b = 0; b < nb; b++)
+        if (hfail[b] != 0) {
+            if (fail_block) *fail_block = b + 1;
+            if (fail_col) *fail_col = hfail[b];
+            return fail(SB_ERR_NOT_PD, "D - X'X is not positive definite in block " + std::to_string(b + 1) +
+                                           " at column " + std::to_string(hfail[b]));
+        }
+    h->aug_built = true;
+    h->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return SB_OK;
+}
+
+}  // namespace sb

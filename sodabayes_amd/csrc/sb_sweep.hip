@@ -1,0 +1,885 @@
+// The SODA Gibbs iteration on gfx950 (replaces soda_bayes, models.f90:9-605).
+//
+// Per iteration:  mu draw -> for each block: k_rhs (X_s'r + A_s'r_a, FP64),
+// k_draw_* (mixture class + effect, per SNP), k_update (r += Z_s delta,
+// r_a += A_s delta) -> k_stats + k_hyper (sigma2_g, sigma2_e, Dirichlet
+// pi, fresh r_a) -> posterior accumulation -> resync every 1000 iterations.
+//
+// Genotypes stay 2-bit packed (16 per 32-bit word, block-position order).
+// X_j'r is evaluated in centred form (sum_i x_ij r_i - 2 f_j sum_i r_i)/s_j;
+// the residual updates use the reference's standardised values
+// z = (x - 2f)/sqrt(2f(1-f)) (utilities.f90:531) from a 3-entry table.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <cmath>
+#include <vector>
+
+#include "sb_device.h"
+#include "sb_internal.h"
+
+namespace sb {
+namespace {
+
+constexpr int NT = 256;
+enum { TAG_MU = 1, TAG_SNP = 2, TAG_HYP = 3, TAG_RA = 4, TAG_INIT = 5 };
+
+__device__ __forceinline__ int bin_of(const Par &P, int g, int k) { return k * P.ngroup + g; }
+
+// ------------------------------------------------------------- mu draw --
+
+// r = r + mu and partial sums of r (models.f90:130)
+__global__ __launch_bounds__(NT) void k_mu_pre(Par P, double *r, const Scal *scal, double *red)
+{
+    __shared__ double lds[NT / WAVE];
+    double mu = scal->mu, s = 0.0;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < P.n; i += gridDim.x * NT) {
+        double v = r[i] + mu;
+        r[i] = v;
+        s += v;
+    }
+    s = block_sum<NT>(s, lds);
+    if (threadIdx.x == 0) red[blockIdx.x] = s;
+}
+
+// mu ~ N(sum(r)/n, vare/n) (models.f90:131) and the per-sweep constants
+// log_p, vare_gp (models.f90:330-338)
+__global__ void k_mu_draw(Par P, Scal *scal, const double *red, int nred, int rep)
+{
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < nred; i++) s += red[i];
+    double nn = (double)P.n;
+    double mean = s / nn, sd = sqrt(scal->vare / nn);
+    if (P.rng_mode == SB_RNG_REPLAY) {
+        XoroSrc src{{scal->rng[0], scal->rng[1]}};
+        scal->mu = draw_normal(src, mean, sd);
+        scal->rng[0] = src.st.s1;
+        scal->rng[1] = src.st.s2;
+    } else {
+        PhiloxSrc src(P.seed, (uint32_t)rep, 0xFFFFFFF0u, TAG_MU);
+        scal->mu = draw_normal(src, mean, sd);
+    }
+    if (P.method == SB_BAYES_B) {
+        scal->log_p[0] = log(scal->P[0]);
+        scal->log_p[1] = log(scal->P[1]);
+    } else if (P.method != SB_BAYES_A) {
+        for (int g = 0; g < P.ngroup; g++)
+            for (int k = 0; k < P.nmix; k++) scal->log_p[bin_of(P, g, k)] = log(scal->P[bin_of(P, g, k)]);
+        for (int k = 1; k < P.nmix; k++) scal->vare_gp[k] = scal->vare / scal->gp[k];
+    }
+}
+
+__global__ void k_mu_post(Par P, double *r, const Scal *scal)
+{
+    double mu = scal->mu;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < P.n; i += gridDim.x * NT) r[i] = r[i] - mu;
+}
+
+// ----------------------------------------------------------- block rhs --
+
+// Workgroups [0, nxg): X part.  WG -> (chunk c, SNP group); each wave holds
+// r for 2048 individuals in registers (32 per lane) and streams `spw` SNP
+// columns (one 8-byte load per lane per SNP, 512 B per wave), reducing each
+// product across the wave.  Workgroups [nxg, ...): A part, one wave per
+// column of the block's packed triangle.
+__global__ __launch_bounds__(NT) void k_rhs(Par P, const uint32_t *__restrict__ geno,
+                                            const double *__restrict__ r,
+                                            const double *__restrict__ ra,
+                                            const double *__restrict__ U, int64_t tri0, int p0, int nb,
+                                            int spw, int ngs, int nxg, double *__restrict__ part,
+                                            double *__restrict__ partA, double *__restrict__ partR)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = blockIdx.x;
+    if (t < nxg) {
+        const int c = t / ngs, sg = t % ngs;
+        const int w0 = c * CHUNK_WORDS + 2 * lane;
+        double rr[32];
+        const double2 *r2 = reinterpret_cast<const double2 *>(r + (int64_t)w0 * 16);
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            double2 v = r2[e];
+            rr[2 * e] = v.x;
+            rr[2 * e + 1] = v.y;
+        }
+        if (sg == 0 && wave == 0) {
+            double s = 0.0;
+#pragma unroll
+            for (int e = 0; e < 32; e++) s += rr[e];
+            s = wave_sum(s);
+            if (lane == 0) partR[c] = s;
+        }
+        const int j0 = (sg * 4 + wave) * spw;
+        const int j1 = min(nb, j0 + spw);
+        const uint32_t *gbase = geno + (int64_t)p0 * P.ldw + w0;
+        int j = j0;
+        for (; j + 1 < j1; j += 2) {
+            uint64_t ga = *reinterpret_cast<const uint64_t *>(gbase + (int64_t)j * P.ldw);
+            uint64_t gb = *reinterpret_cast<const uint64_t *>(gbase + (int64_t)(j + 1) * P.ldw);
+            double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+            for (int e = 0; e < 32; e++) {
+                a0 = fma((double)(uint32_t)((ga >> (2 * e)) & 3u), rr[e], a0);
+                a1 = fma((double)(uint32_t)((gb >> (2 * e)) & 3u), rr[e], a1);
+            }
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
+            if (lane == 0) {
+                part[(int64_t)c * P.blocksize + j] = a0;
+                part[(int64_t)c * P.blocksize + j + 1] = a1;
+            }
+        }
+        if (j < j1) {
+            uint64_t ga = *reinterpret_cast<const uint64_t *>(gbase + (int64_t)j * P.ldw);
+            double a0 = 0.0;
+#pragma unroll
+            for (int e = 0; e < 32; e++) a0 = fma((double)(uint32_t)((ga >> (2 * e)) & 3u), rr[e], a0);
+            a0 = wave_sum(a0);
+            if (lane == 0) part[(int64_t)c * P.blocksize + j] = a0;
+        }
+    } else {
+        const int jk = (t - nxg) * 4 + wave;
+        if (jk >= nb) return;
+        const double *col = U + tri0 + (int64_t)jk * (jk + 1) / 2;
+        double acc = 0.0;
+        for (int i = lane; i <= jk; i += 64) acc = fma(col[i], ra[i], acc);
+        acc = wave_sum(acc);
+        if (lane == 0) partA[jk] = acc;
+    }
+}
+
+// rhs of block position jk (models.f90:358-361) from the partial products.
+__device__ __forceinline__ double block_rhs(const Par &P, const double *part, const double *partA,
+                                            const double *partR, const double *f2, const double *sdv,
+                                            int p0, int jk)
+{
+    double sx = 0.0, sr = 0.0;
+    for (int c = 0; c < P.nchunk; c++) {
+        sx += part[(int64_t)c * P.blocksize + jk];
+        sr += partR[c];
+    }
+    int pos = p0 + jk;
+    double s = sdv[pos];
+    double rx = s > 0.0 ? (sx - f2[pos] * sr) / s : 0.0;
+    return rx + partA[jk];
+}
+
+// Class probabilities (models.f90:363-390) for BayesC/Cpi/R/Rc.
+__device__ __forceinline__ void mix_probs(const Par &P, const Scal *sc, int g, double rhs, double zz,
+                                          double zz_vare, double *stemp)
+{
+    double s[MAXMIX];
+    double vare = sc->vare;
+    s[0] = sc->log_p[bin_of(P, g, 0)];
+    for (int k = 1; k < P.nmix; k++) {
+        double logdetV = log(sc->gp[k] * zz_vare + 1.0);
+        double uhat = rhs / (zz + sc->vare_gp[k]);
+        s[k] = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[bin_of(P, g, k)];
+    }
+    for (int kk = 0; kk < P.nmix; kk++) {
+        double skk = s[kk], sk = 0.0;
+        bool overflow = false;
+        for (int j = 0; j < P.nmix; j++) {
+            if (j == kk) continue;
+            double clike = s[j] - skk;
+            if (clike < -700) continue;
+            if (clike > 700) {
+                overflow = true;
+                break;
+            }
+            sk = sk + exp(clike);
+        }
+        stemp[kk] = overflow ? 0.0 : 1.0 / (1.0 + sk);
+    }
+}
+
+// BayesB two-class probability of class 2 (models.f90:223-234)
+__device__ __forceinline__ double b_prob(const Scal *sc, double rhs, double zz, double zz_vare,
+                                         double vs)
+{
+    double vare = sc->vare;
+    double s1 = sc->log_p[0];
+    double logdetV = log(vs * zz_vare + 1.0);
+    double uhat = rhs / (zz + vare / vs);
+    double s2 = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[1];
+    double clike = s1 - s2;
+    if (clike > 700) return 0.0;
+    if (clike < -700) return 1.0;
+    return 1.0 / (1.0 + exp(clike));
+}
+
+// Per-SNP draw of the chosen method from uniform source `src`; returns the
+// new effect and sets class / variance.  Mirrors models.f90:145-161 (A),
+// 210-255 (B), 349-412 (C/Cpi/R/Rc).
+template <class Src>
+__device__ void draw_snp(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
+                         int pos, int g, double *beta, int32_t *cls, double *vara_s, double *delta,
+                         uint8_t *chg)
+{
+    double gk = beta[pos];
+    int cold = P.method == SB_BAYES_A ? 2 : cls[pos];
+    double vare = sc->vare;
+    if (P.method == SB_BAYES_A) {
+        double rhs = rhs0 + zz * gk;
+        double v1 = zz + vare / vara_s[pos];
+        double gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
+        beta[pos] = gn;
+        double scale = (gn * gn + sc->scale_va * P.df_va) / (P.df_va + 1.0);
+        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
+        delta[pos] = gk - gn;
+        chg[pos] = 1;
+        return;
+    }
+    double rhs = rhs0;
+    if (cold > 1) rhs = rhs + zz * gk;
+    if (P.method == SB_BAYES_B) {
+        double vs = vara_s[pos];
+        double sk = b_prob(sc, rhs, zz, zz_vare, vs);
+        double u = src.next();
+        int flag = u < sk ? 2 : 1;
+        double gv, gn;
+        if (flag == 1) {
+            gv = draw_normal(src, 0.0, sqrt(vs));
+            gn = 0.0;
+        } else {
+            double v1 = zz + vare / vs;
+            gv = draw_normal(src, rhs / v1, sqrt(vare / v1));
+            gn = gv;
+        }
+        beta[pos] = gn;
+        cls[pos] = flag;
+        double scale = (gv * gv + sc->scale_va * P.df_va) / (P.df_va + 1.0);
+        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
+        delta[pos] = gk - gn;
+        chg[pos] = !(flag == 1 && cold == 1);
+        return;
+    }
+    double stemp[MAXMIX];
+    mix_probs(P, sc, g, rhs, zz, zz_vare, stemp);
+    double u = src.next(), ss = 0.0;
+    int flag = 1;
+    for (int k = 0; k < P.nmix; k++) {
+        ss = ss + stemp[k];
+        if (u < ss) {
+            flag = k + 1;
+            break;
+        }
+    }
+    double gn = 0.0;
+    if (flag > 1) {
+        double v1 = zz + vare / sc->gp[flag - 1];
+        gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
+    }
+    beta[pos] = gn;
+    cls[pos] = flag;
+    delta[pos] = gk - gn;
+    chg[pos] = !(flag == 1 && cold == 1);
+}
+
+// COUNTER mode: one thread per SNP of the block.
+__global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, const double *part,
+                                                     const double *partA, const double *partR,
+                                                     const double *f2, const double *sdv,
+                                                     const int32_t *grp, const int32_t *snp_of_pos,
+                                                     int p0, int nb, double zz, int rep, double *beta,
+                                                     int32_t *cls, double *vara_s, double *delta,
+                                                     uint8_t *chg)
+{
+    int jk = blockIdx.x * NT + threadIdx.x;
+    if (jk >= nb) return;
+    int pos = p0 + jk;
+    double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
+    PhiloxSrc src(P.seed, (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
+    draw_snp(P, sc, src, rhs, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta, chg);
+}
+
+// REPLAY mode: rhs in parallel, then every draw from stream 0 in block
+// order on one thread (models.f90:350 with -nthreads 1).  BayesC/Cpi/R/Rc
+// precompute the class probabilities in parallel; the serial walk only
+// consumes uniforms; effects are finished in parallel.
+__global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const double *part,
+                                                    const double *partA, const double *partR,
+                                                    const double *f2, const double *sdv,
+                                                    const int32_t *grp, int p0, int nb, double zz,
+                                                    double *rhs_s, double *stemp_s, double *uu,
+                                                    int32_t *flag_s, double *beta, int32_t *cls,
+                                                    double *vara_s, double *delta, uint8_t *chg)
+{
+    const double zz_vare = zz / sc->vare;
+    const bool mix = P.method != SB_BAYES_A && P.method != SB_BAYES_B;
+    for (int jk = threadIdx.x; jk < nb; jk += NT) {
+        double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
+        rhs_s[jk] = rhs;
+        if (mix) {
+            int pos = p0 + jk;
+            double r2 = rhs;
+            if (cls[pos] > 1) r2 = r2 + zz * beta[pos];
+            rhs_s[jk] = r2;
+            mix_probs(P, sc, grp[pos], r2, zz, zz_vare, stemp_s + (int64_t)jk * P.nmix);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        XoroSrc src{{sc->rng[0], sc->rng[1]}};
+        if (mix) {
+            for (int jk = 0; jk < nb; jk++) {
+                const double *st = stemp_s + (int64_t)jk * P.nmix;
+                double u = src.next(), ss = 0.0;
+                int flag = 1;
+                for (int k = 0; k < P.nmix; k++) {
+                    ss = ss + st[k];
+                    if (u < ss) {
+                        flag = k + 1;
+                        break;
+                    }
+                }
+                flag_s[jk] = flag;
+                if (flag > 1) {
+                    uu[2 * jk] = src.next();
+                    uu[2 * jk + 1] = src.next();
+                }
+            }
+        } else {
+            for (int jk = 0; jk < nb; jk++) {
+                int pos = p0 + jk;
+                draw_snp(P, sc, src, rhs_s[jk], zz, zz_vare, pos, 0, beta, cls, vara_s, delta, chg);
+            }
+        }
+        sc->rng[0] = src.st.s1;
+        sc->rng[1] = src.st.s2;
+    }
+    __syncthreads();
+    if (!mix) return;
+    const double vare = sc->vare;
+    for (int jk = threadIdx.x; jk < nb; jk += NT) {
+        int pos = p0 + jk, flag = flag_s[jk], cold = cls[pos];
+        double gk = beta[pos], gn = 0.0;
+        if (flag > 1) {
+            double rhs = rhs_s[jk];
+            double v1 = zz + vare / sc->gp[flag - 1];
+            gn = box_muller(rhs / v1, sqrt(vare / v1), uu[2 * jk], uu[2 * jk + 1]);
+        }
+        beta[pos] = gn;
+        cls[pos] = flag;
+        delta[pos] = gk - gn;
+        chg[pos] = !(flag == 1 && cold == 1);
+    }
+}
+
+// ------------------------------------------------------- block update --
+
+// Ordered list of the block's changed positions in LDS (models.f90:416-422).
+__device__ int build_list(const uint8_t *chg, const double *delta, int p0, int nb, int *lst, double *dl,
+                          int *cnt)
+{
+    const int R = (nb + NT - 1) / NT;
+    int a = threadIdx.x * R, e = min(nb, a + R), c = 0;
+    for (int j = a; j < e; j++) c += chg[p0 + j] ? 1 : 0;
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < NT; i++) {
+            int v = cnt[i];
+            cnt[i] = s;
+            s += v;
+        }
+        cnt[NT] = s;
+    }
+    __syncthreads();
+    int o = cnt[threadIdx.x];
+    for (int j = a; j < e; j++)
+        if (chg[p0 + j]) {
+            lst[o] = j;
+            dl[o] = delta[p0 + j];
+            o++;
+        }
+    __syncthreads();
+    return cnt[NT];
+}
+
+// r += sum_j z_j delta_j (words [0, nrw)) and r_a += sum_j A_j delta_j
+// (rows after that), models.f90:416-430.  Each thread accumulates its own
+// individuals over the ordered list, then adds once (the reduction's
+// private-copy form).
+__global__ __launch_bounds__(NT) void k_update(Par P, const uint32_t *__restrict__ geno,
+                                               const double *__restrict__ zlut,
+                                               const double *__restrict__ U, int64_t tri0, int p0,
+                                               int nb, int nrw_blocks, const uint8_t *chg,
+                                               const double *delta, double *r, double *ra)
+{
+    __shared__ int lst[4096];
+    __shared__ double dl[4096];
+    __shared__ int cnt[NT + 1];
+    const int m = build_list(chg, delta, p0, nb, lst, dl, cnt);
+    if (m == 0) return;
+    if ((int)blockIdx.x < nrw_blocks) {
+        const int w = blockIdx.x * NT + threadIdx.x;
+        if (w >= P.ldw || w * 16 >= P.n) return;
+        double acc[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[e] = 0.0;
+        for (int q = 0; q < m; q++) {
+            int pos = p0 + lst[q];
+            uint32_t gw = geno[(int64_t)pos * P.ldw + w];
+            const double *zl = zlut + 4 * (int64_t)pos;
+            double z0 = zl[0], z1 = zl[1], z2 = zl[2], d = dl[q];
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                uint32_t x = (gw >> (2 * e)) & 3u;
+                double z = x == 0 ? z0 : (x == 1 ? z1 : z2);
+                acc[e] = __dadd_rn(acc[e], __dmul_rn(z, d));
+            }
+        }
+        int i0 = w * 16;
+#pragma unroll
+        for (int e = 0; e < 16; e++)
+            if (i0 + e < P.n) r[i0 + e] = __dadd_rn(r[i0 + e], acc[e]);
+    } else {
+        const int i = (blockIdx.x - nrw_blocks) * NT + threadIdx.x;
+        if (i >= nb) return;
+        double acc = 0.0;
+        for (int q = 0; q < m; q++) {
+            int jk = lst[q];
+            if (i <= jk) acc = __dadd_rn(acc, __dmul_rn(U[tri0 + (int64_t)jk * (jk + 1) / 2 + i], dl[q]));
+        }
+        ra[i] = __dadd_rn(ra[i], acc);
+    }
+}
+
+// ------------------------------------------------ end-of-iteration stats --
+
+// partial r'r over [0, n)
+__global__ __launch_bounds__(NT) void k_rtr(Par P, const double *r, double *red)
+{
+    __shared__ double lds[NT / WAVE];
+    double s = 0.0;
+    for (int i = blockIdx.x * NT + threadIdx.x; i < P.n; i += gridDim.x * NT) s += r[i] * r[i];
+    s = block_sum<NT>(s, lds);
+    if (threadIdx.x == 0) red[blockIdx.x] = s;
+}
+
+// per (group, class) bin: count and sum of g^2 over a chunk of positions
+// (models.f90:496-503; BayesB 321-325; BayesA 194)
+__global__ __launch_bounds__(NT) void k_bins(Par P, const double *beta, const int32_t *cls,
+                                             const int32_t *grp, int chunk, double *cnt_out,
+                                             double *var_out)
+{
+    __shared__ double lds[NT / WAVE];
+    int bin = blockIdx.y, wg = blockIdx.x;
+    int g = bin % P.ngroup, k = bin / P.ngroup;
+    int a = wg * chunk, e = min(P.p, a + chunk);
+    double c = 0.0, v = 0.0;
+    for (int pos = a + threadIdx.x; pos < e; pos += NT) {
+        bool in = P.method == SB_BAYES_A ? true : (grp[pos] == g && cls[pos] == k + 1);
+        if (in) {
+            c += 1.0;
+            v += beta[pos] * beta[pos];
+        }
+    }
+    c = block_sum<NT>(c, lds);
+    v = block_sum<NT>(v, lds);
+    if (threadIdx.x == 0) {
+        cnt_out[(int64_t)bin * gridDim.x + wg] = c;
+        var_out[(int64_t)bin * gridDim.x + wg] = v;
+    }
+}
+
+// Hyperparameter draws, stores and the trace record (models.f90:496-569).
+__global__ void k_hyper(Par P, Scal *sc, const double *red_rtr, int nrtr, const double *cnt,
+                        const double *var, int nwg, int rep, int store, double *hyp, int hyp_idx)
+{
+    if (threadIdx.x != 0) return;
+    const int nb = P.nbins;
+    for (int b = 0; b < nb; b++) {
+        double c = 0.0, v = 0.0;
+        for (int w = 0; w < nwg; w++) {
+            c += cnt[(int64_t)b * nwg + w];
+            v += var[(int64_t)b * nwg + w];
+        }
+        sc->snpindist[b] = c;
+        sc->varindist[b] = v;
+    }
+    XoroSrc xs{{sc->rng[0], sc->rng[1]}};
+    PhiloxSrc ps(P.seed, (uint32_t)rep, 0xFFFFFFF1u, TAG_HYP);
+    const bool replay = P.rng_mode == SB_RNG_REPLAY;
+    if (P.method == SB_BAYES_A) {
+        sc->snpindist[0] = 0.0;  // BayesA keeps a single variance bin
+    } else if (P.method == SB_BAYES_B) {
+        double inc = 0.0, v2 = 0.0;
+        inc = sc->snpindist[1];
+        v2 = sc->varindist[1];
+        sc->included = inc;
+        sc->snpindist[0] = (double)P.p - inc;
+        sc->snpindist[1] = inc;
+        sc->varindist[0] = 0.0;
+        sc->varindist[1] = v2;
+    } else {
+        double excl = 0.0;
+        for (int g = 0; g < P.ngroup; g++) excl += sc->snpindist[bin_of(P, g, 0)];
+        sc->included = (double)P.p - excl;
+        double scale = 0.0;
+        for (int g = 0; g < P.ngroup; g++)
+            for (int k = 1; k < P.nmix; k++) scale = scale + sc->varindist[bin_of(P, g, k)] / P.mix[k];
+        scale = (scale + sc->scale_va * P.df_va) / (P.df_va + sc->included);
+        double dof = sc->included + P.df_va;
+        sc->vara = replay ? draw_sinvchi2(xs, dof, scale) : draw_sinvchi2(ps, dof, scale);
+        for (int k = 0; k < P.nmix; k++) sc->gp[k] = P.mix[k] * sc->vara;
+    }
+    double rtr = 0.0;
+    for (int i = 0; i < nrtr; i++) rtr += red_rtr[i];
+    double nn = (double)P.n;
+    double scale = (rtr + P.scale_ve * P.df_ve) / (nn + P.df_ve);
+    sc->vare = replay ? draw_sinvchi2(xs, nn + P.df_ve, scale) : draw_sinvchi2(ps, nn + P.df_ve, scale);
+    if (P.method == SB_BAYES_CPI || P.method == SB_BAYES_R || P.method == SB_BAYES_RC) {
+        for (int g = 0; g < P.ngroup; g++) {
+            double x[MAXMIX], sx = 0.0;
+            for (int k = 0; k < P.nmix; k++) {
+                double a = sc->snpindist[bin_of(P, g, k)] + P.alpha[k];
+                x[k] = replay ? draw_gamma(xs, a, 1.0) : draw_gamma(ps, a, 1.0);
+            }
+            for (int k = 0; k < P.nmix; k++) sx += x[k];
+            for (int k = 0; k < P.nmix; k++) sc->P[bin_of(P, g, k)] = x[k] / sx;
+        }
+    }
+    sc->rng[0] = xs.st.s1;
+    sc->rng[1] = xs.st.s2;
+    if (store) {
+        sc->counter += 1.0;
+        sc->mu_store += sc->mu;
+        for (int b = 0; b < nb; b++) sc->varstore[b] += sc->varindist[b];
+        sc->vare_store += sc->vare;
+        if (P.method != SB_BAYES_A) {
+            sc->included_store += sc->included;
+            for (int b = 0; b < nb; b++) sc->snpstore[b] += sc->snpindist[b];
+            if (P.method != SB_BAYES_B && P.method != SB_BAYES_C)
+                for (int b = 0; b < nb; b++) sc->pstore[b] += sc->P[b];
+        }
+        if (P.method != SB_BAYES_A && P.method != SB_BAYES_B) sc->vara_store += sc->vara;
+    }
+    if (hyp && hyp_idx >= 0) {
+        double *h = hyp + (int64_t)hyp_idx * (5 + 2 * nb);
+        h[0] = rep;
+        h[1] = sc->included;
+        h[2] = sc->mu;
+        h[3] = sc->vara;
+        h[4] = sc->vare;
+        for (int b = 0; b < nb; b++) {
+            h[5 + b] = sc->snpindist[b];
+            h[5 + nb + b] = sc->varindist[b];
+        }
+    }
+}
+
+// fresh r_a ~ N(0, vare) (models.f90:523-525; init 118-120)
+__global__ void k_ra_replay(Par P, Scal *sc, double *ra)
+{
+    if (threadIdx.x != 0) return;
+    XoroSrc src{{sc->rng[0], sc->rng[1]}};
+    double sd = sqrt(sc->vare);
+    for (int i = 0; i < P.naind; i++) ra[i] = draw_normal(src, 0.0, sd);
+    sc->rng[0] = src.st.s1;
+    sc->rng[1] = src.st.s2;
+}
+
+__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag)
+{
+    int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= P.naind) return;
+    PhiloxSrc src(P.seed, (uint32_t)rep, (uint32_t)i, tag);
+    ra[i] = draw_normal(src, 0.0, sqrt(sc->vare));
+}
+
+// posterior sums per SNP (models.f90:528-544)
+__global__ void k_accum(Par P, const double *beta, const int32_t *cls, const double *vara_s,
+                        double *gstore, double *indiststore, double *vara_sstore)
+{
+    int pos = blockIdx.x * NT + threadIdx.x;
+    if (pos >= P.p) return;
+    gstore[pos] += beta[pos];
+    if (P.method != SB_BAYES_A) indiststore[(int64_t)(cls[pos] - 1) * P.p + pos] += 1.0;
+    if (P.method == SB_BAYES_A || P.method == SB_BAYES_B) vara_sstore[pos] += vara_s[pos];
+}
+
+// ------------------------------------------------------------- resync --
+
+// Positions with g != 0 in SNP-id order (one workgroup).
+__global__ __launch_bounds__(NT) void k_nonzero(Par P, const int32_t *pos_by_id, const double *beta,
+                                                int32_t *list, int32_t *count)
+{
+    __shared__ int cnt[NT + 1];
+    const int R = (P.p + NT - 1) / NT;
+    int a = threadIdx.x * R, e = min(P.p, a + R), c = 0;
+    for (int id = a; id < e; id++) c += beta[pos_by_id[id]] != 0.0;
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int i = 0; i < NT; i++) {
+            int v = cnt[i];
+            cnt[i] = s;
+            s += v;
+        }
+        *count = s;
+    }
+    __syncthreads();
+    int o = cnt[threadIdx.x];
+    for (int id = a; id < e; id++) {
+        int pos = pos_by_id[id];
+        if (beta[pos] != 0.0) list[o++] = pos;
+    }
+}
+
+// compute_residuals (utilities.f90:609-614): r_i = y_i - sum_j z_ij g_j - mu,
+// the sum over SNP ids in order (zero effects add nothing).
+__global__ __launch_bounds__(NT) void k_resync(Par P, const uint32_t *geno, const double *zlut,
+                                               const double *beta, const int32_t *list,
+                                               const int32_t *count, const double *why,
+                                               const Scal *sc, double *r)
+{
+    int w = blockIdx.x * NT + threadIdx.x;
+    if (w >= P.ldw || w * 16 >= P.n) return;
+    double acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = 0.0;
+    int m = *count;
+    for (int q = 0; q < m; q++) {
+        int pos = list[q];
+        uint32_t gw = geno[(int64_t)pos * P.ldw + w];
+        const double *zl = zlut + 4 * (int64_t)pos;
+        double z0 = zl[0], z1 = zl[1], z2 = zl[2], b = beta[pos];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            uint32_t x = (gw >> (2 * e)) & 3u;
+            double z = x == 0 ? z0 : (x == 1 ? z1 : z2);
+            acc[e] = __dadd_rn(acc[e], __dmul_rn(z, b));
+        }
+    }
+    double mu = sc->mu;
+    int i0 = w * 16;
+#pragma unroll
+    for (int e = 0; e < 16; e++)
+        if (i0 + e < P.n) r[i0 + e] = __dsub_rn(__dsub_rn(why[i0 + e], acc[e]), mu);
+}
+
+// ---------------------------------------------------------------- init --
+
+// Draws of the initial state (models.f90:115-120): sigma2_e from y'y of the
+// initial residuals, then r_a.
+__global__ void k_init_draws(Par P, Scal *sc, double yty)
+{
+    if (threadIdx.x != 0) return;
+    double nn = (double)P.n;
+    double scale = (yty + P.scale_ve * P.df_ve) / (nn + P.df_ve);
+    if (P.rng_mode == SB_RNG_REPLAY) {
+        XoroSrc src{{sc->rng[0], sc->rng[1]}};
+        sc->vare = draw_sinvchi2(src, nn + P.df_ve, scale);
+        sc->rng[0] = src.st.s1;
+        sc->rng[1] = src.st.s2;
+    } else {
+        PhiloxSrc src(P.seed, 0xFFFFFFFEu, 0xFFFFFFF2u, TAG_INIT);
+        sc->vare = draw_sinvchi2(src, nn + P.df_ve, scale);
+    }
+}
+
+__global__ void k_init_r(Par P, const double *why, double mu, double *r)
+{
+    int i = blockIdx.x * NT + threadIdx.x;
+    if (i < P.n) r[i] = (why[i] - 0.0) - mu;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- host side --
+
+int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
+{
+    if (!h->aug_built) return fail(SB_ERR_STATE, "sb_mcmc_init before sb_build_aug");
+    Par &P = h->par;
+    P.pizero = pr->pizero;
+    P.scale_va0 = pr->scale_va;
+    P.scale_ve = pr->scale_ve;
+    P.df_va = pr->df_va;
+    P.df_ve = pr->df_ve;
+    P.vara0 = pr->vara0;
+    for (int k = 0; k < MAXMIX; k++) {
+        P.mix[k] = pr->mix[k];
+        P.alpha[k] = pr->alpha[k];
+    }
+    if (P.method == SB_BAYES_C || P.method == SB_BAYES_CPI) {  // useroptions.f90:503-507
+        P.mix[0] = 0.0;
+        P.mix[1] = 1.0;
+    }
+    P.numit = pr->numit;
+    P.burnin = pr->burnin;
+    P.thin = pr->thin > 0 ? pr->thin : 1;
+    if (P.rng_mode == SB_RNG_COUNTER) P.seed = (uint64_t)pr->seed;
+    const int n = P.n, p = P.p, nmix = P.nmix, ngroup = P.ngroup;
+
+    // Host part of models.f90:56-114 (no draws): sums in the reference's
+    // sequential order so the initial scalars are the reference's doubles.
+    Scal sc;
+    std::memset(&sc, 0, sizeof sc);
+    if (hip_check(hipMemcpy(&sc.rng, &h->scal->rng, sizeof sc.rng, hipMemcpyDeviceToHost), "rng state"))
+        return SB_ERR_HIP;
+    auto IDX = [&](int g, int k) { return k * ngroup + g; };
+    if (P.method != SB_BAYES_A) {
+        if (P.method == SB_BAYES_B || P.method == SB_BAYES_C) {
+            for (int g = 0; g < ngroup; g++) {
+                sc.P[IDX(g, 0)] = P.pizero;
+                sc.P[IDX(g, 1)] = 1.0 - P.pizero;
+            }
+        } else {
+            for (int g = 0; g < ngroup; g++) {
+                sc.P[IDX(g, 0)] = 0.5;
+                double s = 0.0;
+                for (int k = 1; k < nmix; k++) sc.P[IDX(g, k)] = 1.0 / P.mix[k];
+                for (int k = 1; k < nmix; k++) s += sc.P[IDX(g, k)];
+                for (int k = 1; k < nmix; k++) sc.P[IDX(g, k)] = 0.5 * sc.P[IDX(g, k)] / s;
+            }
+        }
+    }
+    double nn = (double)n, sw = 0.0;
+    for (int i = 0; i < n; i++) sw += why[i];
+    sc.mu = sw / nn;
+    double vara = P.vara0, scale_va = P.scale_va0;
+    if (P.df_va <= 0.0) {
+        double yhat = sw / nn, vary = 0.0;
+        for (int i = 0; i < n; i++) vary += (why[i] - yhat) * (why[i] - yhat);
+        vary = vary / (nn - 1.0);
+        if (P.method == SB_BAYES_A) {
+            vara = scale_va * vary / p;
+        } else if (P.method == SB_BAYES_B) {
+            vara = scale_va * vary / p / sc.P[IDX(0, 1)];
+        } else {
+            for (int g = 0; g < ngroup; g++) {
+                double d = 0.0;
+                for (int k = 0; k < nmix; k++) d += sc.P[IDX(g, k)] * P.mix[k];
+                vara = vara + d;
+            }
+            vara = scale_va * vary / p / vara;
+        }
+        scale_va = 0.0;
+    } else {
+        vara = scale_va * P.df_va / (P.df_va + 2.0);
+    }
+    sc.vara = vara;
+    sc.scale_va = scale_va;
+    for (int k = 0; k < nmix; k++) sc.gp[k] = P.mix[k] * vara;
+    double yty = 0.0;
+    for (int i = 0; i < n; i++) {
+        double r = (why[i] - 0.0) - sc.mu;
+        yty += r * r;
+    }
+    if (hip_check(hipMemcpyAsync(h->scal, &sc, sizeof sc, hipMemcpyHostToDevice, h->stream), "init scal"))
+        return SB_ERR_HIP;
+    hipMemsetAsync(h->why, 0, sizeof(double) * (size_t)P.ldw * 16, h->stream);
+    hipMemcpyAsync(h->why, why, sizeof(double) * n, hipMemcpyHostToDevice, h->stream);
+    hipMemsetAsync(h->r, 0, sizeof(double) * (size_t)P.ldw * 16, h->stream);
+    hipMemsetAsync(h->ra, 0, sizeof(double) * P.blocksize, h->stream);
+    hipMemsetAsync(h->beta, 0, sizeof(double) * p, h->stream);
+    hipMemsetAsync(h->gstore, 0, sizeof(double) * p, h->stream);
+    hipMemsetAsync(h->vara_sstore, 0, sizeof(double) * p, h->stream);
+    hipMemsetAsync(h->indiststore, 0, sizeof(double) * p * nmix, h->stream);
+    std::vector<int32_t> two(p, 2);
+    hipMemcpyAsync(h->cls, two.data(), sizeof(int32_t) * p, hipMemcpyHostToDevice, h->stream);
+    std::vector<double> vs(p, vara);
+    hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p, hipMemcpyHostToDevice, h->stream);
+    k_init_r<<<(n + NT - 1) / NT, NT, 0, h->stream>>>(P, h->why, sc.mu, h->r);
+    k_init_draws<<<1, 1, 0, h->stream>>>(P, h->scal, yty);
+    if (P.rng_mode == SB_RNG_REPLAY)
+        k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
+    else
+        k_ra_counter<<<(P.naind + NT - 1) / NT, NT, 0, h->stream>>>(P, h->scal, h->ra, 0x7FFFFFFF,
+                                                                   TAG_INIT);
+    if (hip_check(hipGetLastError(), "init kernels")) return SB_ERR_HIP;
+    if (hip_check(hipStreamSynchronize(h->stream), "init")) return SB_ERR_HIP;
+    h->initialised = true;
+    h->iterations = 0;
+    return SB_OK;
+}
+
+int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
+{
+    if (!h->initialised) return fail(SB_ERR_STATE, "sb_mcmc_run before sb_mcmc_init");
+    if (count < 0 || first < 1) return fail(SB_ERR_ARG, "bad iteration range");
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    const int nred = std::min(1024, (P.n + NT - 1) / NT);
+    const int hw = 5 + 2 * P.nbins;
+    int nrec = 0;
+    for (int rep = first; rep < first + count; rep++) nrec += (rep % P.thin) == 0;
+    double *dhyp = nullptr;
+    if (hyp_host && nrec > 0)
+        if (hip_check(hipMalloc(&dhyp, sizeof(double) * (size_t)nrec * hw), "hipMalloc(hyp)")) return SB_ERR_NOMEM;
+    const int bin_chunk = 8192;
+    const int nwg_bins = (P.p + bin_chunk - 1) / bin_chunk;
+    double *bcnt = h->sc.red + 4096, *bvar = bcnt + (int64_t)P.nbins * nwg_bins;
+    uint8_t *chg = reinterpret_cast<uint8_t *>(h->sc.flag);
+    double *delta = h->sc.delta;
+    std::vector<double> xxd(P.nblocks);
+    hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
+    auto t0 = std::chrono::steady_clock::now();
+    int rec = 0;
+    for (int rep = first; rep < first + count; rep++) {
+        k_mu_pre<<<nred, NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
+        k_mu_draw<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, rep);
+        k_mu_post<<<nred, NT, 0, st>>>(P, h->r, h->scal);
+        for (int b = 0; b < P.nblocks; b++) {
+            const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+            const int64_t tri0 = h->tri_off[b];
+            // X part: aim for >= 2048 waves
+            int spw = (int)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 8192));
+            int ngs = (nb + 4 * spw - 1) / (4 * spw);
+            int nxg = ngs * P.nchunk;
+            int nag = (nb + 3) / 4;
+            k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, p0, nb, spw, ngs, nxg,
+                                            h->sc.part, h->sc.partA, h->sc.partR);
+            if (P.rng_mode == SB_RNG_REPLAY)
+                k_draw_replay<<<1, NT, 0, st>>>(P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2,
+                                                h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp,
+                                                h->sc.uu, h->sc.chg, h->beta, h->cls, h->vara_s, delta,
+                                                chg);
+            else
+                k_draw_counter<<<(nb + NT - 1) / NT, NT, 0, st>>>(
+                    P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
+                    h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg);
+            int nrw = (P.ldw + NT - 1) / NT;
+            int nra = (nb + NT - 1) / NT;
+            k_update<<<nrw + nra, NT, 0, st>>>(P, h->geno, h->zlut, h->U, tri0, p0, nb, nrw, chg, delta,
+                                               h->r, h->ra);
+        }
+        k_rtr<<<nred, NT, 0, st>>>(P, h->r, h->sc.red);
+        k_bins<<<dim3(nwg_bins, P.nbins), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+        const int store = (rep > P.burnin) && (rep % P.thin == 0);
+        const int hidx = (dhyp && rep % P.thin == 0) ? rec++ : -1;
+        k_hyper<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, rep, store, dhyp, hidx);
+        if (P.rng_mode == SB_RNG_REPLAY)
+            k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
+        else
+            k_ra_counter<<<(P.naind + NT - 1) / NT, NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA);
+        if (store)
+            k_accum<<<(P.p + NT - 1) / NT, NT, 0, st>>>(P, h->beta, h->cls, h->vara_s, h->gstore,
+                                                       h->indiststore, h->vara_sstore);
+        if (rep % 1000 == 0) {
+            k_nonzero<<<1, NT, 0, st>>>(P, h->d_pos_by_id, h->beta, h->resync_list, h->d_count);
+            k_resync<<<(P.ldw + NT - 1) / NT, NT, 0, st>>>(P, h->geno, h->zlut, h->beta, h->resync_list,
+                                                          h->d_count, h->why, h->scal, h->r);
+        }
+        if (hip_check(hipGetLastError(), "sweep launch")) return SB_ERR_HIP;
+    }
+    int rc = hip_check(hipStreamSynchronize(st), "sweep");
+    auto t1 = std::chrono::steady_clock::now();
+    if (count > 0) h->last_sweep_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / count;
+    h->iterations += count;
+    if (!rc && dhyp) {
+        rc = hip_check(hipMemcpy(hyp_host, dhyp, sizeof(double) * (size_t)nrec * hw, hipMemcpyDeviceToHost),
+                       "download hyp");
+    }
+    if (dhyp) hipFree(dhyp);
+    return rc;
+}
+
+}  // namespace sb

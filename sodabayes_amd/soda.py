@@ -1,0 +1,326 @@
+"""Host side of the SODA branch, above the C ABI (include/sodabayes.h).
+
+Mirrors what run_program_sodaBayes does around its two SODA calls
+(SODABayes.f90:123-142): option values as the reference parses them, allele
+frequencies, the seeded stream, the block partition (soda.f90:407-417) and
+permutation (soda.f90:429-435), then setup_soda -> :meth:`SodaSampler.setup`
+and soda_bayes -> :meth:`SodaSampler.init` / :meth:`SodaSampler.run`.
+Results come back under the reference's variable names.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .plink import allele_freq
+
+MAXDIST = 20   # globals.f90:8
+MAXGRP = 100   # globals.f90:34
+
+
+def f32(x: float) -> float:
+    """cast_float (utilities.f90:168-171) reads options into a default real."""
+    return float(np.float32(x))
+
+
+@dataclass
+class SodaOptions:
+    """The MCMC options of useroptions.f90:24-54 with the reference defaults.
+
+    Float options are rounded through float32 as cast_float does; ``mix`` and
+    ``alpha`` accept the reference's comma-separated strings.
+    """
+    method: str = "BayesCpi"
+    pi: float = 0.99
+    scale_va: float = 0.01
+    scale_ve: float = 0.01
+    df_va: float = 0.01
+    df_ve: float = 0.01
+    alpha: object = 1.0
+    numit: int = 50000
+    burnin: int = 5000
+    thin: int = 1
+    nmix: int = 2
+    mix: object = "0.0,1.0"
+    seed: int = 0
+    nthreads: int = 1
+    blocksize: int = 0
+    nblocks: int = 0
+    rblocks: bool = False
+    vara0: float = 0.0            # module vara before the job (models.f90:99-101)
+    # build-side choices (no reference counterpart)
+    rng: str = "replay"           # "replay" (xoroshiro stream 0) or "counter" (Philox)
+    precision: str = "fp64"
+
+    def method_code(self) -> int:
+        for k, v in _lib.METHODS.items():
+            if k.upper() == str(self.method).upper():
+                return v
+        raise ValueError("supported methods: BayesA, BayesB, BayesC, BayesCpi, BayesR, and BayesRc")
+
+    def resolved_nmix(self) -> int:
+        """parse_nmix, useroptions.f90:375-400"""
+        m = self.method_code()
+        if self.nmix > MAXDIST:
+            raise ValueError(f"nmix is too big. maximum allowed: {MAXDIST}")
+        if m == 1:
+            return 1
+        if m in (2, 3, 4):
+            return 2
+        return int(self.nmix)
+
+    def mix_values(self) -> list[float]:
+        """parse_priors' -mix handling, useroptions.f90:501-518"""
+        m, nmix = self.method_code(), self.resolved_nmix()
+        if m in (1, 2):
+            return [0.0] * nmix
+        if m in (3, 4):
+            return [0.0, 1.0]
+        vals = self.mix.split(",") if isinstance(self.mix, str) else list(self.mix)
+        if len(vals) != nmix:
+            raise ValueError(f"Number of mixture classes {nmix} but {len(vals)} effect sizes specified (mix)")
+        return [f32(float(v)) for v in vals]
+
+    def alpha_values(self) -> list[float]:
+        """parse_priors' -alpha handling, useroptions.f90:519-535"""
+        nmix = self.resolved_nmix()
+        vals = self.alpha.split(",") if isinstance(self.alpha, str) else (
+            [self.alpha] if np.isscalar(self.alpha) else list(self.alpha))
+        if len(vals) == 1:
+            return [f32(float(vals[0]))] * nmix
+        if len(vals) != nmix:
+            raise ValueError(f"Number of mixture classes {nmix} but {len(vals)} Dirichlet priors (alpha)")
+        return [f32(float(v)) for v in vals]
+
+
+def partition(nloci: int, blocksize: int = 0, nblocks: int = 0, nthreads: int = 1):
+    """Block partition of setup_soda, soda.f90:407-421.
+
+    Returns (blocksize, nblocks, blockt); blocksize <= 1 means SODA is off.
+    """
+    if blocksize > 0:
+        nb = math.ceil(nloci / blocksize)
+        return blocksize, nb, nb
+    if nblocks > 0:
+        bs = math.ceil(nloci / nblocks)
+        return bs, nblocks, nloci - (bs - 1) * nblocks
+    bs = nthreads
+    nb = math.ceil(nloci / bs)
+    return bs, nb, nb
+
+
+def block_bounds(nloci: int, blocksize: int, nblocks: int, blockt: int):
+    """0-based [start, end) of each block (soda.f90:463-469)."""
+    out = []
+    for b in range(1, nblocks + 1):
+        if b <= blockt:
+            ls = (b - 1) * blocksize + 1
+            le = min(ls + blocksize - 1, nloci)
+        else:
+            ls = blocksize * blockt + (b - blockt - 1) * (blocksize - 1) + 1
+            le = min(ls + blocksize - 2, nloci)
+        out.append((ls - 1, le))
+    return out
+
+
+def group_index(labels) -> tuple[np.ndarray, int]:
+    """SNP groups numbered by first appearance (get_size / load_snp_grp,
+    utilities.f90:276-327).  Returns 0-based indices and the group count."""
+    seen: dict[str, int] = {}
+    idx = np.empty(len(labels), dtype=np.int32)
+    for j, lab in enumerate(labels):
+        key = str(lab)
+        if key not in seen:
+            if len(seen) >= MAXGRP:
+                raise ValueError(f"max no. of SNP groups: {MAXGRP}")
+            seen[key] = len(seen)
+        idx[j] = seen[key]
+    return idx, len(seen)
+
+
+def rng_seed(seed: int, stream: int = 0) -> np.ndarray:
+    st = np.zeros(2, dtype=np.uint64)
+    _lib.check(_lib.load().sb_rng_seed(int(seed), stream, st.ctypes.data_as(C.POINTER(C.c_uint64))))
+    return st
+
+
+def permutate(n: int, s: int, state: np.ndarray) -> np.ndarray:
+    p = np.zeros(n, dtype=np.int32)
+    _lib.check(_lib.load().sb_permutate(n, s, state.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                        p.ctypes.data_as(C.POINTER(C.c_int32))))
+    return p
+
+
+@dataclass
+class SodaSampler:
+    """One SODA job on the GPU.
+
+    ``bed`` is the SNP-major PLINK payload (nloci x ceil(nind/4)); ``grp`` the
+    SNP group labels for BayesRc.  ``bed_device_ptr`` (optional) passes a
+    device pointer to the same payload instead of host memory.
+    """
+    opts: SodaOptions
+    nind: int
+    nloci: int
+    why: np.ndarray
+    bed: np.ndarray | None = None
+    grp: object = None
+    freq: np.ndarray | None = None
+    bed_device_ptr: int | None = None
+    handle: C.c_void_p = field(default=None, init=False)
+
+    def __post_init__(self):
+        o = self.opts
+        self.method = o.method_code()
+        self.nmix = o.resolved_nmix()
+        if self.method == 6:
+            if self.grp is None:
+                raise ValueError("BayesRc needs SNP groups")
+            self.grp_idx, self.ngroup = group_index(self.grp)
+        else:
+            self.grp_idx, self.ngroup = None, 1
+        bs, nb, bt = partition(self.nloci, o.blocksize, o.nblocks, o.nthreads)
+        if bs <= 1:
+            raise ValueError("SODA needs -blocksize or -nblocks (blocksize <= 1 switches SODA off, "
+                             "soda.f90:418-421)")
+        self.blocksize, self.nblocks, self.blockt = bs, nb, bt
+        if self.freq is None:
+            if self.bed is None:
+                raise ValueError("freq is required when the genotypes are on the device")
+            self.freq = allele_freq(self.bed, self.nind)
+        if o.seed == 0 and o.rng == "replay":
+            raise ValueError("replay needs a nonzero seed (seed 0 reads the system clock)")
+        self.state = rng_seed(o.seed)
+        if o.rblocks:
+            self.permvec = permutate(self.nloci, bs, self.state)
+        else:
+            self.permvec = np.arange(self.nloci, dtype=np.int32)
+
+    # -- setup_soda --------------------------------------------------------
+    def setup(self) -> np.ndarray:
+        L = _lib.load()
+        o = self.opts
+        d = _lib.SbDims(method=self.method, nind=self.nind, nloci=self.nloci, nmix=self.nmix,
+                        ngroup=self.ngroup, nblocks=self.nblocks, blocksize=self.blocksize,
+                        blockt=self.blockt, nchains=1,
+                        rng_mode=_lib.RNG_REPLAY if o.rng == "replay" else _lib.RNG_COUNTER,
+                        precision=_lib.PREC_FP64 if o.precision == "fp64" else _lib.PREC_MIXED,
+                        bed_on_device=1 if self.bed_device_ptr else 0)
+        freq = np.ascontiguousarray(self.freq, dtype=np.float64)
+        perm = np.ascontiguousarray(self.permvec, dtype=np.int32)
+        if self.bed_device_ptr:
+            bed_ptr = C.c_void_p(self.bed_device_ptr)
+        else:
+            self._bed = np.ascontiguousarray(self.bed, dtype=np.uint8)
+            bed_ptr = self._bed.ctypes.data_as(C.c_void_p)
+        grp = None
+        if self.grp_idx is not None:
+            self._grp = np.ascontiguousarray(self.grp_idx, dtype=np.int32)
+            grp = self._grp.ctypes.data_as(C.POINTER(C.c_int32))
+        h = C.c_void_p()
+        _lib.check(L.sb_create(C.byref(d), bed_ptr, freq.ctypes.data_as(C.POINTER(C.c_double)),
+                               perm.ctypes.data_as(C.POINTER(C.c_int32)), grp, C.byref(h)))
+        self.handle = h
+        _lib.check(L.sb_rng_set(h, self.state.ctypes.data_as(C.POINTER(C.c_uint64))))
+        xxd = np.zeros(self.nblocks)
+        fb, fc = C.c_int32(0), C.c_int32(0)
+        rc = L.sb_build_aug(h, xxd.ctypes.data_as(C.POINTER(C.c_double)), C.byref(fb), C.byref(fc))
+        if rc == _lib.SB_ERR_NOT_PD:
+            raise _lib.SbError(rc, f"D-XX is not positive definite (block {fb.value}, column {fc.value}); "
+                                   "the pivoted-Cholesky fallback of soda.f90:492-510 is not on the GPU path")
+        _lib.check(rc)
+        self.xxd = xxd
+        return xxd
+
+    # -- soda_bayes --------------------------------------------------------
+    def init(self) -> None:
+        o = self.opts
+        pr = _lib.SbPriors()
+        pr.pizero, pr.scale_va, pr.scale_ve = f32(o.pi), f32(o.scale_va), f32(o.scale_ve)
+        pr.df_va, pr.df_ve, pr.vara0 = f32(o.df_va), f32(o.df_ve), float(o.vara0)
+        for k, m in enumerate(o.mix_values()):
+            pr.mix[k] = m
+        for k, a in enumerate(o.alpha_values()):
+            pr.alpha[k] = a
+        pr.numit, pr.burnin, pr.thin, pr.seed = o.numit, o.burnin, o.thin, int(o.seed)
+        self._why = np.ascontiguousarray(self.why, dtype=np.float64)
+        _lib.check(_lib.load().sb_mcmc_init(self.handle, C.byref(pr),
+                                            self._why.ctypes.data_as(C.POINTER(C.c_double))))
+        self.rep = 0
+
+    def run(self, count: int | None = None, trace: bool = True) -> np.ndarray:
+        """Run ``count`` iterations (default: up to numit); returns the trace
+        records [rep, included, mu, vara, vare, snpindist..., varindist...]."""
+        L = _lib.load()
+        if count is None:
+            count = self.opts.numit - self.rep
+        first = self.rep + 1
+        thin = max(1, self.opts.thin)
+        nrec = sum(1 for r in range(first, first + count) if r % thin == 0)
+        w = L.sb_hyp_width(self.handle)
+        hyp = np.zeros((max(nrec, 1), w))
+        ptr = hyp.ctypes.data_as(C.POINTER(C.c_double)) if trace else None
+        _lib.check(L.sb_mcmc_run(self.handle, first, count, ptr))
+        self.rep += count
+        return hyp[:nrec]
+
+    def get(self, name: str, n: int, chain: int = 0) -> np.ndarray:
+        out = np.zeros(n)
+        _lib.check(_lib.load().sb_get(self.handle, _lib.Q[name], chain,
+                                      out.ctypes.data_as(C.POINTER(C.c_double)), n))
+        return out
+
+    def info(self) -> dict:
+        v = np.zeros(4)
+        _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 4))
+        return dict(setup_seconds=v[0], iterations=v[1], sweep_ms=v[2], device_bytes=v[3])
+
+    def results(self) -> dict:
+        """Final state and posterior means under the reference's names
+        (models.f90:575-594)."""
+        p, nb = self.nloci, self.nmix * self.ngroup
+        m = self.method
+        st = self.get("STORES", 5)
+        cnt = st[0]
+        sc = self.get("SCALARS", 4)
+        out = {
+            "nblocks": self.nblocks, "blocksize": self.blocksize, "blockt": self.blockt,
+            "naind": self.blocksize, "freq": self.freq, "permvec": self.permvec + 1,
+            "xxd": self.get("XXD", self.nblocks),
+            "mu": sc[0], "vare": sc[1], "vara": sc[2],
+            "mu_store": st[1] / cnt, "vare_store": st[2] / cnt,
+            "g": self.get("G", p), "gstore": self.get("GSTORE", p) / cnt,
+            "yadj": self.get("YADJ", self.nind), "yAadj": self.get("YAADJ", self.blocksize),
+            "varstore": self.get("VARSTORE", nb) / cnt, "varindist": self.get("VARINDIST", nb),
+        }
+        if m != 1:
+            grp1 = (self.grp_idx + 1) if self.grp_idx is not None else np.ones(p)
+            out["snptracker"] = np.concatenate([grp1, self.get("CLASS", p)])
+            out["snpindist"] = self.get("SNPINDIST", nb)
+            out["snpstore"] = self.get("SNPSTORE", nb) / cnt
+            out["indiststore"] = self.get("INDISTSTORE", p * self.nmix) / cnt
+            out["included_store"] = st[4] / cnt
+            out["p"] = self.get("P", nb)
+        if m not in (1, 2, 3):
+            out["pstore"] = self.get("PSTORE", nb) / cnt
+        if m in (1, 2):
+            out["vara_s"] = self.get("VARA_S", p)
+            out["vara_sstore"] = self.get("VARA_SSTORE", p) / cnt
+        else:
+            out["vara_store"] = st[3] / cnt
+        return out
+
+    def close(self) -> None:
+        if self.handle:
+            _lib.load().sb_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

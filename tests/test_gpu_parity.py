@@ -1,0 +1,109 @@
+"""GPU path (through the C ABI) against the oracle and the reference fixtures.
+
+Bars: integer Gram bit-exact; d (xxd) within 1e-12 relative; augmenting
+matrices within 1e-10 norm-wise and U'U = dI - G to 1e-12 of d; replayed
+chains: every discrete quantity (classes, counts, mixture proportions) equal
+to the reference after the full fixture run, continuous state within 1e-9
+relative to its largest entry.
+"""
+import numpy as np
+import pytest
+
+import orc
+from common import group_ids, load_case, rel_err, sampler_options, soda_cases
+from sodabayes_amd import soda
+
+pytestmark = pytest.mark.gpu
+
+DISCRETE = ["snptracker", "snpindist", "snpstore", "indiststore", "included_store"]
+CONT = ["xxd", "mu_store", "vare", "vara", "vare_store", "g", "gstore", "yadj", "yAadj",
+        "varstore", "varindist", "p", "pstore", "vara_s", "vara_sstore", "vara_store"]
+
+
+def make_sampler(c, **extra):
+    o = sampler_options(c["opts"], **extra)
+    grp = c.get("grp")
+    s = soda.SodaSampler(o, int(c["nind"]), int(c["nloci"]), c["why"], bed=c["bed"],
+                         grp=None if grp is None else [str(g) for g in grp])
+    return s
+
+
+@pytest.mark.parametrize("case", ["bayesr", "ragged", "nblocks", "rblocks", "ldblock"])
+def test_gram_bit_exact(case):
+    c = load_case(case)
+    s = make_sampler(c)
+    s.setup()
+    n = int(c["nind"])
+    f = orc.freq(c["bed"], n)
+    perm = s.permvec
+    bounds = soda.block_bounds(int(c["nloci"]), s.blocksize, s.nblocks, s.blockt)
+    for b, (a, e) in enumerate(bounds):
+        nb = e - a
+        G = s.get("GRAM", nb * nb, chain=b).reshape(nb, nb, order="F")
+        want = orc.gram(c["bed"], n, list(perm[a:e]), f)
+        assert np.array_equal(G, want), f"block {b}"
+    s.close()
+
+
+@pytest.mark.parametrize("case", soda_cases())
+def test_setup_matches_reference(case):
+    c = load_case(case)
+    s = make_sampler(c)
+    xxd = s.setup()
+    gold = c["out"]
+    assert rel_err(xxd, gold["xxd"]) < 1e-12
+    bs, p = s.blocksize, int(c["nloci"])
+    XA = s.get("XA", bs * p).reshape(bs, p, order="F")
+    XAg = gold["XA"].reshape(bs, p, order="F")
+    assert np.linalg.norm(XA - XAg) / np.linalg.norm(XAg) < 1e-10
+    # invariant U'U + G = dI per block
+    n = int(c["nind"])
+    f = orc.freq(c["bed"], n)
+    for b, (a, e) in enumerate(soda.block_bounds(p, bs, s.nblocks, s.blockt)):
+        nb = e - a
+        U = XA[:nb, a:e]
+        G = orc.gram(c["bed"], n, list(s.permvec[a:e]), f)
+        R = U.T @ U + G - xxd[b] * np.eye(nb)
+        assert np.max(np.abs(R)) < 1e-12 * xxd[b] * nb
+    s.close()
+
+
+@pytest.mark.parametrize("case", soda_cases())
+def test_replay_chain_matches_reference(case):
+    c = load_case(case)
+    s = make_sampler(c)
+    s.setup()
+    s.init()
+    hyp = s.run()
+    res = s.results()
+    gold = c["out"]
+    for k in DISCRETE:
+        if k in gold:
+            assert np.array_equal(np.asarray(res[k], dtype=float), gold[k].astype(float)), k
+    for k in CONT:
+        if k in gold:
+            assert rel_err(res[k], gold[k]) < 1e-9, (k, rel_err(res[k], gold[k]))
+    assert hyp.shape[0] == c["opts"]["numit"] // c["opts"].get("thin", 1)
+    s.close()
+
+
+@pytest.mark.parametrize("case", ["bayesr", "bayesb"])
+def test_replay_matches_oracle_trace(case):
+    """Per-iteration mu / vara / vare of the replayed chain against the
+    oracle's full-precision trace."""
+    c = load_case(case)
+    grp, ngroup = (None, 1)
+    if "grp" in c:
+        grp, ngroup = group_ids(c["grp"])
+    from common import oracle_options
+    ref = orc.run(oracle_options(c["opts"], int(c["nind"]), int(c["nloci"]), ngroup),
+                  c["bed"], c["why"], grp)
+    s = make_sampler(c)
+    s.setup()
+    s.init()
+    hyp = s.run()
+    assert rel_err(hyp[:, 2], ref["hyp_mu"]) < 1e-11
+    assert rel_err(hyp[:, 4], ref["hyp_vare"]) < 1e-9
+    if c["opts"]["method"] not in ("BayesA", "BayesB"):
+        assert rel_err(hyp[:, 3], ref["hyp_vara"]) < 1e-9
+    s.close()
