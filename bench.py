@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""SODA Gibbs sweep throughput on one MI355X.
+
+Workload (the metric's configuration): BayesR, n = 50,000 individuals,
+p = 500,000 SNPs, SODA subsets of 4,096 SNPs, 4-class mixture
+{0, 1e-4, 1e-3, 1e-2} sigma2_g.  Synthetic genotypes f_j ~ U(0.01, 0.5),
+x_ij ~ Bin(2, f_j), generated directly in HBM as a SNP-major PLINK payload;
+phenotype from 5,000 t(4) QTL at h2 = 0.4.  One step = one Gibbs iteration
+(p SNP-effect updates).  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line.  Besides value (SNP-effect updates/s) it carries the
+roofline of the dominant kernel (the block product X_s'r + A_s'r_a,
+timed with HIP events on the library's stream in a separate pass) and the
+reference CPU baseline timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nind", type=int, default=50000)
+    ap.add_argument("--nloci", type=int, default=500000)
+    ap.add_argument("--blocksize", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=20240607)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def synth_on_device(torch, n, p, seed, fmin=0.01, fmax=0.5, nqtl=5000, h2=0.4, chunk=2048):
+    """SNP-major PLINK payload (p x ceil(n/4) bytes) generated in HBM."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    f = torch.rand(p, generator=g, device=dev, dtype=torch.float64) * (fmax - fmin) + fmin
+    nb = (n + 3) // 4
+    npad = nb * 4
+    bed = torch.empty((p, nb), dtype=torch.uint8, device=dev)
+    counts = torch.empty(p, dtype=torch.float64, device=dev)
+    rs = np.random.default_rng(seed)
+    qtl = np.sort(rs.choice(p, size=min(nqtl, p), replace=False))
+    eff = rs.standard_t(4, size=len(qtl))
+    qtl_t = torch.from_numpy(qtl).to(dev)
+    eff_t = torch.from_numpy(eff).to(dev)
+    gval = torch.zeros(n, dtype=torch.float64, device=dev)
+    for j0 in range(0, p, chunk):
+        m = min(chunk, p - j0)
+        fc = f[j0:j0 + m].float()
+        u = torch.rand((2, npad, m), generator=g, device=dev)
+        x = (u[0] < fc).to(torch.uint8) + (u[1] < fc).to(torch.uint8)
+        del u
+        if npad > n:
+            x[n:] = 0
+        counts[j0:j0 + m] = x.sum(0, dtype=torch.int64).to(torch.float64)
+        code = x + (x > 0).to(torch.uint8)          # 0 -> 00, 1 -> 10, 2 -> 11
+        c4 = code.view(nb, 4, m)
+        byte = c4[:, 0] | (c4[:, 1] << 2) | (c4[:, 2] << 4) | (c4[:, 3] << 6)
+        bed[j0:j0 + m] = byte.t()
+        sel = (qtl_t >= j0) & (qtl_t < j0 + m)
+        if bool(sel.any()):
+            cols = qtl_t[sel] - j0
+            xq = x[:n, cols].to(torch.float64)
+            gval += (xq - 2.0 * f[j0 + cols]) @ eff_t[sel]
+        del x, code, c4, byte
+    freq = (counts / (2.0 * n)).cpu().numpy()
+    gv = gval.cpu().numpy()
+    gv *= np.sqrt(h2 / gv.var())
+    y = 100.0 + gv + rs.standard_normal(n) * np.sqrt(1.0 - h2)
+    return bed, freq, y
+
+
+def cpu_baseline(seed: int) -> dict | None:
+    """The reference program (oracle/_ref/sodabayes_ref, built from the
+    reference sources) on a bounded sample of the workload: same n, BayesR
+    mixture, p = 2,048 SNPs in subsets of 1,024, 30 iterations, all host
+    cores.  SNP-updates/s = p * numit / the MCMC seconds of its .log."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "sodabayes_ref")
+    if not os.path.exists(exe):
+        log("cpu baseline skipped: oracle/_ref/sodabayes_ref not built")
+        return None
+    from sodabayes_amd import plink, synth
+
+    n, p, numit, bs = 50000, 2048, 30, 1024
+    cores = min(16, os.cpu_count() or 1)
+    with tempfile.TemporaryDirectory() as td:
+        bed, y, _ = synth.dataset(n, p, seed=seed % 100000, nqtl=40, h2=0.4, fmin=0.01, fmax=0.5)
+        plink.write_plink(os.path.join(td, "d"), bed, y)
+        cmd = [exe, "-bfile", "d", "-method", "BayesR", "-nmix", "4", "-mix", "0.0,0.0001,0.001,0.01",
+               "-numit", str(numit), "-burnin", "0", "-blocksize", str(bs), "-nthreads", str(cores),
+               "-seed", "7", "-out", "r"]
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+        log(f"cpu baseline: reference program, n={n} p={p} blocksize={bs} {numit} its, {cores} threads")
+        r = subprocess.run(cmd, cwd=td, env=env, capture_output=True, text=True, timeout=300)
+        txt = open(os.path.join(td, "r.log")).read() if os.path.exists(os.path.join(td, "r.log")) else ""
+        m = re.search(r"Setup:\s*([\d.]+)\s+MCMC:\s*([\d.]+)", txt)
+        if r.returncode != 0 or not m:
+            log(f"cpu baseline failed: {r.stderr[-300:]}")
+            return None
+        mcmc = float(m.group(2))
+    return {"value": p * numit / mcmc, "unit": "SNP-effect updates/s", "cores": cores, "kind": "reference",
+            "sample": f"reference sodabayes (amdflang -O2 -fopenmp) BayesR n={n} p={p} blocksize={bs} "
+                      f"{numit} iterations, -nthreads {cores}; MCMC {mcmc:.2f} s, setup {float(m.group(1)):.2f} s"}
+
+
+def main():
+    a = parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
+              "the sharding design is in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from sodabayes_amd import soda
+
+    n, p, bs = a.nind, a.nloci, a.blocksize
+    t0 = time.time()
+    bed, freq, why = synth_on_device(torch, n, p, a.seed)
+    torch.cuda.synchronize()
+    log(f"genotypes generated in HBM: n={n} p={p} ({bed.numel() / 1e9:.2f} GB) in {time.time() - t0:.1f} s")
+    total = a.warmup + a.steps + (0 if a.no_roofline else max(3, a.steps // 3))
+    opts = soda.SodaOptions(method="BayesR", nmix=4, mix="0.0,0.0001,0.001,0.01", blocksize=bs,
+                            numit=total, burnin=0, seed=a.seed, rng="counter", precision="fp64")
+    s = soda.SodaSampler(opts, n, p, why, freq=freq, bed_device_ptr=bed.data_ptr())
+    t0 = time.time()
+    s.setup()
+    del bed
+    torch.cuda.empty_cache()
+    setup_s = time.time() - t0
+    log(f"setup (gram + power method + cholesky, {s.nblocks} blocks): {setup_s:.1f} s")
+    s.init()
+    s.run(a.warmup, trace=False)
+    torch.cuda.synchronize()
+    log(f"warmup {a.warmup} iterations done")
+    t0 = time.perf_counter()
+    s.run(a.steps, trace=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms = dt * 1e3 / a.steps
+    value = p * a.steps / dt
+    log(f"timed {a.steps} iterations: {ms:.2f} ms/iteration, {value:.4g} SNP-updates/s")
+
+    roof = None
+    if not a.no_roofline:
+        # separate pass with HIP events around every block kernel
+        os.environ["SODABAYES_KERNEL_TIMING"] = "1"
+        before = s.info_raw()
+        k = max(3, a.steps // 3)
+        s.run(k, trace=False)
+        after = s.info_raw()
+        os.environ.pop("SODABAYES_KERNEL_TIMING", None)
+        launches = after[5] - before[5]
+        rhs_ms = (after[4] - before[4]) / launches
+        rhs_bytes = (after[6] - before[6]) / launches
+        achieved = rhs_bytes / (rhs_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_rhs", "avg_launch_us": round(rhs_ms * 1e3, 2),
+                "algorithmic_bytes_per_launch": int(rhs_bytes),
+                "draw_avg_us": round((after[7] - before[7]) / max(1, after[8] - before[8]) * 1e3, 2),
+                "update_avg_us": round((after[9] - before[9]) / max(1, after[10] - before[10]) * 1e3, 2)}
+        log(f"k_rhs: {rhs_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of HBM)")
+    s.close()
+
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.seed)
+    out = {
+        "metric": "SNP-effect updates/s (BayesR n=50k,p=500k): mixed+batched vs FP64 GPU, same acc.",
+        "value": value, "unit": "SNP-effect updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"SODA BayesR n={n} p={p} n_s={bs}, 4-class mixture, 1 chain, "
+                               "counter-based RNG, FP64 sweep", "nind": n, "nloci": p, "blocksize": bs,
+                   "chains": 1, "precision": "fp64", "parallelism": "single GPU"},
+        "setup_seconds": round(setup_s, 2),
+        "iterations_per_s": a.steps / dt,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

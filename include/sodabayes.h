@@ -141,7 +141,11 @@ enum {
 int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count);
 
 /* Diagnostics: fills up to n doubles: [setup seconds, iterations run,
- * last sweep milliseconds, device bytes allocated]. */
+ * last sweep milliseconds, device bytes allocated, then -- with the
+ * environment variable SODABAYES_KERNEL_TIMING=1 during sb_mcmc_run -- summed
+ * HIP-event milliseconds and launch counts of the block kernels:
+ * rhs ms, rhs launches, rhs algorithmic bytes, draw ms, draw launches,
+ * update ms, update launches]. */
 int sb_get_info(sb_handle *h, double *out, int32_t n);
 
 /* Message of the last failure on this thread ("" if none). */

@@ -39,5 +39,27 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.environ.get("FORTRAN", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FDIR, "build")
+
+
+def build_fortran() -> str | None:
+    """Compiles the iso_c_binding module and its host-side check program
+    (fortran/build/binding_check) against libsodabayes.so."""
+    if not os.path.exists(FORTRAN):
+        return None
+    os.makedirs(FBUILD, exist_ok=True)
+    exe = os.path.join(FBUILD, "binding_check")
+    cmd = [FORTRAN, "-O1", "-J", FBUILD, "-o", exe, os.path.join(FDIR, "soda_gpu.f90"),
+           os.path.join(FDIR, "binding_check.f90"), "-L", HERE, "-lsodabayes", "-Wl,-rpath," + HERE]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("Fortran binding failed to build")
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran())

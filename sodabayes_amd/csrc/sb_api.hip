@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -441,8 +442,10 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
 int sb_get_info(sb_handle *h, double *out, int32_t n)
 {
     if (!h || !out) return fail(SB_ERR_ARG, "sb_get_info: null argument");
-    double v[4] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes};
-    for (int i = 0; i < n && i < 4; i++) out[i] = v[i];
+    double v[11] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes,
+                    h->k_ms[0], h->k_launch[0], h->rhs_bytes, h->k_ms[1], h->k_launch[1],
+                    h->k_ms[2], h->k_launch[2]};
+    for (int i = 0; i < n && i < 11; i++) out[i] = v[i];
     return SB_OK;
 }
 

@@ -72,6 +72,12 @@ struct sb_handle {
     bool aug_built = false, initialised = false;
     double setup_seconds = 0.0, last_sweep_ms = 0.0;
     int64_t iterations = 0;
+    // kernel timing with HIP events on the handle's stream (enabled for a
+    // run by the environment variable SODABAYES_KERNEL_TIMING=1)
+    bool ktime = false;
+    double k_ms[3] = {0, 0, 0};       // rhs, draw, update: summed durations
+    double k_launch[3] = {0, 0, 0};   // launches
+    double rhs_bytes = 0.0;           // algorithmic bytes of the timed rhs launches
 
     // genotype data
     uint32_t *geno = nullptr;     // nloci x ldw, position order, 2-bit counts

@@ -12,8 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
-#include <cstring>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "sb_device.h"
@@ -821,6 +822,18 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     double *delta = h->sc.delta;
     std::vector<double> xxd(P.nblocks);
     hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
+    // optional per-kernel HIP events on this stream (rhs, draw, update)
+    const char *kt = std::getenv("SODABAYES_KERNEL_TIMING");
+    h->ktime = kt && kt[0] == '1';
+    std::vector<hipEvent_t> ev;
+    if (h->ktime) {
+        ev.resize((size_t)count * P.nblocks * 6);
+        for (auto &e : ev) hipEventCreate(&e);
+    }
+    size_t evi = 0;
+    auto mark = [&]() {
+        if (h->ktime) hipEventRecord(ev[evi++], st);
+    };
     auto t0 = std::chrono::steady_clock::now();
     int rec = 0;
     for (int rep = first; rep < first + count; rep++) {
@@ -835,8 +848,13 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             int ngs = (nb + 4 * spw - 1) / (4 * spw);
             int nxg = ngs * P.nchunk;
             int nag = (nb + 3) / 4;
+            mark();
             k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, p0, nb, spw, ngs, nxg,
                                             h->sc.part, h->sc.partA, h->sc.partR);
+            mark();
+            if (h->ktime)
+                h->rhs_bytes += (double)nb * (((double)P.n + 3.0) / 4.0) + 8.0 * (double)nb * (nb + 1) / 2.0;
+            mark();
             if (P.rng_mode == SB_RNG_REPLAY)
                 k_draw_replay<<<1, NT, 0, st>>>(P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2,
                                                 h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp,
@@ -846,10 +864,13 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 k_draw_counter<<<(nb + NT - 1) / NT, NT, 0, st>>>(
                     P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
                     h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg);
+            mark();
             int nrw = (P.ldw + NT - 1) / NT;
             int nra = (nb + NT - 1) / NT;
+            mark();
             k_update<<<nrw + nra, NT, 0, st>>>(P, h->geno, h->zlut, h->U, tri0, p0, nb, nrw, chg, delta,
                                                h->r, h->ra);
+            mark();
         }
         k_rtr<<<nred, NT, 0, st>>>(P, h->r, h->sc.red);
         k_bins<<<dim3(nwg_bins, P.nbins), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
@@ -872,6 +893,16 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     }
     int rc = hip_check(hipStreamSynchronize(st), "sweep");
     auto t1 = std::chrono::steady_clock::now();
+    if (h->ktime) {
+        for (size_t i = 0; i + 1 < evi; i += 2) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            int kind = (int)((i / 2) % 3);
+            h->k_ms[kind] += ms;
+            h->k_launch[kind] += 1.0;
+        }
+        for (auto &e : ev) hipEventDestroy(e);
+    }
     if (count > 0) h->last_sweep_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / count;
     h->iterations += count;
     if (!rc && dhyp) {

@@ -274,9 +274,13 @@ class SodaSampler:
                                       out.ctypes.data_as(C.POINTER(C.c_double)), n))
         return out
 
+    def info_raw(self) -> np.ndarray:
+        v = np.zeros(11)
+        _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 11))
+        return v
+
     def info(self) -> dict:
-        v = np.zeros(4)
-        _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 4))
+        v = self.info_raw()
         return dict(setup_seconds=v[0], iterations=v[1], sweep_ms=v[2], device_bytes=v[3])
 
     def results(self) -> dict:

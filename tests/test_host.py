@@ -80,3 +80,24 @@ def test_library_errors_are_reported():
     with pytest.raises(_lib.SbError):
         _lib.check(L.sb_rng_seed(1, -1, None))
     assert b"sb_rng_seed" in L.sb_last_error()
+
+
+def test_fortran_binding_calls_the_library():
+    """The iso_c_binding module (sodabayes_amd/fortran/soda_gpu.f90) compiled
+    with the image's Fortran compiler calls the library's host entry points."""
+    import subprocess
+    from sodabayes_amd import build as b
+    exe = os.path.join(b.FBUILD, "binding_check")
+    if not os.path.exists(exe):
+        exe = b.build_fortran()
+        if exe is None:
+            pytest.skip("no Fortran compiler")
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    r = orc.Rng(123456789)
+    s0 = [int(v) if v < 2**63 else int(v) - 2**64 for v in r.state()]
+    assert [int(t) for t in out[0].split()] == s0
+    perm = orc.permutate(1000, 64, r) - 1
+    assert [int(t) for t in out[1].split()] == list(perm[:10])
+    s1 = [int(v) if v < 2**63 else int(v) - 2**64 for v in r.state()]
+    assert [int(t) for t in out[2].split()] == s1
+    assert out[3].startswith("-1 sb_rng_seed")
