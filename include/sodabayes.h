@@ -140,12 +140,18 @@ enum {
 };
 int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count);
 
+/* Diagnostics (no counterpart in the reference): X_s'r + A_s'r_a of every SNP
+ * of block `block` for the current state, through the handle's product path
+ * (FP64 or mixed), without drawing.  out: n_b x nchains, chain-major. */
+int sb_block_rhs(sb_handle *h, int32_t block, double *out);
+
 /* Diagnostics: fills up to n doubles: [setup seconds, iterations run,
  * last sweep milliseconds, device bytes allocated, then -- with the
  * environment variable SODABAYES_KERNEL_TIMING=1 during sb_mcmc_run -- summed
  * HIP-event milliseconds and launch counts of the block kernels:
  * rhs ms, rhs launches, rhs algorithmic bytes, draw ms, draw launches,
- * update ms, update launches]. */
+ * update ms, update launches; then the largest power-method iteration
+ * count of sb_build_aug]. */
 int sb_get_info(sb_handle *h, double *out, int32_t n);
 
 /* Message of the last failure on this thread ("" if none). */

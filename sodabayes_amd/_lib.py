@@ -23,8 +23,8 @@ Q = dict(G=1, CLASS=2, VARA_S=3, YADJ=4, YAADJ=5, P=6, SCALARS=7, GSTORE=8, INDI
 
 # every function include/sodabayes.h declares
 EXPORTS = ["sb_rng_seed", "sb_permutate", "sb_create", "sb_destroy", "sb_rng_set", "sb_rng_get",
-           "sb_build_aug", "sb_mcmc_init", "sb_mcmc_run", "sb_hyp_width", "sb_get", "sb_get_info",
-           "sb_last_error"]
+           "sb_build_aug", "sb_mcmc_init", "sb_mcmc_run", "sb_hyp_width", "sb_get", "sb_block_rhs",
+           "sb_get_info", "sb_last_error"]
 
 
 class SbDims(C.Structure):
@@ -73,6 +73,7 @@ def load():
     L.sb_hyp_width.restype = C.c_int32
     L.sb_get.argtypes = [vp, C.c_int32, C.c_int32, dp, C.c_int64]
     L.sb_get_info.argtypes = [vp, dp, C.c_int32]
+    L.sb_block_rhs.argtypes = [vp, C.c_int32, dp]
     L.sb_last_error.restype = C.c_char_p
     for f in EXPORTS:
         if f not in ("sb_hyp_width", "sb_last_error"):

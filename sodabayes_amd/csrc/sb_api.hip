@@ -138,9 +138,13 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     if (d.method == SB_BAYES_RC && !snpgrp) return fail(SB_ERR_ARG, "BayesRc needs snpgrp");
     if (d.blocksize < 2 || d.blocksize > 4096) return fail(SB_ERR_ARG, "blocksize must be in [2, 4096]");
     if (d.nblocks < 1 || d.blockt < 0 || d.blockt > d.nblocks) return fail(SB_ERR_ARG, "bad block partition");
-    if (d.nchains != 1) return fail(SB_ERR_ARG, "this build runs one chain per handle");
     if (d.rng_mode != SB_RNG_REPLAY && d.rng_mode != SB_RNG_COUNTER) return fail(SB_ERR_ARG, "bad rng_mode");
-    if (d.precision != SB_PREC_FP64) return fail(SB_ERR_ARG, "precision mode not available in this build");
+    if (d.precision != SB_PREC_FP64 && d.precision != SB_PREC_MIXED) return fail(SB_ERR_ARG, "bad precision");
+    if (d.nchains < 1 || d.nchains > 64) return fail(SB_ERR_ARG, "nchains must be in [1, 64]");
+    if (d.nchains > 1 && d.precision != SB_PREC_MIXED)
+        return fail(SB_ERR_ARG, "several chains per handle need the mixed-precision batched path");
+    if (d.rng_mode == SB_RNG_REPLAY && (d.nchains != 1 || d.precision != SB_PREC_FP64))
+        return fail(SB_ERR_ARG, "replay runs one FP64 chain (the reference's -nthreads 1 stream)");
 
     // block bounds, soda.f90:463-469
     std::vector<int> ls(d.nblocks + 1);
@@ -183,8 +187,10 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     int words = (d.nind + 15) / 16;
     P.ldw = ((words + CHUNK_WORDS - 1) / CHUNK_WORDS) * CHUNK_WORDS;
     P.nchunk = P.ldw / CHUNK_WORDS;
+    P.ldr = (int64_t)P.ldw * 16;
     P.rng_mode = d.rng_mode;
     P.nchains = d.nchains;
+    P.precision = d.precision;
     h->ls = ls;
     h->tri_off.assign(d.nblocks + 1, 0);
     for (int b = 0; b < d.nblocks; b++) {
@@ -202,7 +208,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         h->pos_of[s] = k;
     }
 
-    const int64_t p = d.nloci, bs = d.blocksize, ldr = (int64_t)P.ldw * 16;
+    const int64_t p = d.nloci, bs = d.blocksize, ldr = (int64_t)P.ldw * 16, C = d.nchains;
     const int bin_chunk = 8192, nwg_bins = (int)((p + bin_chunk - 1) / bin_chunk);
     int rc = 0;
     auto A = [&](void **ptr, size_t bytes) {
@@ -218,16 +224,16 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->U, sizeof(double) * (size_t)h->tri_off[d.nblocks]);
     A((void **)&h->d_tri_off, sizeof(int64_t) * (d.nblocks + 1));
     A((void **)&h->d_ls, sizeof(int32_t) * (d.nblocks + 1));
-    A((void **)&h->r, sizeof(double) * ldr);
-    A((void **)&h->ra, sizeof(double) * bs);
+    A((void **)&h->r, sizeof(double) * ldr * C);
+    A((void **)&h->ra, sizeof(double) * bs * C);
     A((void **)&h->why, sizeof(double) * ldr);
-    A((void **)&h->beta, sizeof(double) * p);
-    A((void **)&h->cls, sizeof(int32_t) * p);
-    A((void **)&h->vara_s, sizeof(double) * p);
-    A((void **)&h->gstore, sizeof(double) * p);
-    A((void **)&h->vara_sstore, sizeof(double) * p);
-    A((void **)&h->indiststore, sizeof(double) * p * d.nmix);
-    A((void **)&h->scal, sizeof(Scal));
+    A((void **)&h->beta, sizeof(double) * p * C);
+    A((void **)&h->cls, sizeof(int32_t) * p * C);
+    A((void **)&h->vara_s, sizeof(double) * p * C);
+    A((void **)&h->gstore, sizeof(double) * p * C);
+    A((void **)&h->vara_sstore, sizeof(double) * p * C);
+    A((void **)&h->indiststore, sizeof(double) * p * d.nmix * C);
+    A((void **)&h->scal, sizeof(Scal) * C);
     A((void **)&h->sc.part, sizeof(double) * P.nchunk * bs);
     A((void **)&h->sc.partA, sizeof(double) * bs);
     A((void **)&h->sc.partR, sizeof(double) * P.nchunk);
@@ -235,11 +241,31 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->sc.stemp, sizeof(double) * bs * d.nmix);
     A((void **)&h->sc.uu, sizeof(double) * 2 * bs);
     A((void **)&h->sc.chg, sizeof(int32_t) * bs);
-    A((void **)&h->sc.flag, sizeof(uint8_t) * p + 64);
-    A((void **)&h->sc.delta, sizeof(double) * p);
-    A((void **)&h->sc.red, sizeof(double) * (4096 + 2 * (size_t)P.nbins * nwg_bins));
-    A((void **)&h->resync_list, sizeof(int32_t) * p);
-    A((void **)&h->d_count, sizeof(int32_t) * 4);
+    A((void **)&h->sc.flag, sizeof(uint8_t) * p * C + 64);
+    A((void **)&h->sc.delta, sizeof(double) * p * C);
+    A((void **)&h->sc.red, sizeof(double) * C * (1024 + 2 * (size_t)P.nbins * nwg_bins) + 64);
+    A((void **)&h->resync_list, sizeof(int32_t) * p * C);
+    A((void **)&h->d_count, sizeof(int32_t) * (C + 4));
+    if (d.precision == SB_PREC_MIXED) {
+        h->nwg_r = (int)std::min<int64_t>(64, (d.nind + 255) / 256);
+        int need = (int)((C * h->limbs + 15) / 16);
+        static const int tiles[] = {1, 2, 3, 4, 6, 8, 12};
+        h->b_ntile = 12;
+        for (int t : tiles)
+            if (t >= need) {
+                h->b_ntile = t;
+                break;
+            }
+        h->b_ldq = ldr;
+        h->b_ldt = 16 * h->b_ntile;
+        int nrg = (int)((bs + h->rows_per_group - 1) / h->rows_per_group);
+        A((void **)&h->b_rsum, sizeof(double) * C * h->nwg_r);
+        A((void **)&h->b_rmax, sizeof(uint64_t) * C * h->nwg_r);
+        A((void **)&h->b_Q, (size_t)16 * h->b_ntile * h->b_ldq);
+        A((void **)&h->b_expo, sizeof(int) * C);
+        A((void **)&h->b_T, sizeof(int) * (size_t)bs * h->b_ldt);
+        A((void **)&h->b_rapart, sizeof(double) * (size_t)nrg * bs * C);
+    }
     if (rc) {
         sb_destroy(h);
         return rc;
@@ -279,7 +305,11 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     hipMemcpyAsync(h->d_pos_by_id, pb.data(), sizeof(int32_t) * 2 * p, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(h->d_tri_off, h->tri_off.data(), sizeof(int64_t) * (d.nblocks + 1), hipMemcpyHostToDevice, st);
     hipMemcpyAsync(h->d_ls, ls.data(), sizeof(int32_t) * (d.nblocks + 1), hipMemcpyHostToDevice, st);
-    hipMemsetAsync(h->scal, 0, sizeof(Scal), st);
+    hipMemsetAsync(h->scal, 0, sizeof(Scal) * C, st);
+    if (h->b_Q) {
+        hipMemsetAsync(h->b_Q, 0, (size_t)16 * h->b_ntile * h->b_ldq, st);
+        hipMemsetAsync(h->b_T, 0, sizeof(int) * (size_t)bs * h->b_ldt, st);
+    }
     hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
     rc = setup_pack(h, bed, d.bed_on_device != 0);
     if (rc) {
@@ -341,6 +371,8 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
     if (!h || !out) return fail(SB_ERR_ARG, "sb_get: null argument");
     const Par &P = h->par;
     const int64_t p = P.p;
+    if (what != SB_Q_GRAM && (chain < 0 || chain >= P.nchains)) return fail(SB_ERR_ARG, "sb_get: bad chain");
+    const int64_t cp = what == SB_Q_GRAM ? 0 : (int64_t)chain * p;
     hipStream_t st = h->stream;
     auto need = [&](int64_t n) { return count >= n ? SB_OK : fail(SB_ERR_ARG, "sb_get: output too small"); };
     auto by_id = [&](const void *dsrc, bool is_int, int64_t ncol) -> int {
@@ -364,21 +396,24 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
     bool need_scal = what == SB_Q_P || what == SB_Q_SCALARS || what == SB_Q_STORES || what == SB_Q_SNPSTORE ||
                      what == SB_Q_VARSTORE || what == SB_Q_PSTORE || what == SB_Q_SNPINDIST ||
                      what == SB_Q_VARINDIST;
-    if (need_scal && hip_check(hipMemcpy(&sc, h->scal, sizeof sc, hipMemcpyDeviceToHost), "get scal"))
+    if (need_scal && hip_check(hipMemcpy(&sc, h->scal + chain, sizeof sc, hipMemcpyDeviceToHost), "get scal"))
         return SB_ERR_HIP;
     switch (what) {
-    case SB_Q_G: return by_id(h->beta, false, 1);
-    case SB_Q_CLASS: return by_id(h->cls, true, 1);
-    case SB_Q_VARA_S: return by_id(h->vara_s, false, 1);
-    case SB_Q_GSTORE: return by_id(h->gstore, false, 1);
-    case SB_Q_VARA_SSTORE: return by_id(h->vara_sstore, false, 1);
-    case SB_Q_INDISTSTORE: return by_id(h->indiststore, false, P.nmix);
+    case SB_Q_G: return by_id(h->beta + cp, false, 1);
+    case SB_Q_CLASS: return by_id(h->cls + cp, true, 1);
+    case SB_Q_VARA_S: return by_id(h->vara_s + cp, false, 1);
+    case SB_Q_GSTORE: return by_id(h->gstore + cp, false, 1);
+    case SB_Q_VARA_SSTORE: return by_id(h->vara_sstore + cp, false, 1);
+    case SB_Q_INDISTSTORE: return by_id(h->indiststore + cp * P.nmix, false, P.nmix);
     case SB_Q_YADJ:
         if (need(P.n)) return SB_ERR_ARG;
-        return hip_check(hipMemcpy(out, h->r, sizeof(double) * P.n, hipMemcpyDeviceToHost), "get r");
+        return hip_check(hipMemcpy(out, h->r + (int64_t)chain * P.ldr, sizeof(double) * P.n, hipMemcpyDeviceToHost),
+                         "get r");
     case SB_Q_YAADJ:
         if (need(P.blocksize)) return SB_ERR_ARG;
-        return hip_check(hipMemcpy(out, h->ra, sizeof(double) * P.blocksize, hipMemcpyDeviceToHost), "get ra");
+        return hip_check(hipMemcpy(out, h->ra + (int64_t)chain * P.blocksize, sizeof(double) * P.blocksize,
+                                   hipMemcpyDeviceToHost),
+                         "get ra");
     case SB_Q_XXD:
         if (need(P.nblocks)) return SB_ERR_ARG;
         return hip_check(hipMemcpy(out, h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost), "get xxd");
@@ -439,13 +474,19 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
     }
 }
 
+int sb_block_rhs(sb_handle *h, int32_t block, double *out)
+{
+    if (!h || !out) return fail(SB_ERR_ARG, "sb_block_rhs: null argument");
+    return block_rhs(h, block, out);
+}
+
 int sb_get_info(sb_handle *h, double *out, int32_t n)
 {
     if (!h || !out) return fail(SB_ERR_ARG, "sb_get_info: null argument");
-    double v[11] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes,
+    double v[12] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes,
                     h->k_ms[0], h->k_launch[0], h->rhs_bytes, h->k_ms[1], h->k_launch[1],
-                    h->k_ms[2], h->k_launch[2]};
-    for (int i = 0; i < n && i < 11; i++) out[i] = v[i];
+                    h->k_ms[2], h->k_launch[2], h->pm_iters_max};
+    for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
     return SB_OK;
 }
 

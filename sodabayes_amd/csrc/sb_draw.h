@@ -1,0 +1,130 @@
+// Per-SNP conditional draws of the SODA sweep, shared by the FP64 and the
+// chain-batched paths (models.f90:145-161 BayesA, 210-255 BayesB,
+// 349-412 BayesC/Cpi/R/Rc).
+#pragma once
+#include "sb_device.h"
+#include "sb_internal.h"
+
+namespace sb {
+
+__device__ __forceinline__ int bin_of(const Par &P, int g, int k) { return k * P.ngroup + g; }
+
+// Philox key of chain c (COUNTER mode): chains draw independent streams.
+__device__ __forceinline__ uint64_t chain_key(const Par &P, int c)
+{
+    return P.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(c + 1));
+}
+
+// Class probabilities (models.f90:363-390) for BayesC/Cpi/R/Rc.
+__device__ inline void mix_probs(const Par &P, const Scal *sc, int g, double rhs, double zz,
+                                          double zz_vare, double *stemp)
+{
+    double s[MAXMIX];
+    double vare = sc->vare;
+    s[0] = sc->log_p[bin_of(P, g, 0)];
+    for (int k = 1; k < P.nmix; k++) {
+        double logdetV = log(sc->gp[k] * zz_vare + 1.0);
+        double uhat = rhs / (zz + sc->vare_gp[k]);
+        s[k] = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[bin_of(P, g, k)];
+    }
+    for (int kk = 0; kk < P.nmix; kk++) {
+        double skk = s[kk], sk = 0.0;
+        bool overflow = false;
+        for (int j = 0; j < P.nmix; j++) {
+            if (j == kk) continue;
+            double clike = s[j] - skk;
+            if (clike < -700) continue;
+            if (clike > 700) {
+                overflow = true;
+                break;
+            }
+            sk = sk + exp(clike);
+        }
+        stemp[kk] = overflow ? 0.0 : 1.0 / (1.0 + sk);
+    }
+}
+
+// BayesB two-class probability of class 2 (models.f90:223-234)
+__device__ __forceinline__ double b_prob(const Scal *sc, double rhs, double zz, double zz_vare,
+                                         double vs)
+{
+    double vare = sc->vare;
+    double s1 = sc->log_p[0];
+    double logdetV = log(vs * zz_vare + 1.0);
+    double uhat = rhs / (zz + vare / vs);
+    double s2 = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[1];
+    double clike = s1 - s2;
+    if (clike > 700) return 0.0;
+    if (clike < -700) return 1.0;
+    return 1.0 / (1.0 + exp(clike));
+}
+
+// Per-SNP draw of the chosen method from uniform source `src`; returns the
+// new effect and sets class / variance.  Mirrors models.f90:145-161 (A),
+// 210-255 (B), 349-412 (C/Cpi/R/Rc).
+template <class Src>
+__device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
+                         int pos, int g, double *beta, int32_t *cls, double *vara_s, double *delta,
+                         uint8_t *chg)
+{
+    double gk = beta[pos];
+    int cold = P.method == SB_BAYES_A ? 2 : cls[pos];
+    double vare = sc->vare;
+    if (P.method == SB_BAYES_A) {
+        double rhs = rhs0 + zz * gk;
+        double v1 = zz + vare / vara_s[pos];
+        double gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
+        beta[pos] = gn;
+        double scale = (gn * gn + sc->scale_va * P.df_va) / (P.df_va + 1.0);
+        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
+        delta[pos] = gk - gn;
+        chg[pos] = 1;
+        return;
+    }
+    double rhs = rhs0;
+    if (cold > 1) rhs = rhs + zz * gk;
+    if (P.method == SB_BAYES_B) {
+        double vs = vara_s[pos];
+        double sk = b_prob(sc, rhs, zz, zz_vare, vs);
+        double u = src.next();
+        int flag = u < sk ? 2 : 1;
+        double gv, gn;
+        if (flag == 1) {
+            gv = draw_normal(src, 0.0, sqrt(vs));
+            gn = 0.0;
+        } else {
+            double v1 = zz + vare / vs;
+            gv = draw_normal(src, rhs / v1, sqrt(vare / v1));
+            gn = gv;
+        }
+        beta[pos] = gn;
+        cls[pos] = flag;
+        double scale = (gv * gv + sc->scale_va * P.df_va) / (P.df_va + 1.0);
+        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
+        delta[pos] = gk - gn;
+        chg[pos] = !(flag == 1 && cold == 1);
+        return;
+    }
+    double stemp[MAXMIX];
+    mix_probs(P, sc, g, rhs, zz, zz_vare, stemp);
+    double u = src.next(), ss = 0.0;
+    int flag = 1;
+    for (int k = 0; k < P.nmix; k++) {
+        ss = ss + stemp[k];
+        if (u < ss) {
+            flag = k + 1;
+            break;
+        }
+    }
+    double gn = 0.0;
+    if (flag > 1) {
+        double v1 = zz + vare / sc->gp[flag - 1];
+        gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
+    }
+    beta[pos] = gn;
+    cls[pos] = flag;
+    delta[pos] = gk - gn;
+    chg[pos] = !(flag == 1 && cold == 1);
+}
+
+}  // namespace sb

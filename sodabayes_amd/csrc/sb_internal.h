@@ -22,7 +22,8 @@ struct Par {
     int nblocks, blocksize, blockt, naind;
     int ldw;           // 32-bit genotype words per SNP column (multiple of CHUNK_WORDS)
     int nchunk;        // ldw / CHUNK_WORDS
-    int rng_mode, nchains;
+    int64_t ldr;       // residual stride per chain (ldw * 16)
+    int rng_mode, nchains, precision;
     int numit, burnin, thin;
     double pizero, scale_va0, scale_ve, df_va, df_ve, vara0;
     double mix[MAXMIX], alpha[MAXMIX];
@@ -78,6 +79,7 @@ struct sb_handle {
     double k_ms[3] = {0, 0, 0};       // rhs, draw, update: summed durations
     double k_launch[3] = {0, 0, 0};   // launches
     double rhs_bytes = 0.0;           // algorithmic bytes of the timed rhs launches
+    double pm_iters_max = 0.0;        // largest power-method iteration count of the setup
 
     // genotype data
     uint32_t *geno = nullptr;     // nloci x ldw, position order, 2-bit counts
@@ -106,6 +108,21 @@ struct sb_handle {
     int32_t *resync_list = nullptr; // positions with beta != 0 in SNP-id order
     int32_t *d_count = nullptr;
 
+    // mixed-precision batched products (precision SB_PREC_MIXED)
+    int limbs = 3;                // int8 limbs per residual (8L-2 fixed-point bits)
+    int nwg_r = 0;                // workgroups of the residual statistics
+    double *b_rsum = nullptr;     // nchains * nwg_r partial sums of r
+    uint64_t *b_rmax = nullptr;   // nchains * nwg_r partial max |r| (bit patterns)
+    int8_t *b_Q = nullptr;        // (16 * b_ntile) x ldq int8 limbs, limb-major
+    int64_t b_ldq = 0;
+    int *b_expo = nullptr;        // fixed-point exponent per chain
+    int kwords = 128;             // genotype words per MFMA workgroup chunk
+    int b_ntile = 1;              // 16-column MFMA tiles covering nchains * limbs
+    int *b_T = nullptr;           // blocksize x b_ldt int32 limb products
+    int b_ldt = 16;
+    int rows_per_group = 256;     // row split of the augmented product
+    double *b_rapart = nullptr;   // (blocksize / rows_per_group) x blocksize x nchains
+
     std::vector<void *> allocs;
 };
 
@@ -123,4 +140,10 @@ int setup_gram_block(sb_handle *h, int block, double *G_host);
 // MCMC (sb_sweep.hip)
 int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why);
 int mcmc_run(sb_handle *h, int first, int count, double *hyp);
+
+// chain-batched mixed-precision block products and draws (sb_batch.hip)
+int batch_block(sb_handle *h, int b, double zz, int rep);
+int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *chg,
+               double *rhs_out = nullptr);
+int block_rhs(sb_handle *h, int b, double *out_host);
 }  // namespace sb

@@ -592,9 +592,10 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     }
     k_pack_tri<<<dim3(P.blocksize, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->d_tri_off, h->U);
     rc = hip_check(hipGetLastError(), "cholesky");
-    std::vector<int32_t> hfail(nb);
+    std::vector<int32_t> hfail(nb), hit(nb);
     if (!rc) {
         hipMemcpyAsync(hfail.data(), dfail, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
+        hipMemcpyAsync(hit.data(), iters, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
         if (xxd_out) hipMemcpyAsync(xxd_out, h->xxd, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream);
         rc = hip_check(hipStreamSynchronize(h->stream), "build_aug");
     }
@@ -614,6 +615,8 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
             return fail(SB_ERR_NOT_PD, "D - X'X is not positive definite in block " + std::to_string(b + 1) +
                                            " at column " + std::to_string(hfail[b]));
         }
+    h->pm_iters_max = 0;
+    for (int b = 0; b < nb; b++) h->pm_iters_max = std::max(h->pm_iters_max, (double)hit[b]);
     h->aug_built = true;
     h->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return SB_OK;

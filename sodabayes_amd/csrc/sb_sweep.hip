@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "sb_device.h"
+#include "sb_draw.h"
 #include "sb_internal.h"
 
 namespace sb {
@@ -26,7 +27,7 @@ namespace {
 constexpr int NT = 256;
 enum { TAG_MU = 1, TAG_SNP = 2, TAG_HYP = 3, TAG_RA = 4, TAG_INIT = 5 };
 
-__device__ __forceinline__ int bin_of(const Par &P, int g, int k) { return k * P.ngroup + g; }
+
 
 // ------------------------------------------------------------- mu draw --
 
@@ -34,7 +35,10 @@ __device__ __forceinline__ int bin_of(const Par &P, int g, int k) { return k * P
 __global__ __launch_bounds__(NT) void k_mu_pre(Par P, double *r, const Scal *scal, double *red)
 {
     __shared__ double lds[NT / WAVE];
-    double mu = scal->mu, s = 0.0;
+    const int c = blockIdx.z;
+    r += (int64_t)c * P.ldr;
+    red += (int64_t)c * gridDim.x;
+    double mu = scal[c].mu, s = 0.0;
     for (int i = blockIdx.x * NT + threadIdx.x; i < P.n; i += gridDim.x * NT) {
         double v = r[i] + mu;
         r[i] = v;
@@ -48,7 +52,10 @@ __global__ __launch_bounds__(NT) void k_mu_pre(Par P, double *r, const Scal *sca
 // log_p, vare_gp (models.f90:330-338)
 __global__ void k_mu_draw(Par P, Scal *scal, const double *red, int nred, int rep)
 {
-    if (threadIdx.x != 0) return;
+    const int c = threadIdx.x;
+    if (c >= P.nchains) return;
+    scal += c;
+    red += (int64_t)c * nred;
     double s = 0.0;
     for (int i = 0; i < nred; i++) s += red[i];
     double nn = (double)P.n;
@@ -59,7 +66,7 @@ __global__ void k_mu_draw(Par P, Scal *scal, const double *red, int nred, int re
         scal->rng[0] = src.st.s1;
         scal->rng[1] = src.st.s2;
     } else {
-        PhiloxSrc src(P.seed, (uint32_t)rep, 0xFFFFFFF0u, TAG_MU);
+        PhiloxSrc src(chain_key(P, c), (uint32_t)rep, 0xFFFFFFF0u, TAG_MU);
         scal->mu = draw_normal(src, mean, sd);
     }
     if (P.method == SB_BAYES_B) {
@@ -74,7 +81,9 @@ __global__ void k_mu_draw(Par P, Scal *scal, const double *red, int nred, int re
 
 __global__ void k_mu_post(Par P, double *r, const Scal *scal)
 {
-    double mu = scal->mu;
+    const int c = blockIdx.z;
+    r += (int64_t)c * P.ldr;
+    double mu = scal[c].mu;
     for (int i = blockIdx.x * NT + threadIdx.x; i < P.n; i += gridDim.x * NT) r[i] = r[i] - mu;
 }
 
@@ -167,118 +176,6 @@ __device__ __forceinline__ double block_rhs(const Par &P, const double *part, co
     return rx + partA[jk];
 }
 
-// Class probabilities (models.f90:363-390) for BayesC/Cpi/R/Rc.
-__device__ __forceinline__ void mix_probs(const Par &P, const Scal *sc, int g, double rhs, double zz,
-                                          double zz_vare, double *stemp)
-{
-    double s[MAXMIX];
-    double vare = sc->vare;
-    s[0] = sc->log_p[bin_of(P, g, 0)];
-    for (int k = 1; k < P.nmix; k++) {
-        double logdetV = log(sc->gp[k] * zz_vare + 1.0);
-        double uhat = rhs / (zz + sc->vare_gp[k]);
-        s[k] = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[bin_of(P, g, k)];
-    }
-    for (int kk = 0; kk < P.nmix; kk++) {
-        double skk = s[kk], sk = 0.0;
-        bool overflow = false;
-        for (int j = 0; j < P.nmix; j++) {
-            if (j == kk) continue;
-            double clike = s[j] - skk;
-            if (clike < -700) continue;
-            if (clike > 700) {
-                overflow = true;
-                break;
-            }
-            sk = sk + exp(clike);
-        }
-        stemp[kk] = overflow ? 0.0 : 1.0 / (1.0 + sk);
-    }
-}
-
-// BayesB two-class probability of class 2 (models.f90:223-234)
-__device__ __forceinline__ double b_prob(const Scal *sc, double rhs, double zz, double zz_vare,
-                                         double vs)
-{
-    double vare = sc->vare;
-    double s1 = sc->log_p[0];
-    double logdetV = log(vs * zz_vare + 1.0);
-    double uhat = rhs / (zz + vare / vs);
-    double s2 = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[1];
-    double clike = s1 - s2;
-    if (clike > 700) return 0.0;
-    if (clike < -700) return 1.0;
-    return 1.0 / (1.0 + exp(clike));
-}
-
-// Per-SNP draw of the chosen method from uniform source `src`; returns the
-// new effect and sets class / variance.  Mirrors models.f90:145-161 (A),
-// 210-255 (B), 349-412 (C/Cpi/R/Rc).
-template <class Src>
-__device__ void draw_snp(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
-                         int pos, int g, double *beta, int32_t *cls, double *vara_s, double *delta,
-                         uint8_t *chg)
-{
-    double gk = beta[pos];
-    int cold = P.method == SB_BAYES_A ? 2 : cls[pos];
-    double vare = sc->vare;
-    if (P.method == SB_BAYES_A) {
-        double rhs = rhs0 + zz * gk;
-        double v1 = zz + vare / vara_s[pos];
-        double gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
-        beta[pos] = gn;
-        double scale = (gn * gn + sc->scale_va * P.df_va) / (P.df_va + 1.0);
-        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
-        delta[pos] = gk - gn;
-        chg[pos] = 1;
-        return;
-    }
-    double rhs = rhs0;
-    if (cold > 1) rhs = rhs + zz * gk;
-    if (P.method == SB_BAYES_B) {
-        double vs = vara_s[pos];
-        double sk = b_prob(sc, rhs, zz, zz_vare, vs);
-        double u = src.next();
-        int flag = u < sk ? 2 : 1;
-        double gv, gn;
-        if (flag == 1) {
-            gv = draw_normal(src, 0.0, sqrt(vs));
-            gn = 0.0;
-        } else {
-            double v1 = zz + vare / vs;
-            gv = draw_normal(src, rhs / v1, sqrt(vare / v1));
-            gn = gv;
-        }
-        beta[pos] = gn;
-        cls[pos] = flag;
-        double scale = (gv * gv + sc->scale_va * P.df_va) / (P.df_va + 1.0);
-        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
-        delta[pos] = gk - gn;
-        chg[pos] = !(flag == 1 && cold == 1);
-        return;
-    }
-    double stemp[MAXMIX];
-    mix_probs(P, sc, g, rhs, zz, zz_vare, stemp);
-    double u = src.next(), ss = 0.0;
-    int flag = 1;
-    for (int k = 0; k < P.nmix; k++) {
-        ss = ss + stemp[k];
-        if (u < ss) {
-            flag = k + 1;
-            break;
-        }
-    }
-    double gn = 0.0;
-    if (flag > 1) {
-        double v1 = zz + vare / sc->gp[flag - 1];
-        gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
-    }
-    beta[pos] = gn;
-    cls[pos] = flag;
-    delta[pos] = gk - gn;
-    chg[pos] = !(flag == 1 && cold == 1);
-}
-
 // COUNTER mode: one thread per SNP of the block.
 __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, const double *part,
                                                      const double *partA, const double *partR,
@@ -292,7 +189,7 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
     if (jk >= nb) return;
     int pos = p0 + jk;
     double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
-    PhiloxSrc src(P.seed, (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
+    PhiloxSrc src(chain_key(P, 0), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
     draw_snp(P, sc, src, rhs, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta, chg);
 }
 
@@ -414,6 +311,11 @@ __global__ __launch_bounds__(NT) void k_update(Par P, const uint32_t *__restrict
     __shared__ int lst[4096];
     __shared__ double dl[4096];
     __shared__ int cnt[NT + 1];
+    const int c = blockIdx.z;
+    chg += (int64_t)c * P.p;
+    delta += (int64_t)c * P.p;
+    r += (int64_t)c * P.ldr;
+    ra += (int64_t)c * P.blocksize;
     const int m = build_list(chg, delta, p0, nb, lst, dl, cnt);
     if (m == 0) return;
     if ((int)blockIdx.x < nrw_blocks) {
@@ -456,6 +358,9 @@ __global__ __launch_bounds__(NT) void k_update(Par P, const uint32_t *__restrict
 __global__ __launch_bounds__(NT) void k_rtr(Par P, const double *r, double *red)
 {
     __shared__ double lds[NT / WAVE];
+    const int c = blockIdx.z;
+    r += (int64_t)c * P.ldr;
+    red += (int64_t)c * gridDim.x;
     double s = 0.0;
     for (int i = blockIdx.x * NT + threadIdx.x; i < P.n; i += gridDim.x * NT) s += r[i] * r[i];
     s = block_sum<NT>(s, lds);
@@ -470,6 +375,11 @@ __global__ __launch_bounds__(NT) void k_bins(Par P, const double *beta, const in
 {
     __shared__ double lds[NT / WAVE];
     int bin = blockIdx.y, wg = blockIdx.x;
+    const int ch = blockIdx.z;
+    beta += (int64_t)ch * P.p;
+    cls += (int64_t)ch * P.p;
+    cnt_out += (int64_t)ch * P.nbins * gridDim.x;
+    var_out += (int64_t)ch * P.nbins * gridDim.x;
     int g = bin % P.ngroup, k = bin / P.ngroup;
     int a = wg * chunk, e = min(P.p, a + chunk);
     double c = 0.0, v = 0.0;
@@ -492,8 +402,14 @@ __global__ __launch_bounds__(NT) void k_bins(Par P, const double *beta, const in
 __global__ void k_hyper(Par P, Scal *sc, const double *red_rtr, int nrtr, const double *cnt,
                         const double *var, int nwg, int rep, int store, double *hyp, int hyp_idx)
 {
-    if (threadIdx.x != 0) return;
+    const int ch = threadIdx.x;
+    if (ch >= P.nchains) return;
     const int nb = P.nbins;
+    sc += ch;
+    red_rtr += (int64_t)ch * nrtr;
+    cnt += (int64_t)ch * nb * nwg;
+    var += (int64_t)ch * nb * nwg;
+    if (ch != 0) hyp = nullptr;
     for (int b = 0; b < nb; b++) {
         double c = 0.0, v = 0.0;
         for (int w = 0; w < nwg; w++) {
@@ -504,7 +420,7 @@ __global__ void k_hyper(Par P, Scal *sc, const double *red_rtr, int nrtr, const 
         sc->varindist[b] = v;
     }
     XoroSrc xs{{sc->rng[0], sc->rng[1]}};
-    PhiloxSrc ps(P.seed, (uint32_t)rep, 0xFFFFFFF1u, TAG_HYP);
+    PhiloxSrc ps(chain_key(P, ch), (uint32_t)rep, 0xFFFFFFF1u, TAG_HYP);
     const bool replay = P.rng_mode == SB_RNG_REPLAY;
     if (P.method == SB_BAYES_A) {
         sc->snpindist[0] = 0.0;  // BayesA keeps a single variance bin
@@ -588,9 +504,10 @@ __global__ void k_ra_replay(Par P, Scal *sc, double *ra)
 __global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag)
 {
     int i = blockIdx.x * NT + threadIdx.x;
+    const int c = blockIdx.z;
     if (i >= P.naind) return;
-    PhiloxSrc src(P.seed, (uint32_t)rep, (uint32_t)i, tag);
-    ra[i] = draw_normal(src, 0.0, sqrt(sc->vare));
+    PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)i, tag);
+    ra[(int64_t)c * P.blocksize + i] = draw_normal(src, 0.0, sqrt(sc[c].vare));
 }
 
 // posterior sums per SNP (models.f90:528-544)
@@ -599,6 +516,13 @@ __global__ void k_accum(Par P, const double *beta, const int32_t *cls, const dou
 {
     int pos = blockIdx.x * NT + threadIdx.x;
     if (pos >= P.p) return;
+    const int64_t c = blockIdx.z;
+    beta += c * P.p;
+    cls += c * P.p;
+    vara_s += c * P.p;
+    gstore += c * P.p;
+    vara_sstore += c * P.p;
+    indiststore += c * P.p * P.nmix;
     gstore[pos] += beta[pos];
     if (P.method != SB_BAYES_A) indiststore[(int64_t)(cls[pos] - 1) * P.p + pos] += 1.0;
     if (P.method == SB_BAYES_A || P.method == SB_BAYES_B) vara_sstore[pos] += vara_s[pos];
@@ -611,6 +535,10 @@ __global__ __launch_bounds__(NT) void k_nonzero(Par P, const int32_t *pos_by_id,
                                                 int32_t *list, int32_t *count)
 {
     __shared__ int cnt[NT + 1];
+    const int64_t ch = blockIdx.z;
+    beta += ch * P.p;
+    list += ch * P.p;
+    count += ch;
     const int R = (P.p + NT - 1) / NT;
     int a = threadIdx.x * R, e = min(P.p, a + R), c = 0;
     for (int id = a; id < e; id++) c += beta[pos_by_id[id]] != 0.0;
@@ -642,6 +570,12 @@ __global__ __launch_bounds__(NT) void k_resync(Par P, const uint32_t *geno, cons
 {
     int w = blockIdx.x * NT + threadIdx.x;
     if (w >= P.ldw || w * 16 >= P.n) return;
+    const int64_t ch = blockIdx.z;
+    beta += ch * P.p;
+    list += ch * P.p;
+    count += ch;
+    sc += ch;
+    r += ch * P.ldr;
     double acc[16];
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[e] = 0.0;
@@ -671,7 +605,9 @@ __global__ __launch_bounds__(NT) void k_resync(Par P, const uint32_t *geno, cons
 // initial residuals, then r_a.
 __global__ void k_init_draws(Par P, Scal *sc, double yty)
 {
-    if (threadIdx.x != 0) return;
+    const int c = threadIdx.x;
+    if (c >= P.nchains) return;
+    sc += c;
     double nn = (double)P.n;
     double scale = (yty + P.scale_ve * P.df_ve) / (nn + P.df_ve);
     if (P.rng_mode == SB_RNG_REPLAY) {
@@ -680,7 +616,7 @@ __global__ void k_init_draws(Par P, Scal *sc, double yty)
         sc->rng[0] = src.st.s1;
         sc->rng[1] = src.st.s2;
     } else {
-        PhiloxSrc src(P.seed, 0xFFFFFFFEu, 0xFFFFFFF2u, TAG_INIT);
+        PhiloxSrc src(chain_key(P, c), 0xFFFFFFFEu, 0xFFFFFFF2u, TAG_INIT);
         sc->vare = draw_sinvchi2(src, nn + P.df_ve, scale);
     }
 }
@@ -688,10 +624,18 @@ __global__ void k_init_draws(Par P, Scal *sc, double yty)
 __global__ void k_init_r(Par P, const double *why, double mu, double *r)
 {
     int i = blockIdx.x * NT + threadIdx.x;
-    if (i < P.n) r[i] = (why[i] - 0.0) - mu;
+    if (i < P.n) r[(int64_t)blockIdx.z * P.ldr + i] = (why[i] - 0.0) - mu;
 }
 
 }  // namespace
+
+// X_s'r + U_s'r_a of block position jk for the FP64 path (diagnostics)
+__global__ void k_rhs_out(Par P, const double *part, const double *partA, const double *partR,
+                          const double *f2, const double *sdv, int p0, int nb, double *out)
+{
+    int jk = blockIdx.x * NT + threadIdx.x;
+    if (jk < nb) out[jk] = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
+}
 
 // ------------------------------------------------------------- host side --
 
@@ -774,32 +718,67 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
         double r = (why[i] - 0.0) - sc.mu;
         yty += r * r;
     }
-    if (hip_check(hipMemcpyAsync(h->scal, &sc, sizeof sc, hipMemcpyHostToDevice, h->stream), "init scal"))
-        return SB_ERR_HIP;
-    hipMemsetAsync(h->why, 0, sizeof(double) * (size_t)P.ldw * 16, h->stream);
+    // every chain starts from the same state (models.f90:56-114); the draws
+    // that follow differ per chain
+    const int C = P.nchains;
+    const size_t Cs = (size_t)C;
+    for (int c = 0; c < C; c++)
+        if (hip_check(hipMemcpyAsync(h->scal + c, &sc, sizeof sc, hipMemcpyHostToDevice, h->stream), "init scal"))
+            return SB_ERR_HIP;
+    hipMemsetAsync(h->why, 0, sizeof(double) * (size_t)P.ldr, h->stream);
     hipMemcpyAsync(h->why, why, sizeof(double) * n, hipMemcpyHostToDevice, h->stream);
-    hipMemsetAsync(h->r, 0, sizeof(double) * (size_t)P.ldw * 16, h->stream);
-    hipMemsetAsync(h->ra, 0, sizeof(double) * P.blocksize, h->stream);
-    hipMemsetAsync(h->beta, 0, sizeof(double) * p, h->stream);
-    hipMemsetAsync(h->gstore, 0, sizeof(double) * p, h->stream);
-    hipMemsetAsync(h->vara_sstore, 0, sizeof(double) * p, h->stream);
-    hipMemsetAsync(h->indiststore, 0, sizeof(double) * p * nmix, h->stream);
-    std::vector<int32_t> two(p, 2);
-    hipMemcpyAsync(h->cls, two.data(), sizeof(int32_t) * p, hipMemcpyHostToDevice, h->stream);
-    std::vector<double> vs(p, vara);
-    hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p, hipMemcpyHostToDevice, h->stream);
-    k_init_r<<<(n + NT - 1) / NT, NT, 0, h->stream>>>(P, h->why, sc.mu, h->r);
-    k_init_draws<<<1, 1, 0, h->stream>>>(P, h->scal, yty);
+    hipMemsetAsync(h->r, 0, sizeof(double) * (size_t)P.ldr * Cs, h->stream);
+    hipMemsetAsync(h->ra, 0, sizeof(double) * P.blocksize * Cs, h->stream);
+    hipMemsetAsync(h->beta, 0, sizeof(double) * p * Cs, h->stream);
+    hipMemsetAsync(h->gstore, 0, sizeof(double) * p * Cs, h->stream);
+    hipMemsetAsync(h->vara_sstore, 0, sizeof(double) * p * Cs, h->stream);
+    hipMemsetAsync(h->indiststore, 0, sizeof(double) * p * nmix * Cs, h->stream);
+    std::vector<int32_t> two(p * Cs, 2);
+    hipMemcpyAsync(h->cls, two.data(), sizeof(int32_t) * p * Cs, hipMemcpyHostToDevice, h->stream);
+    std::vector<double> vs(p * Cs, vara);
+    hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p * Cs, hipMemcpyHostToDevice, h->stream);
+    k_init_r<<<dim3((n + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->why, sc.mu, h->r);
+    k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal, yty);
     if (P.rng_mode == SB_RNG_REPLAY)
         k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
     else
-        k_ra_counter<<<(P.naind + NT - 1) / NT, NT, 0, h->stream>>>(P, h->scal, h->ra, 0x7FFFFFFF,
-                                                                   TAG_INIT);
+        k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->scal, h->ra,
+                                                                               0x7FFFFFFF, TAG_INIT);
     if (hip_check(hipGetLastError(), "init kernels")) return SB_ERR_HIP;
     if (hip_check(hipStreamSynchronize(h->stream), "init")) return SB_ERR_HIP;
     h->initialised = true;
     h->iterations = 0;
     return SB_OK;
+}
+
+int block_rhs(sb_handle *h, int b, double *out_host)
+{
+    if (!h->initialised) return fail(SB_ERR_STATE, "sb_block_rhs before sb_mcmc_init");
+    if (b < 0 || b >= h->par.nblocks) return fail(SB_ERR_ARG, "block out of range");
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    double *dout = nullptr;
+    if (hip_check(hipMalloc(&dout, sizeof(double) * (size_t)nb * P.nchains), "hipMalloc(rhs)")) return SB_ERR_NOMEM;
+    int rc = SB_OK;
+    if (P.precision == SB_PREC_MIXED) {
+        rc = batch_block(h, b, 0.0, 0);
+        if (!rc) rc = batch_draw(h, b, 0.0, 0, h->sc.delta, reinterpret_cast<uint8_t *>(h->sc.flag), dout);
+    } else {
+        int spw = (int)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 8192));
+        int ngs = (nb + 4 * spw - 1) / (4 * spw);
+        int nxg = ngs * P.nchunk;
+        k_rhs<<<nxg + (nb + 3) / 4, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, h->tri_off[b], p0, nb, spw, ngs,
+                                                 nxg, h->sc.part, h->sc.partA, h->sc.partR);
+        k_rhs_out<<<(nb + NT - 1) / NT, NT, 0, st>>>(P, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, p0,
+                                                     nb, dout);
+    }
+    if (!rc) rc = hip_check(hipMemcpyAsync(out_host, dout, sizeof(double) * (size_t)nb * P.nchains,
+                                           hipMemcpyDeviceToHost, st),
+                            "download rhs");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "block rhs");
+    hipFree(dout);
+    return rc;
 }
 
 int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
@@ -815,9 +794,11 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     double *dhyp = nullptr;
     if (hyp_host && nrec > 0)
         if (hip_check(hipMalloc(&dhyp, sizeof(double) * (size_t)nrec * hw), "hipMalloc(hyp)")) return SB_ERR_NOMEM;
+    const int C = P.nchains;
+    const bool mixed = P.precision == SB_PREC_MIXED;
     const int bin_chunk = 8192;
     const int nwg_bins = (P.p + bin_chunk - 1) / bin_chunk;
-    double *bcnt = h->sc.red + 4096, *bvar = bcnt + (int64_t)P.nbins * nwg_bins;
+    double *bcnt = h->sc.red + (int64_t)1024 * C, *bvar = bcnt + (int64_t)C * P.nbins * nwg_bins;
     uint8_t *chg = reinterpret_cast<uint8_t *>(h->sc.flag);
     double *delta = h->sc.delta;
     std::vector<double> xxd(P.nblocks);
@@ -837,9 +818,9 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     auto t0 = std::chrono::steady_clock::now();
     int rec = 0;
     for (int rep = first; rep < first + count; rep++) {
-        k_mu_pre<<<nred, NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
+        k_mu_pre<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
         k_mu_draw<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, rep);
-        k_mu_post<<<nred, NT, 0, st>>>(P, h->r, h->scal);
+        k_mu_post<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal);
         for (int b = 0; b < P.nblocks; b++) {
             const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
             const int64_t tri0 = h->tri_off[b];
@@ -849,13 +830,21 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             int nxg = ngs * P.nchunk;
             int nag = (nb + 3) / 4;
             mark();
-            k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, p0, nb, spw, ngs, nxg,
-                                            h->sc.part, h->sc.partA, h->sc.partR);
+            if (mixed) {
+                int rc = batch_block(h, b, xxd[b], rep);
+                if (rc) return rc;
+            } else {
+                k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, p0, nb, spw, ngs, nxg,
+                                                h->sc.part, h->sc.partA, h->sc.partR);
+            }
             mark();
             if (h->ktime)
                 h->rhs_bytes += (double)nb * (((double)P.n + 3.0) / 4.0) + 8.0 * (double)nb * (nb + 1) / 2.0;
             mark();
-            if (P.rng_mode == SB_RNG_REPLAY)
+            if (mixed) {
+                int rc = batch_draw(h, b, xxd[b], rep, delta, chg);
+                if (rc) return rc;
+            } else if (P.rng_mode == SB_RNG_REPLAY)
                 k_draw_replay<<<1, NT, 0, st>>>(P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2,
                                                 h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp,
                                                 h->sc.uu, h->sc.chg, h->beta, h->cls, h->vara_s, delta,
@@ -868,26 +857,27 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             int nrw = (P.ldw + NT - 1) / NT;
             int nra = (nb + NT - 1) / NT;
             mark();
-            k_update<<<nrw + nra, NT, 0, st>>>(P, h->geno, h->zlut, h->U, tri0, p0, nb, nrw, chg, delta,
-                                               h->r, h->ra);
+            k_update<<<dim3(nrw + nra, 1, C), NT, 0, st>>>(P, h->geno, h->zlut, h->U, tri0, p0, nb, nrw, chg,
+                                                           delta, h->r, h->ra);
             mark();
         }
-        k_rtr<<<nred, NT, 0, st>>>(P, h->r, h->sc.red);
-        k_bins<<<dim3(nwg_bins, P.nbins), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+        k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);
+        k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
         const int store = (rep > P.burnin) && (rep % P.thin == 0);
         const int hidx = (dhyp && rep % P.thin == 0) ? rec++ : -1;
         k_hyper<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, rep, store, dhyp, hidx);
         if (P.rng_mode == SB_RNG_REPLAY)
             k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
         else
-            k_ra_counter<<<(P.naind + NT - 1) / NT, NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA);
+            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA);
         if (store)
-            k_accum<<<(P.p + NT - 1) / NT, NT, 0, st>>>(P, h->beta, h->cls, h->vara_s, h->gstore,
-                                                       h->indiststore, h->vara_sstore);
+            k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->vara_s, h->gstore,
+                                                                   h->indiststore, h->vara_sstore);
         if (rep % 1000 == 0) {
-            k_nonzero<<<1, NT, 0, st>>>(P, h->d_pos_by_id, h->beta, h->resync_list, h->d_count);
-            k_resync<<<(P.ldw + NT - 1) / NT, NT, 0, st>>>(P, h->geno, h->zlut, h->beta, h->resync_list,
-                                                          h->d_count, h->why, h->scal, h->r);
+            k_nonzero<<<dim3(1, 1, C), NT, 0, st>>>(P, h->d_pos_by_id, h->beta, h->resync_list, h->d_count);
+            k_resync<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->geno, h->zlut, h->beta,
+                                                                      h->resync_list, h->d_count, h->why,
+                                                                      h->scal, h->r);
         }
         if (hip_check(hipGetLastError(), "sweep launch")) return SB_ERR_HIP;
     }

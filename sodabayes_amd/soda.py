@@ -54,7 +54,8 @@ class SodaOptions:
     vara0: float = 0.0            # module vara before the job (models.f90:99-101)
     # build-side choices (no reference counterpart)
     rng: str = "replay"           # "replay" (xoroshiro stream 0) or "counter" (Philox)
-    precision: str = "fp64"
+    precision: str = "fp64"       # "fp64" or "mixed" (int8-limb MFMA products, chains batched)
+    chains: int = 1               # independent chains sharing the genotypes (mixed only)
 
     def method_code(self) -> int:
         for k, v in _lib.METHODS.items():
@@ -206,7 +207,7 @@ class SodaSampler:
         o = self.opts
         d = _lib.SbDims(method=self.method, nind=self.nind, nloci=self.nloci, nmix=self.nmix,
                         ngroup=self.ngroup, nblocks=self.nblocks, blocksize=self.blocksize,
-                        blockt=self.blockt, nchains=1,
+                        blockt=self.blockt, nchains=int(o.chains),
                         rng_mode=_lib.RNG_REPLAY if o.rng == "replay" else _lib.RNG_COUNTER,
                         precision=_lib.PREC_FP64 if o.precision == "fp64" else _lib.PREC_MIXED,
                         bed_on_device=1 if self.bed_device_ptr else 0)
@@ -275,45 +276,53 @@ class SodaSampler:
         return out
 
     def info_raw(self) -> np.ndarray:
-        v = np.zeros(11)
-        _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 11))
+        v = np.zeros(12)
+        _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 12))
         return v
+
+    def block_rhs(self, block: int) -> np.ndarray:
+        """X_s'r + A_s'r_a of a block for the current state: (chains, n_b)."""
+        a, e = block_bounds(self.nloci, self.blocksize, self.nblocks, self.blockt)[block]
+        out = np.zeros((int(self.opts.chains), e - a))
+        _lib.check(_lib.load().sb_block_rhs(self.handle, block, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
     def info(self) -> dict:
         v = self.info_raw()
         return dict(setup_seconds=v[0], iterations=v[1], sweep_ms=v[2], device_bytes=v[3])
 
-    def results(self) -> dict:
+    def results(self, chain: int = 0) -> dict:
         """Final state and posterior means under the reference's names
         (models.f90:575-594)."""
         p, nb = self.nloci, self.nmix * self.ngroup
         m = self.method
-        st = self.get("STORES", 5)
+        get = lambda name, n: self.get(name, n, chain)  # noqa: E731
+        st = get("STORES", 5)
         cnt = st[0]
-        sc = self.get("SCALARS", 4)
+        sc = get("SCALARS", 4)
         out = {
             "nblocks": self.nblocks, "blocksize": self.blocksize, "blockt": self.blockt,
             "naind": self.blocksize, "freq": self.freq, "permvec": self.permvec + 1,
             "xxd": self.get("XXD", self.nblocks),
             "mu": sc[0], "vare": sc[1], "vara": sc[2],
             "mu_store": st[1] / cnt, "vare_store": st[2] / cnt,
-            "g": self.get("G", p), "gstore": self.get("GSTORE", p) / cnt,
-            "yadj": self.get("YADJ", self.nind), "yAadj": self.get("YAADJ", self.blocksize),
-            "varstore": self.get("VARSTORE", nb) / cnt, "varindist": self.get("VARINDIST", nb),
+            "g": get("G", p), "gstore": get("GSTORE", p) / cnt,
+            "yadj": get("YADJ", self.nind), "yAadj": get("YAADJ", self.blocksize),
+            "varstore": get("VARSTORE", nb) / cnt, "varindist": get("VARINDIST", nb),
         }
         if m != 1:
             grp1 = (self.grp_idx + 1) if self.grp_idx is not None else np.ones(p)
-            out["snptracker"] = np.concatenate([grp1, self.get("CLASS", p)])
-            out["snpindist"] = self.get("SNPINDIST", nb)
-            out["snpstore"] = self.get("SNPSTORE", nb) / cnt
-            out["indiststore"] = self.get("INDISTSTORE", p * self.nmix) / cnt
+            out["snptracker"] = np.concatenate([grp1, get("CLASS", p)])
+            out["snpindist"] = get("SNPINDIST", nb)
+            out["snpstore"] = get("SNPSTORE", nb) / cnt
+            out["indiststore"] = get("INDISTSTORE", p * self.nmix) / cnt
             out["included_store"] = st[4] / cnt
-            out["p"] = self.get("P", nb)
+            out["p"] = get("P", nb)
         if m not in (1, 2, 3):
-            out["pstore"] = self.get("PSTORE", nb) / cnt
+            out["pstore"] = get("PSTORE", nb) / cnt
         if m in (1, 2):
-            out["vara_s"] = self.get("VARA_S", p)
-            out["vara_sstore"] = self.get("VARA_SSTORE", p) / cnt
+            out["vara_s"] = get("VARA_S", p)
+            out["vara_sstore"] = get("VARA_SSTORE", p) / cnt
         else:
             out["vara_store"] = st[3] / cnt
         return out

@@ -1,0 +1,96 @@
+"""Mixed-precision batched path and the counter-based RNG on the GPU.
+
+Bars:
+* block products X_s'r + A_s'r_a of every chain against an FP64 evaluation of
+  the same state: norm-wise relative error <= 1e-6 (mixed, int8 limbs) and
+  <= 1e-12 (FP64 path);
+* posterior means of mu, sigma2_e, sigma2_g and of the effects from batched
+  mixed chains against long oracle chains (the reference's algorithm) within
+  4 combined Monte Carlo standard errors.
+"""
+import numpy as np
+import pytest
+
+import orc
+from sodabayes_amd import soda, synth
+
+pytestmark = pytest.mark.gpu
+
+R4 = "0.0,0.0001,0.001,0.01"
+
+
+def dataset(n=1001, p=300, seed=41):
+    bed, why, _ = synth.dataset(n, p, seed=seed, nqtl=15, h2=0.5)
+    return bed, why
+
+
+def fp64_products(s, bed, chain):
+    """X_s'r + U_s'r_a in numpy FP64 from the sampler's downloaded state."""
+    n, p = s.nind, s.nloci
+    f = orc.freq(bed, n)
+    Z = orc.standardize(bed, n, f)
+    r = s.get("YADJ", n, chain)
+    ra = s.get("YAADJ", s.blocksize, chain)
+    XA = s.get("XA", s.blocksize * p).reshape(s.blocksize, p, order="F")
+    out = []
+    for a, e in soda.block_bounds(p, s.blocksize, s.nblocks, s.blockt):
+        cols = s.permvec[a:e]
+        nb = e - a
+        out.append(Z[:, cols].T @ r + XA[:nb, a:e].T @ ra[:nb])
+    return out
+
+
+@pytest.mark.parametrize("precision,chains,bar", [("mixed", 4, 1e-6), ("mixed", 1, 1e-6),
+                                                  ("fp64", 1, 1e-12)])
+def test_block_products_against_fp64(precision, chains, bar):
+    bed, why = dataset()
+    o = soda.SodaOptions(method="BayesR", nmix=4, mix=R4, blocksize=100, numit=50, burnin=0, seed=5,
+                         rng="counter", precision=precision, chains=chains)
+    s = soda.SodaSampler(o, 1001, 300, why, bed=bed)
+    s.setup()
+    s.init()
+    s.run(3, trace=False)
+    worst = 0.0
+    for b in range(s.nblocks):
+        got = s.block_rhs(b)
+        for c in range(chains):
+            ref = fp64_products(s, bed, c)[b]
+            err = np.linalg.norm(got[c] - ref) / np.linalg.norm(ref)
+            worst = max(worst, err)
+    print(f"{precision} x{chains}: worst norm-wise relative error of X'r + A'r_a = {worst:.3e}")
+    assert worst <= bar
+    s.close()
+
+
+def batch_means_se(x, nb=10):
+    x = np.asarray(x)
+    m = len(x) // nb
+    bm = x[: m * nb].reshape(nb, m).mean(1)
+    return bm.std(ddof=1) / np.sqrt(nb)
+
+
+def test_mixed_chains_match_reference_posterior():
+    n, p = 400, 200
+    bed, why, _ = synth.dataset(n, p, seed=77, nqtl=10, h2=0.5)
+    numit, burn = 1500, 300
+    ref = orc.run(orc.make_opts("BayesR", n, p, nmix=4, mix=(0.0, 0.0001, 0.001, 0.01), blocksize=50,
+                                numit=numit, burnin=burn, seed=3), bed, why)
+    C, it = 16, 400
+    o = soda.SodaOptions(method="BayesR", nmix=4, mix=R4, blocksize=50, numit=it, burnin=100, seed=9,
+                         rng="counter", precision="mixed", chains=C)
+    s = soda.SodaSampler(o, n, p, why, bed=bed)
+    s.setup()
+    s.init()
+    s.run(trace=False)
+    res = [s.results(c) for c in range(C)]
+    s.close()
+    for key, hk in (("mu_store", "hyp_mu"), ("vare_store", "hyp_vare"), ("vara_store", "hyp_vara")):
+        chain_means = np.array([float(np.ravel(r[key])[0]) for r in res])
+        m_gpu, se_gpu = chain_means.mean(), chain_means.std(ddof=1) / np.sqrt(C)
+        trace = ref[hk][burn:]
+        m_ref, se_ref = trace.mean(), batch_means_se(trace)
+        z = abs(m_gpu - m_ref) / np.sqrt(se_gpu ** 2 + se_ref ** 2)
+        print(f"{key}: gpu {m_gpu:.6g} +- {se_gpu:.2g}, reference {m_ref:.6g} +- {se_ref:.2g}, z={z:.2f}")
+        assert z < 4.0, key
+    g_gpu = np.mean([r["gstore"] for r in res], axis=0)
+    assert np.corrcoef(g_gpu, ref["gstore"])[0, 1] > 0.9

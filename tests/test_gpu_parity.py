@@ -79,7 +79,7 @@ def test_replay_chain_matches_reference(case):
     gold = c["out"]
     for k in DISCRETE:
         if k in gold:
-            assert np.array_equal(np.asarray(res[k], dtype=float), gold[k].astype(float)), k
+            assert np.array_equal(np.ravel(np.asarray(res[k], dtype=float)), gold[k].astype(float).ravel()), k
     for k in CONT:
         if k in gold:
             assert rel_err(res[k], gold[k]) < 1e-9, (k, rel_err(res[k], gold[k]))
