@@ -8,10 +8,13 @@ x_ij ~ Bin(2, f_j), generated directly in HBM as a SNP-major PLINK payload;
 phenotype from 5,000 t(4) QTL at h2 = 0.4.  One step = one Gibbs iteration
 (p SNP-effect updates).  Inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  Besides value (SNP-effect updates/s) it carries the
-roofline of the dominant kernel (the block product X_s'r + A_s'r_a,
-timed with HIP events on the library's stream in a separate pass) and the
-reference CPU baseline timed on this host.
+Two passes share the genotypes: the plain FP64 single-chain path, then the
+mixed-precision batched path (``--chains`` chains per launch, int8-limb MFMA
+products).  ``value`` is the batched path's SNP-effect updates/s summed over
+chains; ``fp64_gpu`` and ``speedup_vs_fp64_gpu`` give the FP64 path beside
+it.  The roofline is that of the dominant kernel group (the block products),
+timed with HIP events on the library's stream in a separate pass; the
+reference CPU baseline is timed on this host.
 """
 from __future__ import annotations
 
@@ -33,14 +36,16 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--nind", type=int, default=50000)
     ap.add_argument("--nloci", type=int, default=500000)
     ap.add_argument("--blocksize", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=20240607)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
     return ap.parse_args()
 
 
@@ -125,6 +130,56 @@ def cpu_baseline(seed: int) -> dict | None:
                       f"{numit} iterations, -nthreads {cores}; MCMC {mcmc:.2f} s, setup {float(m.group(1)):.2f} s"}
 
 
+def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int):
+    """Setup + timed sweeps of one product path; returns its metrics."""
+    n, p, bs = a.nind, a.nloci, a.blocksize
+    troof = 0 if a.no_roofline else max(3, a.steps // 3)
+    opts = soda.SodaOptions(method="BayesR", nmix=4, mix="0.0,0.0001,0.001,0.01", blocksize=bs,
+                            numit=a.warmup + a.steps + troof, burnin=0, seed=a.seed, rng="counter",
+                            precision=precision, chains=chains)
+    s = soda.SodaSampler(opts, n, p, why, freq=freq, bed_device_ptr=bed_ptr)
+    t0 = time.time()
+    s.setup()
+    setup_s = time.time() - t0
+    info = s.info_raw()
+    log(f"[{precision} x{chains}] setup (gram + power method + cholesky, {s.nblocks} blocks): "
+        f"{setup_s:.1f} s, power iterations <= {int(info[11])}")
+    s.init()
+    s.run(a.warmup, trace=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(a.steps, trace=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"setup_s": setup_s, "power_iters": int(info[11]), "dt": dt, "ms": dt * 1e3 / a.steps,
+           "value": p * chains * a.steps / dt}
+    log(f"[{precision} x{chains}] {out['ms']:.2f} ms/iteration, {out['value']:.4g} SNP-updates/s")
+    if troof:
+        # separate pass with HIP events around every block kernel
+        os.environ["SODABAYES_KERNEL_TIMING"] = "1"
+        before = s.info_raw()
+        s.run(troof, trace=False)
+        after = s.info_raw()
+        os.environ.pop("SODABAYES_KERNEL_TIMING", None)
+        launches = after[5] - before[5]
+        rhs_ms = (after[4] - before[4]) / launches
+        rhs_bytes = (after[6] - before[6]) / launches
+        achieved = rhs_bytes / (rhs_ms * 1e-3) / 1e9
+        out["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "block products X_s'r + A_s'r_a (k_rhs)" if precision == "fp64"
+            else "block products X_s'Q + U_s'R_a (k_rstat+k_quant+k_xq+k_ua)",
+            "avg_launch_us": round(rhs_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(rhs_bytes),
+            "draw_avg_us": round((after[7] - before[7]) / max(1, after[8] - before[8]) * 1e3, 2),
+            "update_avg_us": round((after[9] - before[9]) / max(1, after[10] - before[10]) * 1e3, 2)}
+        log(f"[{precision} x{chains}] block products: {rhs_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s "
+            f"({achieved / HBM_PEAK_GBS:.1%} of HBM); draw {out['roofline']['draw_avg_us']} us, "
+            f"update {out['roofline']['update_avg_us']} us")
+    s.close()
+    return out
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -140,62 +195,29 @@ def main():
     bed, freq, why = synth_on_device(torch, n, p, a.seed)
     torch.cuda.synchronize()
     log(f"genotypes generated in HBM: n={n} p={p} ({bed.numel() / 1e9:.2f} GB) in {time.time() - t0:.1f} s")
-    total = a.warmup + a.steps + (0 if a.no_roofline else max(3, a.steps // 3))
-    opts = soda.SodaOptions(method="BayesR", nmix=4, mix="0.0,0.0001,0.001,0.01", blocksize=bs,
-                            numit=total, burnin=0, seed=a.seed, rng="counter", precision="fp64")
-    s = soda.SodaSampler(opts, n, p, why, freq=freq, bed_device_ptr=bed.data_ptr())
-    t0 = time.time()
-    s.setup()
+    fp64 = None if a.chains_only else run_path(soda, torch, a, why, freq, bed.data_ptr(), "fp64", 1)
+    mixed = run_path(soda, torch, a, why, freq, bed.data_ptr(), "mixed", a.chains)
     del bed
     torch.cuda.empty_cache()
-    setup_s = time.time() - t0
-    log(f"setup (gram + power method + cholesky, {s.nblocks} blocks): {setup_s:.1f} s")
-    s.init()
-    s.run(a.warmup, trace=False)
-    torch.cuda.synchronize()
-    log(f"warmup {a.warmup} iterations done")
-    t0 = time.perf_counter()
-    s.run(a.steps, trace=False)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms = dt * 1e3 / a.steps
-    value = p * a.steps / dt
-    log(f"timed {a.steps} iterations: {ms:.2f} ms/iteration, {value:.4g} SNP-updates/s")
-
-    roof = None
-    if not a.no_roofline:
-        # separate pass with HIP events around every block kernel
-        os.environ["SODABAYES_KERNEL_TIMING"] = "1"
-        before = s.info_raw()
-        k = max(3, a.steps // 3)
-        s.run(k, trace=False)
-        after = s.info_raw()
-        os.environ.pop("SODABAYES_KERNEL_TIMING", None)
-        launches = after[5] - before[5]
-        rhs_ms = (after[4] - before[4]) / launches
-        rhs_bytes = (after[6] - before[6]) / launches
-        achieved = rhs_bytes / (rhs_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_rhs", "avg_launch_us": round(rhs_ms * 1e3, 2),
-                "algorithmic_bytes_per_launch": int(rhs_bytes),
-                "draw_avg_us": round((after[7] - before[7]) / max(1, after[8] - before[8]) * 1e3, 2),
-                "update_avg_us": round((after[9] - before[9]) / max(1, after[10] - before[10]) * 1e3, 2)}
-        log(f"k_rhs: {rhs_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of HBM)")
-    s.close()
 
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.seed)
     out = {
         "metric": "SNP-effect updates/s (BayesR n=50k,p=500k): mixed+batched vs FP64 GPU, same acc.",
-        "value": value, "unit": "SNP-effect updates/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"SODA BayesR n={n} p={p} n_s={bs}, 4-class mixture, 1 chain, "
-                               "counter-based RNG, FP64 sweep", "nind": n, "nloci": p, "blocksize": bs,
-                   "chains": 1, "precision": "fp64", "parallelism": "single GPU"},
-        "setup_seconds": round(setup_s, 2),
-        "iterations_per_s": a.steps / dt,
-        "roofline": roof,
+        "value": mixed["value"], "unit": "SNP-effect updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": mixed["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int8 MFMA limbs (int32 sums) + f64", "data": "synthetic",
+        "config": {"workload": f"SODA BayesR n={n} p={p} n_s={bs}, 4-class mixture, {a.chains} chains "
+                               "batched, counter-based RNG, mixed-precision products",
+                   "nind": n, "nloci": p, "blocksize": bs, "chains": a.chains, "limbs": 3,
+                   "parallelism": "single GPU"},
+        "fp64_gpu": None if fp64 is None else {
+            "value": fp64["value"], "ms_per_step": fp64["ms"], "chains": 1,
+            "roofline": fp64.get("roofline"), "setup_seconds": round(fp64["setup_s"], 2)},
+        "speedup_vs_fp64_gpu": None if fp64 is None else mixed["value"] / fp64["value"],
+        "setup_seconds": round(mixed["setup_s"], 2),
+        "power_method_iterations": mixed["power_iters"],
+        "iterations_per_s": a.steps / mixed["dt"],
+        "roofline": mixed.get("roofline"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -245,6 +245,11 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->sc.delta, sizeof(double) * p * C);
     A((void **)&h->sc.red, sizeof(double) * C * (1024 + 2 * (size_t)P.nbins * nwg_bins) + 64);
     A((void **)&h->resync_list, sizeof(int32_t) * p * C);
+    A((void **)&h->u_lst, sizeof(int) * bs * C);
+    A((void **)&h->u_dl, sizeof(double) * bs * C);
+    A((void **)&h->u_cl, sizeof(double) * bs * C);
+    A((void **)&h->u_kc, sizeof(double) * (C + 4));
+    A((void **)&h->u_cnt, sizeof(int) * (C + 4));
     A((void **)&h->d_count, sizeof(int32_t) * (C + 4));
     if (d.precision == SB_PREC_MIXED) {
         h->nwg_r = (int)std::min<int64_t>(64, (d.nind + 255) / 256);
@@ -265,6 +270,15 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         A((void **)&h->b_expo, sizeof(int) * C);
         A((void **)&h->b_T, sizeof(int) * (size_t)bs * h->b_ldt);
         A((void **)&h->b_rapart, sizeof(double) * (size_t)nrg * bs * C);
+        A((void **)&h->b_uapart, sizeof(double) * (size_t)nrg * bs * C);
+        h->b_ldd = ((bs + 15) / 16) * 16;
+        A((void **)&h->b_Dq, (size_t)16 * h->b_ntile * h->b_ldd);
+        A((void **)&h->b_expo2, sizeof(int) * C);
+        A((void **)&h->b_kc, sizeof(double) * C);
+        h->gt_off.assign(d.nblocks + 1, 0);
+        for (int b = 0; b < d.nblocks; b++)
+            h->gt_off[b + 1] = h->gt_off[b] + (int64_t)((ls[b + 1] - ls[b] + 15) / 16) * ldr;
+        A((void **)&h->geno_t, sizeof(uint32_t) * (size_t)h->gt_off[d.nblocks]);
     }
     if (rc) {
         sb_destroy(h);
@@ -309,9 +323,11 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     if (h->b_Q) {
         hipMemsetAsync(h->b_Q, 0, (size_t)16 * h->b_ntile * h->b_ldq, st);
         hipMemsetAsync(h->b_T, 0, sizeof(int) * (size_t)bs * h->b_ldt, st);
+        hipMemsetAsync(h->b_Dq, 0, (size_t)16 * h->b_ntile * h->b_ldd, st);
     }
     hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
     rc = setup_pack(h, bed, d.bed_on_device != 0);
+    if (!rc && h->geno_t) rc = batch_transpose(h);
     if (rc) {
         sb_destroy(h);
         return rc;

@@ -151,49 +151,222 @@ __global__ __launch_bounds__(NT) void k_xq(Par P, const uint32_t *__restrict__ g
         }
 }
 
-// RA_part[rg][jk][c] = sum over rows i of row group rg (i <= jk) of
-// U(i, jk) * ra_c(i): 64 columns x all chains per workgroup, U and R_a
-// tiles staged in LDS.
-__global__ __launch_bounds__(NT) void k_ua(Par P, const double *__restrict__ U, int64_t tri0, int nb,
-                                           const double *__restrict__ ra, int rows_per_group,
-                                           double *__restrict__ part)
+// Products with a block's packed upper triangle U on FP64 MFMA
+// (v_mfma_f64_16x16x4f64), 64 x 64 U tiles staged in LDS:
+//   mode 0 (rhs):    part[g][jk][c] = sum_{i in row group g, i <= jk} U(i, jk) R(i, c)
+//   mode 1 (update): part[g][i][c]  = sum_{j in col group g, j >= i} U(i, j) R(j, c)
+// R is n_b x C (chain stride rstride) -- r_a for mode 0, delta for mode 1.
+// Workgroup: 64 outputs (4 waves x 16) x all chains over one group of
+// `span` rows (mode 0) or columns (mode 1).
+typedef double v4d __attribute__((ext_vector_type(4)));
+template <int MODE>
+__global__ __launch_bounds__(NT) void k_utile(Par P, const double *__restrict__ U, int64_t tri0, int nb,
+                                              const double *__restrict__ R, int64_t rstride, int span,
+                                              double *__restrict__ part)
 {
     __shared__ double Ut[64][65];
     __shared__ double Rt[64][65];
     const int C = P.nchains;
-    const int col0 = blockIdx.x * 64;
-    const int rg = blockIdx.y;
-    const int r0 = rg * rows_per_group;
-    if (r0 >= nb || r0 > col0 + 63) return;  // group entirely below the diagonal
-    const int r1 = min(min(nb, r0 + rows_per_group), col0 + 64);
-    const int tc = threadIdx.x & 63, cg = threadIdx.x >> 6;  // column, chain group
-    double acc[16];
+    const int o0 = blockIdx.x * 64;          // first output index (column jk or row i)
+    const int g = blockIdx.y;
+    int k0 = g * span, k1 = min(nb, k0 + span);
+    if (MODE == 0) {
+        if (k0 > o0 + 63) return;            // rows entirely below the diagonal
+        k1 = min(k1, o0 + 64);
+    } else {
+        if (k1 <= o0) return;                // columns entirely left of the diagonal
+        k0 = max(k0, o0);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, lg = lane >> 4;
+    const int ntc = (C + 15) / 16;
+    v4d acc[4];
 #pragma unroll
-    for (int k = 0; k < 16; k++) acc[k] = 0.0;
-    for (int rt = r0; rt < r1; rt += 64) {
+    for (int t = 0; t < 4; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int kt = k0; kt < k1; kt += 64) {
+        // U tile: rows r (mode 0: k index; mode 1: output index), cols likewise
         for (int e = threadIdx.x; e < 64 * 64; e += NT) {
             int rr = e & 63, cc = e >> 6;
-            int i = rt + rr, j = col0 + cc;
-            Ut[rr][cc] = (j < nb && i <= j && i < r1) ? U[tri0 + (int64_t)j * (j + 1) / 2 + i] : 0.0;
+            int i = MODE == 0 ? kt + rr : o0 + rr;
+            int j = MODE == 0 ? o0 + cc : kt + cc;
+            bool ok = j < nb && i <= j && (MODE == 0 ? i < k1 : (i < nb && j < k1));
+            Ut[rr][cc] = ok ? U[tri0 + (int64_t)j * (j + 1) / 2 + i] : 0.0;
         }
         for (int e = threadIdx.x; e < 64 * 64; e += NT) {
-            int rr = e & 63, ch = e >> 6;
-            int i = rt + rr;
-            Rt[rr][ch] = (ch < C && i < r1) ? ra[(int64_t)ch * P.blocksize + i] : 0.0;
+            int kk = e & 63, ch = e >> 6;
+            int k = kt + kk;
+            Rt[kk][ch] = (ch < C && k < k1) ? R[(int64_t)ch * rstride + k] : 0.0;
         }
         __syncthreads();
-        for (int rr = 0; rr < 64; rr++) {
-            double u = Ut[rr][tc];
-#pragma unroll
-            for (int k = 0; k < 16; k++) acc[k] = fma(u, Rt[rr][cg * 16 + k], acc[k]);
+#pragma unroll 4
+        for (int ks = 0; ks < 64; ks += 4) {
+            // A[m][k]: m = output within the wave's 16, k = ks + lg
+            const int m = wave * 16 + l16, k = ks + lg;
+            const double a = MODE == 0 ? Ut[k][m] : Ut[m][k];
+            for (int t = 0; t < ntc; t++) {
+                const double bv = Rt[k][t * 16 + l16];
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
+            }
         }
         __syncthreads();
     }
-    const int j = col0 + tc;
-    if (j >= nb) return;
-    for (int k = 0; k < 16; k++) {
-        int ch = cg * 16 + k;
-        if (ch < C) part[((int64_t)rg * P.blocksize + j) * C + ch] = acc[k];
+    // D[m][n] of the f64 16x16x4 MFMA: m = lg + 4*v (rows interleaved across
+    // the 4 lane groups), n = l16 (chain within tile t)
+    for (int t = 0; t < ntc; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            int o = o0 + wave * 16 + lg + 4 * v;
+            int ch = t * 16 + l16;
+            if (o < nb && ch < C) part[((int64_t)g * P.blocksize + o) * C + ch] = acc[t][v];
+        }
+}
+
+// r_a(i, c) += sum of the mode-1 partials of the column groups that reach row
+// i's tile (groups left of the diagonal wrote nothing).
+__global__ void k_ra_reduce(Par P, const double *part, int span, int ngroups, int nb, double *ra)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= nb) return;
+    const int C = P.nchains;
+    const int i0 = (i / 64) * 64;
+    double s = 0.0;
+    for (int g = i0 / span; g < ngroups; g++) s += part[((int64_t)g * P.blocksize + i) * C + c];
+    ra[(int64_t)c * P.blocksize + i] += s;
+}
+
+// Genotypes in individual-major form per block for the update GEMM:
+// word (block b, SNP word sw, individual i) holds the 2-bit counts of block
+// positions 16 sw .. 16 sw + 15 of individual i (position t at bits 2t).
+__global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, int nb, int64_t ldi,
+                            uint32_t *__restrict__ gt)
+{
+    const int sw = blockIdx.y;
+    const int iw = blockIdx.x * blockDim.x + threadIdx.x;  // word of 16 individuals
+    if (iw >= ldw) return;
+    uint32_t in[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        int pos = 16 * sw + t;
+        in[t] = pos < nb ? geno[(int64_t)(p0 + pos) * ldw + iw] : 0u;
+    }
+    uint32_t out[16];
+#pragma unroll
+    for (int li = 0; li < 16; li++) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int t = 0; t < 16; t++) w |= ((in[t] >> (2 * li)) & 3u) << (2 * t);
+        out[li] = w;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(gt + (int64_t)sw * ldi + (int64_t)iw * 16);
+#pragma unroll
+    for (int q = 0; q < 4; q++) dst[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+}
+
+// Per chain: c_j = delta_j / s_j over the block, K_c = sum_j 2 f_j c_j, and
+// c quantised to L balanced int8 limbs in the MFMA slot order (Dq rows
+// c*L + k, 16 slots per 16-SNP word).
+__global__ __launch_bounds__(NT) void k_dquant(Par P, int L, const double *delta, const double *f2,
+                                               const double *sdv, int p0, int nb, int nsw, int8_t *Dq,
+                                               int64_t ldd, int *expo, double *kconst)
+{
+    __shared__ double lds[NT / WAVE];
+    __shared__ unsigned long long mx[NT / WAVE];
+    __shared__ int e_s;
+    const int c = blockIdx.x;
+    const double *dc = delta + (int64_t)c * P.p + p0;
+    double kc = 0.0;
+    unsigned long long m = 0;
+    for (int j = threadIdx.x; j < nb; j += NT) {
+        double s = sdv[p0 + j];
+        double cj = s > 0.0 ? dc[j] / s : 0.0;
+        kc = fma(f2[p0 + j], cj, kc);
+        unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(cj));
+        m = bits > m ? bits : m;
+    }
+    kc = block_sum<NT>(kc, lds);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(m, off, WAVE);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) mx[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / WAVE; w++) m = mx[w] > m ? mx[w] : m;
+        double mv = __longlong_as_double((long long)m);
+        int E = 0;
+        if (mv > 0.0) frexp(mv, &E);
+        e_s = mv > 0.0 ? 8 * L - 2 - E : 0;
+        expo[c] = e_s;
+        kconst[c] = kc;
+    }
+    __syncthreads();
+    const int e = e_s;
+    for (int j = threadIdx.x; j < nsw * 16; j += NT) {
+        int32_t q = 0;
+        if (j < nb) {
+            double s = sdv[p0 + j];
+            double cj = s > 0.0 ? dc[j] / s : 0.0;
+            q = (int32_t)rint(ldexp(cj, e));
+        }
+        const int t = j & 15, slot = 4 * (t & 3) + (t >> 2);
+        const int64_t col = (int64_t)(j & ~15) + slot;
+        for (int k = 0; k < L; k++) {
+            int8_t d = (int8_t)(q & 0xFF);
+            Dq[(int64_t)(c * L + k) * ldd + col] = d;
+            q = (q - (int32_t)d) >> 8;
+        }
+    }
+}
+
+// r_c += X_s c - K_c for every chain, as the int8 MFMA product of the
+// individual-major genotypes with the limbs of c (exact int32 sums), then
+// the FP64 limb combination.  Workgroup = 64 individuals (4 waves x 16).
+template <int NTILE>
+__global__ __launch_bounds__(NT) void k_xupd(Par P, const uint32_t *__restrict__ gt, int64_t ldi, int nsw,
+                                             const int8_t *__restrict__ Dq, int64_t ldd, int L,
+                                             const int *__restrict__ expo, const double *__restrict__ kconst,
+                                             double *__restrict__ r)
+{
+    __shared__ int Ts[4][16][16 * NTILE + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int i0 = blockIdx.x * 64 + wave * 16;
+    v4i acc[NTILE];
+#pragma unroll
+    for (int t = 0; t < NTILE; t++) acc[t] = v4i{0, 0, 0, 0};
+    constexpr uint32_t M = 0x03030303u;
+    for (int s0 = 0; s0 < nsw; s0 += 4) {
+        const int sw = s0 + g;
+        const uint32_t wd = sw < nsw ? gt[(int64_t)sw * ldi + i0 + l16] : 0u;
+        const long a0 = (long)(((uint64_t)((wd >> 2) & M) << 32) | (wd & M));
+        const long a1 = (long)(((uint64_t)((wd >> 6) & M) << 32) | ((wd >> 4) & M));
+#pragma unroll
+        for (int t = 0; t < NTILE; t++) {
+            uint4 qb = make_uint4(0, 0, 0, 0);
+            if (sw < nsw) qb = *reinterpret_cast<const uint4 *>(Dq + (int64_t)(16 * t + l16) * ldd + (int64_t)sw * 16);
+            const long b0 = (long)(((uint64_t)qb.y << 32) | qb.x);
+            const long b1 = (long)(((uint64_t)qb.w << 32) | qb.z);
+            acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b0, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b1, acc[t], 0, 0, 0);
+        }
+    }
+    // acc[t][e]: individual i0 + 4g + e, column 16 t + l16
+#pragma unroll
+    for (int t = 0; t < NTILE; t++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) Ts[wave][4 * g + e][16 * t + l16] = acc[t][e];
+    __syncthreads();
+    const int C = P.nchains;
+    for (int pr = lane; pr < 16 * C; pr += 64) {
+        const int li = pr & 15, c = pr >> 4;
+        const int i = i0 + li;
+        if (i >= P.n) continue;
+        long long S = 0;
+        for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)Ts[wave][li][c * L + k];
+        double* rc = r + (int64_t)c * P.ldr;
+        rc[i] = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
     }
 }
 
@@ -261,9 +434,52 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     default: return SB_ERR_ARG;
     }
     const int nrg = (nb + h->rows_per_group - 1) / h->rows_per_group;
-    k_ua<<<dim3((nb + 63) / 64, nrg), NT, 0, st>>>(P, h->U, h->tri_off[b], nb, h->ra, h->rows_per_group,
-                                                   h->b_rapart);
+    k_utile<0><<<dim3((nb + 63) / 64, nrg), NT, 0, st>>>(P, h->U, h->tri_off[b], nb, h->ra, P.blocksize,
+                                                        h->rows_per_group, h->b_rapart);
     return hip_check(hipGetLastError(), "batch products");
+}
+
+int batch_update(sb_handle *h, int b, const double *delta)
+{
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    const int C = P.nchains, L = h->limbs;
+    const int nsw = (nb + 15) / 16;
+    k_dquant<<<C, NT, 0, st>>>(P, L, delta, h->f2, h->sdv, p0, nb, nsw, h->b_Dq, h->b_ldd, h->b_expo2,
+                               h->b_kc);
+    const int64_t ldi = P.ldr;
+    const uint32_t *gt = h->geno_t + h->gt_off[b];
+    const int nit = (int)((ldi + 63) / 64);
+    switch (h->b_ntile) {
+    case 1: k_xupd<1><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
+    case 2: k_xupd<2><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
+    case 3: k_xupd<3><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
+    case 4: k_xupd<4><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
+    case 6: k_xupd<6><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
+    case 8: k_xupd<8><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
+    case 12: k_xupd<12><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
+    default: return SB_ERR_ARG;
+    }
+    const int ncg = (nb + h->rows_per_group - 1) / h->rows_per_group;
+    k_utile<1><<<dim3((nb + 63) / 64, ncg), NT, 0, st>>>(P, h->U, h->tri_off[b], nb, delta + p0, P.p,
+                                                        h->rows_per_group, h->b_uapart);
+    k_ra_reduce<<<dim3((nb + NT - 1) / NT, C), NT, 0, st>>>(P, h->b_uapart, h->rows_per_group, ncg, nb, h->ra);
+    return hip_check(hipGetLastError(), "batch update");
+}
+
+int batch_transpose(sb_handle *h)
+{
+    const Par &P = h->par;
+    for (int b = 0; b < P.nblocks; b++) {
+        const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+        const int nsw = (nb + 15) / 16;
+        k_transpose<<<dim3((P.ldw + NT - 1) / NT, nsw), NT, 0, h->stream>>>(h->geno, P.ldw, p0, nb, P.ldr,
+                                                                          h->geno_t + h->gt_off[b]);
+    }
+    int rc = hip_check(hipGetLastError(), "transpose");
+    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "transpose");
+    return rc;
 }
 
 int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *chg, double *rhs_out)

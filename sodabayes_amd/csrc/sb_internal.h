@@ -105,6 +105,11 @@ struct sb_handle {
     double *hyp = nullptr;        // device hyp ring
     int hyp_cap = 0;
     sb::Scratch sc{};
+    int *u_lst = nullptr;         // changed positions of the current block, per chain (blocksize each)
+    double *u_dl = nullptr;       // their beta_old - beta_new
+    double *u_cl = nullptr;       // delta / s per listed position
+    double *u_kc = nullptr;       // centring constant sum 2 f c per chain
+    int *u_cnt = nullptr;         // list length per chain
     int32_t *resync_list = nullptr; // positions with beta != 0 in SNP-id order
     int32_t *d_count = nullptr;
 
@@ -122,6 +127,13 @@ struct sb_handle {
     int b_ldt = 16;
     int rows_per_group = 256;     // row split of the augmented product
     double *b_rapart = nullptr;   // (blocksize / rows_per_group) x blocksize x nchains
+    double *b_uapart = nullptr;   // same shape: partial U_s delta of the update
+    int8_t *b_Dq = nullptr;       // (16 * b_ntile) x b_ldd int8 limbs of delta / s
+    int64_t b_ldd = 0;
+    int *b_expo2 = nullptr;       // fixed-point exponent of delta / s per chain
+    double *b_kc = nullptr;       // centring constant per chain
+    uint32_t *geno_t = nullptr;   // individual-major genotypes per block (update GEMM)
+    std::vector<int64_t> gt_off;  // word offset of each block in geno_t
 
     std::vector<void *> allocs;
 };
@@ -145,5 +157,7 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp);
 int batch_block(sb_handle *h, int b, double zz, int rep);
 int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *chg,
                double *rhs_out = nullptr);
+int batch_update(sb_handle *h, int b, const double *delta);
+int batch_transpose(sb_handle *h);
 int block_rhs(sb_handle *h, int b, double *out_host);
 }  // namespace sb

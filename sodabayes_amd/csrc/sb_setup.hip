@@ -10,6 +10,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "sb_device.h"
@@ -198,9 +200,10 @@ __global__ __launch_bounds__(256) void k_pm_gemv(const double *__restrict__ G,
                                                  const int64_t *__restrict__ goff,
                                                  const int32_t *__restrict__ ls,
                                                  const int32_t *__restrict__ active,
-                                                 const double *__restrict__ v, double *__restrict__ y)
+                                                 const double *__restrict__ v, double *__restrict__ y,
+                                                 const int32_t *__restrict__ blist)
 {
-    int b = blockIdx.y;
+    int b = blist ? blist[blockIdx.y] : (int)blockIdx.y;
     if (!active[b]) return;
     int p0 = ls[b], nb = ls[b + 1] - p0;
     int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -216,12 +219,13 @@ __global__ __launch_bounds__(256) void k_pm_gemv(const double *__restrict__ G,
 // lambda = y.v, delta = sum((y - lambda v)**2)/|lambda|, stop at
 // delta <= tol (f32 1e-14) or maxit = 1000 n; else v = y/|y| (soda.f90:391-398)
 __global__ __launch_bounds__(256) void k_pm_post(const int32_t *ls, int32_t *active, int32_t *iters,
-                                                 double *lam_out, const double *y, double *v)
+                                                 double *lam_out, const double *y, double *v,
+                                                 const int32_t *blist)
 {
     __shared__ double lds[8];
     __shared__ double lam_s, nrm_s;
     __shared__ int stop_s;
-    int b = blockIdx.x;
+    int b = blist ? blist[blockIdx.x] : (int)blockIdx.x;
     if (!active[b]) return;
     int p0 = ls[b], nb = ls[b + 1] - p0;
     double s = 0.0;
@@ -253,12 +257,54 @@ __global__ __launch_bounds__(256) void k_pm_post(const int32_t *ls, int32_t *act
     for (int i = threadIdx.x; i < nb; i += 256) v[p0 + i] = y[p0 + i] / nrm_s;
 }
 
+// One Lanczos step per block (COUNTER mode bound, see setup_build_aug):
+// w = G q_k (in y) -> alpha_k = w.q_k, full re-orthogonalisation of w against
+// q_0..q_k (twice), beta_k = |w|, q_(k+1) = w / beta_k.  Q holds the Lanczos
+// vectors, vector j of every block at Q + j * p.
+__global__ __launch_bounds__(256) void k_lz_step(const int32_t *ls, int p, int k, double *y, double *Q,
+                                                 double *alpha, double *beta, int K1)
+{
+    __shared__ double lds[8];
+    __shared__ double coef;
+    const int b = blockIdx.x, p0 = ls[b], nb = ls[b + 1] - p0;
+    double *w = y + p0;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int j = 0; j <= k; j++) {
+            const double *qj = Q + (int64_t)j * p + p0;
+            double s = 0.0;
+            for (int i = threadIdx.x; i < nb; i += 256) s += w[i] * qj[i];
+            s = block_sum<256>(s, lds);
+            if (threadIdx.x == 0) {
+                coef = s;
+                if (pass == 0 && j == k) alpha[b * K1 + k] = s;
+            }
+            __syncthreads();
+            const double cj = coef;
+            for (int i = threadIdx.x; i < nb; i += 256) w[i] -= cj * qj[i];
+            __syncthreads();
+        }
+    }
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += w[i] * w[i];
+    s = block_sum<256>(s, lds);
+    if (threadIdx.x == 0) {
+        coef = sqrt(s);
+        beta[b * K1 + k] = coef;
+    }
+    __syncthreads();
+    const double bk = coef;
+    double *qn = Q + (int64_t)(k + 1) * p + p0;
+    for (int i = threadIdx.x; i < nb; i += 256) qn[i] = bk > 0.0 ? w[i] / bk : 0.0;
+}
+
 // ------------------------------------------------------------- cholesky --
 
 // M = dI - G in place (soda.f90:487-490: xxtemp = -xxtemp; diag = d + diag)
-__global__ void k_form_m(double *G, const int64_t *goff, const int32_t *ls, const double *xxd)
+__global__ void k_form_m(double *G, const int64_t *goff, const int32_t *ls, const double *xxd,
+                         const int32_t *skip)
 {
     int b = blockIdx.y, nb = ls[b + 1] - ls[b];
+    if (skip && skip[b] != 0) return;
     double d = xxd[b];
     double *A = G + goff[b];
     int64_t n2 = (int64_t)nb * nb;
@@ -420,6 +466,13 @@ __global__ __launch_bounds__(256) void k_chol_update(double *G, const int64_t *g
         }
 }
 
+// dfail = -1 marked blocks that were skipped (already factorised): reset to 0
+__global__ void k_clear_skip(int32_t *fail, int nb)
+{
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        if (fail[b] < 0) fail[b] = 0;
+}
+
 // Dense upper factor (ld n_b) -> packed column-major triangle.
 __global__ void k_pack_tri(const double *G, const int64_t *goff, const int32_t *ls,
                            const int64_t *tri_off, double *U)
@@ -511,6 +564,35 @@ int launch_gram(sb_handle *h, GramBuf &gb, const double *freqp, int only_block)
 }
 }  // namespace
 
+// Largest eigenvalue of the symmetric tridiagonal matrix (a[0..m), b[0..m-1))
+// by Sturm-count bisection.
+static double tridiag_max(const double *a, const double *b, int m)
+{
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < m; i++) {
+        double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    auto count_below = [&](double x) {
+        int c = 0;
+        double d = 1.0;
+        for (int i = 0; i < m; i++) {
+            double bb = i > 0 ? b[i - 1] * b[i - 1] : 0.0;
+            d = a[i] - x - (i > 0 ? bb / d : 0.0);
+            if (d == 0.0) d = -1e-300;
+            if (d < 0.0) c++;
+        }
+        return c;
+    };
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(1.0, std::fabs(hi)); it++) {
+        double mid = 0.5 * (lo + hi);
+        if (count_below(mid) >= m) hi = mid;  // every eigenvalue below mid
+        else lo = mid;
+    }
+    return hi;
+}
+
 // freq per position is kept in the handle's zlut slot 3 (see sb_api.hip).
 static const double *freq_pos(sb_handle *h) { return h->f2 + h->par.p; }
 
@@ -544,6 +626,16 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
         gram_free(gb);
         return rc;
     }
+    const char *vb = std::getenv("SODABAYES_VERBOSE");
+    const bool verbose = vb && vb[0] == '1';
+    auto progress = [&](const char *what, long it, int active) {
+        if (!verbose) return;
+        hipStreamSynchronize(h->stream);
+        double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "[sodabayes] setup %.1f s: %s (iteration %ld, %d blocks active)\n", el, what, it,
+                     active);
+    };
+    progress("gram done", 0, nb);
     // power method
     double *y = nullptr, *v = nullptr;
     int32_t *active = nullptr, *iters = nullptr, *dfail = nullptr;
@@ -563,34 +655,121 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     k_pm_normalize<<<nb, 256, 0, h->stream>>>(h->d_ls, y, v);
     dim3 ggrid((P.blocksize + 3) / 4, nb);
     std::vector<int32_t> hact(nb);
-    const int64_t maxit = 1000LL * P.blocksize;
-    for (int64_t it = 0; it < maxit; it += 16) {
-        for (int q = 0; q < 16; q++) {
-            k_pm_gemv<<<ggrid, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, active, v, y);
-            k_pm_post<<<nb, 256, 0, h->stream>>>(h->d_ls, active, iters, h->xxd, y, v);
+    // REPLAY reproduces the reference's power iteration (its d to 1e-12).
+    // COUNTER needs only a valid bound d = lambda_max + 0.001: the largest
+    // Ritz value of K Lanczos steps (full re-orthogonalisation, exact for
+    // K = n_b); the Cholesky below still checks that dI - G is positive
+    // definite.
+    const bool lanczos = P.rng_mode == SB_RNG_COUNTER;
+    const int64_t maxit = lanczos ? 0 : 1000LL * P.blocksize;
+    if (lanczos) {
+        const int K = std::min(P.blocksize, 96), K1 = K + 1;
+        double *Qv = nullptr, *al = nullptr, *be = nullptr;
+        hipMalloc(&Qv, sizeof(double) * (size_t)P.p * (K + 1));
+        hipMalloc(&al, sizeof(double) * nb * K1);
+        hipMalloc(&be, sizeof(double) * nb * K1);
+        hipMemcpyAsync(Qv, v, sizeof(double) * P.p, hipMemcpyDeviceToDevice, h->stream);
+        hipMemsetAsync(active, 0xFF, sizeof(int32_t) * nb, h->stream);
+        for (int k = 0; k < K; k++) {
+            k_pm_gemv<<<ggrid, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, active, Qv + (int64_t)k * P.p, y,
+                                                    nullptr);
+            k_lz_step<<<nb, 256, 0, h->stream>>>(h->d_ls, P.p, k, y, Qv, al, be, K1);
         }
-        if (hip_check(hipMemcpyAsync(hact.data(), active, sizeof(int32_t) * nb, hipMemcpyDeviceToHost,
-                                     h->stream),
-                      "power method"))
-            break;
+        std::vector<double> ha((size_t)nb * K1), hb((size_t)nb * K1), hx(nb);
+        hipMemcpyAsync(ha.data(), al, sizeof(double) * nb * K1, hipMemcpyDeviceToHost, h->stream);
+        hipMemcpyAsync(hb.data(), be, sizeof(double) * nb * K1, hipMemcpyDeviceToHost, h->stream);
+        rc = hip_check(hipStreamSynchronize(h->stream), "lanczos");
+        for (int b = 0; b < nb && !rc; b++) {
+            int nbb = h->ls[b + 1] - h->ls[b];
+            int m = std::min(K, nbb);
+            // stop at an invariant subspace (beta ~ 0)
+            for (int k = 0; k + 1 < m; k++)
+                if (hb[(size_t)b * K1 + k] <= 1e-13 * std::fabs(ha[(size_t)b * K1])) {
+                    m = k + 1;
+                    break;
+                }
+            hx[b] = std::fabs(tridiag_max(&ha[(size_t)b * K1], &hb[(size_t)b * K1], m)) + D_EPS_ADD;
+        }
+        hipMemcpyAsync(h->xxd, hx.data(), sizeof(double) * nb, hipMemcpyHostToDevice, h->stream);
         hipStreamSynchronize(h->stream);
-        bool any = false;
-        for (int b = 0; b < nb; b++) any |= hact[b] != 0;
-        if (!any) break;
+        hipFree(Qv);
+        hipFree(al);
+        hipFree(be);
+        progress("lanczos bound done", K, 0);
     }
-    // Cholesky of dI - G
-    dim3 fgrid(64, nb);
-    k_form_m<<<fgrid, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->xxd);
-    int T = (P.blocksize + TILE - 1) / TILE;
-    for (int k = 0; k < T; k++) {
-        k_chol_diag<<<nb, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
-        if (T - k - 1 > 0) {
-            k_chol_panel<<<dim3(T - k - 1, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
-            int R = T - k - 1;
-            k_chol_update<<<dim3(R * (R + 1) / 2, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
+    // launches cover the still-active blocks only (blist), refreshed at
+    // every host check
+    int32_t *blist = nullptr;
+    hipMalloc(&blist, sizeof(int32_t) * nb);
+    auto power_loop = [&](int64_t limit) {
+        hipMemcpyAsync(hact.data(), active, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
+        hipStreamSynchronize(h->stream);
+        std::vector<int32_t> hl;
+        for (int b = 0; b < nb; b++)
+            if (hact[b]) hl.push_back(b);
+        for (int64_t it = 0; it < limit && !hl.empty(); it += 16) {
+            hipMemcpyAsync(blist, hl.data(), sizeof(int32_t) * hl.size(), hipMemcpyHostToDevice, h->stream);
+            const int na = (int)hl.size();
+            for (int q = 0; q < 16; q++) {
+                k_pm_gemv<<<dim3(ggrid.x, na), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, active, v, y, blist);
+                k_pm_post<<<na, 256, 0, h->stream>>>(h->d_ls, active, iters, h->xxd, y, v, blist);
+            }
+            if (hip_check(hipMemcpyAsync(hact.data(), active, sizeof(int32_t) * nb, hipMemcpyDeviceToHost,
+                                         h->stream),
+                          "power method"))
+                break;
+            hipStreamSynchronize(h->stream);
+            hl.clear();
+            for (int b = 0; b < nb; b++)
+                if (hact[b]) hl.push_back(b);
+            if (it % 2048 == 0) progress("power method", it + 16, (int)hl.size());
+        }
+    };
+    power_loop(maxit);
+    // Cholesky of dI - G (blocks with dfail != 0 are skipped)
+    const int T = (P.blocksize + TILE - 1) / TILE;
+    auto factor = [&]() {
+        k_form_m<<<dim3(64, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->xxd, dfail);
+        for (int k = 0; k < T; k++) {
+            k_chol_diag<<<nb, 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
+            if (T - k - 1 > 0) {
+                k_chol_panel<<<dim3(T - k - 1, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k, dfail);
+                int R = T - k - 1;
+                k_chol_update<<<dim3(R * (R + 1) / 2, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, k,
+                                                                               dfail);
+            }
+        }
+    };
+    factor();
+    if (lanczos) {
+        // a block whose Lanczos bound left dI - G indefinite gets the
+        // reference's power iteration: rebuild its Gram, iterate, refactor
+        std::vector<int32_t> hf(nb), mask(nb);
+        hipMemcpyAsync(hf.data(), dfail, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
+        hipStreamSynchronize(h->stream);
+        int nretry = 0;
+        for (int b = 0; b < nb; b++) {
+            mask[b] = hf[b] != 0;
+            nretry += mask[b];
+        }
+        if (nretry) {
+            progress("lanczos bound too low, power method for the failed blocks", 0, nretry);
+            for (int b = 0; b < nb; b++)
+                if (mask[b]) launch_gram(h, gb, freq_pos(h), b);
+            hipMemcpyAsync(active, mask.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, h->stream);
+            k_start_counter<<<(P.p + 255) / 256, 256, 0, h->stream>>>(P.seed, P.p, y);
+            k_pm_normalize<<<nb, 256, 0, h->stream>>>(h->d_ls, y, v);
+            power_loop(1000LL * P.blocksize);
+            std::vector<int32_t> skip(nb);
+            for (int b = 0; b < nb; b++) skip[b] = mask[b] ? 0 : -1;
+            hipMemcpyAsync(dfail, skip.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, h->stream);
+            factor();
+            // blocks factorised in the first pass are fine
+            k_clear_skip<<<1, 256, 0, h->stream>>>(dfail, nb);
         }
     }
     k_pack_tri<<<dim3(P.blocksize, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->d_tri_off, h->U);
+    progress("cholesky done", 0, 0);
     rc = hip_check(hipGetLastError(), "cholesky");
     std::vector<int32_t> hfail(nb), hit(nb);
     if (!rc) {
@@ -599,6 +778,7 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
         if (xxd_out) hipMemcpyAsync(xxd_out, h->xxd, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream);
         rc = hip_check(hipStreamSynchronize(h->stream), "build_aug");
     }
+    hipFree(blist);
     hipFree(y);
     hipFree(v);
     hipFree(active);

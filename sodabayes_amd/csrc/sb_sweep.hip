@@ -268,87 +268,146 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
 
 // ------------------------------------------------------- block update --
 
-// Ordered list of the block's changed positions in LDS (models.f90:416-422).
-__device__ int build_list(const uint8_t *chg, const double *delta, int p0, int nb, int *lst, double *dl,
-                          int *cnt)
+// Ordered list of the block's changed positions of chain blockIdx.z
+// (models.f90:416-422: the SNPs with gflag /= 1, in block order), written
+// once per block for the update kernel, with delta_j, c_j = delta_j / s_j
+// and the centring constant sum_j 2 f_j c_j.
+__global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const double *delta,
+                                             const double *f2, const double *sdv, int p0, int nb, int *lst,
+                                             double *dl, double *cl, int *count, double *kconst)
 {
+    __shared__ int cnt[NT + 1];
+    __shared__ int wsum[NT / WAVE];
+    __shared__ double lds[NT / WAVE];
+    const int64_t c = blockIdx.z;
+    chg += c * P.p;
+    delta += c * P.p;
+    lst += c * P.blocksize;
+    dl += c * P.blocksize;
+    cl += c * P.blocksize;
     const int R = (nb + NT - 1) / NT;
-    int a = threadIdx.x * R, e = min(nb, a + R), c = 0;
-    for (int j = a; j < e; j++) c += chg[p0 + j] ? 1 : 0;
-    cnt[threadIdx.x] = c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int i = 0; i < NT; i++) {
-            int v = cnt[i];
-            cnt[i] = s;
-            s += v;
-        }
-        cnt[NT] = s;
+    int a = threadIdx.x * R, e = min(nb, a + R), k = 0;
+    for (int j = a; j < e; j++) k += chg[p0 + j] ? 1 : 0;
+    // exclusive scan of the per-thread counts: wave scan, then wave offsets
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    int incl = k;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        int t = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += t;
     }
+    if (lane == WAVE - 1) wsum[wv] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wv; w++) base += wsum[w];
+    cnt[threadIdx.x] = base + incl - k;
+    if (threadIdx.x == NT - 1) count[c] = base + incl;
     __syncthreads();
     int o = cnt[threadIdx.x];
+    double kc = 0.0;
     for (int j = a; j < e; j++)
         if (chg[p0 + j]) {
+            const int pos = p0 + j;
+            const double d = delta[pos], sd = sdv[pos];
+            const double cj = sd > 0.0 ? d / sd : 0.0;
             lst[o] = j;
-            dl[o] = delta[p0 + j];
+            dl[o] = d;
+            cl[o] = cj;
+            kc = fma(f2[pos], cj, kc);
             o++;
         }
-    __syncthreads();
-    return cnt[NT];
+    kc = block_sum<NT>(kc, lds);
+    if (threadIdx.x == 0) kconst[c] = kc;
 }
 
-// r += sum_j z_j delta_j (words [0, nrw)) and r_a += sum_j A_j delta_j
-// (rows after that), models.f90:416-430.  Each thread accumulates its own
-// individuals over the ordered list, then adds once (the reduction's
-// private-copy form).
-__global__ __launch_bounds__(NT) void k_update(Par P, const uint32_t *__restrict__ geno,
-                                               const double *__restrict__ zlut,
-                                               const double *__restrict__ U, int64_t tri0, int p0,
-                                               int nb, int nrw_blocks, const uint8_t *chg,
-                                               const double *delta, double *r, double *ra)
+// r += sum_j z_j delta_j and r_a += sum_j A_j delta_j over the changed list
+// (models.f90:416-430), z = (x - 2f)/s.  The genotype part is evaluated in
+// centred form: sum_j x_ij c_j - sum_j 2 f_j c_j with c_j = delta_j / s_j.
+// Workgroups [0, nrw): 256 individuals each; lane l of every wave owns
+// individuals 4l..4l+3; the UW waves take interleaved runs of the list
+// (UNR loads in flight per wave) and their partial sums are added in wave
+// order through LDS.  Workgroups after that: 64 rows of r_a each.
+constexpr int UW = 16;   // waves per update workgroup
+constexpr int UNR = 8;   // list entries in flight per wave
+__global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__restrict__ geno,
+                                                      const double *__restrict__ U, int64_t tri0, int p0,
+                                                      int nb, int nrw, const int *lst, const double *dl,
+                                                      const double *cl, const int *count,
+                                                      const double *kconst, double *r, double *ra)
 {
-    __shared__ int lst[4096];
-    __shared__ double dl[4096];
-    __shared__ int cnt[NT + 1];
-    const int c = blockIdx.z;
-    chg += (int64_t)c * P.p;
-    delta += (int64_t)c * P.p;
-    r += (int64_t)c * P.ldr;
-    ra += (int64_t)c * P.blocksize;
-    const int m = build_list(chg, delta, p0, nb, lst, dl, cnt);
+    __shared__ double red[UW][WAVE][4];
+    const int64_t c = blockIdx.z;
+    const int m = count[c];
     if (m == 0) return;
-    if ((int)blockIdx.x < nrw_blocks) {
-        const int w = blockIdx.x * NT + threadIdx.x;
-        if (w >= P.ldw || w * 16 >= P.n) return;
-        double acc[16];
+    lst += c * P.blocksize;
+    dl += c * P.blocksize;
+    cl += c * P.blocksize;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if ((int)blockIdx.x < nrw) {
+        r += c * P.ldr;
+        const int i0 = blockIdx.x * 256 + 4 * lane;       // first of 4 individuals
+        const int64_t byte = i0 >> 2;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q0 = wave * UNR; q0 < m; q0 += UW * UNR) {
+            uint32_t gb[UNR];
+            double cj[UNR];
 #pragma unroll
-        for (int e = 0; e < 16; e++) acc[e] = 0.0;
-        for (int q = 0; q < m; q++) {
-            int pos = p0 + lst[q];
-            uint32_t gw = geno[(int64_t)pos * P.ldw + w];
-            const double *zl = zlut + 4 * (int64_t)pos;
-            double z0 = zl[0], z1 = zl[1], z2 = zl[2], d = dl[q];
+            for (int u = 0; u < UNR; u++) {
+                const int q = q0 + u;
+                gb[u] = 0;
+                cj[u] = 0.0;
+                if (q < m) {
+                    const int pos = p0 + lst[q];
+                    cj[u] = cl[q];
+                    gb[u] = reinterpret_cast<const uint8_t *>(geno + (int64_t)pos * P.ldw)[byte];
+                }
+            }
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                uint32_t x = (gw >> (2 * e)) & 3u;
-                double z = x == 0 ? z0 : (x == 1 ? z1 : z2);
-                acc[e] = __dadd_rn(acc[e], __dmul_rn(z, d));
+            for (int u = 0; u < UNR; u++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    acc[e] = fma((double)((gb[u] >> (2 * e)) & 3u), cj[u], acc[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) red[wave][lane][e] = acc[e];
+        __syncthreads();
+        if (wave == 0 && i0 < P.n) {
+            const double kc = kconst[c];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                double s = red[0][lane][e];
+                for (int w = 1; w < UW; w++) s += red[w][lane][e];
+                if (i0 + e < P.n) r[i0 + e] = r[i0 + e] + (s - kc);
             }
         }
-        int i0 = w * 16;
-#pragma unroll
-        for (int e = 0; e < 16; e++)
-            if (i0 + e < P.n) r[i0 + e] = __dadd_rn(r[i0 + e], acc[e]);
     } else {
-        const int i = (blockIdx.x - nrw_blocks) * NT + threadIdx.x;
-        if (i >= nb) return;
+        ra += c * P.blocksize;
+        const int i = (blockIdx.x - nrw) * WAVE + lane;
         double acc = 0.0;
-        for (int q = 0; q < m; q++) {
-            int jk = lst[q];
-            if (i <= jk) acc = __dadd_rn(acc, __dmul_rn(U[tri0 + (int64_t)jk * (jk + 1) / 2 + i], dl[q]));
+        if (i < nb)
+            for (int q0 = wave * UNR; q0 < m; q0 += UW * UNR) {
+                double uv[UNR], dv[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++) {
+                    const int q = q0 + u;
+                    uv[u] = 0.0;
+                    dv[u] = 0.0;
+                    if (q < m) {
+                        const int jk = lst[q];
+                        dv[u] = dl[q];
+                        if (i <= jk) uv[u] = U[tri0 + (int64_t)jk * (jk + 1) / 2 + i];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; u++) acc = fma(uv[u], dv[u], acc);
+            }
+        red[wave][lane][0] = acc;
+        __syncthreads();
+        if (wave == 0 && i < nb) {
+            double s = red[0][lane][0];
+            for (int w = 1; w < UW; w++) s += red[w][lane][0];
+            ra[i] = ra[i] + s;
         }
-        ra[i] = __dadd_rn(ra[i], acc);
     }
 }
 
@@ -765,7 +824,7 @@ int block_rhs(sb_handle *h, int b, double *out_host)
         rc = batch_block(h, b, 0.0, 0);
         if (!rc) rc = batch_draw(h, b, 0.0, 0, h->sc.delta, reinterpret_cast<uint8_t *>(h->sc.flag), dout);
     } else {
-        int spw = (int)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 8192));
+        int spw = (int)std::max<int64_t>(2, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 2048));
         int ngs = (nb + 4 * spw - 1) / (4 * spw);
         int nxg = ngs * P.nchunk;
         k_rhs<<<nxg + (nb + 3) / 4, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, h->tri_off[b], p0, nb, spw, ngs,
@@ -824,8 +883,8 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
         for (int b = 0; b < P.nblocks; b++) {
             const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
             const int64_t tri0 = h->tri_off[b];
-            // X part: aim for >= 2048 waves
-            int spw = (int)std::max<int64_t>(1, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 8192));
+            // X part: ~2048 waves, each reusing its residual registers over spw SNPs
+            int spw = (int)std::max<int64_t>(2, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 2048));
             int ngs = (nb + 4 * spw - 1) / (4 * spw);
             int nxg = ngs * P.nchunk;
             int nag = (nb + 3) / 4;
@@ -854,11 +913,19 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                     P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
                     h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg);
             mark();
-            int nrw = (P.ldw + NT - 1) / NT;
-            int nra = (nb + NT - 1) / NT;
             mark();
-            k_update<<<dim3(nrw + nra, 1, C), NT, 0, st>>>(P, h->geno, h->zlut, h->U, tri0, p0, nb, nrw, chg,
-                                                           delta, h->r, h->ra);
+            if (mixed) {
+                int rc = batch_update(h, b, delta);
+                if (rc) return rc;
+            } else {
+                const int nrw = (P.n + 255) / 256;
+                const int nra = (nb + WAVE - 1) / WAVE;
+                k_list<<<dim3(1, 1, C), NT, 0, st>>>(P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
+                                                     h->u_cl, h->u_cnt, h->u_kc);
+                k_update<<<dim3(nrw + nra, 1, C), UW * WAVE, 0, st>>>(P, h->geno, h->U, tri0, p0, nb, nrw,
+                                                                      h->u_lst, h->u_dl, h->u_cl, h->u_cnt,
+                                                                      h->u_kc, h->r, h->ra);
+            }
             mark();
         }
         k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);

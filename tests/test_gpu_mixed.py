@@ -94,3 +94,43 @@ def test_mixed_chains_match_reference_posterior():
         assert z < 4.0, key
     g_gpu = np.mean([r["gstore"] for r in res], axis=0)
     assert np.corrcoef(g_gpu, ref["gstore"])[0, 1] > 0.9
+
+
+@pytest.mark.parametrize("case", ["bayesr", "nblocks", "ldblock", "ragged"])
+def test_counter_mode_bound_matches_reference_d(case):
+    """COUNTER mode takes d from Lanczos instead of the power iteration: on
+    blocks of <= 96 SNPs the Ritz value is lambda_max, so d matches the
+    reference's power-method d."""
+    from common import load_case, rel_err, sampler_options
+    c = load_case(case)
+    o = sampler_options(c["opts"], rng="counter")
+    s = soda.SodaSampler(o, int(c["nind"]), int(c["nloci"]), c["why"], bed=c["bed"])
+    xxd = s.setup()
+    assert rel_err(xxd, c["out"]["xxd"]) < 1e-10
+    s.close()
+
+
+@pytest.mark.parametrize("precision,chains,bar", [("fp64", 1, 1e-11), ("mixed", 8, 1e-5)])
+def test_residuals_stay_consistent(precision, chains, bar):
+    """After the sweeps' incremental updates r_c = y - X beta_c - mu_c still
+    holds for every chain (before the every-1000-iterations resync)."""
+    bed, why = dataset()
+    n, p = 1001, 300
+    o = soda.SodaOptions(method="BayesR", nmix=4, mix=R4, blocksize=100, numit=50, burnin=0, seed=8,
+                         rng="counter", precision=precision, chains=chains)
+    s = soda.SodaSampler(o, n, p, why, bed=bed)
+    s.setup()
+    s.init()
+    s.run(20, trace=False)
+    f = orc.freq(bed, n)
+    Z = orc.standardize(bed, n, f)
+    worst = 0.0
+    for c in range(chains):
+        g = s.get("G", p, c)
+        mu = s.get("SCALARS", 4, c)[0]
+        r = s.get("YADJ", n, c)
+        want = why - Z @ g - mu
+        worst = max(worst, np.linalg.norm(r - want) / np.linalg.norm(want))
+    print(f"{precision} x{chains}: residual identity error {worst:.3e}")
+    assert worst < bar
+    s.close()
