@@ -122,20 +122,27 @@ __global__ __launch_bounds__(NT) void k_xq(Par P, const uint32_t *__restrict__ g
     for (int t = 0; t < NTILE; t++) acc[t] = v4i{0, 0, 0, 0};
     constexpr uint32_t M = 0x03030303u;
     // super-step: 16 words (256 individuals); lane group g owns words 4g..4g+3
+    // (all loads of a super-step are issued before its MFMAs)
     for (int ws = w0; ws < w1; ws += 16) {
         uint4 gw = valid ? *reinterpret_cast<const uint4 *>(grow + ws + 4 * g) : make_uint4(0, 0, 0, 0);
+        uint4 qb[4][NTILE];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int64_t qoff = (int64_t)(ws + 4 * g + s) * 16;
+#pragma unroll
+            for (int t = 0; t < NTILE; t++)
+                qb[s][t] = *reinterpret_cast<const uint4 *>(Q + (int64_t)(16 * t + l16) * ldq + qoff);
+        }
         uint32_t words[4] = {gw.x, gw.y, gw.z, gw.w};
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             const uint32_t wd = words[s];
             const long a0 = (long)(((uint64_t)((wd >> 2) & M) << 32) | (wd & M));
             const long a1 = (long)(((uint64_t)((wd >> 6) & M) << 32) | ((wd >> 4) & M));
-            const int64_t qoff = (int64_t)(ws + 4 * g + s) * 16;
 #pragma unroll
             for (int t = 0; t < NTILE; t++) {
-                const uint4 qb = *reinterpret_cast<const uint4 *>(Q + (int64_t)(16 * t + l16) * ldq + qoff);
-                const long b0 = (long)(((uint64_t)qb.y << 32) | qb.x);
-                const long b1 = (long)(((uint64_t)qb.w << 32) | qb.z);
+                const long b0 = (long)(((uint64_t)qb[s][t].y << 32) | qb[s][t].x);
+                const long b1 = (long)(((uint64_t)qb[s][t].w << 32) | qb[s][t].z);
                 acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b0, acc[t], 0, 0, 0);
                 acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b1, acc[t], 0, 0, 0);
             }
@@ -164,58 +171,65 @@ __global__ __launch_bounds__(NT) void k_utile(Par P, const double *__restrict__ 
                                               const double *__restrict__ R, int64_t rstride, int span,
                                               double *__restrict__ part)
 {
-    __shared__ double Ut[64][65];
-    __shared__ double Rt[64][65];
     const int C = P.nchains;
-    const int o0 = blockIdx.x * 64;          // first output index (column jk or row i)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, lg = lane >> 4;
+    const int o0 = blockIdx.x * 64 + wave * 16;   // this wave's 16 outputs
     const int g = blockIdx.y;
     int k0 = g * span, k1 = min(nb, k0 + span);
     if (MODE == 0) {
-        if (k0 > o0 + 63) return;            // rows entirely below the diagonal
-        k1 = min(k1, o0 + 64);
+        if (k0 > blockIdx.x * 64 + 63) return;    // rows entirely below the diagonal
+        k1 = min(k1, o0 + 16);
     } else {
-        if (k1 <= o0) return;                // columns entirely left of the diagonal
+        if (k1 <= blockIdx.x * 64) return;        // columns entirely left of the diagonal
         k0 = max(k0, o0);
     }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int l16 = lane & 15, lg = lane >> 4;
+    if (o0 >= nb) return;
     const int ntc = (C + 15) / 16;
     v4d acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int kt = k0; kt < k1; kt += 64) {
-        // U tile: rows r (mode 0: k index; mode 1: output index), cols likewise
-        for (int e = threadIdx.x; e < 64 * 64; e += NT) {
-            int rr = e & 63, cc = e >> 6;
-            int i = MODE == 0 ? kt + rr : o0 + rr;
-            int j = MODE == 0 ? o0 + cc : kt + cc;
-            bool ok = j < nb && i <= j && (MODE == 0 ? i < k1 : (i < nb && j < k1));
-            Ut[rr][cc] = ok ? U[tri0 + (int64_t)j * (j + 1) / 2 + i] : 0.0;
-        }
-        for (int e = threadIdx.x; e < 64 * 64; e += NT) {
-            int kk = e & 63, ch = e >> 6;
-            int k = kt + kk;
-            Rt[kk][ch] = (ch < C && k < k1) ? R[(int64_t)ch * rstride + k] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int ks = 0; ks < 64; ks += 4) {
-            // A[m][k]: m = output within the wave's 16, k = ks + lg
-            const int m = wave * 16 + l16, k = ks + lg;
-            const double a = MODE == 0 ? Ut[k][m] : Ut[m][k];
-            for (int t = 0; t < ntc; t++) {
-                const double bv = Rt[k][t * 16 + l16];
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc[t], 0, 0, 0);
+    // k-step of 16: lane group lg owns k indices kt + 4 lg + s, s = 0..3
+    for (int kt = k0 - (k0 & 3); kt < k1; kt += 16) {
+        double a[4];
+        const int kb = kt + 4 * lg;
+        if (MODE == 0) {
+            // A[m][k] = U(row kb+s, col o0+m): 4 consecutive rows of column o0+l16
+            const int j = o0 + l16;
+            const double *col = U + tri0 + (int64_t)j * (j + 1) / 2;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int i = kb + s;
+                a[s] = (j < nb && i >= k0 && i < k1 && i <= j) ? col[i] : 0.0;
+            }
+        } else {
+            // A[m][k] = U(row o0+m, col kb+s): row o0+l16 of 4 consecutive columns
+            const int i = o0 + l16;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int j = kb + s;
+                a[s] = (i < nb && j >= k0 && j < k1 && i <= j) ? U[tri0 + (int64_t)j * (j + 1) / 2 + i] : 0.0;
             }
         }
-        __syncthreads();
+        for (int t = 0; t < ntc; t++) {
+            const int ch = t * 16 + l16;
+            double bv[4];
+            const double *rc = R + (int64_t)(ch < C ? ch : 0) * rstride;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int k = kb + s;
+                bv[s] = (ch < C && k >= k0 && k < k1) ? rc[k] : 0.0;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bv[s], acc[t], 0, 0, 0);
+        }
     }
     // D[m][n] of the f64 16x16x4 MFMA: m = lg + 4*v (rows interleaved across
     // the 4 lane groups), n = l16 (chain within tile t)
     for (int t = 0; t < ntc; t++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            int o = o0 + wave * 16 + lg + 4 * v;
+            int o = o0 + lg + 4 * v;
             int ch = t * 16 + l16;
             if (o < nb && ch < C) part[((int64_t)g * P.blocksize + o) * C + ch] = acc[t][v];
         }
@@ -337,19 +351,30 @@ __global__ __launch_bounds__(NT) void k_xupd(Par P, const uint32_t *__restrict__
 #pragma unroll
     for (int t = 0; t < NTILE; t++) acc[t] = v4i{0, 0, 0, 0};
     constexpr uint32_t M = 0x03030303u;
-    for (int s0 = 0; s0 < nsw; s0 += 4) {
-        const int sw = s0 + g;
-        const uint32_t wd = sw < nsw ? gt[(int64_t)sw * ldi + i0 + l16] : 0u;
-        const long a0 = (long)(((uint64_t)((wd >> 2) & M) << 32) | (wd & M));
-        const long a1 = (long)(((uint64_t)((wd >> 6) & M) << 32) | ((wd >> 4) & M));
+    constexpr int PF = 4;  // super-steps fetched ahead of their MFMAs
+    for (int s0 = 0; s0 < nsw; s0 += 4 * PF) {
+        uint32_t wd[PF];
+        uint4 qb[PF][NTILE];
 #pragma unroll
-        for (int t = 0; t < NTILE; t++) {
-            uint4 qb = make_uint4(0, 0, 0, 0);
-            if (sw < nsw) qb = *reinterpret_cast<const uint4 *>(Dq + (int64_t)(16 * t + l16) * ldd + (int64_t)sw * 16);
-            const long b0 = (long)(((uint64_t)qb.y << 32) | qb.x);
-            const long b1 = (long)(((uint64_t)qb.w << 32) | qb.z);
-            acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b0, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b1, acc[t], 0, 0, 0);
+        for (int f = 0; f < PF; f++) {
+            const int sw = s0 + 4 * f + g;
+            wd[f] = sw < nsw ? gt[(int64_t)sw * ldi + i0 + l16] : 0u;
+#pragma unroll
+            for (int t = 0; t < NTILE; t++)
+                qb[f][t] = sw < nsw ? *reinterpret_cast<const uint4 *>(Dq + (int64_t)(16 * t + l16) * ldd + (int64_t)sw * 16)
+                                    : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int f = 0; f < PF; f++) {
+            const long a0 = (long)(((uint64_t)((wd[f] >> 2) & M) << 32) | (wd[f] & M));
+            const long a1 = (long)(((uint64_t)((wd[f] >> 6) & M) << 32) | ((wd[f] >> 4) & M));
+#pragma unroll
+            for (int t = 0; t < NTILE; t++) {
+                const long b0 = (long)(((uint64_t)qb[f][t].y << 32) | qb[f][t].x);
+                const long b1 = (long)(((uint64_t)qb[f][t].w << 32) | qb[f][t].z);
+                acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b0, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b1, acc[t], 0, 0, 0);
+            }
         }
     }
     // acc[t][e]: individual i0 + 4g + e, column 16 t + l16

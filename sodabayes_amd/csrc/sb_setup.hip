@@ -195,7 +195,8 @@ __global__ __launch_bounds__(256) void k_pm_normalize(const int32_t *ls, const d
 }
 
 // y = A v for active blocks; one wave per output row (A symmetric, so row
-// i of A is its contiguous column i).
+// i of A is its contiguous column i), four independent partial sums per
+// lane so that four 512-byte loads per wave are in flight.
 __global__ __launch_bounds__(256) void k_pm_gemv(const double *__restrict__ G,
                                                  const int64_t *__restrict__ goff,
                                                  const int32_t *__restrict__ ls,
@@ -210,9 +211,16 @@ __global__ __launch_bounds__(256) void k_pm_gemv(const double *__restrict__ G,
     if (i >= nb) return;
     const double *col = G + goff[b] + (int64_t)i * nb;
     const double *vb = v + p0;
-    double acc = 0.0;
-    for (int j = lane; j < nb; j += 64) acc = fma(col[j], vb[j], acc);
-    acc = wave_sum(acc);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int j = lane;
+    for (; j + 192 < nb; j += 256) {
+        a0 = fma(col[j], vb[j], a0);
+        a1 = fma(col[j + 64], vb[j + 64], a1);
+        a2 = fma(col[j + 128], vb[j + 128], a2);
+        a3 = fma(col[j + 192], vb[j + 192], a3);
+    }
+    for (; j < nb; j += 64) a0 = fma(col[j], vb[j], a0);
+    double acc = wave_sum((a0 + a1) + (a2 + a3));
     if (lane == 0) y[p0 + i] = acc;
 }
 
@@ -566,6 +574,8 @@ int launch_gram(sb_handle *h, GramBuf &gb, const double *freqp, int only_block)
 
 // Largest eigenvalue of the symmetric tridiagonal matrix (a[0..m), b[0..m-1))
 // by Sturm-count bisection.
+constexpr int LANCZOS_MAX = 192;  // Lanczos steps of the COUNTER-mode bound
+
 static double tridiag_max(const double *a, const double *b, int m)
 {
     double lo = a[0], hi = a[0];
@@ -591,6 +601,59 @@ static double tridiag_max(const double *a, const double *b, int m)
         else lo = mid;
     }
     return hi;
+}
+
+// Last component of the unit eigenvector of the symmetric tridiagonal T_m
+// (diagonal a, off-diagonal b) for its eigenvalue theta, by two steps of
+// inverse iteration.  (T - sigma I) is factored by Gaussian elimination with
+// partial pivoting in the tridiagonal form of LAPACK's dgttrf (fill-in on a
+// second superdiagonal).
+static double tridiag_last_component(const double *a, const double *b, int m, double theta)
+{
+    if (m == 1) return 1.0;
+    const double scale = std::max(std::fabs(theta), 1e-300);
+    const double sigma = theta + 1e-10 * scale;
+    std::vector<double> dl(m - 1), d(m), du(m - 1), du2(m > 2 ? m - 2 : 1, 0.0), x(m, 1.0);
+    std::vector<int> piv(m - 1);
+    for (int i = 0; i < m; i++) d[i] = a[i] - sigma;
+    for (int i = 0; i + 1 < m; i++) dl[i] = du[i] = b[i];
+    for (int i = 0; i + 1 < m; i++) {
+        if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+            piv[i] = 0;
+            const double l = d[i] != 0.0 ? dl[i] / d[i] : 0.0;
+            dl[i] = l;
+            d[i + 1] -= l * du[i];
+        } else {
+            piv[i] = 1;
+            const double l = d[i] / dl[i];
+            d[i] = dl[i];
+            dl[i] = l;
+            const double t = d[i + 1];
+            d[i + 1] = du[i] - l * t;
+            du[i] = t;
+            if (i + 2 < m) {
+                du2[i] = du[i + 1];
+                du[i + 1] = -l * du[i + 1];
+            }
+        }
+    }
+    for (int i = 0; i < m; i++)
+        if (d[i] == 0.0) d[i] = 1e-300 * scale;
+    for (int step = 0; step < 2; step++) {
+        for (int i = 0; i + 1 < m; i++) {
+            if (piv[i]) std::swap(x[i], x[i + 1]);
+            x[i + 1] -= dl[i] * x[i];
+        }
+        x[m - 1] /= d[m - 1];
+        x[m - 2] = (x[m - 2] - du[m - 2] * x[m - 1]) / d[m - 2];
+        for (int i = m - 3; i >= 0; i--) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
+        double nrm = 0.0;
+        for (int i = 0; i < m; i++) nrm = std::max(nrm, std::fabs(x[i]));
+        for (int i = 0; i < m; i++) x[i] /= nrm;
+    }
+    double s2 = 0.0;
+    for (int i = 0; i < m; i++) s2 += x[i] * x[i];
+    return x[m - 1] / std::sqrt(s2);
 }
 
 // freq per position is kept in the handle's zlut slot 3 (see sb_api.hip).
@@ -656,14 +719,15 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     dim3 ggrid((P.blocksize + 3) / 4, nb);
     std::vector<int32_t> hact(nb);
     // REPLAY reproduces the reference's power iteration (its d to 1e-12).
-    // COUNTER needs only a valid bound d = lambda_max + 0.001: the largest
-    // Ritz value of K Lanczos steps (full re-orthogonalisation, exact for
-    // K = n_b); the Cholesky below still checks that dI - G is positive
-    // definite.
+    // COUNTER needs only a valid bound d > lambda_max: K Lanczos steps with
+    // full re-orthogonalisation give the top Ritz value theta (exact for
+    // K = n_b) and its residual |beta_K s_K|, an interval around theta that
+    // holds an eigenvalue; d = theta + residual + 0.001.  The Cholesky below
+    // still checks that dI - G is positive definite.
     const bool lanczos = P.rng_mode == SB_RNG_COUNTER;
     const int64_t maxit = lanczos ? 0 : 1000LL * P.blocksize;
     if (lanczos) {
-        const int K = std::min(P.blocksize, 96), K1 = K + 1;
+        const int K = std::min(P.blocksize, LANCZOS_MAX), K1 = K + 1;
         double *Qv = nullptr, *al = nullptr, *be = nullptr;
         hipMalloc(&Qv, sizeof(double) * (size_t)P.p * (K + 1));
         hipMalloc(&al, sizeof(double) * nb * K1);
@@ -680,15 +744,20 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
         hipMemcpyAsync(hb.data(), be, sizeof(double) * nb * K1, hipMemcpyDeviceToHost, h->stream);
         rc = hip_check(hipStreamSynchronize(h->stream), "lanczos");
         for (int b = 0; b < nb && !rc; b++) {
-            int nbb = h->ls[b + 1] - h->ls[b];
+            const int nbb = h->ls[b + 1] - h->ls[b];
+            const double *ab = &ha[(size_t)b * K1], *bb = &hb[(size_t)b * K1];
             int m = std::min(K, nbb);
+            bool exact = m == nbb;
             // stop at an invariant subspace (beta ~ 0)
             for (int k = 0; k + 1 < m; k++)
-                if (hb[(size_t)b * K1 + k] <= 1e-13 * std::fabs(ha[(size_t)b * K1])) {
+                if (bb[k] <= 1e-13 * std::fabs(ab[0])) {
                     m = k + 1;
+                    exact = true;
                     break;
                 }
-            hx[b] = std::fabs(tridiag_max(&ha[(size_t)b * K1], &hb[(size_t)b * K1], m)) + D_EPS_ADD;
+            const double theta = tridiag_max(ab, bb, m);
+            const double resid = exact ? 0.0 : std::fabs(bb[m - 1] * tridiag_last_component(ab, bb, m, theta));
+            hx[b] = std::fabs(theta) + resid + D_EPS_ADD;
         }
         hipMemcpyAsync(h->xxd, hx.data(), sizeof(double) * nb, hipMemcpyHostToDevice, h->stream);
         hipStreamSynchronize(h->stream);
