@@ -263,22 +263,41 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
             }
         h->b_ldq = ldr;
         h->b_ldt = 16 * h->b_ntile;
+        const int nS = P.ldw / 16;
+        const int nch = (nS + h->kss - 1) / h->kss;
+        const int nwg_d = (int)((bs + 255) / 256);
+        const int ntc = (int)((C + 15) / 16);
+        const int64_t rows16 = 16 * ((bs + 15) / 16);
         int nrg = (int)((bs + h->rows_per_group - 1) / h->rows_per_group);
         A((void **)&h->b_rsum, sizeof(double) * C * h->nwg_r);
         A((void **)&h->b_rmax, sizeof(uint64_t) * C * h->nwg_r);
+        A((void **)&h->b_rtot, sizeof(double) * C);
         A((void **)&h->b_Q, (size_t)16 * h->b_ntile * h->b_ldq);
         A((void **)&h->b_expo, sizeof(int) * C);
-        A((void **)&h->b_T, sizeof(int) * (size_t)bs * h->b_ldt);
+        A((void **)&h->b_T, sizeof(int) * (size_t)nch * bs * h->b_ldt);
         A((void **)&h->b_rapart, sizeof(double) * (size_t)nrg * bs * C);
         A((void **)&h->b_uapart, sizeof(double) * (size_t)nrg * bs * C);
-        h->b_ldd = ((bs + 15) / 16) * 16;
+        h->b_ldd = (int64_t)16 * sb::batch_words((int)bs);
         A((void **)&h->b_Dq, (size_t)16 * h->b_ntile * h->b_ldd);
         A((void **)&h->b_expo2, sizeof(int) * C);
         A((void **)&h->b_kc, sizeof(double) * C);
+        A((void **)&h->b_dmax, sizeof(uint64_t) * C * nwg_d);
+        A((void **)&h->b_dk, sizeof(double) * C * nwg_d);
+        A((void **)&h->ra_sw, sizeof(double) * rows16 * 16 * ntc);
+        A((void **)&h->dl_sw, sizeof(double) * rows16 * 16 * ntc);
         h->gt_off.assign(d.nblocks + 1, 0);
-        for (int b = 0; b < d.nblocks; b++)
-            h->gt_off[b + 1] = h->gt_off[b] + (int64_t)((ls[b + 1] - ls[b] + 15) / 16) * ldr;
-        A((void **)&h->geno_t, sizeof(uint32_t) * (size_t)h->gt_off[d.nblocks]);
+        h->gs_off.assign(d.nblocks + 1, 0);
+        h->tt_off.assign(d.nblocks + 1, 0);
+        for (int b = 0; b < d.nblocks; b++) {
+            const int64_t nbb = ls[b + 1] - ls[b], nbt = (nbb + 15) / 16;
+            h->gt_off[b + 1] = h->gt_off[b] + (ldr / 16) * (sb::batch_words((int)nbb) / 16) * 64;
+            h->gs_off[b + 1] = h->gs_off[b] + nbt * nS * 64;
+            h->tt_off[b + 1] = h->tt_off[b] + nbt * (nbt + 1) / 2;
+        }
+        A((void **)&h->geno_t, (size_t)16 * h->gt_off[d.nblocks]);
+        A((void **)&h->geno_sw, (size_t)16 * h->gs_off[d.nblocks]);
+        A((void **)&h->Ut0, sizeof(double) * 256 * (size_t)h->tt_off[d.nblocks]);
+        A((void **)&h->Ut1, sizeof(double) * 256 * (size_t)h->tt_off[d.nblocks]);
     }
     if (rc) {
         sb_destroy(h);
@@ -322,8 +341,10 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     hipMemsetAsync(h->scal, 0, sizeof(Scal) * C, st);
     if (h->b_Q) {
         hipMemsetAsync(h->b_Q, 0, (size_t)16 * h->b_ntile * h->b_ldq, st);
-        hipMemsetAsync(h->b_T, 0, sizeof(int) * (size_t)bs * h->b_ldt, st);
         hipMemsetAsync(h->b_Dq, 0, (size_t)16 * h->b_ntile * h->b_ldd, st);
+        const int64_t rows16 = 16 * ((bs + 15) / 16), ntc = (C + 15) / 16;
+        hipMemsetAsync(h->ra_sw, 0, sizeof(double) * rows16 * 16 * ntc, st);
+        hipMemsetAsync(h->dl_sw, 0, sizeof(double) * rows16 * 16 * ntc, st);
     }
     hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
     rc = setup_pack(h, bed, d.bed_on_device != 0);

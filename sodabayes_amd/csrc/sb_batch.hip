@@ -4,12 +4,17 @@
 // r_c is written as a fixed-point integer q = rint(r_c * 2^e_c) with
 // |q| < 2^(8L-2), split into L balanced radix-256 digits (int8), and
 //   sum_i x_ij r_ic  ~=  2^-e_c * sum_k 256^k * T_jck,   T_jck = sum_i x_ij d_ick
-// where every T is an exact int32 sum from int8 MFMA (v_mfma_i32_16x16x32_i8).
-// The only error is the rounding of r to 8L-2 bits of its largest entry.
-// The augmented part U_s' R_a stays FP64 (LDS-tiled over all chains).
+// where every T is an exact int32 sum from int8 MFMA
+// (v_mfma_i32_16x16x64_i8: one 32-bit word of sixteen 2-bit genotypes is
+// one lane's 16-byte operand).  The only error is the rounding of r to
+// 8L-2 bits of its largest entry.  The residual update r_c += Z_s delta_c
+// is the same int8 product the other way round (individual-major genotypes
+// times the limbs of delta / s).  The augmented parts U_s' R_a and U_s delta
+// stay FP64 (v_mfma_f64_16x16x4f64 over all chains).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "sb_device.h"
 #include "sb_draw.h"
@@ -20,6 +25,52 @@ namespace sb {
 namespace {
 constexpr int NT = 256;
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr uint32_t M2 = 0x03030303u;
+
+// One 32-bit word of 2-bit counts as a 16-byte MFMA operand: dword m holds
+// the counts of positions m, m+4, m+8, m+12 (bytes 0..3).  Hence the limb
+// slot order {0,4,8,12, 1,5,9,13, 2,6,10,14, 3,7,11,15} of the other operand.
+__device__ __forceinline__ v4i unpack_word(uint32_t w)
+{
+    return v4i{(int)(w & M2), (int)((w >> 2) & M2), (int)((w >> 4) & M2), (int)((w >> 6) & M2)};
+}
+__device__ __forceinline__ v4i as_v4i(const uint4 &u) { return v4i{(int)u.x, (int)u.y, (int)u.z, (int)u.w}; }
+__device__ __forceinline__ uint32_t word_of(const uint4 &u, int s)
+{
+    return s == 0 ? u.x : (s == 1 ? u.y : (s == 2 ? u.z : u.w));
+}
+__device__ __forceinline__ int slot_of(int t) { return 4 * (t & 3) + (t >> 2); }
+
+// rows of 16 per wave for a given number of 16-column tiles (register budget)
+constexpr int rows_tiles(int ntile) { return ntile <= 4 ? 4 : (ntile <= 8 ? 2 : 1); }
+
+template <class F>
+int with_ntile(int nt, F &&f)
+{
+    switch (nt) {
+    case 1: f(std::integral_constant<int, 1>{}); return 0;
+    case 2: f(std::integral_constant<int, 2>{}); return 0;
+    case 3: f(std::integral_constant<int, 3>{}); return 0;
+    case 4: f(std::integral_constant<int, 4>{}); return 0;
+    case 6: f(std::integral_constant<int, 6>{}); return 0;
+    case 8: f(std::integral_constant<int, 8>{}); return 0;
+    case 12: f(std::integral_constant<int, 12>{}); return 0;
+    default: return SB_ERR_ARG;
+    }
+}
+
+template <class F>
+int with_ntc(int ntc, F &&f)
+{
+    switch (ntc) {
+    case 1: f(std::integral_constant<int, 1>{}); return 0;
+    case 2: f(std::integral_constant<int, 2>{}); return 0;
+    case 3: f(std::integral_constant<int, 3>{}); return 0;
+    case 4: f(std::integral_constant<int, 4>{}); return 0;
+    default: return SB_ERR_ARG;
+    }
+}
 
 }  // namespace
 
@@ -54,34 +105,59 @@ __global__ __launch_bounds__(NT) void k_rstat(Par P, const double *r, double *rs
     }
 }
 
-// Fixed-point exponent of chain c from the k_rstat partials.
-__device__ __forceinline__ int chain_exponent(const uint64_t *rmax, int nwg, int c, int L)
+// Fixed-point exponent e with max|v| * 2^e < 2^(8L-2) from a maximum given
+// as the bit pattern of a non-negative double.
+__device__ __forceinline__ int fixed_exponent(unsigned long long m, int L)
 {
-    unsigned long long m = 0;
-    for (int w = 0; w < nwg; w++) m = rmax[(int64_t)c * nwg + w] > m ? rmax[(int64_t)c * nwg + w] : m;
     double mv = __longlong_as_double((long long)m);
     if (mv == 0.0) return 0;
     int E;
     frexp(mv, &E);              // mv = f * 2^E, 0.5 <= f < 1
-    return 8 * L - 2 - E;       // |r| * 2^e < 2^(8L-2)
+    return 8 * L - 2 - E;
 }
 
 // One thread per (16-individual word, chain): balanced radix-256 digits of
-// q = rint(r * 2^e), written limb-major as Q[c*L + k][16 w + slot] with the
-// slot order {0,4,8,12, 1,5,9,13, 2,6,10,14, 3,7,11,15}: the order in which
-// (word >> 2m) & 0x03030303 unpacks a word of 2-bit genotypes.
-__global__ __launch_bounds__(NT) void k_quant(Par P, int L, const double *r, const uint64_t *rmax,
-                                              int nwg, int8_t *Q, int64_t ldq, int *expo)
+// q = rint(r * 2^e) for limb rows c*L + k, written as MFMA B-operand
+// blocks: Q[((S * NTILE + t) * 4 + s) * 64 + lane] (16 bytes each) holds,
+// for lane = l16 + 16 g, the 16 slots (order of unpack_word) of row
+// 16 t + l16 at word 16 S + 4 g + s.  The first wave of every workgroup
+// reduces the k_rstat partials of its chain; workgroup 0 stores e, sum r.
+__global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const double *r, const double *rsum,
+                                              const uint64_t *rmax, int nwg, uint4 *Q, int *expo, double *rtot)
 {
+    __shared__ int e_s;
     const int c = blockIdx.z;
+    if (threadIdx.x < WAVE) {
+        unsigned long long m = 0;
+        double s = 0.0;
+        for (int w = threadIdx.x; w < nwg; w += WAVE) {
+            unsigned long long v = rmax[(int64_t)c * nwg + w];
+            m = v > m ? v : m;
+            s += rsum[(int64_t)c * nwg + w];
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long o = __shfl_xor(m, off, WAVE);
+            m = o > m ? o : m;
+        }
+        s = wave_sum(s);
+        if (threadIdx.x == 0) {
+            e_s = fixed_exponent(m, L);
+            if (blockIdx.x == 0) {
+                expo[c] = e_s;
+                rtot[c] = s;
+            }
+        }
+    }
+    __syncthreads();
+    const int e = e_s;
     const int w = blockIdx.x * NT + threadIdx.x;
-    const int e = chain_exponent(rmax, nwg, c, L);
-    if (w == 0) expo[c] = e;
     if (w >= P.ldw) return;
     const double *rc = r + (int64_t)c * P.ldr + (int64_t)w * 16;
     int32_t q[16];
 #pragma unroll
     for (int t = 0; t < 16; t++) q[t] = (int32_t)rint(ldexp(rc[t], e));
+    const int S = w >> 4, g = (w >> 2) & 3, s = w & 3;
     for (int k = 0; k < L; k++) {
         uint32_t out[4];
 #pragma unroll
@@ -97,166 +173,210 @@ __global__ __launch_bounds__(NT) void k_quant(Par P, int L, const double *r, con
         }
 #pragma unroll
         for (int t = 0; t < 16; t++) q[t] = (q[t] - (int32_t)(int8_t)(q[t] & 0xFF)) >> 8;
-        uint4 *dst = reinterpret_cast<uint4 *>(Q + (int64_t)(c * L + k) * ldq + (int64_t)w * 16);
-        *dst = make_uint4(out[0], out[1], out[2], out[3]);
+        const int row = c * L + k;
+        Q[(((int64_t)S * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
+            make_uint4(out[0], out[1], out[2], out[3]);
     }
 }
 
-// T[jk][col] += sum_i x_(p0+jk, i) * Q[col][i] for the block's SNPs.
-// Workgroup = 64 SNPs (16 per wave) x all NT16 columns over one chunk of
-// `kwords` genotype words; int32 partial sums are added atomically (exact).
-template <int NTILE>
-__global__ __launch_bounds__(NT) void k_xq(Par P, const uint32_t *__restrict__ geno, int p0, int nb,
-                                           const int8_t *__restrict__ Q, int64_t ldq, int kwords,
-                                           int *__restrict__ T, int ldt)
+// Tp[chunk][jk][col] = sum over the chunk's individuals of x_(p0+jk, i) * Q[col][i].
+// Workgroup: 16*IT SNPs x all 16*NTILE columns over one chunk of `kss`
+// super-steps (16 words = 256 individuals).  The four waves split the
+// chunk's super-steps; every operand load is one contiguous 1 KB per wave
+// (A tiles of geno_sw, B blocks of Q); the waves are summed exactly in LDS.
+template <int NTILE, int IT>
+__global__ __launch_bounds__(NT) void k_xq(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
+                                           const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp, int ldt,
+                                           int64_t tstride)
 {
+    constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
+    __shared__ int red[ROWS][COLS + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int snp = blockIdx.x * 64 + wave * 16 + l16;
-    const bool valid = snp < nb;
-    const int w0 = blockIdx.y * kwords;
-    const int w1 = min(P.ldw, w0 + kwords);
-    const uint32_t *grow = geno + (int64_t)(p0 + (valid ? snp : 0)) * P.ldw;
-    v4i acc[NTILE];
+    const int r0 = blockIdx.x * ROWS;
+    for (int k = threadIdx.x; k < ROWS * (COLS + 1); k += NT) (&red[0][0])[k] = 0;
+    const int nS = P.ldw / 16;
+    const int S0 = blockIdx.y * kss, nss = min(nS, S0 + kss) - S0;
+    const int sb = S0 + wave * nss / 4, se = S0 + (wave + 1) * nss / 4;
+    bool valid[IT];
 #pragma unroll
-    for (int t = 0; t < NTILE; t++) acc[t] = v4i{0, 0, 0, 0};
-    constexpr uint32_t M = 0x03030303u;
-    // super-step: 16 words (256 individuals); lane group g owns words 4g..4g+3
-    // (all loads of a super-step are issued before its MFMAs)
-    for (int ws = w0; ws < w1; ws += 16) {
-        uint4 gw = valid ? *reinterpret_cast<const uint4 *>(grow + ws + 4 * g) : make_uint4(0, 0, 0, 0);
-        uint4 qb[4][NTILE];
+    for (int it = 0; it < IT; it++) valid[it] = blockIdx.x * IT + it < nbt;
+    const uint4 *abase = gsw + (int64_t)blockIdx.x * IT * nS * 64 + lane;
+    v4i acc[IT][NTILE];
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const int64_t qoff = (int64_t)(ws + 4 * g + s) * 16;
+    for (int it = 0; it < IT; it++)
 #pragma unroll
-            for (int t = 0; t < NTILE; t++)
-                qb[s][t] = *reinterpret_cast<const uint4 *>(Q + (int64_t)(16 * t + l16) * ldq + qoff);
-        }
-        uint32_t words[4] = {gw.x, gw.y, gw.z, gw.w};
+        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
+    for (int S = sb; S < se; S++) {
+        uint4 a[IT];
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const uint32_t wd = words[s];
-            const long a0 = (long)(((uint64_t)((wd >> 2) & M) << 32) | (wd & M));
-            const long a1 = (long)(((uint64_t)((wd >> 6) & M) << 32) | ((wd >> 4) & M));
+        for (int it = 0; it < IT; it++)
+            a[it] = valid[it] ? abase[((int64_t)it * nS + S) * 64] : make_uint4(0, 0, 0, 0);
+        uint4 b[4][NTILE];
 #pragma unroll
-            for (int t = 0; t < NTILE; t++) {
-                const long b0 = (long)(((uint64_t)qb[s][t].y << 32) | qb[s][t].x);
-                const long b1 = (long)(((uint64_t)qb[s][t].w << 32) | qb[s][t].z);
-                acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b0, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b1, acc[t], 0, 0, 0);
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int t = 0; t < NTILE; t++) b[s][t] = Q[(((int64_t)S * NTILE + t) * 4 + s) * 64 + lane];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const v4i av = unpack_word(word_of(a[it], s));
+#pragma unroll
+                for (int t = 0; t < NTILE; t++)
+                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[s][t]), acc[it][t], 0, 0, 0);
             }
-        }
     }
-    // acc[t][e]: row = SNP blockIdx.x*64 + wave*16 + 4g + e, column 16 t + l16
+    __syncthreads();
+    // acc[it][t][e]: SNP row 16 it + 4 g + e, column 16 t + l16
 #pragma unroll
-    for (int t = 0; t < NTILE; t++)
+    for (int it = 0; it < IT; it++)
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            int row = blockIdx.x * 64 + wave * 16 + 4 * g + e;
-            if (row < nb && acc[t][e] != 0) atomicAdd(T + (int64_t)row * ldt + 16 * t + l16, acc[t][e]);
-        }
+        for (int t = 0; t < NTILE; t++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (acc[it][t][e] != 0) atomicAdd(&red[16 * it + 4 * g + e][16 * t + l16], acc[it][t][e]);
+    __syncthreads();
+    int *out = Tp + (int64_t)blockIdx.y * tstride;
+    for (int k = threadIdx.x; k < ROWS * COLS; k += NT) {
+        const int row = k / COLS, col = k - row * COLS;
+        if (r0 + row < nb) out[(int64_t)(r0 + row) * ldt + col] = red[row][col];
+    }
 }
 
-// Products with a block's packed upper triangle U on FP64 MFMA
-// (v_mfma_f64_16x16x4f64), 64 x 64 U tiles staged in LDS:
-//   mode 0 (rhs):    part[g][jk][c] = sum_{i in row group g, i <= jk} U(i, jk) R(i, c)
-//   mode 1 (update): part[g][i][c]  = sum_{j in col group g, j >= i} U(i, j) R(j, c)
-// R is n_b x C (chain stride rstride) -- r_a for mode 0, delta for mode 1.
-// Workgroup: 64 outputs (4 waves x 16) x all chains over one group of
-// `span` rows (mode 0) or columns (mode 1).
-typedef double v4d __attribute__((ext_vector_type(4)));
-template <int MODE>
-__global__ __launch_bounds__(NT) void k_utile(Par P, const double *__restrict__ U, int64_t tri0, int nb,
-                                              const double *__restrict__ R, int64_t rstride, int span,
-                                              double *__restrict__ part)
+// Products with a block's upper factor U on FP64 MFMA (v_mfma_f64_16x16x4f64),
+// U held as 16 x 16 tiles (I <= J at tile J(J+1)/2 + I) in the operand order
+// of the mode, so that a wave reads a tile as two contiguous 1 KB loads:
+//   mode 0 (rhs):    part[g][c][jk] = sum_{row tiles I of group g, I <= J} U(i, jk) R(i, c)
+//   mode 1 (update): part[g][c][i]  = sum_{col tiles J of group g, J >= I} U(i, j) R(j, c)
+// R (r_a for mode 0, delta for mode 1) is a swizzled panel (rsw_index).  A
+// wave owns one 16-row/column output tile; a workgroup four of them over one
+// group of `spant` tiles; waves with no tile of the group on their side of
+// the diagonal write nothing.
+template <int MODE, int NTC>
+__global__ __launch_bounds__(NT) void k_utile(Par P, const double2 *__restrict__ Ut, int nbt, int nb,
+                                              const double2 *__restrict__ Rsw, int spant, double *__restrict__ part)
 {
     const int C = P.nchains;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int l16 = lane & 15, lg = lane >> 4;
-    const int o0 = blockIdx.x * 64 + wave * 16;   // this wave's 16 outputs
-    const int g = blockIdx.y;
-    int k0 = g * span, k1 = min(nb, k0 + span);
+    const int O = blockIdx.x * 4 + wave;
+    if (O >= nbt) return;
+    int K0 = blockIdx.y * spant, K1 = min(nbt, K0 + spant);
     if (MODE == 0) {
-        if (k0 > blockIdx.x * 64 + 63) return;    // rows entirely below the diagonal
-        k1 = min(k1, o0 + 16);
+        if (K0 > O) return;
+        K1 = min(K1, O + 1);
     } else {
-        if (k1 <= blockIdx.x * 64) return;        // columns entirely left of the diagonal
-        k0 = max(k0, o0);
+        if (K1 <= O) return;
+        K0 = max(K0, O);
     }
-    if (o0 >= nb) return;
-    const int ntc = (C + 15) / 16;
-    v4d acc[4];
+    v4d acc[NTC];
 #pragma unroll
-    for (int t = 0; t < 4; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    // k-step of 16: lane group lg owns k indices kt + 4 lg + s, s = 0..3
-    for (int kt = k0 - (k0 & 3); kt < k1; kt += 16) {
-        double a[4];
-        const int kb = kt + 4 * lg;
-        if (MODE == 0) {
-            // A[m][k] = U(row kb+s, col o0+m): 4 consecutive rows of column o0+l16
-            const int j = o0 + l16;
-            const double *col = U + tri0 + (int64_t)j * (j + 1) / 2;
+    for (int t = 0; t < NTC; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int K = K0; K < K1; K += 2) {
+        double2 a[2][2], r[2][NTC][2];
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int i = kb + s;
-                a[s] = (j < nb && i >= k0 && i < k1 && i <= j) ? col[i] : 0.0;
-            }
-        } else {
-            // A[m][k] = U(row o0+m, col kb+s): row o0+l16 of 4 consecutive columns
-            const int i = o0 + l16;
+        for (int u = 0; u < 2; u++) {
+            const int KK = min(K + u, K1 - 1);
+            const int I = MODE == 0 ? KK : O, J = MODE == 0 ? O : KK;
+            const double2 *tp = Ut + ((int64_t)J * (J + 1) / 2 + I) * 128 + lane;
+            a[u][0] = tp[0];
+            a[u][1] = tp[64];
+            if (K + u >= K1) a[u][0] = a[u][1] = make_double2(0.0, 0.0);
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int j = kb + s;
-                a[s] = (i < nb && j >= k0 && j < k1 && i <= j) ? U[tri0 + (int64_t)j * (j + 1) / 2 + i] : 0.0;
+            for (int t = 0; t < NTC; t++) {
+                const double2 *rp = Rsw + ((int64_t)KK * NTC + t) * 128 + lane;
+                r[u][t][0] = rp[0];
+                r[u][t][1] = rp[64];
             }
         }
-        for (int t = 0; t < ntc; t++) {
-            const int ch = t * 16 + l16;
-            double bv[4];
-            const double *rc = R + (int64_t)(ch < C ? ch : 0) * rstride;
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int k = kb + s;
-                bv[s] = (ch < C && k >= k0 && k < k1) ? rc[k] : 0.0;
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int t = 0; t < NTC; t++) {
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][0].x, r[u][t][0].x, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][0].y, r[u][t][0].y, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][1].x, r[u][t][1].x, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][1].y, r[u][t][1].y, acc[t], 0, 0, 0);
             }
-#pragma unroll
-            for (int s = 0; s < 4; s++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bv[s], acc[t], 0, 0, 0);
-        }
     }
     // D[m][n] of the f64 16x16x4 MFMA: m = lg + 4*v (rows interleaved across
     // the 4 lane groups), n = l16 (chain within tile t)
-    for (int t = 0; t < ntc; t++)
+#pragma unroll
+    for (int t = 0; t < NTC; t++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            int o = o0 + lg + 4 * v;
-            int ch = t * 16 + l16;
-            if (o < nb && ch < C) part[((int64_t)g * P.blocksize + o) * C + ch] = acc[t][v];
+            const int o = 16 * O + lg + 4 * v, ch = 16 * t + l16;
+            if (o < nb && ch < C) part[((int64_t)blockIdx.y * C + ch) * P.blocksize + o] = acc[t][v];
         }
 }
 
+// U tiles of every block from the packed triangles: tile (I, J), element
+// th = (h * 64 + lane) * 2 + s2 (s = 2 h + s2, lane = l16 + 16 lg) holds
+//   Ut0: U(16 I + 4 lg + s, 16 J + l16)     Ut1: U(16 I + l16, 16 J + 4 lg + s)
+// (zero below the diagonal and beyond the block).
+__global__ __launch_bounds__(256) void k_pack_tiles(const double *U, const int64_t *tri_off, const int32_t *ls,
+                                                    const int64_t *tt_off, double *Ut0, double *Ut1)
+{
+    const int b = blockIdx.y, nb = ls[b + 1] - ls[b], nbt = (nb + 15) / 16;
+    const int64_t T = blockIdx.x;
+    if (T >= (int64_t)nbt * (nbt + 1) / 2) return;
+    int J = (int)((sqrt(8.0 * (double)T + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(J + 1) * (J + 2) / 2 <= T) J++;
+    while ((int64_t)J * (J + 1) / 2 > T) J--;
+    const int I = (int)(T - (int64_t)J * (J + 1) / 2);
+    const int th = threadIdx.x, s = 2 * (th >> 7) + (th & 1), lane = (th >> 1) & 63;
+    const int l16 = lane & 15, lg = lane >> 4;
+    const double *Ub = U + tri_off[b];
+    auto u = [&](int i, int j) -> double {
+        return (i <= j && j < nb) ? Ub[(int64_t)j * (j + 1) / 2 + i] : 0.0;
+    };
+    const int64_t o = (tt_off[b] + T) * 256 + th;
+    Ut0[o] = u(16 * I + 4 * lg + s, 16 * J + l16);
+    Ut1[o] = u(16 * I + l16, 16 * J + 4 * lg + s);
+}
+
 // r_a(i, c) += sum of the mode-1 partials of the column groups that reach row
-// i's tile (groups left of the diagonal wrote nothing).
-__global__ void k_ra_reduce(Par P, const double *part, int span, int ngroups, int nb, double *ra)
+// i's tile (groups left of the diagonal wrote nothing); plain and swizzled.
+__global__ void k_ra_reduce(Par P, const double *part, int spant, int ngroups, int nb, double *ra, double *ra_sw)
 {
     const int i = blockIdx.x * NT + threadIdx.x;
     const int c = blockIdx.y;
     if (i >= nb) return;
     const int C = P.nchains;
-    const int i0 = (i / 64) * 64;
     double s = 0.0;
-    for (int g = i0 / span; g < ngroups; g++) s += part[((int64_t)g * P.blocksize + i) * C + c];
-    ra[(int64_t)c * P.blocksize + i] += s;
+    for (int g = (i >> 4) / spant; g < ngroups; g++) s += part[((int64_t)g * C + c) * P.blocksize + i];
+    const double v = ra[(int64_t)c * P.blocksize + i] + s;
+    ra[(int64_t)c * P.blocksize + i] = v;
+    ra_sw[rsw_index(i, c, (C + 15) / 16)] = v;
 }
 
-// Genotypes in individual-major form per block for the update GEMM:
-// word (block b, SNP word sw, individual i) holds the 2-bit counts of block
-// positions 16 sw .. 16 sw + 15 of individual i (position t at bits 2t).
-__global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, int nb, int64_t ldi,
+// SNP-major genotype tiles of one block for k_xq: element ((tile * nS + S) *
+// 64 + lane), lane = l16 + 16 g, holds words 16 S + 4 g .. + 3 of SNP
+// 16 tile + l16 (zero beyond the block).
+__global__ void k_swz_geno(const uint32_t *__restrict__ geno, int ldw, int p0, int nb, uint4 *__restrict__ gsw)
+{
+    const int nS = ldw / 16;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (S, lane) within the tile
+    if (e >= (int64_t)nS * 64) return;
+    const int tile = blockIdx.y, S = (int)(e >> 6), lane = (int)(e & 63);
+    const int snp = 16 * tile + (lane & 15);
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (snp < nb) v = *reinterpret_cast<const uint4 *>(geno + (int64_t)(p0 + snp) * ldw + 16 * S + 4 * (lane >> 4));
+    gsw[((int64_t)tile * nS + S) * 64 + lane] = v;
+}
+
+// Individual-major genotype tiles of one block for k_xupd: SNP word sw
+// holds the 2-bit counts of block positions 16 sw .. 16 sw + 15 of one
+// individual (position t at bits 2t).  Element ((itile * nss + ss) * 64 +
+// lane), lane = l16 + 16 g, holds words 16 ss + 4 g .. + 3 of individual
+// 16 itile + l16 (positions >= nb are 0).  One thread per (16-individual
+// word iw = itile, SNP word sw).
+__global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, int nb, int nss,
                             uint32_t *__restrict__ gt)
 {
     const int sw = blockIdx.y;
-    const int iw = blockIdx.x * blockDim.x + threadIdx.x;  // word of 16 individuals
+    const int iw = blockIdx.x * blockDim.x + threadIdx.x;
     if (iw >= ldw) return;
     uint32_t in[16];
 #pragma unroll
@@ -264,41 +384,189 @@ __global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, 
         int pos = 16 * sw + t;
         in[t] = pos < nb ? geno[(int64_t)(p0 + pos) * ldw + iw] : 0u;
     }
-    uint32_t out[16];
+    const int ss = sw >> 4, g = (sw >> 2) & 3, s = sw & 3;
+    uint32_t *dst = gt + (((int64_t)iw * nss + ss) * 64 + 16 * g) * 4 + s;
 #pragma unroll
     for (int li = 0; li < 16; li++) {
         uint32_t w = 0;
 #pragma unroll
         for (int t = 0; t < 16; t++) w |= ((in[t] >> (2 * li)) & 3u) << (2 * t);
-        out[li] = w;
+        dst[4 * li] = w;
     }
-    uint4 *dst = reinterpret_cast<uint4 *>(gt + (int64_t)sw * ldi + (int64_t)iw * 16);
-#pragma unroll
-    for (int q = 0; q < 4; q++) dst[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
 }
 
-// Per chain: c_j = delta_j / s_j over the block, K_c = sum_j 2 f_j c_j, and
-// c quantised to L balanced int8 limbs in the MFMA slot order (Dq rows
-// c*L + k, 16 slots per 16-SNP word).
-__global__ __launch_bounds__(NT) void k_dquant(Par P, int L, const double *delta, const double *f2,
-                                               const double *sdv, int p0, int nb, int nsw, int8_t *Dq,
-                                               int64_t ldd, int *expo, double *kconst)
+// One thread per (16-SNP word, chain): c_j = delta_j / s_j quantised to L
+// balanced int8 limbs (slot order of unpack_word), written as B-operand
+// blocks like Q: Dq[((ss * NTILE + t) * 4 + s) * 64 + l16 + 16 g] for limb row
+// 16 t + l16 and SNP word 16 ss + 4 g + s (zero beyond the block).  The
+// exponent comes from the per-workgroup maxima of |c_j| that k_draw_b left;
+// the thread of word 0 stores it and K_c = sum_j 2 f_j c_j.
+__global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const double *delta, const double *sdv,
+                                               int p0, int nb, int nsw, const uint64_t *dmax, const double *dk,
+                                               int nwg_d, uint4 *Dq, int *expo, double *kconst)
+{
+    const int c = blockIdx.y;
+    const int sw = blockIdx.x * NT + threadIdx.x;
+    if (sw >= nsw) return;
+    unsigned long long m = 0;
+    for (int w = 0; w < nwg_d; w++) {
+        unsigned long long v = dmax[(int64_t)c * nwg_d + w];
+        m = v > m ? v : m;
+    }
+    const int e = fixed_exponent(m, L);
+    if (sw == 0) {
+        double kc = 0.0;
+        for (int w = 0; w < nwg_d; w++) kc += dk[(int64_t)c * nwg_d + w];
+        expo[c] = e;
+        kconst[c] = kc;
+    }
+    const double *dc = delta + (int64_t)c * P.p + p0;
+    int32_t q[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int j = 16 * sw + t;
+        double cj = 0.0;
+        if (j < nb) {
+            const double s = sdv[p0 + j];
+            cj = s > 0.0 ? dc[j] / s : 0.0;
+        }
+        q[slot_of(t)] = (int32_t)rint(ldexp(cj, e));
+    }
+    const int ss = sw >> 4, g = (sw >> 2) & 3, s = sw & 3;
+    for (int k = 0; k < L; k++) {
+        uint32_t out[4];
+#pragma unroll
+        for (int m4 = 0; m4 < 4; m4++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int8_t d = (int8_t)(q[4 * m4 + b] & 0xFF);
+                v |= ((uint32_t)(uint8_t)d) << (8 * b);
+            }
+            out[m4] = v;
+        }
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[t] = (q[t] - (int32_t)(int8_t)(q[t] & 0xFF)) >> 8;
+        const int row = c * L + k;
+        Dq[(((int64_t)ss * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
+            make_uint4(out[0], out[1], out[2], out[3]);
+    }
+}
+
+// r_c += X_s c - K_c for every chain, as the int8 MFMA product of the
+// individual-major genotype tiles with the limbs of c (exact int32 sums),
+// then the FP64 limb combination.  Workgroup = 16*IT individuals; the four
+// waves split the block's super-steps (16 SNP words) and are summed exactly
+// in LDS; every operand load is one contiguous 1 KB per wave.
+template <int NTILE, int IT>
+__global__ __launch_bounds__(NT) void k_xupd(Par P, const uint4 *__restrict__ gt, int nss,
+                                             const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
+                                             const double *__restrict__ kconst, double *__restrict__ r)
+{
+    constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
+    __shared__ int red[ROWS][COLS + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int i0 = blockIdx.x * ROWS;
+    for (int k = threadIdx.x; k < ROWS * (COLS + 1); k += NT) (&red[0][0])[k] = 0;
+    const int sb = wave * nss / 4, se = (wave + 1) * nss / 4;
+    v4i acc[IT][NTILE];
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
+    const uint4 *abase = gt + (int64_t)blockIdx.x * IT * nss * 64 + lane;
+    for (int ss = sb; ss < se; ss++) {
+        uint4 a[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) a[it] = abase[((int64_t)it * nss + ss) * 64];
+        uint4 b[4][NTILE];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int t = 0; t < NTILE; t++) b[s][t] = Dq[(((int64_t)ss * NTILE + t) * 4 + s) * 64 + lane];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const v4i av = unpack_word(word_of(a[it], s));
+#pragma unroll
+                for (int t = 0; t < NTILE; t++)
+                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[s][t]), acc[it][t], 0, 0, 0);
+            }
+    }
+    __syncthreads();
+    // acc[it][t][e]: individual 16 it + 4 g + e, column 16 t + l16
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (acc[it][t][e] != 0) atomicAdd(&red[16 * it + 4 * g + e][16 * t + l16], acc[it][t][e]);
+    __syncthreads();
+    const int C = P.nchains;
+    for (int pr = threadIdx.x; pr < ROWS * C; pr += NT) {
+        const int c = pr / ROWS, li = pr - c * ROWS;
+        const int i = i0 + li;
+        if (i >= P.n) continue;
+        long long S = 0;
+        for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
+        double *rc = r + (int64_t)c * P.ldr;
+        rc[i] = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
+    }
+}
+
+// One thread per (SNP, chain): exact limb combination of the chunk
+// partials, rhs, draw (Philox, counter (iteration, SNP id, tag)); delta
+// also into the swizzled panel for U delta; per workgroup, max |delta / s|
+// and sum 2 f delta / s for the update.
+__global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, int L, const int *Tp, int ldt,
+                                               int64_t tstride, int nch, const int *expo, const double *rtot,
+                                               const double *rapart, int spant, const double *f2,
+                                               const double *sdv, const int32_t *grp,
+                                               const int32_t *snp_of_pos, int p0, int nb, double zz,
+                                               int rep, double *beta, int32_t *cls, double *vara_s,
+                                               double *delta, uint8_t *chg, double *rhs_out, uint64_t *dmax,
+                                               double *dk, double *dl_sw)
 {
     __shared__ double lds[NT / WAVE];
     __shared__ unsigned long long mx[NT / WAVE];
-    __shared__ int e_s;
-    const int c = blockIdx.x;
-    const double *dc = delta + (int64_t)c * P.p + p0;
-    double kc = 0.0;
-    unsigned long long m = 0;
-    for (int j = threadIdx.x; j < nb; j += NT) {
-        double s = sdv[p0 + j];
-        double cj = s > 0.0 ? dc[j] / s : 0.0;
-        kc = fma(f2[p0 + j], cj, kc);
-        unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(cj));
-        m = bits > m ? bits : m;
+    const int jk = blockIdx.x * NT + threadIdx.x;
+    const int c = blockIdx.y;
+    const int C = P.nchains;
+    double cj = 0.0, kc = 0.0;
+    if (jk < nb) {
+        const int *trow = Tp + (int64_t)jk * ldt + c * L;
+        long long S = 0;
+        for (int k = L - 1; k >= 0; k--) {
+            long long t = 0;
+            for (int ch = 0; ch < nch; ch++) t += trow[(int64_t)ch * tstride + k];
+            S = S * 256 + t;
+        }
+        const double sx = ldexp((double)S, -expo[c]);
+        const int pos = p0 + jk;
+        const double s = sdv[pos];
+        double rhs = s > 0.0 ? (sx - f2[pos] * rtot[c]) / s : 0.0;
+        // row groups entirely below this column's diagonal tile hold no partial
+        for (int g = 0; g <= (jk >> 4) / spant; g++) rhs += rapart[((int64_t)g * C + c) * P.blocksize + jk];
+        if (rhs_out) {  // diagnostics: the products only, no draw
+            rhs_out[(int64_t)c * nb + jk] = rhs;
+        } else {
+            const Scal *scc = sc + c;
+            PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)snp_of_pos[pos], 2u);
+            const int64_t o = (int64_t)c * P.p;
+            draw_snp(P, scc, src, rhs, zz, zz / scc->vare, pos, grp[pos], beta + o, cls + o, vara_s + o,
+                     delta + o, chg + o);
+            const double dlt = delta[o + pos];
+            dl_sw[rsw_index(jk, c, (C + 15) / 16)] = dlt;
+            cj = s > 0.0 ? dlt / s : 0.0;
+            kc = f2[pos] * cj;
+        }
     }
+    if (rhs_out) return;
     kc = block_sum<NT>(kc, lds);
+    unsigned long long m = (unsigned long long)__double_as_longlong(fabs(cj));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(m, off, WAVE);
@@ -308,159 +576,46 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, const double *delta
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < NT / WAVE; w++) m = mx[w] > m ? mx[w] : m;
-        double mv = __longlong_as_double((long long)m);
-        int E = 0;
-        if (mv > 0.0) frexp(mv, &E);
-        e_s = mv > 0.0 ? 8 * L - 2 - E : 0;
-        expo[c] = e_s;
-        kconst[c] = kc;
+        dmax[(int64_t)c * gridDim.x + blockIdx.x] = m;
+        dk[(int64_t)c * gridDim.x + blockIdx.x] = kc;
     }
-    __syncthreads();
-    const int e = e_s;
-    for (int j = threadIdx.x; j < nsw * 16; j += NT) {
-        int32_t q = 0;
-        if (j < nb) {
-            double s = sdv[p0 + j];
-            double cj = s > 0.0 ? dc[j] / s : 0.0;
-            q = (int32_t)rint(ldexp(cj, e));
-        }
-        const int t = j & 15, slot = 4 * (t & 3) + (t >> 2);
-        const int64_t col = (int64_t)(j & ~15) + slot;
-        for (int k = 0; k < L; k++) {
-            int8_t d = (int8_t)(q & 0xFF);
-            Dq[(int64_t)(c * L + k) * ldd + col] = d;
-            q = (q - (int32_t)d) >> 8;
-        }
-    }
-}
-
-// r_c += X_s c - K_c for every chain, as the int8 MFMA product of the
-// individual-major genotypes with the limbs of c (exact int32 sums), then
-// the FP64 limb combination.  Workgroup = 64 individuals (4 waves x 16).
-template <int NTILE>
-__global__ __launch_bounds__(NT) void k_xupd(Par P, const uint32_t *__restrict__ gt, int64_t ldi, int nsw,
-                                             const int8_t *__restrict__ Dq, int64_t ldd, int L,
-                                             const int *__restrict__ expo, const double *__restrict__ kconst,
-                                             double *__restrict__ r)
-{
-    __shared__ int Ts[4][16][16 * NTILE + 1];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = lane >> 4, l16 = lane & 15;
-    const int i0 = blockIdx.x * 64 + wave * 16;
-    v4i acc[NTILE];
-#pragma unroll
-    for (int t = 0; t < NTILE; t++) acc[t] = v4i{0, 0, 0, 0};
-    constexpr uint32_t M = 0x03030303u;
-    constexpr int PF = 4;  // super-steps fetched ahead of their MFMAs
-    for (int s0 = 0; s0 < nsw; s0 += 4 * PF) {
-        uint32_t wd[PF];
-        uint4 qb[PF][NTILE];
-#pragma unroll
-        for (int f = 0; f < PF; f++) {
-            const int sw = s0 + 4 * f + g;
-            wd[f] = sw < nsw ? gt[(int64_t)sw * ldi + i0 + l16] : 0u;
-#pragma unroll
-            for (int t = 0; t < NTILE; t++)
-                qb[f][t] = sw < nsw ? *reinterpret_cast<const uint4 *>(Dq + (int64_t)(16 * t + l16) * ldd + (int64_t)sw * 16)
-                                    : make_uint4(0, 0, 0, 0);
-        }
-#pragma unroll
-        for (int f = 0; f < PF; f++) {
-            const long a0 = (long)(((uint64_t)((wd[f] >> 2) & M) << 32) | (wd[f] & M));
-            const long a1 = (long)(((uint64_t)((wd[f] >> 6) & M) << 32) | ((wd[f] >> 4) & M));
-#pragma unroll
-            for (int t = 0; t < NTILE; t++) {
-                const long b0 = (long)(((uint64_t)qb[f][t].y << 32) | qb[f][t].x);
-                const long b1 = (long)(((uint64_t)qb[f][t].w << 32) | qb[f][t].z);
-                acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b0, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b1, acc[t], 0, 0, 0);
-            }
-        }
-    }
-    // acc[t][e]: individual i0 + 4g + e, column 16 t + l16
-#pragma unroll
-    for (int t = 0; t < NTILE; t++)
-#pragma unroll
-        for (int e = 0; e < 4; e++) Ts[wave][4 * g + e][16 * t + l16] = acc[t][e];
-    __syncthreads();
-    const int C = P.nchains;
-    for (int pr = lane; pr < 16 * C; pr += 64) {
-        const int li = pr & 15, c = pr >> 4;
-        const int i = i0 + li;
-        if (i >= P.n) continue;
-        long long S = 0;
-        for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)Ts[wave][li][c * L + k];
-        double* rc = r + (int64_t)c * P.ldr;
-        rc[i] = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
-    }
-}
-
-// One thread per (SNP, chain): exact limb combination, rhs, draw
-// (Philox, counter (iteration, SNP id, tag)), and clearing of T.
-__global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, int L, int *T, int ldt,
-                                               const int *expo, const double *rsum, int nwg_r,
-                                               const double *rapart, int rpg, const double *f2,
-                                               const double *sdv, const int32_t *grp,
-                                               const int32_t *snp_of_pos, int p0, int nb, double zz,
-                                               int rep, double *beta, int32_t *cls, double *vara_s,
-                                               double *delta, uint8_t *chg, double *rhs_out)
-{
-    const int jk = blockIdx.x * NT + threadIdx.x;
-    const int c = blockIdx.y;
-    if (jk >= nb) return;
-    const int C = P.nchains;
-    int *trow = T + (int64_t)jk * ldt + c * L;
-    long long S = 0;
-    for (int k = L - 1; k >= 0; k--) {
-        S = S * 256 + (long long)trow[k];
-        trow[k] = 0;
-    }
-    double sx = ldexp((double)S, -expo[c]);
-    double sr = 0.0;
-    for (int w = 0; w < nwg_r; w++) sr += rsum[(int64_t)c * nwg_r + w];
-    const int pos = p0 + jk;
-    const double s = sdv[pos];
-    double rhs = s > 0.0 ? (sx - f2[pos] * sr) / s : 0.0;
-    // row groups entirely below this column's diagonal hold no partial
-    for (int g = 0; g <= jk / rpg; g++) rhs += rapart[((int64_t)g * P.blocksize + jk) * C + c];
-    if (rhs_out) {  // diagnostics: the products only, no draw
-        rhs_out[(int64_t)c * nb + jk] = rhs;
-        return;
-    }
-    const Scal *scc = sc + c;
-    PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)snp_of_pos[pos], 2u);
-    const int64_t o = (int64_t)c * P.p;
-    draw_snp(P, scc, src, rhs, zz, zz / scc->vare, pos, grp[pos], beta + o, cls + o, vara_s + o,
-                    delta + o, chg + o);
 }
 
 // ----------------------------------------------------------------- host --
 
+// SNP words of a block rounded up to whole super-steps (16 words)
+int batch_words(int nb) { return ((nb + 255) / 256) * 16; }
+
 int batch_block(sb_handle *h, int b, double zz, int rep)
 {
+    (void)zz;
+    (void)rep;
     const Par &P = h->par;
     hipStream_t st = h->stream;
-    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
     const int C = P.nchains, L = h->limbs;
     const int nwg_r = h->nwg_r;
+    uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
     k_rstat<<<dim3(nwg_r, 1, C), NT, 0, st>>>(P, h->r, h->b_rsum, h->b_rmax);
-    k_quant<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, L, h->r, h->b_rmax, nwg_r, h->b_Q,
-                                                             h->b_ldq, h->b_expo);
-    const int kwords = h->kwords;
-    dim3 gx((nb + 63) / 64, (P.ldw + kwords - 1) / kwords);
-    switch (h->b_ntile) {
-    case 1: k_xq<1><<<gx, NT, 0, st>>>(P, h->geno, p0, nb, h->b_Q, h->b_ldq, kwords, h->b_T, h->b_ldt); break;
-    case 2: k_xq<2><<<gx, NT, 0, st>>>(P, h->geno, p0, nb, h->b_Q, h->b_ldq, kwords, h->b_T, h->b_ldt); break;
-    case 3: k_xq<3><<<gx, NT, 0, st>>>(P, h->geno, p0, nb, h->b_Q, h->b_ldq, kwords, h->b_T, h->b_ldt); break;
-    case 4: k_xq<4><<<gx, NT, 0, st>>>(P, h->geno, p0, nb, h->b_Q, h->b_ldq, kwords, h->b_T, h->b_ldt); break;
-    case 6: k_xq<6><<<gx, NT, 0, st>>>(P, h->geno, p0, nb, h->b_Q, h->b_ldq, kwords, h->b_T, h->b_ldt); break;
-    case 8: k_xq<8><<<gx, NT, 0, st>>>(P, h->geno, p0, nb, h->b_Q, h->b_ldq, kwords, h->b_T, h->b_ldt); break;
-    case 12: k_xq<12><<<gx, NT, 0, st>>>(P, h->geno, p0, nb, h->b_Q, h->b_ldq, kwords, h->b_T, h->b_ldt); break;
-    default: return SB_ERR_ARG;
-    }
-    const int nrg = (nb + h->rows_per_group - 1) / h->rows_per_group;
-    k_utile<0><<<dim3((nb + 63) / 64, nrg), NT, 0, st>>>(P, h->U, h->tri_off[b], nb, h->ra, P.blocksize,
-                                                        h->rows_per_group, h->b_rapart);
+    k_quant<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax, nwg_r,
+                                                             Q, h->b_expo, h->b_rtot);
+    const int kss = h->kss, nS = P.ldw / 16;
+    const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
+    const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
+    int rc = with_ntile(h->b_ntile, [&](auto ntl) {
+        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
+        dim3 gx((nbt + IT - 1) / IT, (nS + kss - 1) / kss);
+        k_xq<NTL, IT><<<gx, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, h->b_ldt, tstride);
+    });
+    if (rc) return fail(rc, "unsupported number of limb columns");
+    const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
+    const double2 *ut = reinterpret_cast<const double2 *>(h->Ut0) + h->tt_off[b] * 128;
+    rc = with_ntc((C + 15) / 16, [&](auto ntc) {
+        constexpr int NTC = decltype(ntc)::value;
+        k_utile<0, NTC><<<dim3((nbt + 3) / 4, nrg), NT, 0, st>>>(
+            P, ut, nbt, nb, reinterpret_cast<const double2 *>(h->ra_sw), spant, h->b_rapart);
+    });
+    if (rc) return fail(rc, "unsupported chain count");
     return hip_check(hipGetLastError(), "batch products");
 }
 
@@ -468,42 +623,62 @@ int batch_update(sb_handle *h, int b, const double *delta)
 {
     const Par &P = h->par;
     hipStream_t st = h->stream;
-    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
     const int C = P.nchains, L = h->limbs;
-    const int nsw = (nb + 15) / 16;
-    k_dquant<<<C, NT, 0, st>>>(P, L, delta, h->f2, h->sdv, p0, nb, nsw, h->b_Dq, h->b_ldd, h->b_expo2,
-                               h->b_kc);
-    const int64_t ldi = P.ldr;
-    const uint32_t *gt = h->geno_t + h->gt_off[b];
-    const int nit = (int)((ldi + 63) / 64);
-    switch (h->b_ntile) {
-    case 1: k_xupd<1><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
-    case 2: k_xupd<2><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
-    case 3: k_xupd<3><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
-    case 4: k_xupd<4><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
-    case 6: k_xupd<6><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
-    case 8: k_xupd<8><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
-    case 12: k_xupd<12><<<nit, NT, 0, st>>>(P, gt, ldi, nsw, h->b_Dq, h->b_ldd, L, h->b_expo2, h->b_kc, h->r); break;
-    default: return SB_ERR_ARG;
-    }
-    const int ncg = (nb + h->rows_per_group - 1) / h->rows_per_group;
-    k_utile<1><<<dim3((nb + 63) / 64, ncg), NT, 0, st>>>(P, h->U, h->tri_off[b], nb, delta + p0, P.p,
-                                                        h->rows_per_group, h->b_uapart);
-    k_ra_reduce<<<dim3((nb + NT - 1) / NT, C), NT, 0, st>>>(P, h->b_uapart, h->rows_per_group, ncg, nb, h->ra);
+    const int nsw = batch_words(nb), nwg_d = (nb + NT - 1) / NT;
+    uint4 *Dq = reinterpret_cast<uint4 *>(h->b_Dq);
+    k_dquant<<<dim3((nsw + NT - 1) / NT, C), NT, 0, st>>>(P, L, h->b_ntile, delta, h->sdv, p0, nb, nsw, h->b_dmax,
+                                                           h->b_dk, nwg_d, Dq, h->b_expo2, h->b_kc);
+    const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
+    int rc = with_ntile(h->b_ntile, [&](auto ntl) {
+        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
+        const int nit = (int)(P.ldr / (16 * IT));
+        k_xupd<NTL, IT><<<nit, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r);
+    });
+    if (rc) return fail(rc, "unsupported number of limb columns");
+    const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
+    const double2 *ut = reinterpret_cast<const double2 *>(h->Ut1) + h->tt_off[b] * 128;
+    rc = with_ntc((C + 15) / 16, [&](auto ntc) {
+        constexpr int NTC = decltype(ntc)::value;
+        k_utile<1, NTC><<<dim3((nbt + 3) / 4, ncg), NT, 0, st>>>(
+            P, ut, nbt, nb, reinterpret_cast<const double2 *>(h->dl_sw), spant, h->b_uapart);
+    });
+    if (rc) return fail(rc, "unsupported chain count");
+    k_ra_reduce<<<dim3((nb + NT - 1) / NT, C), NT, 0, st>>>(P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
     return hip_check(hipGetLastError(), "batch update");
 }
 
+// Genotype operand layouts of the batched path, built once at sb_create.
 int batch_transpose(sb_handle *h)
 {
     const Par &P = h->par;
+    const int nS = P.ldw / 16;
     for (int b = 0; b < P.nblocks; b++) {
-        const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
-        const int nsw = (nb + 15) / 16;
-        k_transpose<<<dim3((P.ldw + NT - 1) / NT, nsw), NT, 0, h->stream>>>(h->geno, P.ldw, p0, nb, P.ldr,
-                                                                          h->geno_t + h->gt_off[b]);
+        const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
+        const int nsw = batch_words(nb);
+        k_transpose<<<dim3((P.ldw + NT - 1) / NT, nsw), NT, 0, h->stream>>>(
+            h->geno, P.ldw, p0, nb, nsw / 16, h->geno_t + 4 * h->gt_off[b]);
+        k_swz_geno<<<dim3((nS * 64 + NT - 1) / NT, nbt), NT, 0, h->stream>>>(
+            h->geno, P.ldw, p0, nb, reinterpret_cast<uint4 *>(h->geno_sw) + h->gs_off[b]);
     }
-    int rc = hip_check(hipGetLastError(), "transpose");
-    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "transpose");
+    int rc = hip_check(hipGetLastError(), "genotype layouts");
+    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "genotype layouts");
+    return rc;
+}
+
+// U tiles of every block (after the factorisation has filled h->U).
+int batch_tiles(sb_handle *h)
+{
+    const Par &P = h->par;
+    const int bt = (P.blocksize + 15) / 16;
+    int64_t *dtt = nullptr;
+    if (hip_check(hipMalloc(&dtt, sizeof(int64_t) * (P.nblocks + 1)), "hipMalloc(tile offsets)")) return SB_ERR_NOMEM;
+    hipMemcpyAsync(dtt, h->tt_off.data(), sizeof(int64_t) * (P.nblocks + 1), hipMemcpyHostToDevice, h->stream);
+    k_pack_tiles<<<dim3(bt * (bt + 1) / 2, P.nblocks), 256, 0, h->stream>>>(h->U, h->d_tri_off, h->d_ls, dtt,
+                                                                          h->Ut0, h->Ut1);
+    int rc = hip_check(hipGetLastError(), "U tiles");
+    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "U tiles");
+    hipFree(dtt);
     return rc;
 }
 
@@ -511,11 +686,13 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
 {
     const Par &P = h->par;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
+    const int nS = P.ldw / 16;
+    const int nch = (nS + h->kss - 1) / h->kss;
     k_draw_b<<<dim3((nb + NT - 1) / NT, P.nchains), NT, 0, h->stream>>>(
-        P, h->scal, h->limbs, h->b_T, h->b_ldt, h->b_expo, h->b_rsum, h->nwg_r, h->b_rapart,
-        h->rows_per_group, h->f2,
-        h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls, h->vara_s, delta, chg,
-        rhs_out);
+        P, h->scal, h->limbs, h->b_T, h->b_ldt, tstride, nch, h->b_expo, h->b_rtot, h->b_rapart,
+        h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls,
+        h->vara_s, delta, chg, rhs_out, h->b_dmax, h->b_dk, h->dl_sw);
     return hip_check(hipGetLastError(), "batch draw");
 }
 

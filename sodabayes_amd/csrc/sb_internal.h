@@ -121,9 +121,10 @@ struct sb_handle {
     int8_t *b_Q = nullptr;        // (16 * b_ntile) x ldq int8 limbs, limb-major
     int64_t b_ldq = 0;
     int *b_expo = nullptr;        // fixed-point exponent per chain
-    int kwords = 128;             // genotype words per MFMA workgroup chunk
+    double *b_rtot = nullptr;     // sum of r per chain
+    int kss = 32;                 // super-steps (16 genotype words) per MFMA workgroup chunk
     int b_ntile = 1;              // 16-column MFMA tiles covering nchains * limbs
-    int *b_T = nullptr;           // blocksize x b_ldt int32 limb products
+    int *b_T = nullptr;           // chunks x blocksize x b_ldt int32 limb products per individual chunk
     int b_ldt = 16;
     int rows_per_group = 256;     // row split of the augmented product
     double *b_rapart = nullptr;   // (blocksize / rows_per_group) x blocksize x nchains
@@ -132,8 +133,17 @@ struct sb_handle {
     int64_t b_ldd = 0;
     int *b_expo2 = nullptr;       // fixed-point exponent of delta / s per chain
     double *b_kc = nullptr;       // centring constant per chain
-    uint32_t *geno_t = nullptr;   // individual-major genotypes per block (update GEMM)
-    std::vector<int64_t> gt_off;  // word offset of each block in geno_t
+    uint64_t *b_dmax = nullptr;   // per chain, per draw workgroup: max |delta / s| (bit patterns)
+    double *b_dk = nullptr;       // per chain, per draw workgroup: sum 2 f delta / s
+    uint32_t *geno_t = nullptr;   // individual-major genotypes per block, MFMA operand tiles (update GEMM)
+    std::vector<int64_t> gt_off;  // 16-byte element offset of each block in geno_t
+    uint32_t *geno_sw = nullptr;  // SNP-major genotypes per block, MFMA operand tiles (product GEMM)
+    std::vector<int64_t> gs_off;  // 16-byte element offset of each block in geno_sw
+    double *Ut0 = nullptr;        // U as 16x16 tiles in the operand order of U' R (mode 0)
+    double *Ut1 = nullptr;        // U as 16x16 tiles in the operand order of U D (mode 1)
+    std::vector<int64_t> tt_off;  // tile offset of each block in Ut0 / Ut1 (256 doubles per tile)
+    double *ra_sw = nullptr;      // r_a of every chain as a swizzled panel (rsw_index)
+    double *dl_sw = nullptr;      // delta of the current block, swizzled panel
 
     std::vector<void *> allocs;
 };
@@ -159,5 +169,7 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
                double *rhs_out = nullptr);
 int batch_update(sb_handle *h, int b, const double *delta);
 int batch_transpose(sb_handle *h);
+int batch_words(int nb);
+int batch_tiles(sb_handle *h);
 int block_rhs(sb_handle *h, int b, double *out_host);
 }  // namespace sb

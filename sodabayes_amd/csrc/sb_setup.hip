@@ -840,6 +840,7 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     k_pack_tri<<<dim3(P.blocksize, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->d_tri_off, h->U);
     progress("cholesky done", 0, 0);
     rc = hip_check(hipGetLastError(), "cholesky");
+    if (!rc && h->Ut0) rc = batch_tiles(h);
     std::vector<int32_t> hfail(nb), hit(nb);
     if (!rc) {
         hipMemcpyAsync(hfail.data(), dfail, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);

@@ -560,13 +560,16 @@ __global__ void k_ra_replay(Par P, Scal *sc, double *ra)
     sc->rng[1] = src.st.s2;
 }
 
-__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag)
+// (ra_sw, when given, receives the same values as a swizzled panel)
+__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag, double *ra_sw)
 {
     int i = blockIdx.x * NT + threadIdx.x;
     const int c = blockIdx.z;
     if (i >= P.naind) return;
     PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)i, tag);
-    ra[(int64_t)c * P.blocksize + i] = draw_normal(src, 0.0, sqrt(sc[c].vare));
+    const double v = draw_normal(src, 0.0, sqrt(sc[c].vare));
+    ra[(int64_t)c * P.blocksize + i] = v;
+    if (ra_sw) ra_sw[rsw_index(i, c, (P.nchains + 15) / 16)] = v;
 }
 
 // posterior sums per SNP (models.f90:528-544)
@@ -802,7 +805,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
         k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
     else
         k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->scal, h->ra,
-                                                                               0x7FFFFFFF, TAG_INIT);
+                                                                               0x7FFFFFFF, TAG_INIT, h->ra_sw);
     if (hip_check(hipGetLastError(), "init kernels")) return SB_ERR_HIP;
     if (hip_check(hipStreamSynchronize(h->stream), "init")) return SB_ERR_HIP;
     h->initialised = true;
@@ -936,7 +939,8 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
         if (P.rng_mode == SB_RNG_REPLAY)
             k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
         else
-            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA);
+            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA,
+                                                                             h->ra_sw);
         if (store)
             k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->vara_s, h->gstore,
                                                                    h->indiststore, h->vara_sstore);
