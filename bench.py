@@ -32,6 +32,21 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# kernel timing slots (sodabayes_amd._lib.KT_SLOTS) -> kernel names
+KERNEL_NAMES = {
+    "rhs": "k_rhs (X_s'r + U_s'r_a, FP64)", "draw": "k_draw_counter", "list": "k_list",
+    "update": "k_update (r += Z delta, r_a += U delta)", "rquant": "k_quant (+ k_rstat per iteration; r limbs)",
+    "xq": "k_prod (X_s'Q int8 MFMA | U_s'R_a FP64 MFMA)", "ut0": "k_utile<0> (U_s'R_a, FP64 MFMA)",
+    "drawb": "k_draw_b (limb sums + draw)", "dquant": "k_dquant (delta limbs)",
+    "xupd": "k_upd (R += X_s C int8 MFMA | U_s D FP64 MFMA)", "ut1": "k_utile<1> (U_s D, FP64 MFMA)",
+    "rared": "k_ra_reduce",
+}
+
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2, the factor a 1 GiB
+# device copy shows on this GPU) + WRITE_SIZE (x1), per kernel slot, measured
+# at this workload (see profiles/); None where not measured
+TRAFFIC = {}
+
 
 def parse_args():
     ap = argparse.ArgumentParser()
@@ -46,6 +61,9 @@ def parse_args():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one plain 1 GiB device-to-device copy (a known-size reference for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
     return ap.parse_args()
 
 
@@ -157,25 +175,33 @@ def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int):
     if troof:
         # separate pass with HIP events around every block kernel
         os.environ["SODABAYES_KERNEL_TIMING"] = "1"
-        before = s.info_raw()
+        before = s.kernel_times()
         s.run(troof, trace=False)
-        after = s.info_raw()
+        after = s.kernel_times()
         os.environ.pop("SODABAYES_KERNEL_TIMING", None)
-        launches = after[5] - before[5]
-        rhs_ms = (after[4] - before[4]) / launches
-        rhs_bytes = (after[6] - before[6]) / launches
-        achieved = rhs_bytes / (rhs_ms * 1e-3) / 1e9
+        kernels = {}
+        for k, (ms, n, b) in after.items():
+            dn = n - before[k][1]
+            if dn <= 0:
+                continue
+            avg_us = (ms - before[k][0]) / dn * 1e3
+            bpl = (b - before[k][2]) / dn
+            kernels[k] = {"name": KERNEL_NAMES[k], "avg_us": round(avg_us, 2), "launches": int(dn),
+                          "algorithmic_bytes": int(bpl), "gbs": round(bpl / (avg_us * 1e-6) / 1e9, 1),
+                          "share": 0.0}
+        total = sum(v["avg_us"] * v["launches"] for v in kernels.values())
+        for v in kernels.values():
+            v["share"] = round(v["avg_us"] * v["launches"] / total, 4)
+        dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
+        d = kernels[dom]
+        out["kernels"] = kernels
         out["roofline"] = {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "block products X_s'r + A_s'r_a (k_rhs)" if precision == "fp64"
-            else "block products X_s'Q + U_s'R_a (k_rstat+k_quant+k_xq+k_ua)",
-            "avg_launch_us": round(rhs_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(rhs_bytes),
-            "draw_avg_us": round((after[7] - before[7]) / max(1, after[8] - before[8]) * 1e3, 2),
-            "update_avg_us": round((after[9] - before[9]) / max(1, after[10] - before[10]) * 1e3, 2)}
-        log(f"[{precision} x{chains}] block products: {rhs_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s "
-            f"({achieved / HBM_PEAK_GBS:.1%} of HBM); draw {out['roofline']['draw_avg_us']} us, "
-            f"update {out['roofline']['update_avg_us']} us")
+            "bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": TRAFFIC.get(dom),
+            "kernel": d["name"], "avg_launch_us": d["avg_us"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
+        for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["share"]):
+            log(f"[{precision} x{chains}]   {v['name']:<34s} {v['avg_us']:8.2f} us/launch  {v['gbs']:7.0f} GB/s"
+                f"  {v['share']:6.1%} of kernel time")
     s.close()
     return out
 
@@ -191,6 +217,13 @@ def main():
     from sodabayes_amd import soda
 
     n, p, bs = a.nind, a.nloci, a.blocksize
+    if a.copy_probe:
+        src = torch.ones(2 ** 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy probe: 1 GiB read + 1 GiB written")
     t0 = time.time()
     bed, freq, why = synth_on_device(torch, n, p, a.seed)
     torch.cuda.synchronize()

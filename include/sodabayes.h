@@ -136,7 +136,20 @@ enum {
     SB_Q_XXD,            /* d per block (nblocks) */
     SB_Q_GRAM,           /* standardised Gram of block `chain` (n_b x n_b), recomputed */
     SB_Q_SNPINDIST,      /* last iteration's SNPs per (group, class) (ngroup*nmix) */
-    SB_Q_VARINDIST       /* last iteration's sum of g^2 per (group, class) (ngroup*nmix) */
+    SB_Q_VARINDIST,      /* last iteration's sum of g^2 per (group, class) (ngroup*nmix) */
+    SB_Q_KTIMES          /* per-kernel timing of the runs made with SODABAYES_KERNEL_TIMING=1:
+                            SB_KT_N x (summed ms, launches, algorithmic bytes), slot order below */
+};
+
+/* Kernel timing slots of SB_Q_KTIMES (diagnostics; no counterpart in the
+ * reference).  FP64 path: block products, draw, changed-SNP list, update.
+ * Batched path: residual statistics + limb quantisation, X_s'Q, U_s'R_a,
+ * limb combination + draw, delta quantisation, X_s C update, U_s D, r_a
+ * reduction. */
+enum {
+    SB_KT_RHS = 0, SB_KT_DRAW, SB_KT_LIST, SB_KT_UPDATE,
+    SB_KT_RQUANT, SB_KT_XQ, SB_KT_UT0, SB_KT_DRAWB, SB_KT_DQUANT, SB_KT_XUPD, SB_KT_UT1, SB_KT_RARED,
+    SB_KT_N
 };
 int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count);
 

@@ -269,8 +269,9 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         const int ntc = (int)((C + 15) / 16);
         const int64_t rows16 = 16 * ((bs + 15) / 16);
         int nrg = (int)((bs + h->rows_per_group - 1) / h->rows_per_group);
-        A((void **)&h->b_rsum, sizeof(double) * C * h->nwg_r);
-        A((void **)&h->b_rmax, sizeof(uint64_t) * C * h->nwg_r);
+        const int64_t nwg_stat = std::max<int64_t>(h->nwg_r, ldr / 16);  // k_rstat or update workgroups
+        A((void **)&h->b_rsum, sizeof(double) * C * nwg_stat);
+        A((void **)&h->b_rmax, sizeof(uint64_t) * C * nwg_stat);
         A((void **)&h->b_rtot, sizeof(double) * C);
         A((void **)&h->b_Q, (size_t)16 * h->b_ntile * h->b_ldq);
         A((void **)&h->b_expo, sizeof(int) * C);
@@ -507,6 +508,16 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
         if (need(nb * nb)) return SB_ERR_ARG;
         return setup_gram_block(h, chain, out);
     }
+    case SB_Q_KTIMES: {
+        static_assert((int)SB_KT_N == (int)sb::KS_N, "kernel timing slots");
+        if (need(3 * SB_KT_N)) return SB_ERR_ARG;
+        for (int k = 0; k < SB_KT_N; k++) {
+            out[3 * k] = h->kc.ms[k];
+            out[3 * k + 1] = h->kc.launches[k];
+            out[3 * k + 2] = h->kc.tbytes[k];
+        }
+        return SB_OK;
+    }
     default: return fail(SB_ERR_ARG, "sb_get: unknown quantity");
     }
 }
@@ -520,9 +531,18 @@ int sb_block_rhs(sb_handle *h, int32_t block, double *out)
 int sb_get_info(sb_handle *h, double *out, int32_t n)
 {
     if (!h || !out) return fail(SB_ERR_ARG, "sb_get_info: null argument");
+    // grouped kernel times: block products, draws, updates
+    const sb::KClock &k = h->kc;
+    const double prod_ms = k.ms[sb::KS_RHS] + k.ms[sb::KS_RQUANT] + k.ms[sb::KS_XQ] + k.ms[sb::KS_UT0];
+    const double prod_n = k.launches[sb::KS_RHS] + k.launches[sb::KS_XQ];
+    const double prod_b = k.tbytes[sb::KS_RHS] + k.tbytes[sb::KS_RQUANT] + k.tbytes[sb::KS_XQ] + k.tbytes[sb::KS_UT0];
+    const double draw_ms = k.ms[sb::KS_DRAW] + k.ms[sb::KS_DRAWB];
+    const double draw_n = k.launches[sb::KS_DRAW] + k.launches[sb::KS_DRAWB];
+    const double upd_ms = k.ms[sb::KS_LIST] + k.ms[sb::KS_UPDATE] + k.ms[sb::KS_DQUANT] + k.ms[sb::KS_XUPD] +
+                          k.ms[sb::KS_UT1] + k.ms[sb::KS_RARED];
+    const double upd_n = k.launches[sb::KS_UPDATE] + k.launches[sb::KS_XUPD];
     double v[12] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes,
-                    h->k_ms[0], h->k_launch[0], h->rhs_bytes, h->k_ms[1], h->k_launch[1],
-                    h->k_ms[2], h->k_launch[2], h->pm_iters_max};
+                    prod_ms, prod_n, prod_b, draw_ms, draw_n, upd_ms, upd_n, h->pm_iters_max};
     for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
     return SB_OK;
 }

@@ -27,6 +27,7 @@ constexpr int NT = 256;
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr uint32_t M2 = 0x03030303u;
+constexpr int LIMBS = 3;  // int8 limbs per fixed-point value (22 bits)
 
 // One 32-bit word of 2-bit counts as a 16-byte MFMA operand: dword m holds
 // the counts of positions m, m+4, m+8, m+12 (bytes 0..3).  Hence the limb
@@ -45,31 +46,26 @@ __device__ __forceinline__ int slot_of(int t) { return 4 * (t & 3) + (t >> 2); }
 // rows of 16 per wave for a given number of 16-column tiles (register budget)
 constexpr int rows_tiles(int ntile) { return ntile <= 4 ? 4 : (ntile <= 8 ? 2 : 1); }
 
+// (16-column limb tiles, 16-chain tiles) pairs that 3 limbs per chain give
 template <class F>
-int with_ntile(int nt, F &&f)
+int with_tiles(int ntile, int ntc, F &&f)
 {
-    switch (nt) {
-    case 1: f(std::integral_constant<int, 1>{}); return 0;
-    case 2: f(std::integral_constant<int, 2>{}); return 0;
-    case 3: f(std::integral_constant<int, 3>{}); return 0;
-    case 4: f(std::integral_constant<int, 4>{}); return 0;
-    case 6: f(std::integral_constant<int, 6>{}); return 0;
-    case 8: f(std::integral_constant<int, 8>{}); return 0;
-    case 12: f(std::integral_constant<int, 12>{}); return 0;
-    default: return SB_ERR_ARG;
+    using std::integral_constant;
+#define SB_TILE_CASE(a, b)                                              \
+    if (ntile == a && ntc == b) {                                       \
+        f(integral_constant<int, a>{}, integral_constant<int, b>{});    \
+        return 0;                                                       \
     }
-}
-
-template <class F>
-int with_ntc(int ntc, F &&f)
-{
-    switch (ntc) {
-    case 1: f(std::integral_constant<int, 1>{}); return 0;
-    case 2: f(std::integral_constant<int, 2>{}); return 0;
-    case 3: f(std::integral_constant<int, 3>{}); return 0;
-    case 4: f(std::integral_constant<int, 4>{}); return 0;
-    default: return SB_ERR_ARG;
-    }
+    SB_TILE_CASE(1, 1)
+    SB_TILE_CASE(2, 1)
+    SB_TILE_CASE(3, 1)
+    SB_TILE_CASE(4, 2)
+    SB_TILE_CASE(6, 2)
+    SB_TILE_CASE(8, 3)
+    SB_TILE_CASE(12, 3)
+    SB_TILE_CASE(12, 4)
+#undef SB_TILE_CASE
+    return SB_ERR_ARG;
 }
 
 }  // namespace
@@ -120,33 +116,52 @@ __device__ __forceinline__ int fixed_exponent(unsigned long long m, int L)
 // q = rint(r * 2^e) for limb rows c*L + k, written as MFMA B-operand
 // blocks: Q[((S * NTILE + t) * 4 + s) * 64 + lane] (16 bytes each) holds,
 // for lane = l16 + 16 g, the 16 slots (order of unpack_word) of row
-// 16 t + l16 at word 16 S + 4 g + s.  The first wave of every workgroup
-// reduces the k_rstat partials of its chain; workgroup 0 stores e, sum r.
+// 16 t + l16 at word 16 S + 4 g + s.  Every workgroup reduces the nwg
+// partials (max |r|, sum r) of its chain; workgroup 0 stores e, sum r.
 __global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const double *r, const double *rsum,
                                               const uint64_t *rmax, int nwg, uint4 *Q, int *expo, double *rtot)
 {
     __shared__ int e_s;
+    __shared__ double sl[NT / WAVE];
+    __shared__ unsigned long long ml[NT / WAVE];
     const int c = blockIdx.z;
-    if (threadIdx.x < WAVE) {
-        unsigned long long m = 0;
-        double s = 0.0;
-        for (int w = threadIdx.x; w < nwg; w += WAVE) {
-            unsigned long long v = rmax[(int64_t)c * nwg + w];
-            m = v > m ? v : m;
-            s += rsum[(int64_t)c * nwg + w];
+    unsigned long long m = 0;
+    double sr = 0.0;
+    for (int w0 = 0; w0 < nwg; w0 += 4 * NT) {
+        unsigned long long v[4];
+        double sv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int w = w0 + u * NT + threadIdx.x;
+            v[u] = w < nwg ? rmax[(int64_t)c * nwg + w] : 0ull;
+            sv[u] = w < nwg ? rsum[(int64_t)c * nwg + w] : 0.0;
         }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            unsigned long long o = __shfl_xor(m, off, WAVE);
-            m = o > m ? o : m;
+        for (int u = 0; u < 4; u++) {
+            m = v[u] > m ? v[u] : m;
+            sr += sv[u];
         }
-        s = wave_sum(s);
-        if (threadIdx.x == 0) {
-            e_s = fixed_exponent(m, L);
-            if (blockIdx.x == 0) {
-                expo[c] = e_s;
-                rtot[c] = s;
-            }
+    }
+    sr = wave_sum(sr);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(m, off, WAVE);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sl[threadIdx.x >> 6] = sr;
+        ml[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / WAVE; w++) {
+            sr += sl[w];
+            m = ml[w] > m ? ml[w] : m;
+        }
+        e_s = fixed_exponent(m, L);
+        if (blockIdx.x == 0) {
+            expo[c] = e_s;
+            rtot[c] = sr;
         }
     }
     __syncthreads();
@@ -179,29 +194,30 @@ __global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const dou
     }
 }
 
-// Tp[chunk][jk][col] = sum over the chunk's individuals of x_(p0+jk, i) * Q[col][i].
+// Tp[chunk][col][jk] = sum over the chunk's individuals of x_(p0+jk, i) * Q[col][i]
+// (columns of length blocksize).
 // Workgroup: 16*IT SNPs x all 16*NTILE columns over one chunk of `kss`
 // super-steps (16 words = 256 individuals).  The four waves split the
 // chunk's super-steps; every operand load is one contiguous 1 KB per wave
 // (A tiles of geno_sw, B blocks of Q); the waves are summed exactly in LDS.
 template <int NTILE, int IT>
-__global__ __launch_bounds__(NT) void k_xq(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                           const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp, int ldt,
-                                           int64_t tstride)
+__device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ gsw, int nbt, int nb,
+                                        const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp,
+                                        int64_t tstride, int bx, int by)
 {
     constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
     __shared__ int red[ROWS][COLS + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int r0 = blockIdx.x * ROWS;
+    const int r0 = bx * ROWS;
     for (int k = threadIdx.x; k < ROWS * (COLS + 1); k += NT) (&red[0][0])[k] = 0;
     const int nS = P.ldw / 16;
-    const int S0 = blockIdx.y * kss, nss = min(nS, S0 + kss) - S0;
+    const int S0 = by * kss, nss = min(nS, S0 + kss) - S0;
     const int sb = S0 + wave * nss / 4, se = S0 + (wave + 1) * nss / 4;
     bool valid[IT];
 #pragma unroll
-    for (int it = 0; it < IT; it++) valid[it] = blockIdx.x * IT + it < nbt;
-    const uint4 *abase = gsw + (int64_t)blockIdx.x * IT * nS * 64 + lane;
+    for (int it = 0; it < IT; it++) valid[it] = bx * IT + it < nbt;
+    const uint4 *abase = gsw + (int64_t)bx * IT * nS * 64 + lane;
     v4i acc[IT][NTILE];
 #pragma unroll
     for (int it = 0; it < IT; it++)
@@ -237,10 +253,10 @@ __global__ __launch_bounds__(NT) void k_xq(Par P, const uint4 *__restrict__ gsw,
             for (int e = 0; e < 4; e++)
                 if (acc[it][t][e] != 0) atomicAdd(&red[16 * it + 4 * g + e][16 * t + l16], acc[it][t][e]);
     __syncthreads();
-    int *out = Tp + (int64_t)blockIdx.y * tstride;
+    int *out = Tp + (int64_t)by * tstride;
     for (int k = threadIdx.x; k < ROWS * COLS; k += NT) {
-        const int row = k / COLS, col = k - row * COLS;
-        if (r0 + row < nb) out[(int64_t)(r0 + row) * ldt + col] = red[row][col];
+        const int col = k / ROWS, row = k - col * ROWS;
+        if (r0 + row < nb) out[(int64_t)col * P.blocksize + r0 + row] = red[row][col];
     }
 }
 
@@ -254,15 +270,16 @@ __global__ __launch_bounds__(NT) void k_xq(Par P, const uint4 *__restrict__ gsw,
 // group of `spant` tiles; waves with no tile of the group on their side of
 // the diagonal write nothing.
 template <int MODE, int NTC>
-__global__ __launch_bounds__(NT) void k_utile(Par P, const double2 *__restrict__ Ut, int nbt, int nb,
-                                              const double2 *__restrict__ Rsw, int spant, double *__restrict__ part)
+__device__ __forceinline__ void ut_tile(const Par &P, const double2 *__restrict__ Ut, int nbt, int nb,
+                                        const double2 *__restrict__ Rsw, int spant, double *__restrict__ part,
+                                        int bx, int by)
 {
     const int C = P.nchains;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int l16 = lane & 15, lg = lane >> 4;
-    const int O = blockIdx.x * 4 + wave;
+    const int O = bx * 4 + wave;
     if (O >= nbt) return;
-    int K0 = blockIdx.y * spant, K1 = min(nbt, K0 + spant);
+    int K0 = by * spant, K1 = min(nbt, K0 + spant);
     if (MODE == 0) {
         if (K0 > O) return;
         K1 = min(K1, O + 1);
@@ -307,7 +324,7 @@ __global__ __launch_bounds__(NT) void k_utile(Par P, const double2 *__restrict__
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int o = 16 * O + lg + 4 * v, ch = 16 * t + l16;
-            if (o < nb && ch < C) part[((int64_t)blockIdx.y * C + ch) * P.blocksize + o] = acc[t][v];
+            if (o < nb && ch < C) part[((int64_t)by * C + ch) * P.blocksize + o] = acc[t][v];
         }
 }
 
@@ -344,8 +361,16 @@ __global__ void k_ra_reduce(Par P, const double *part, int spant, int ngroups, i
     const int c = blockIdx.y;
     if (i >= nb) return;
     const int C = P.nchains;
+    const int64_t gstride = (int64_t)C * P.blocksize;
+    const double *pp = part + (int64_t)c * P.blocksize + i;
     double s = 0.0;
-    for (int g = (i >> 4) / spant; g < ngroups; g++) s += part[((int64_t)g * C + c) * P.blocksize + i];
+    for (int g0 = (i >> 4) / spant; g0 < ngroups; g0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = g0 + u < ngroups ? pp[(g0 + u) * gstride] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
     const double v = ra[(int64_t)c * P.blocksize + i] + s;
     ra[(int64_t)c * P.blocksize + i] = v;
     ra_sw[rsw_index(i, c, (C + 15) / 16)] = v;
@@ -409,14 +434,23 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
     const int sw = blockIdx.x * NT + threadIdx.x;
     if (sw >= nsw) return;
     unsigned long long m = 0;
-    for (int w = 0; w < nwg_d; w++) {
-        unsigned long long v = dmax[(int64_t)c * nwg_d + w];
-        m = v > m ? v : m;
+    double kc = 0.0;
+    for (int w0 = 0; w0 < nwg_d; w0 += 8) {
+        unsigned long long v[8];
+        double kv[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            v[u] = w0 + u < nwg_d ? dmax[(int64_t)c * nwg_d + w0 + u] : 0ull;
+            kv[u] = (sw == 0 && w0 + u < nwg_d) ? dk[(int64_t)c * nwg_d + w0 + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            m = v[u] > m ? v[u] : m;
+            kc += kv[u];
+        }
     }
     const int e = fixed_exponent(m, L);
     if (sw == 0) {
-        double kc = 0.0;
-        for (int w = 0; w < nwg_d; w++) kc += dk[(int64_t)c * nwg_d + w];
         expo[c] = e;
         kconst[c] = kc;
     }
@@ -459,15 +493,16 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
 // waves split the block's super-steps (16 SNP words) and are summed exactly
 // in LDS; every operand load is one contiguous 1 KB per wave.
 template <int NTILE, int IT>
-__global__ __launch_bounds__(NT) void k_xupd(Par P, const uint4 *__restrict__ gt, int nss,
-                                             const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
-                                             const double *__restrict__ kconst, double *__restrict__ r)
+__device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict__ gt, int nss,
+                                          const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
+                                          const double *__restrict__ kconst, double *__restrict__ r,
+                                          double *__restrict__ rsum, uint64_t *__restrict__ rmax, int nwg, int bx)
 {
     constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
     __shared__ int red[ROWS][COLS + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int i0 = blockIdx.x * ROWS;
+    const int i0 = bx * ROWS;
     for (int k = threadIdx.x; k < ROWS * (COLS + 1); k += NT) (&red[0][0])[k] = 0;
     const int sb = wave * nss / 4, se = (wave + 1) * nss / 4;
     v4i acc[IT][NTILE];
@@ -475,7 +510,7 @@ __global__ __launch_bounds__(NT) void k_xupd(Par P, const uint4 *__restrict__ gt
     for (int it = 0; it < IT; it++)
 #pragma unroll
         for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
-    const uint4 *abase = gt + (int64_t)blockIdx.x * IT * nss * 64 + lane;
+    const uint4 *abase = gt + (int64_t)bx * IT * nss * 64 + lane;
     for (int ss = sb; ss < se; ss++) {
         uint4 a[IT];
 #pragma unroll
@@ -505,23 +540,99 @@ __global__ __launch_bounds__(NT) void k_xupd(Par P, const uint4 *__restrict__ gt
             for (int e = 0; e < 4; e++)
                 if (acc[it][t][e] != 0) atomicAdd(&red[16 * it + 4 * g + e][16 * t + l16], acc[it][t][e]);
     __syncthreads();
+    // r update, then per (workgroup, chain) max |r| and sum r of the new
+    // values over segments of ROWS lanes (one chain per segment) for the
+    // quantisation of the next block (nwg = number of xupd workgroups)
     const int C = P.nchains;
     for (int pr = threadIdx.x; pr < ROWS * C; pr += NT) {
         const int c = pr / ROWS, li = pr - c * ROWS;
         const int i = i0 + li;
-        if (i >= P.n) continue;
-        long long S = 0;
-        for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
-        double *rc = r + (int64_t)c * P.ldr;
-        rc[i] = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
+        double nv = 0.0;
+        if (i < P.n) {
+            long long S = 0;
+            for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
+            double *rc = r + (int64_t)c * P.ldr;
+            nv = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
+            rc[i] = nv;
+        }
+        double sm = nv;
+        unsigned long long mx = (unsigned long long)__double_as_longlong(fabs(nv));
+#pragma unroll
+        for (int off = ROWS / 2; off > 0; off >>= 1) {
+            sm += __shfl_xor(sm, off, WAVE);
+            const unsigned long long o = __shfl_xor(mx, off, WAVE);
+            mx = o > mx ? o : mx;
+        }
+        if (li == 0) {
+            rsum[(int64_t)c * nwg + bx] = sm;
+            rmax[(int64_t)c * nwg + bx] = mx;
+        }
     }
+}
+
+// Workgroup id -> (kind, local id) for two horizontally fused grids of n0
+// and n1 workgroups: interleaved in runs of 8 (one run per XCD, which take
+// workgroups round-robin), then the rest of the larger grid.
+__device__ __forceinline__ void split_ids(int id, int n0, int n1, int &kind, int &local)
+{
+    const int m = (min(n0, n1) / 8) * 8;
+    if (id < 2 * m) {
+        kind = (id >> 3) & 1;
+        local = ((id >> 4) << 3) + (id & 7);
+        return;
+    }
+    id -= 2 * m;
+    if (id < n0 - m) {
+        kind = 0;
+        local = m + id;
+    } else {
+        kind = 1;
+        local = m + id - (n0 - m);
+    }
+}
+
+// Block products of one block in one launch: the X_s'Q workgroups (n0, grid
+// g0x x .) and the U_s'R_a workgroups (n1, grid g1x x .) run side by side.
+template <int NTILE, int IT, int NTC>
+__global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
+                                             const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp,
+                                             int64_t tstride, int g0x, int n0, const double2 *__restrict__ Ut,
+                                             const double2 *__restrict__ Rsw, int spant, double *__restrict__ part,
+                                             int g1x, int n1)
+{
+    int kind, local;
+    split_ids(blockIdx.x, n0, n1, kind, local);
+    if (kind == 0)
+        xq_tile<NTILE, IT>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
+    else
+        ut_tile<0, NTC>(P, Ut, nbt, nb, Rsw, spant, part, local % g1x, local / g1x);
+}
+
+// Updates of one block in one launch: r += X_s C - K (n0 workgroups) beside
+// U_s D (n1 workgroups, grid g1x x .).
+template <int NTILE, int IT, int NTC>
+__global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss,
+                                            const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
+                                            const double *__restrict__ kconst, double *__restrict__ r,
+                                            double *__restrict__ rsum, uint64_t *__restrict__ rmax, int n0,
+                                            const double2 *__restrict__ Ut, int nbt, int nb,
+                                            const double2 *__restrict__ Dsw, int spant, double *__restrict__ part,
+                                            int g1x, int n1)
+{
+    int kind, local;
+    split_ids(blockIdx.x, n0, n1, kind, local);
+    if (kind == 0)
+        xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, n0, local);
+    else
+        ut_tile<1, NTC>(P, Ut, nbt, nb, Dsw, spant, part, local % g1x, local / g1x);
 }
 
 // One thread per (SNP, chain): exact limb combination of the chunk
 // partials, rhs, draw (Philox, counter (iteration, SNP id, tag)); delta
 // also into the swizzled panel for U delta; per workgroup, max |delta / s|
 // and sum 2 f delta / s for the update.
-__global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, int L, const int *Tp, int ldt,
+template <int NM>
+__global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int *Tp,
                                                int64_t tstride, int nch, const int *expo, const double *rtot,
                                                const double *rapart, int spant, const double *f2,
                                                const double *sdv, const int32_t *grp,
@@ -532,32 +643,54 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, int L, con
 {
     __shared__ double lds[NT / WAVE];
     __shared__ unsigned long long mx[NT / WAVE];
+    constexpr int L = LIMBS;
     const int jk = blockIdx.x * NT + threadIdx.x;
     const int c = blockIdx.y;
     const int C = P.nchains;
     double cj = 0.0, kc = 0.0;
     if (jk < nb) {
-        const int *trow = Tp + (int64_t)jk * ldt + c * L;
-        long long S = 0;
-        for (int k = L - 1; k >= 0; k--) {
-            long long t = 0;
-            for (int ch = 0; ch < nch; ch++) t += trow[(int64_t)ch * tstride + k];
-            S = S * 256 + t;
+        // chunk partials and r_a partials: 8 independent loads in flight per
+        // round (runtime-bounded loops would wait on each load in turn)
+        const int *tcol = Tp + (int64_t)c * L * P.blocksize + jk;
+        long long t0 = 0, t1 = 0, t2 = 0;
+        for (int ch0 = 0; ch0 < nch; ch0 += 8) {
+            int v[8][3];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    v[u][k] = ch0 + u < nch ? tcol[(int64_t)(ch0 + u) * tstride + (int64_t)k * P.blocksize] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                t0 += v[u][0];
+                t1 += v[u][1];
+                t2 += v[u][2];
+            }
         }
+        const long long S = (t2 * 256 + t1) * 256 + t0;
         const double sx = ldexp((double)S, -expo[c]);
         const int pos = p0 + jk;
         const double s = sdv[pos];
         double rhs = s > 0.0 ? (sx - f2[pos] * rtot[c]) / s : 0.0;
         // row groups entirely below this column's diagonal tile hold no partial
-        for (int g = 0; g <= (jk >> 4) / spant; g++) rhs += rapart[((int64_t)g * C + c) * P.blocksize + jk];
+        const int ng = (jk >> 4) / spant + 1;
+        const double *rp = rapart + (int64_t)c * P.blocksize + jk;
+        const int64_t gstride = (int64_t)C * P.blocksize;
+        for (int g0 = 0; g0 < ng; g0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = g0 + u < ng ? rp[(g0 + u) * gstride] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) rhs += v[u];
+        }
         if (rhs_out) {  // diagnostics: the products only, no draw
             rhs_out[(int64_t)c * nb + jk] = rhs;
         } else {
             const Scal *scc = sc + c;
             PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)snp_of_pos[pos], 2u);
             const int64_t o = (int64_t)c * P.p;
-            draw_snp(P, scc, src, rhs, zz, zz / scc->vare, pos, grp[pos], beta + o, cls + o, vara_s + o,
-                     delta + o, chg + o);
+            draw_snp<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, pos, grp[pos], beta + o, cls + o,
+                                    vara_s + o, delta + o, chg + o);
             const double dlt = delta[o + pos];
             dl_sw[rsw_index(jk, c, (C + 15) / 16)] = dlt;
             cj = s > 0.0 ? dlt / s : 0.0;
@@ -596,26 +729,33 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     const int C = P.nchains, L = h->limbs;
     const int nwg_r = h->nwg_r;
     uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
-    k_rstat<<<dim3(nwg_r, 1, C), NT, 0, st>>>(P, h->r, h->b_rsum, h->b_rmax);
-    k_quant<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax, nwg_r,
-                                                             Q, h->b_expo, h->b_rtot);
+    KClock &kc = h->kc;
+    const double n = P.n, cl = (double)C * L, gbytes = (double)nb * ((n + 3.0) / 4.0);
+    const double tbytes = 8.0 * 256.0 * nbt * (nbt + 1) / 2.0;   // U tiles of the block
+    kc.begin(st);
+    if (h->rstat_nwg == 0) {   // r changed since the last batched update
+        k_rstat<<<dim3(nwg_r, 1, C), NT, 0, st>>>(P, h->r, h->b_rsum, h->b_rmax);
+        h->rstat_nwg = nwg_r;
+    }
+    k_quant<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax,
+                                                             h->rstat_nwg, Q, h->b_expo, h->b_rtot);
+    kc.end(st, KS_RQUANT, 16.0 * C * n + cl * n);
     const int kss = h->kss, nS = P.ldw / 16;
     const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
     const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
-    int rc = with_ntile(h->b_ntile, [&](auto ntl) {
-        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
-        dim3 gx((nbt + IT - 1) / IT, (nS + kss - 1) / kss);
-        k_xq<NTL, IT><<<gx, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, h->b_ldt, tstride);
-    });
-    if (rc) return fail(rc, "unsupported number of limb columns");
     const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
     const double2 *ut = reinterpret_cast<const double2 *>(h->Ut0) + h->tt_off[b] * 128;
-    rc = with_ntc((C + 15) / 16, [&](auto ntc) {
-        constexpr int NTC = decltype(ntc)::value;
-        k_utile<0, NTC><<<dim3((nbt + 3) / 4, nrg), NT, 0, st>>>(
-            P, ut, nbt, nb, reinterpret_cast<const double2 *>(h->ra_sw), spant, h->b_rapart);
+    kc.begin(st);
+    int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
+        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
+        const int g0x = (nbt + IT - 1) / IT, n0 = g0x * ((nS + kss - 1) / kss);
+        const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
+        k_prod<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, tstride, g0x, n0, ut,
+                                                     reinterpret_cast<const double2 *>(h->ra_sw), spant,
+                                                     h->b_rapart, g1x, n1);
     });
     if (rc) return fail(rc, "unsupported chain count");
+    kc.end(st, KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 16.0 * C * nb);
     return hip_check(hipGetLastError(), "batch products");
 }
 
@@ -627,24 +767,32 @@ int batch_update(sb_handle *h, int b, const double *delta)
     const int C = P.nchains, L = h->limbs;
     const int nsw = batch_words(nb), nwg_d = (nb + NT - 1) / NT;
     uint4 *Dq = reinterpret_cast<uint4 *>(h->b_Dq);
+    KClock &kc = h->kc;
+    const double n = P.n, cl = (double)C * L, gbytes = (double)nb * ((n + 3.0) / 4.0);
+    const double tbytes = 8.0 * 256.0 * nbt * (nbt + 1) / 2.0;
+    kc.begin(st);
     k_dquant<<<dim3((nsw + NT - 1) / NT, C), NT, 0, st>>>(P, L, h->b_ntile, delta, h->sdv, p0, nb, nsw, h->b_dmax,
                                                            h->b_dk, nwg_d, Dq, h->b_expo2, h->b_kc);
+    kc.end(st, KS_DQUANT, 16.0 * C * nb + cl * nb);
     const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
-    int rc = with_ntile(h->b_ntile, [&](auto ntl) {
-        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
-        const int nit = (int)(P.ldr / (16 * IT));
-        k_xupd<NTL, IT><<<nit, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r);
-    });
-    if (rc) return fail(rc, "unsupported number of limb columns");
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
     const double2 *ut = reinterpret_cast<const double2 *>(h->Ut1) + h->tt_off[b] * 128;
-    rc = with_ntc((C + 15) / 16, [&](auto ntc) {
-        constexpr int NTC = decltype(ntc)::value;
-        k_utile<1, NTC><<<dim3((nbt + 3) / 4, ncg), NT, 0, st>>>(
-            P, ut, nbt, nb, reinterpret_cast<const double2 *>(h->dl_sw), spant, h->b_uapart);
+    kc.begin(st);
+    int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
+        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
+        const int n0 = (int)(P.ldr / (16 * IT));
+        const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
+        k_upd<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r, h->b_rsum,
+                                                    h->b_rmax, n0, ut, nbt,
+                                                    nb, reinterpret_cast<const double2 *>(h->dl_sw), spant,
+                                                    h->b_uapart, g1x, n1);
     });
     if (rc) return fail(rc, "unsupported chain count");
+    h->rstat_nwg = (int)(P.ldr / (16 * rows_tiles(h->b_ntile)));  // partials of the new r for the next block
+    kc.end(st, KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 16.0 * C * nb);
+    kc.begin(st);
     k_ra_reduce<<<dim3((nb + NT - 1) / NT, C), NT, 0, st>>>(P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
+    kc.end(st, KS_RARED, 8.0 * C * nb * (ncg + 3.0));
     return hip_check(hipGetLastError(), "batch update");
 }
 
@@ -689,10 +837,17 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
     const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
     const int nS = P.ldw / 16;
     const int nch = (nS + h->kss - 1) / h->kss;
-    k_draw_b<<<dim3((nb + NT - 1) / NT, P.nchains), NT, 0, h->stream>>>(
-        P, h->scal, h->limbs, h->b_T, h->b_ldt, tstride, nch, h->b_expo, h->b_rtot, h->b_rapart,
-        h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls,
-        h->vara_s, delta, chg, rhs_out, h->b_dmax, h->b_dk, h->dl_sw);
+    const double C = P.nchains;
+    h->kc.begin(h->stream);
+    with_nmix(P.nmix, [&](auto nm) {
+        k_draw_b<decltype(nm)::value><<<dim3((nb + NT - 1) / NT, P.nchains), NT, 0, h->stream>>>(
+            P, h->scal, h->b_T, tstride, nch, h->b_expo, h->b_rtot, h->b_rapart,
+            h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta,
+            h->cls, h->vara_s, delta, chg, rhs_out, h->b_dmax, h->b_dk, h->dl_sw);
+    });
+    // limb partials, r_a partials, effect state read and written
+    h->kc.end(h->stream, KS_DRAWB,
+              4.0 * nch * C * h->limbs * nb + 8.0 * C * nb * ((nb + 255) / 256 / 2.0 + 1.0) + 60.0 * C * nb);
     return hip_check(hipGetLastError(), "batch draw");
 }
 

@@ -2,6 +2,8 @@
 // chain-batched paths (models.f90:145-161 BayesA, 210-255 BayesB,
 // 349-412 BayesC/Cpi/R/Rc).
 #pragma once
+#include <type_traits>
+
 #include "sb_device.h"
 #include "sb_internal.h"
 
@@ -15,22 +17,32 @@ __device__ __forceinline__ uint64_t chain_key(const Par &P, int c)
     return P.seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(c + 1));
 }
 
-// Class probabilities (models.f90:363-390) for BayesC/Cpi/R/Rc.
+// Class probabilities (models.f90:363-390) for BayesC/Cpi/R/Rc.  NM is a
+// compile-time bound on the number of classes (>= P.nmix), so that the
+// per-class terms stay in registers when it is small.
+template <int NM>
 __device__ inline void mix_probs(const Par &P, const Scal *sc, int g, double rhs, double zz,
-                                          double zz_vare, double *stemp)
+                                 double zz_vare, double *stemp)
 {
-    double s[MAXMIX];
+    double s[NM];
     double vare = sc->vare;
     s[0] = sc->log_p[bin_of(P, g, 0)];
-    for (int k = 1; k < P.nmix; k++) {
-        double logdetV = log(sc->gp[k] * zz_vare + 1.0);
-        double uhat = rhs / (zz + sc->vare_gp[k]);
-        s[k] = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[bin_of(P, g, k)];
+#pragma unroll
+    for (int k = 1; k < NM; k++) {
+        if (k < P.nmix) {
+            double logdetV = log(sc->gp[k] * zz_vare + 1.0);
+            double uhat = rhs / (zz + sc->vare_gp[k]);
+            s[k] = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[bin_of(P, g, k)];
+        }
     }
-    for (int kk = 0; kk < P.nmix; kk++) {
+#pragma unroll
+    for (int kk = 0; kk < NM; kk++) {
+        if (kk >= P.nmix) break;
         double skk = s[kk], sk = 0.0;
         bool overflow = false;
-        for (int j = 0; j < P.nmix; j++) {
+#pragma unroll
+        for (int j = 0; j < NM; j++) {
+            if (j >= P.nmix) break;
             if (j == kk) continue;
             double clike = s[j] - skk;
             if (clike < -700) continue;
@@ -62,7 +74,7 @@ __device__ __forceinline__ double b_prob(const Scal *sc, double rhs, double zz, 
 // Per-SNP draw of the chosen method from uniform source `src`; returns the
 // new effect and sets class / variance.  Mirrors models.f90:145-161 (A),
 // 210-255 (B), 349-412 (C/Cpi/R/Rc).
-template <class Src>
+template <class Src, int NM = MAXMIX>
 __device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
                          int pos, int g, double *beta, int32_t *cls, double *vara_s, double *delta,
                          uint8_t *chg)
@@ -105,11 +117,13 @@ __device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double r
         chg[pos] = !(flag == 1 && cold == 1);
         return;
     }
-    double stemp[MAXMIX];
-    mix_probs(P, sc, g, rhs, zz, zz_vare, stemp);
+    double stemp[NM];
+    mix_probs<NM>(P, sc, g, rhs, zz, zz_vare, stemp);
     double u = src.next(), ss = 0.0;
     int flag = 1;
-    for (int k = 0; k < P.nmix; k++) {
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        if (k >= P.nmix) break;
         ss = ss + stemp[k];
         if (u < ss) {
             flag = k + 1;
@@ -125,6 +139,16 @@ __device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double r
     cls[pos] = flag;
     delta[pos] = gk - gn;
     chg[pos] = !(flag == 1 && cold == 1);
+}
+
+// Host-side dispatch on a compile-time bound NM >= nmix of the class count
+// (4, 8 or MAXMIX) for the draw kernels.
+template <class F>
+inline void with_nmix(int nmix, F &&f)
+{
+    if (nmix <= 4) f(std::integral_constant<int, 4>{});
+    else if (nmix <= 8) f(std::integral_constant<int, 8>{});
+    else f(std::integral_constant<int, MAXMIX>{});
 }
 
 }  // namespace sb

@@ -58,6 +58,60 @@ struct Scratch {
     double *red;       // generic reduction partials
 };
 
+// Per-kernel HIP-event timing on a handle's stream (enabled for a run of
+// sb_mcmc_run by the environment variable SODABAYES_KERNEL_TIMING=1).  Each
+// slot sums the elapsed ms, the launches and the algorithmic bytes of its
+// timed launches.
+enum KSlot {
+    KS_RHS, KS_DRAW, KS_LIST, KS_UPDATE,                          // FP64 path
+    KS_RQUANT, KS_XQ, KS_UT0, KS_DRAWB, KS_DQUANT, KS_XUPD, KS_UT1, KS_RARED,  // batched path
+    KS_N
+};
+
+struct KClock {
+    bool on = false;
+    std::vector<hipEvent_t> ev;   // event pool, reused across runs
+    std::vector<int> slot;        // slot of each recorded pair
+    std::vector<double> bytes;    // algorithmic bytes of each recorded pair
+    size_t used = 0;
+    double ms[KS_N] = {}, launches[KS_N] = {}, tbytes[KS_N] = {};
+    void begin(hipStream_t st)
+    {
+        if (!on) return;
+        if (used == ev.size()) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            ev.push_back(e);
+        }
+        hipEventRecord(ev[used++], st);
+    }
+    void end(hipStream_t st, int s, double b)
+    {
+        if (!on) return;
+        begin(st);
+        slot.push_back(s);
+        bytes.push_back(b);
+    }
+    // after the stream has drained
+    void collect()
+    {
+        for (size_t k = 0; k < slot.size(); k++) {
+            float t = 0.f;
+            hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]);
+            ms[slot[k]] += t;
+            launches[slot[k]] += 1.0;
+            tbytes[slot[k]] += bytes[k];
+        }
+        slot.clear();
+        bytes.clear();
+        used = 0;
+    }
+    ~KClock()
+    {
+        for (auto e : ev) hipEventDestroy(e);
+    }
+};
+
 }  // namespace sb
 
 struct sb_handle {
@@ -73,12 +127,7 @@ struct sb_handle {
     bool aug_built = false, initialised = false;
     double setup_seconds = 0.0, last_sweep_ms = 0.0;
     int64_t iterations = 0;
-    // kernel timing with HIP events on the handle's stream (enabled for a
-    // run by the environment variable SODABAYES_KERNEL_TIMING=1)
-    bool ktime = false;
-    double k_ms[3] = {0, 0, 0};       // rhs, draw, update: summed durations
-    double k_launch[3] = {0, 0, 0};   // launches
-    double rhs_bytes = 0.0;           // algorithmic bytes of the timed rhs launches
+    sb::KClock kc;                // per-kernel timing (SODABAYES_KERNEL_TIMING=1)
     double pm_iters_max = 0.0;        // largest power-method iteration count of the setup
 
     // genotype data
@@ -115,7 +164,8 @@ struct sb_handle {
 
     // mixed-precision batched products (precision SB_PREC_MIXED)
     int limbs = 3;                // int8 limbs per residual (8L-2 fixed-point bits)
-    int nwg_r = 0;                // workgroups of the residual statistics
+    int nwg_r = 0;                // workgroups of the residual statistics (k_rstat)
+    int rstat_nwg = 0;            // valid partials of (max |r|, sum r) in b_rsum / b_rmax (0: stale)
     double *b_rsum = nullptr;     // nchains * nwg_r partial sums of r
     uint64_t *b_rmax = nullptr;   // nchains * nwg_r partial max |r| (bit patterns)
     int8_t *b_Q = nullptr;        // (16 * b_ntile) x ldq int8 limbs, limb-major

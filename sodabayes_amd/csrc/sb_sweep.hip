@@ -177,6 +177,7 @@ __device__ __forceinline__ double block_rhs(const Par &P, const double *part, co
 }
 
 // COUNTER mode: one thread per SNP of the block.
+template <int NM>
 __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, const double *part,
                                                      const double *partA, const double *partR,
                                                      const double *f2, const double *sdv,
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
     int pos = p0 + jk;
     double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
     PhiloxSrc src(chain_key(P, 0), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
-    draw_snp(P, sc, src, rhs, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta, chg);
+    draw_snp<PhiloxSrc, NM>(P, sc, src, rhs, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta, chg);
 }
 
 // REPLAY mode: rhs in parallel, then every draw from stream 0 in block
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
             double r2 = rhs;
             if (cls[pos] > 1) r2 = r2 + zz * beta[pos];
             rhs_s[jk] = r2;
-            mix_probs(P, sc, grp[pos], r2, zz, zz_vare, stemp_s + (int64_t)jk * P.nmix);
+            mix_probs<MAXMIX>(P, sc, grp[pos], r2, zz, zz_vare, stemp_s + (int64_t)jk * P.nmix);
         }
     }
     __syncthreads();
@@ -800,6 +801,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
     std::vector<double> vs(p * Cs, vara);
     hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p * Cs, hipMemcpyHostToDevice, h->stream);
     k_init_r<<<dim3((n + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->why, sc.mu, h->r);
+    h->rstat_nwg = 0;
     k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal, yty);
     if (P.rng_mode == SB_RNG_REPLAY)
         k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
@@ -824,6 +826,7 @@ int block_rhs(sb_handle *h, int b, double *out_host)
     if (hip_check(hipMalloc(&dout, sizeof(double) * (size_t)nb * P.nchains), "hipMalloc(rhs)")) return SB_ERR_NOMEM;
     int rc = SB_OK;
     if (P.precision == SB_PREC_MIXED) {
+        h->rstat_nwg = 0;
         rc = batch_block(h, b, 0.0, 0);
         if (!rc) rc = batch_draw(h, b, 0.0, 0, h->sc.delta, reinterpret_cast<uint8_t *>(h->sc.flag), dout);
     } else {
@@ -865,24 +868,17 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     double *delta = h->sc.delta;
     std::vector<double> xxd(P.nblocks);
     hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
-    // optional per-kernel HIP events on this stream (rhs, draw, update)
+    // optional per-kernel HIP events on this stream
     const char *kt = std::getenv("SODABAYES_KERNEL_TIMING");
-    h->ktime = kt && kt[0] == '1';
-    std::vector<hipEvent_t> ev;
-    if (h->ktime) {
-        ev.resize((size_t)count * P.nblocks * 6);
-        for (auto &e : ev) hipEventCreate(&e);
-    }
-    size_t evi = 0;
-    auto mark = [&]() {
-        if (h->ktime) hipEventRecord(ev[evi++], st);
-    };
+    h->kc.on = kt && kt[0] == '1';
+    KClock &kc = h->kc;
     auto t0 = std::chrono::steady_clock::now();
     int rec = 0;
     for (int rep = first; rep < first + count; rep++) {
         k_mu_pre<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
         k_mu_draw<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, rep);
         k_mu_post<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal);
+        h->rstat_nwg = 0;   // r changed: the batched path recomputes its statistics
         for (int b = 0; b < P.nblocks; b++) {
             const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
             const int64_t tri0 = h->tri_off[b];
@@ -891,45 +887,51 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             int ngs = (nb + 4 * spw - 1) / (4 * spw);
             int nxg = ngs * P.nchunk;
             int nag = (nb + 3) / 4;
-            mark();
+            const double gbytes = (double)nb * (((double)P.n + 3.0) / 4.0);   // 2-bit genotypes of the block
+            const double ubytes = 8.0 * (double)nb * (nb + 1) / 2.0;           // packed upper factor
             if (mixed) {
                 int rc = batch_block(h, b, xxd[b], rep);
                 if (rc) return rc;
             } else {
+                kc.begin(st);
                 k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, p0, nb, spw, ngs, nxg,
                                                 h->sc.part, h->sc.partA, h->sc.partR);
+                kc.end(st, KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
             }
-            mark();
-            if (h->ktime)
-                h->rhs_bytes += (double)nb * (((double)P.n + 3.0) / 4.0) + 8.0 * (double)nb * (nb + 1) / 2.0;
-            mark();
             if (mixed) {
                 int rc = batch_draw(h, b, xxd[b], rep, delta, chg);
                 if (rc) return rc;
-            } else if (P.rng_mode == SB_RNG_REPLAY)
-                k_draw_replay<<<1, NT, 0, st>>>(P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2,
-                                                h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp,
-                                                h->sc.uu, h->sc.chg, h->beta, h->cls, h->vara_s, delta,
-                                                chg);
-            else
-                k_draw_counter<<<(nb + NT - 1) / NT, NT, 0, st>>>(
-                    P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
-                    h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg);
-            mark();
-            mark();
+            } else {
+                kc.begin(st);
+                if (P.rng_mode == SB_RNG_REPLAY)
+                    k_draw_replay<<<1, NT, 0, st>>>(P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2,
+                                                    h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp,
+                                                    h->sc.uu, h->sc.chg, h->beta, h->cls, h->vara_s, delta,
+                                                    chg);
+                else
+                    with_nmix(P.nmix, [&](auto nm) {
+                        k_draw_counter<decltype(nm)::value><<<(nb + NT - 1) / NT, NT, 0, st>>>(
+                            P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
+                            h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg);
+                    });
+                kc.end(st, KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
+            }
             if (mixed) {
                 int rc = batch_update(h, b, delta);
                 if (rc) return rc;
             } else {
                 const int nrw = (P.n + 255) / 256;
                 const int nra = (nb + WAVE - 1) / WAVE;
+                kc.begin(st);
                 k_list<<<dim3(1, 1, C), NT, 0, st>>>(P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
                                                      h->u_cl, h->u_cnt, h->u_kc);
+                kc.end(st, KS_LIST, 25.0 * nb);
+                kc.begin(st);
                 k_update<<<dim3(nrw + nra, 1, C), UW * WAVE, 0, st>>>(P, h->geno, h->U, tri0, p0, nb, nrw,
                                                                       h->u_lst, h->u_dl, h->u_cl, h->u_cnt,
                                                                       h->u_kc, h->r, h->ra);
+                kc.end(st, KS_UPDATE, 16.0 * P.n + 16.0 * nb);
             }
-            mark();
         }
         k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);
         k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
@@ -949,21 +951,14 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             k_resync<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->geno, h->zlut, h->beta,
                                                                       h->resync_list, h->d_count, h->why,
                                                                       h->scal, h->r);
+            h->rstat_nwg = 0;
         }
         if (hip_check(hipGetLastError(), "sweep launch")) return SB_ERR_HIP;
     }
     int rc = hip_check(hipStreamSynchronize(st), "sweep");
     auto t1 = std::chrono::steady_clock::now();
-    if (h->ktime) {
-        for (size_t i = 0; i + 1 < evi; i += 2) {
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
-            int kind = (int)((i / 2) % 3);
-            h->k_ms[kind] += ms;
-            h->k_launch[kind] += 1.0;
-        }
-        for (auto &e : ev) hipEventDestroy(e);
-    }
+    if (kc.on) kc.collect();
+    kc.on = false;
     if (count > 0) h->last_sweep_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / count;
     h->iterations += count;
     if (!rc && dhyp) {

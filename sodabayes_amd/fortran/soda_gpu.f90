@@ -17,7 +17,8 @@ module soda_gpu
         SB_Q_YADJ = 4, SB_Q_YAADJ = 5, SB_Q_P = 6, SB_Q_SCALARS = 7, SB_Q_GSTORE = 8, &
         SB_Q_INDISTSTORE = 9, SB_Q_VARA_SSTORE = 10, SB_Q_STORES = 11, &
         SB_Q_SNPSTORE = 12, SB_Q_VARSTORE = 13, SB_Q_PSTORE = 14, SB_Q_XA = 15, &
-        SB_Q_XXD = 16, SB_Q_GRAM = 17, SB_Q_SNPINDIST = 18, SB_Q_VARINDIST = 19
+        SB_Q_XXD = 16, SB_Q_GRAM = 17, SB_Q_SNPINDIST = 18, SB_Q_VARINDIST = 19, &
+        SB_Q_KTIMES = 20
 
     type, bind(C) :: sb_dims
         integer(c_int32_t) :: method, nind, nloci, nmix, ngroup

@@ -280,6 +280,12 @@ class SodaSampler:
         _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 12))
         return v
 
+    def kernel_times(self) -> dict:
+        """Per-kernel HIP-event timing accumulated over the runs made with
+        SODABAYES_KERNEL_TIMING=1: {slot: (total ms, launches, algorithmic bytes)}."""
+        v = self.get("KTIMES", 3 * len(_lib.KT_SLOTS))
+        return {k: (v[3 * i], v[3 * i + 1], v[3 * i + 2]) for i, k in enumerate(_lib.KT_SLOTS)}
+
     def block_rhs(self, block: int) -> np.ndarray:
         """X_s'r + A_s'r_a of a block for the current state: (chains, n_b)."""
         a, e = block_bounds(self.nloci, self.blocksize, self.nblocks, self.blockt)[block]
