@@ -40,13 +40,33 @@ def fp64_products(s, bed, chain):
     return out
 
 
-@pytest.mark.parametrize("precision,chains,bar", [("mixed", 4, 1e-6), ("mixed", 1, 1e-6),
-                                                  ("fp64", 1, 1e-12)])
-def test_block_products_against_fp64(precision, chains, bar):
-    bed, why = dataset()
-    o = soda.SodaOptions(method="BayesR", nmix=4, mix=R4, blocksize=100, numit=50, burnin=0, seed=5,
-                         rng="counter", precision=precision, chains=chains)
-    s = soda.SodaSampler(o, 1001, 300, why, bed=bed)
+METHOD_OPTS = {
+    "BayesR": dict(method="BayesR", nmix=4, mix=R4),
+    "BayesA": dict(method="BayesA"),
+    "BayesB": dict(method="BayesB", pi=0.9),
+    "BayesC": dict(method="BayesC", pi=0.9),
+    "BayesCpi": dict(method="BayesCpi"),
+    "BayesRc": dict(method="BayesRc", nmix=4, mix=R4),
+}
+
+
+def sampler(method, precision, chains, seed, blocksize=100, n=1001, p=300):
+    bed, why = dataset(n, p)
+    grp = (np.arange(p) % 3) + 1 if method == "BayesRc" else None
+    o = soda.SodaOptions(blocksize=blocksize, numit=50, burnin=0, seed=seed, rng="counter",
+                         precision=precision, chains=chains, **METHOD_OPTS[method])
+    return soda.SodaSampler(o, n, p, why, bed=bed, grp=grp), bed, why
+
+
+# chain counts cover every (limb tile, chain tile) kernel configuration:
+# 1-16 chains (1-3 limb tiles, 1 chain tile), 20 (4, 2), 40 (8, 3), 64 (12, 4)
+@pytest.mark.parametrize("method,precision,chains,bar", [
+    ("BayesR", "mixed", 4, 1e-6), ("BayesR", "mixed", 1, 1e-6), ("BayesR", "fp64", 1, 1e-12),
+    ("BayesA", "mixed", 4, 1e-6), ("BayesB", "mixed", 3, 1e-6), ("BayesC", "mixed", 5, 1e-6),
+    ("BayesCpi", "mixed", 6, 1e-6), ("BayesRc", "mixed", 4, 1e-6),
+    ("BayesR", "mixed", 20, 1e-6), ("BayesR", "mixed", 40, 1e-6), ("BayesR", "mixed", 64, 1e-6)])
+def test_block_products_against_fp64(method, precision, chains, bar):
+    s, bed, why = sampler(method, precision, chains, 5)
     s.setup()
     s.init()
     s.run(3, trace=False)
@@ -57,7 +77,7 @@ def test_block_products_against_fp64(precision, chains, bar):
             ref = fp64_products(s, bed, c)[b]
             err = np.linalg.norm(got[c] - ref) / np.linalg.norm(ref)
             worst = max(worst, err)
-    print(f"{precision} x{chains}: worst norm-wise relative error of X'r + A'r_a = {worst:.3e}")
+    print(f"{method} {precision} x{chains}: worst norm-wise relative error of X'r + A'r_a = {worst:.3e}")
     assert worst <= bar
     s.close()
 
@@ -140,15 +160,15 @@ def test_counter_mode_bound_matches_reference_d(case):
     s.close()
 
 
-@pytest.mark.parametrize("precision,chains,bar", [("fp64", 1, 1e-11), ("mixed", 8, 1e-5)])
-def test_residuals_stay_consistent(precision, chains, bar):
+@pytest.mark.parametrize("method,precision,chains,bar", [
+    ("BayesR", "fp64", 1, 1e-11), ("BayesR", "mixed", 8, 1e-5), ("BayesA", "mixed", 4, 1e-5),
+    ("BayesB", "mixed", 4, 1e-5), ("BayesRc", "mixed", 4, 1e-5), ("BayesR", "mixed", 64, 1e-5),
+    ("BayesR", "mixed", 40, 1e-5)])
+def test_residuals_stay_consistent(method, precision, chains, bar):
     """After the sweeps' incremental updates r_c = y - X beta_c - mu_c still
     holds for every chain (before the every-1000-iterations resync)."""
-    bed, why = dataset()
     n, p = 1001, 300
-    o = soda.SodaOptions(method="BayesR", nmix=4, mix=R4, blocksize=100, numit=50, burnin=0, seed=8,
-                         rng="counter", precision=precision, chains=chains)
-    s = soda.SodaSampler(o, n, p, why, bed=bed)
+    s, bed, why = sampler(method, precision, chains, 8)
     s.setup()
     s.init()
     s.run(20, trace=False)
@@ -161,6 +181,27 @@ def test_residuals_stay_consistent(precision, chains, bar):
         r = s.get("YADJ", n, c)
         want = why - Z @ g - mu
         worst = max(worst, np.linalg.norm(r - want) / np.linalg.norm(want))
-    print(f"{precision} x{chains}: residual identity error {worst:.3e}")
+    print(f"{method} {precision} x{chains}: residual identity error {worst:.3e}")
     assert worst < bar
+    s.close()
+
+
+@pytest.mark.parametrize("precision,slots,absent", [
+    ("mixed", ("rquant", "xq", "drawb", "dquant", "xupd", "rared"), ("rhs", "draw", "list", "update")),
+    ("fp64", ("rhs", "draw", "list", "update"), ("rquant", "xq", "drawb", "dquant", "xupd", "rared"))])
+def test_kernel_timing_slots(monkeypatch, precision, slots, absent):
+    """SODABAYES_KERNEL_TIMING=1 times every block kernel of the path in use
+    (one launch per block and iteration) and nothing of the other path."""
+    s, bed, why = sampler("BayesR", precision, 4 if precision == "mixed" else 1, 3)
+    s.setup()
+    s.init()
+    s.run(1, trace=False)
+    assert all(v[1] == 0 for v in s.kernel_times().values())   # off unless asked for
+    monkeypatch.setenv("SODABAYES_KERNEL_TIMING", "1")
+    s.run(2, trace=False)
+    kt = s.kernel_times()
+    for k in slots:
+        assert kt[k][1] == 2 * s.nblocks and kt[k][0] > 0 and kt[k][2] > 0, k
+    for k in absent:
+        assert kt[k][1] == 0, k
     s.close()
