@@ -151,9 +151,11 @@ contains
     ! run_program_sodaBayes (SODABayes.f90:118-143) with dumps around the
     ! two calls the GPU path replaces: setup_soda and soda_bayes.
     subroutine run_soda()
-        integer :: k, npar, ios2
+        integer :: k, npar, ios2, dblock
         character(len=1024) :: parameters(100)
         character(len=512) :: optpath
+        character(len=64) :: sblock, sshift
+        real(real64) :: dshift
         ! Options come one token per line from GOLDEN_OPTS and go through the
         ! reference's .par option parser, as a .par job does
         ! (SODABayes.f90:68-70): on the command line a 3-character key such
@@ -186,6 +188,13 @@ contains
         if (SODAOFF) stop 'golden_driver: soda mode needs SODA on'
         call setup_soda
         if (SODAOFF) stop 'golden_driver: SODA switched off by setup_soda'
+        call get_environment_variable('GOLDEN_DSHIFT_BLOCK', sblock, status=ios2)
+        if (ios2 == 0 .and. len_trim(sblock) > 0) then
+            read(sblock, *) dblock
+            call get_environment_variable('GOLDEN_DSHIFT', sshift)
+            read(sshift, *) dshift
+            call shifted_aug(dblock + 1, dshift)
+        end if
         call put_n('nind', nind)
         call put_n('nloci', nloci)
         call put_n('maxthreads', maxthreads)
@@ -235,6 +244,77 @@ contains
         k = 0
         close(21)
     end subroutine run_soda
+
+    ! Test hook (no counterpart in the reference): get_aug_matrix's loop
+    ! (soda.f90:445-518) again, with block `bshift`'s d lowered by `shift`, so
+    ! that dI - G is indefinite there and the reference's own choleskyPD /
+    ! choleskyPSD / skip bookkeeping run.  d is taken from xxd, so the
+    ! power method (and its stream-0 draws) is not repeated.
+    subroutine shifted_aug(bshift, shift)
+        integer, intent(in) :: bshift
+        real(real64), intent(in) :: shift
+        integer :: i, j, k, jj, kk, ls, le, n, rank, status
+        integer, allocatable :: piv(:)
+        real(real64), allocatable :: xxtemp(:,:)
+        real(real64) :: d
+        allocate(xxtemp(blocksize, blocksize), piv(blocksize))
+        ! setup_soda frees the packed genotypes on return (soda.f90:436-441)
+        nbatch = (nind - 1) / (int64 * 4) + 1
+        allocate(genbits(nbatch, nloci))
+        call load_geno_bits
+        skip_j = .false.
+        i_skipped = 0
+        nskipped = 0
+        naind = naind + naless
+        naless = nloci
+        XA = 0.0d0
+        do i = 1, nblocks
+            if (i <= blockt) then
+                ls = (i - 1) * blocksize + 1
+                le = min(ls + blocksize - 1, nloci)
+            else
+                ls = blocksize * blockt + (i - blockt - 1) * (blocksize - 1) + 1
+                le = min(ls + blocksize - 2, nloci)
+            end if
+            n = le - ls + 1
+            xxtemp = 0.0d0
+            do j = 1, n
+                jj = permvec(ls + j - 1)
+                do k = 1, j
+                    kk = permvec(ls + k - 1)
+                    xxtemp(k, j) = xxproduct(jj, kk)
+                    xxtemp(j, k) = xxtemp(k, j)
+                end do
+            end do
+            d = xxd(i)
+            if (i == bshift) d = d - shift
+            xxd(i) = d
+            xxtemp(:,:) = -xxtemp(:,:)
+            do j = 1, n
+                xxtemp(j, j) = d + xxtemp(j, j)
+            end do
+            call choleskyPD(xxtemp(1:n,1:n), XA(1:n,ls:le), n, status)
+            if (status /= 0) then
+                call choleskyPSD(xxtemp(1:n,1:n), piv, n, rank, status)
+                if (status /= 0) stop 'golden_driver: D-XX is not positive semidefinite'
+                do j = 1, n
+                    jj = piv(j)
+                    if (j <= rank) then
+                        XA(1:j, ls+jj-1) = xxtemp(1:j, j)
+                    else
+                        skip_j(ls+jj-1) = .true.
+                        nskipped = nskipped + 1
+                        i_skipped(nskipped) = permvec(ls+jj-1)
+                    end if
+                end do
+            else
+                naless = 0
+            end if
+            naless = min(naless, blocksize - n + nskipped)
+        end do
+        naind = naind - naless
+        deallocate(xxtemp, piv, genbits)
+    end subroutine shifted_aug
 
     subroutine run_matrix(which)
         character(len=*), intent(in) :: which

@@ -29,11 +29,12 @@ class OrcOpts(C.Structure):
         ("numit", C.c_int32), ("burnin", C.c_int32), ("thin", C.c_int32),
         ("blocksize", C.c_int32), ("nblocks", C.c_int32),
         ("rblocks", C.c_int32), ("nthreads", C.c_int32), ("maxthreads", C.c_int32),
-        ("pad0", C.c_int32),
+        ("dshift_block", C.c_int32),
         ("seed", C.c_int64),
         ("pizero", C.c_double), ("scale_va", C.c_double), ("scale_ve", C.c_double),
         ("df_va", C.c_double), ("df_ve", C.c_double), ("vara0", C.c_double),
         ("mix", C.c_double * 20), ("alpha", C.c_double * 20),
+        ("dshift", C.c_double),
     ]
 
 
@@ -204,7 +205,7 @@ def permutate(n: int, s: int, rng: Rng) -> np.ndarray:
 def make_opts(method: str, nind: int, nloci: int, *, nmix=2, mix=(0.0, 1.0), alpha=1.0,
               ngroup=1, numit=10, burnin=2, thin=1, blocksize=0, nblocks=0, rblocks=False,
               nthreads=1, maxthreads=1, seed=11, pizero=0.99, scale_va=0.01, scale_ve=0.01,
-              df_va=0.01, df_ve=0.01, vara0=0.0) -> OrcOpts:
+              df_va=0.01, df_ve=0.01, vara0=0.0, dshift_block=-1, dshift=0.0) -> OrcOpts:
     # cast_float (utilities.f90:168-171) is a default-real function: every
     # floating option passes through float32 before it reaches the sampler
     f32 = lambda v: float(np.float32(v))  # noqa: E731
@@ -219,6 +220,7 @@ def make_opts(method: str, nind: int, nloci: int, *, nmix=2, mix=(0.0, 1.0), alp
     o.nthreads, o.maxthreads, o.seed = nthreads, maxthreads, seed
     o.pizero, o.scale_va, o.scale_ve, o.df_va, o.df_ve = pizero, scale_va, scale_ve, df_va, df_ve
     o.vara0 = vara0
+    o.dshift_block, o.dshift = dshift_block, dshift
     for k, m in enumerate(mix):
         o.mix[k] = m
     al = [alpha] * nmix if np.isscalar(alpha) else list(alpha)

@@ -526,6 +526,10 @@ static int setup_soda(const orc_opts *o, const uint8_t *bed, orc_soda *S, orc_rn
         double d;
         orc_power_method(n, xx, rng0, &d);
         d = d + D_EPS_ADD;
+        /* test hook (no counterpart in the reference): a lowered d makes
+         * dI - G indefinite, so the pivoted-Cholesky fallback and the
+         * skipped-SNP updates run; the golden driver applies the same shift */
+        if (b - 1 == o->dshift_block) d = d - o->dshift;
         S->xxd[b - 1] = d;
         for (int64_t k = 0; k < (int64_t)n * n; k++) xx[k] = -xx[k];
         for (int j = 0; j < n; j++) xx[(int64_t)j * n + j] = d + xx[(int64_t)j * n + j];

@@ -56,11 +56,12 @@ typedef struct {
     int32_t numit, burnin, thin;
     int32_t blocksize, nblocks;   /* options; 0 = not given */
     int32_t rblocks, nthreads, maxthreads;
-    int32_t pad0;
+    int32_t dshift_block;         /* test hook: block whose d is lowered (-1: none) */
     int64_t seed;
     double  pizero, scale_va, scale_ve, df_va, df_ve;
     double  vara0;                /* value of module vara before the job */
     double  mix[20], alpha[20];
+    double  dshift;               /* amount subtracted from that block's d */
 } orc_opts;
 
 /* Runs the job and writes a tagged dump (same records as the golden

@@ -45,7 +45,8 @@ def oracle_options(o: dict, nind: int, nloci: int, ngroup: int = 1):
                          rblocks=bool(o.get("rblocks", False)), seed=o["seed"],
                          pizero=o.get("pi", 0.99), scale_va=o.get("scale_va", 0.01),
                          scale_ve=o.get("scale_ve", 0.01), df_va=o.get("df_va", 0.01),
-                         df_ve=o.get("df_ve", 0.01))
+                         df_ve=o.get("df_ve", 0.01), dshift_block=o.get("dshift_block", -1),
+                         dshift=o.get("dshift", 0.0))
 
 
 def group_ids(grp):

@@ -55,6 +55,13 @@ CASES = {
     "resync": (96, 80, 16, {}, dict(method="BayesCpi", blocksize=20, numit=1002, burnin=900, seed=22)),
     "ldblock": (220, 200, 17, {"ld_block": 25},
                 dict(method="BayesR", blocksize=50, numit=20, burnin=5, seed=23, **R4)),
+    # d of block 1 lowered by 30% (test hook): dI - G is indefinite there, so
+    # the reference's pivoted-Cholesky fallback, skip lists and skipped-SNP
+    # updates run (BayesA: the one method whose skipped-SNP update reads no
+    # per-thread-only state, models.f90:178-193)
+    "psd_bayesa": (150, 180, 24, {"dshift": (1, 0.3)},
+                   dict(method="BayesA", blocksize=45, numit=20, burnin=5, df_va=4.0, scale_va=0.002,
+                        seed=25)),
 }
 
 
@@ -115,7 +122,15 @@ def soda_case(name, spec):
         if grp is not None:
             gfile = os.path.join(td, "grp.txt")
             plink.write_groups(gfile, grp)
-        dump = run_driver("soda", ref_args(opts, prefix, gfile))
+        env = None
+        if "dshift" in kw:
+            blk, frac = kw["dshift"]
+            d0 = run_driver("soda", ref_args(opts, prefix, gfile))
+            shift = float(np.round(frac * float(np.ravel(d0["xxd"])[blk]), 3))
+            opts = dict(opts, dshift_block=blk, dshift=shift)
+            env = {"GOLDEN_DSHIFT_BLOCK": str(blk), "GOLDEN_DSHIFT": repr(shift)}
+        dump = run_driver("soda", ref_args({k: v for k, v in opts.items() if not k.startswith("dshift")},
+                                            prefix, gfile), env)
     rec = {f"out_{k}": v for k, v in dump.items()}
     rec.update(bed=bed, why=y, options=json.dumps(opts), nind=nind, nloci=nloci)
     if grp is not None:
