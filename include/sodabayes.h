@@ -97,9 +97,16 @@ int sb_rng_get(sb_handle *h, uint64_t state[2]);
  * xxproduct (soda.f90:520-556), power-method bound d = lambda + 0.001
  * (soda.f90:378-401, start vectors drawn from the replay stream), and the
  * FP64 Cholesky of dI - X_s'X_s (soda.f90:8-47).  xxd: nblocks values.
- * Returns SB_ERR_NOT_PD with *fail_block / *fail_col (1-based) set when a
- * block is not positive definite (the reference then switches to pivoted
- * Cholesky, soda.f90:492-510). */
+ * A block whose dI - X_s'X_s is not positive definite takes the reference's
+ * fallback (soda.f90:492-510): pivoted Cholesky (soda.f90:49-124) on the
+ * host, a dense augmenting block, and the SNPs beyond its rank skipped in
+ * the block sweep and updated one by one after it (SB_Q_SKIPPED,
+ * SB_Q_NAIND).  Returns SB_ERR_NOT_PD when the pivoted factorisation fails
+ * too (the reference stops there), or, with *fail_block / *fail_col
+ * (1-based) set, for any failing block under SB_PREC_MIXED: the fallback
+ * runs on the FP64 path only.  The environment variable
+ * SODABAYES_TEST_D_SHIFT="<block>:<amount>" (0-based block) lowers that
+ * block's d before the factorisation, to exercise the fallback in tests. */
 int sb_build_aug(sb_handle *h, double *xxd, int32_t *fail_block, int32_t *fail_col);
 
 /* ---- MCMC (replaces soda_bayes, models.f90:9-605) ---- */
@@ -137,8 +144,11 @@ enum {
     SB_Q_GRAM,           /* standardised Gram of block `chain` (n_b x n_b), recomputed */
     SB_Q_SNPINDIST,      /* last iteration's SNPs per (group, class) (ngroup*nmix) */
     SB_Q_VARINDIST,      /* last iteration's sum of g^2 per (group, class) (ngroup*nmix) */
-    SB_Q_KTIMES          /* per-kernel timing of the runs made with SODABAYES_KERNEL_TIMING=1:
+    SB_Q_KTIMES,         /* per-kernel timing of the runs made with SODABAYES_KERNEL_TIMING=1:
                             SB_KT_N x (summed ms, launches, algorithmic bytes), slot order below */
+    SB_Q_SKIPPED,        /* SNPs skipped by the pivoted-Cholesky fallback, 1-based ids in update
+                            order (i_skipped, soda.f90:505-507) (nskipped) */
+    SB_Q_NAIND           /* naind, nskipped (2) (soda.f90:440, 505, 514) */
 };
 
 /* Kernel timing slots of SB_Q_KTIMES (diagnostics; no counterpart in the

@@ -660,6 +660,7 @@ int orc_run(const orc_opts *o, const uint8_t *bed, const double *why, const int3
     put_d(f, "xxd", S.xxd, S.nblocks);
     put_d(f, "XA", S.XA, (int64_t)S.naind_alloc * p);
     put_i(f, "skip_j", S.skip_j, p);
+    if (S.nskipped > 0) put_i(f, "i_skipped", S.i_skipped, S.nskipped);
 
     /* ------------------------------------------- soda_bayes init 56-128 -- */
     const double nnind = (double)n;

@@ -222,6 +222,8 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->d_pos_by_id, sizeof(int32_t) * 2 * p);
     A((void **)&h->xxd, sizeof(double) * d.nblocks);
     A((void **)&h->U, sizeof(double) * (size_t)h->tri_off[d.nblocks]);
+    A((void **)&h->skip, (size_t)p + 64);
+    h->dense_off.assign(d.nblocks, -1);
     A((void **)&h->d_tri_off, sizeof(int64_t) * (d.nblocks + 1));
     A((void **)&h->d_ls, sizeof(int32_t) * (d.nblocks + 1));
     A((void **)&h->r, sizeof(double) * ldr * C);
@@ -348,6 +350,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         hipMemsetAsync(h->dl_sw, 0, sizeof(double) * rows16 * 16 * ntc, st);
     }
     hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
+    hipMemsetAsync(h->skip, 0, (size_t)p + 64, st);
     rc = setup_pack(h, bed, d.bed_on_device != 0);
     if (!rc && h->geno_t) rc = batch_transpose(h);
     if (rc) {
@@ -496,6 +499,16 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
         std::memset(out, 0, sizeof(double) * (size_t)P.blocksize * p);
         for (int b = 0; b < P.nblocks; b++) {
             int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+            if (b < (int)h->dense_off.size() && h->dense_off[b] >= 0) {  // pivoted-Cholesky block
+                std::vector<double> D((size_t)nb * nb);
+                if (hip_check(hipMemcpy(D.data(), h->Ud + h->dense_off[b], sizeof(double) * D.size(),
+                                        hipMemcpyDeviceToHost),
+                              "get dense XA"))
+                    return SB_ERR_HIP;
+                for (int j = 0; j < nb; j++)
+                    for (int i = 0; i < nb; i++) out[(int64_t)(p0 + j) * P.blocksize + i] = D[(size_t)j * nb + i];
+                continue;
+            }
             for (int j = 0; j < nb; j++)
                 for (int i = 0; i <= j; i++)
                     out[(int64_t)(p0 + j) * P.blocksize + i] = U[h->tri_off[b] + (int64_t)j * (j + 1) / 2 + i];
@@ -518,6 +531,16 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
         }
         return SB_OK;
     }
+    case SB_Q_SKIPPED: {
+        if (need(h->nskipped)) return SB_ERR_ARG;
+        for (int k = 0; k < h->nskipped; k++) out[k] = h->permvec[h->skip_list[k]] + 1;
+        return SB_OK;
+    }
+    case SB_Q_NAIND:
+        if (need(2)) return SB_ERR_ARG;
+        out[0] = P.naind;
+        out[1] = h->nskipped;
+        return SB_OK;
     default: return fail(SB_ERR_ARG, "sb_get: unknown quantity");
     }
 }

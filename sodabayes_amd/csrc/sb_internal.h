@@ -141,6 +141,14 @@ struct sb_handle {
     double *xxd = nullptr;        // nblocks
     double *U = nullptr;          // packed upper triangles
     int64_t *d_tri_off = nullptr;
+    // pivoted-Cholesky fallback (blocks whose dI - G failed the factorisation)
+    double *Ud = nullptr;          // dense n_b x n_b XA blocks of the fallback, column = block position
+    std::vector<int64_t> dense_off; // per block: offset in Ud, or -1 (packed triangle in U)
+    uint8_t *skip = nullptr;       // per position: 1 = skipped in the block sweep (skip_j)
+    std::vector<int32_t> skip_list; // skipped positions in the reference's i_skipped order
+    int32_t *d_skip_list = nullptr;
+    double *xpx = nullptr;         // per skipped SNP: z'z
+    int nskipped = 0;
     int32_t *d_ls = nullptr;
     // chain state (nchains copies)
     double *r = nullptr;          // ldw*16 per chain

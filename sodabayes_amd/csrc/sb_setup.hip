@@ -8,6 +8,7 @@
 //   k_pack_tri    upper factors -> packed column-major triangles
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -676,6 +677,178 @@ int setup_gram_block(sb_handle *h, int block, double *G_host)
     return rc;
 }
 
+// --------------------------------------------------------- PSD fallback --
+
+// choleskyPD (soda.f90:8-47) on the host, row by row as the reference runs
+// it: on a failing pivot, rows 1..i-1 of U are complete and the rest zero;
+// with a non-positive diagonal entry U is left untouched (returns -1).
+// Column-major, 0-based: A(r, c) = A[c * n + r].
+static int host_chol_pd(int n, const double *A, double *U)
+{
+    for (int i = 0; i < n; i++)
+        if (A[(int64_t)i * n + i] <= 0.0) return -1;
+    std::fill(U, U + (int64_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        double dot = 0.0;
+        for (int k = 0; k < i; k++) dot += U[(int64_t)i * n + k] * U[(int64_t)i * n + k];
+        const double tmp = A[(int64_t)i * n + i] - dot;
+        if (tmp <= 0.0 || std::isnan(tmp)) return i + 1;
+        U[(int64_t)i * n + i] = std::sqrt(tmp);
+        for (int j = i + 1; j < n; j++) {
+            double d2 = 0.0;
+            for (int k = 0; k < i; k++) d2 += U[(int64_t)j * n + k] * U[(int64_t)i * n + k];
+            U[(int64_t)j * n + i] = (A[(int64_t)i * n + j] - d2) / U[(int64_t)i * n + i];
+        }
+    }
+    return 0;
+}
+
+// choleskyPSD (soda.f90:49-124): complete-pivoting Cholesky in place with
+// the reference helpers' operation order (IDMAX takes the first maximum;
+// the transposed DGEMV sums rows in order; DSCAL multiplies by 1/Ajj).
+// piv is 1-based; returns 0, or -1 when the largest diagonal is <= eps.
+static int host_chol_psd(int n, double *A, int *piv, int *rank)
+{
+    auto a = [&](int r, int c) -> double & { return A[(int64_t)c * n + r]; };
+    std::vector<double> work(2 * (size_t)n, 0.0);
+    for (int q = 0; q < n; q++) piv[q] = q + 1;
+    int pvt = 0;
+    double best = a(0, 0);
+    for (int i = 1; i < n; i++)
+        if (a(i, i) > best) {
+            pvt = i;
+            best = a(i, i);
+        }
+    double Ajj = a(pvt, pvt);
+    if (Ajj <= DEPS) {
+        *rank = 0;
+        return -1;
+    }
+    const double DSTOP = n * DEPS * Ajj;
+    for (int j = 0; j < n; j++) {
+        for (int q = j; q < n; q++) {
+            if (j > 0) work[q] = work[q] + a(j - 1, q) * a(j - 1, q);
+            work[n + q] = a(q, q) - work[q];
+        }
+        if (j > 0) {
+            int it = 0;
+            double r = work[n + j];
+            for (int i = 1; i < n - j; i++)
+                if (work[n + j + i] > r) {
+                    it = i;
+                    r = work[n + j + i];
+                }
+            pvt = it + j;
+            Ajj = work[n + pvt];
+            if (Ajj <= DSTOP) {
+                a(j, j) = Ajj;
+                *rank = j;
+                return 0;
+            }
+        }
+        if (j != pvt) {
+            a(pvt, pvt) = a(j, j);
+            for (int i = 0; i < j; i++) std::swap(a(i, j), a(i, pvt));
+            for (int i = 0; i < pvt - j - 1; i++) std::swap(a(j, j + 1 + i), a(j + 1 + i, pvt));
+            for (int i = 0; i < n - pvt - 1; i++) std::swap(a(j, pvt + 1 + i), a(pvt, pvt + 1 + i));
+            std::swap(work[j], work[pvt]);
+            std::swap(piv[pvt], piv[j]);
+        }
+        Ajj = std::sqrt(Ajj);
+        a(j, j) = Ajj;
+        if (j < n - 1) {
+            if (j > 0)
+                for (int c = 0; c < n - j - 1; c++) {
+                    double temp = 0.0;
+                    for (int i = 0; i < j; i++) temp = temp + a(i, j + 1 + c) * a(i, j);
+                    a(j, j + 1 + c) = a(j, j + 1 + c) + (-1.0) * temp;
+                }
+            const double da = 1.0 / Ajj;
+            for (int c = 0; c < n - j - 1; c++) a(j, j + 1 + c) = da * a(j, j + 1 + c);
+        }
+    }
+    *rank = n;
+    return 0;
+}
+
+// Blocks whose dI - G is not positive definite take the reference's fallback
+// (get_aug_matrix, soda.f90:490-510) on the host: the Gram is recomputed on
+// the GPU and downloaded, choleskyPD runs (its partial factor is what the
+// reference leaves in XA), then choleskyPSD; rank columns of the pivoted
+// factor go to their block positions, the rest are skipped.  XA of such a
+// block is dense (positions are permuted by the pivoting), stored in Ud.
+static int psd_fallback(sb_handle *h, GramBuf &gb, const std::vector<int> &failed, const std::vector<double> &xxd)
+{
+    const Par &P = h->par;
+    int64_t dense = 0;
+    for (int b : failed) {
+        const int64_t nbb = h->ls[b + 1] - h->ls[b];
+        h->dense_off[b] = dense;
+        dense += nbb * nbb;
+    }
+    int rc = dev_alloc(h, (void **)&h->Ud, sizeof(double) * (size_t)dense);
+    if (rc) return rc;
+    // the handle's stream does not synchronise with the null stream: every
+    // copy below goes through it
+    std::vector<uint8_t> hskip(P.p, 0);
+    for (int b : failed) {
+        const int p0 = h->ls[b], n = h->ls[b + 1] - p0;
+        rc = launch_gram(h, gb, freq_pos(h), b);
+        std::vector<double> M((size_t)n * n), U((size_t)n * n, 0.0);
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(M.data(), gb.G + gb.hoff[b], sizeof(double) * M.size(),
+                                          hipMemcpyDeviceToHost, h->stream),
+                           "download gram");
+        if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "download gram");
+        if (rc) return rc;
+        for (auto &v : M) v = -v;
+        for (int j = 0; j < n; j++) M[(size_t)j * n + j] = xxd[b] + M[(size_t)j * n + j];
+        host_chol_pd(n, M.data(), U.data());
+        std::vector<int> piv(n);
+        int rank = 0;
+        if (host_chol_psd(n, M.data(), piv.data(), &rank) != 0)
+            return fail(SB_ERR_NOT_PD, "D - X'X is not positive semidefinite in block " + std::to_string(b + 1) +
+                                           " (soda.f90:496-500)");
+        for (int j = 0; j < n; j++) {
+            const int pos = piv[j] - 1;
+            if (j < rank) {
+                for (int r = 0; r <= j; r++) U[(size_t)pos * n + r] = M[(size_t)j * n + r];
+            } else {
+                hskip[p0 + pos] = 1;
+                h->skip_list.push_back(p0 + pos);
+            }
+        }
+        rc = hip_check(hipMemcpyAsync(h->Ud + h->dense_off[b], U.data(), sizeof(double) * U.size(),
+                                      hipMemcpyHostToDevice, h->stream),
+                       "upload fallback factor");
+        if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "upload fallback factor");
+        if (rc) return rc;
+    }
+    h->nskipped = (int)h->skip_list.size();
+    rc = hip_check(hipMemcpyAsync(h->skip, hskip.data(), P.p, hipMemcpyHostToDevice, h->stream), "upload skip flags");
+    if (!rc && h->nskipped) {
+        rc = dev_alloc(h, (void **)&h->d_skip_list, sizeof(int32_t) * h->nskipped);
+        if (!rc) rc = dev_alloc(h, (void **)&h->xpx, sizeof(double) * h->nskipped);
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(h->d_skip_list, h->skip_list.data(), sizeof(int32_t) * h->nskipped,
+                                          hipMemcpyHostToDevice, h->stream),
+                           "upload skipped list");
+    }
+    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "fallback");
+    return rc;
+}
+
+// Test hook (no counterpart in the reference): SODABAYES_TEST_D_SHIFT =
+// "<block>:<amount>" lowers that block's d before the factorisation, so that
+// dI - G is indefinite there and the fallback above runs.
+static bool test_dshift(int nblocks, int &block, double &amount)
+{
+    const char *e = std::getenv("SODABAYES_TEST_D_SHIFT");
+    if (!e || !*e) return false;
+    if (std::sscanf(e, "%d:%lf", &block, &amount) != 2 || block < 0 || block >= nblocks) return false;
+    return true;
+}
+
 int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t *fail_col)
 {
     auto t0 = std::chrono::steady_clock::now();
@@ -699,6 +872,22 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
                      active);
     };
     progress("gram done", 0, nb);
+    // fallback state of an earlier call on this handle
+    h->skip_list.clear();
+    h->nskipped = 0;
+    h->dense_off.assign(nb, -1);
+    hipMemsetAsync(h->skip, 0, (size_t)P.p, h->stream);
+    int shift_block = -1;
+    double shift_amt = 0.0;
+    const bool dshift = test_dshift(nb, shift_block, shift_amt);
+    auto apply_dshift = [&]() {
+        double v = 0.0;
+        hipMemcpyAsync(&v, h->xxd + shift_block, sizeof v, hipMemcpyDeviceToHost, h->stream);
+        hipStreamSynchronize(h->stream);
+        v = v - shift_amt;
+        hipMemcpyAsync(h->xxd + shift_block, &v, sizeof v, hipMemcpyHostToDevice, h->stream);
+        hipStreamSynchronize(h->stream);
+    };
     // power method
     double *y = nullptr, *v = nullptr;
     int32_t *active = nullptr, *iters = nullptr, *dfail = nullptr;
@@ -795,6 +984,7 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
         }
     };
     power_loop(maxit);
+    if (dshift) apply_dshift();
     // Cholesky of dI - G (blocks with dfail != 0 are skipped)
     const int T = (P.blocksize + TILE - 1) / TILE;
     auto factor = [&]() {
@@ -829,6 +1019,7 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
             k_start_counter<<<(P.p + 255) / 256, 256, 0, h->stream>>>(P.seed, P.p, y);
             k_pm_normalize<<<nb, 256, 0, h->stream>>>(h->d_ls, y, v);
             power_loop(1000LL * P.blocksize);
+            if (dshift && mask[shift_block]) apply_dshift();
             std::vector<int32_t> skip(nb);
             for (int b = 0; b < nb; b++) skip[b] = mask[b] ? 0 : -1;
             hipMemcpyAsync(dfail, skip.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, h->stream);
@@ -840,14 +1031,35 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     k_pack_tri<<<dim3(P.blocksize, nb), 256, 0, h->stream>>>(gb.G, gb.goff, h->d_ls, h->d_tri_off, h->U);
     progress("cholesky done", 0, 0);
     rc = hip_check(hipGetLastError(), "cholesky");
-    if (!rc && h->Ut0) rc = batch_tiles(h);
     std::vector<int32_t> hfail(nb), hit(nb);
+    std::vector<double> hxxd(nb);
     if (!rc) {
         hipMemcpyAsync(hfail.data(), dfail, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
         hipMemcpyAsync(hit.data(), iters, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
-        if (xxd_out) hipMemcpyAsync(xxd_out, h->xxd, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream);
+        hipMemcpyAsync(hxxd.data(), h->xxd, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream);
         rc = hip_check(hipStreamSynchronize(h->stream), "build_aug");
     }
+    std::vector<int> failed;
+    for (int b = 0; b < nb; b++)
+        if (hfail[b] != 0) failed.push_back(b);
+    if (fail_block) *fail_block = 0;
+    if (fail_col) *fail_col = 0;
+    if (!rc && !failed.empty()) {
+        if (P.precision == SB_PREC_MIXED) {
+            // the batched kernels keep triangular factor tiles and no skip lists
+            const int b = failed[0];
+            if (fail_block) *fail_block = b + 1;
+            if (fail_col) *fail_col = hfail[b];
+            rc = fail(SB_ERR_NOT_PD, "D - X'X is not positive definite in block " + std::to_string(b + 1) +
+                                         " at column " + std::to_string(hfail[b]) +
+                                         "; the pivoted-Cholesky fallback runs on the FP64 path only");
+        } else {
+            progress("pivoted-Cholesky fallback", 0, (int)failed.size());
+            rc = psd_fallback(h, gb, failed, hxxd);
+        }
+    }
+    if (!rc && h->Ut0) rc = batch_tiles(h);
+    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "build_aug");
     hipFree(blist);
     hipFree(y);
     hipFree(v);
@@ -856,15 +1068,20 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     hipFree(dfail);
     gram_free(gb);
     if (rc) return rc;
-    if (fail_block) *fail_block = 0;
-    if (fail_col) *fail_col = 0;
-    for (int b = 0; b < nb; b++)
+    if (xxd_out) std::copy(hxxd.begin(), hxxd.end(), xxd_out);
+    // naind = blocksize - naless (soda.f90:440, 460, 511-514): naless is 0
+    // once a block factorises; otherwise the least blocksize - n_b + nskipped
+    int naless = P.p, nsk = 0;
+    for (int b = 0; b < nb; b++) {
+        const int p0 = h->ls[b], nbb = h->ls[b + 1] - p0;
         if (hfail[b] != 0) {
-            if (fail_block) *fail_block = b + 1;
-            if (fail_col) *fail_col = hfail[b];
-            return fail(SB_ERR_NOT_PD, "D - X'X is not positive definite in block " + std::to_string(b + 1) +
-                                           " at column " + std::to_string(hfail[b]));
+            for (int q : h->skip_list) nsk += (q >= p0 && q < p0 + nbb);
+        } else {
+            naless = 0;
         }
+        naless = std::min(naless, P.blocksize - nbb + nsk);
+    }
+    h->par.naind = P.blocksize - naless;
     h->pm_iters_max = 0;
     for (int b = 0; b < nb; b++) h->pm_iters_max = std::max(h->pm_iters_max, (double)hit[b]);
     h->aug_built = true;

@@ -93,11 +93,13 @@ __global__ void k_mu_post(Par P, double *r, const Scal *scal)
 // r for 2048 individuals in registers (32 per lane) and streams `spw` SNP
 // columns (one 8-byte load per lane per SNP, 512 B per wave), reducing each
 // product across the wave.  Workgroups [nxg, ...): A part, one wave per
-// column of the block's packed triangle.
+// column of the block's packed triangle (Ud == nullptr) or of its dense
+// n_b x n_b factor (blocks that took the pivoted-Cholesky fallback).
 __global__ __launch_bounds__(NT) void k_rhs(Par P, const uint32_t *__restrict__ geno,
                                             const double *__restrict__ r,
                                             const double *__restrict__ ra,
-                                            const double *__restrict__ U, int64_t tri0, int p0, int nb,
+                                            const double *__restrict__ U, int64_t tri0,
+                                            const double *__restrict__ Ud, int p0, int nb,
                                             int spw, int ngs, int nxg, double *__restrict__ part,
                                             double *__restrict__ partA, double *__restrict__ partR)
 {
@@ -152,9 +154,11 @@ __global__ __launch_bounds__(NT) void k_rhs(Par P, const uint32_t *__restrict__ 
     } else {
         const int jk = (t - nxg) * 4 + wave;
         if (jk >= nb) return;
-        const double *col = U + tri0 + (int64_t)jk * (jk + 1) / 2;
+        // packed triangle, or (pivoted-Cholesky blocks) a dense n_b column
+        const double *col = Ud ? Ud + (int64_t)jk * nb : U + tri0 + (int64_t)jk * (jk + 1) / 2;
+        const int len = Ud ? nb : jk + 1;
         double acc = 0.0;
-        for (int i = lane; i <= jk; i += 64) acc = fma(col[i], ra[i], acc);
+        for (int i = lane; i < len; i += 64) acc = fma(col[i], ra[i], acc);
         acc = wave_sum(acc);
         if (lane == 0) partA[jk] = acc;
     }
@@ -184,11 +188,15 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
                                                      const int32_t *grp, const int32_t *snp_of_pos,
                                                      int p0, int nb, double zz, int rep, double *beta,
                                                      int32_t *cls, double *vara_s, double *delta,
-                                                     uint8_t *chg)
+                                                     uint8_t *chg, const uint8_t *skip)
 {
     int jk = blockIdx.x * NT + threadIdx.x;
     if (jk >= nb) return;
     int pos = p0 + jk;
+    if (skip[pos]) {  // skip_j (models.f90:352): updated after the block loop
+        chg[pos] = 0;
+        return;
+    }
     double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
     PhiloxSrc src(chain_key(P, 0), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
     draw_snp<PhiloxSrc, NM>(P, sc, src, rhs, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta, chg);
@@ -204,11 +212,16 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
                                                     const int32_t *grp, int p0, int nb, double zz,
                                                     double *rhs_s, double *stemp_s, double *uu,
                                                     int32_t *flag_s, double *beta, int32_t *cls,
-                                                    double *vara_s, double *delta, uint8_t *chg)
+                                                    double *vara_s, double *delta, uint8_t *chg,
+                                                    const uint8_t *skip)
 {
     const double zz_vare = zz / sc->vare;
     const bool mix = P.method != SB_BAYES_A && P.method != SB_BAYES_B;
     for (int jk = threadIdx.x; jk < nb; jk += NT) {
+        if (skip[p0 + jk]) {  // skip_j: no draw here, no stream-0 uniforms
+            chg[p0 + jk] = 0;
+            continue;
+        }
         double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
         rhs_s[jk] = rhs;
         if (mix) {
@@ -224,6 +237,7 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
         XoroSrc src{{sc->rng[0], sc->rng[1]}};
         if (mix) {
             for (int jk = 0; jk < nb; jk++) {
+                if (skip[p0 + jk]) continue;
                 const double *st = stemp_s + (int64_t)jk * P.nmix;
                 double u = src.next(), ss = 0.0;
                 int flag = 1;
@@ -243,6 +257,7 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
         } else {
             for (int jk = 0; jk < nb; jk++) {
                 int pos = p0 + jk;
+                if (skip[pos]) continue;
                 draw_snp(P, sc, src, rhs_s[jk], zz, zz_vare, pos, 0, beta, cls, vara_s, delta, chg);
             }
         }
@@ -253,6 +268,7 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
     if (!mix) return;
     const double vare = sc->vare;
     for (int jk = threadIdx.x; jk < nb; jk += NT) {
+        if (skip[p0 + jk]) continue;
         int pos = p0 + jk, flag = flag_s[jk], cold = cls[pos];
         double gk = beta[pos], gn = 0.0;
         if (flag > 1) {
@@ -331,7 +347,8 @@ __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const do
 constexpr int UW = 16;   // waves per update workgroup
 constexpr int UNR = 8;   // list entries in flight per wave
 __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__restrict__ geno,
-                                                      const double *__restrict__ U, int64_t tri0, int p0,
+                                                      const double *__restrict__ U, int64_t tri0,
+                                                      const double *__restrict__ Ud, int p0,
                                                       int nb, int nrw, const int *lst, const double *dl,
                                                       const double *cl, const int *count,
                                                       const double *kconst, double *r, double *ra)
@@ -396,7 +413,10 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
                     if (q < m) {
                         const int jk = lst[q];
                         dv[u] = dl[q];
-                        if (i <= jk) uv[u] = U[tri0 + (int64_t)jk * (jk + 1) / 2 + i];
+                        if (Ud)
+                            uv[u] = Ud[(int64_t)jk * nb + i];
+                        else if (i <= jk)
+                            uv[u] = U[tri0 + (int64_t)jk * (jk + 1) / 2 + i];
                     }
                 }
 #pragma unroll
@@ -690,6 +710,95 @@ __global__ void k_init_r(Par P, const double *why, double mu, double *r)
     if (i < P.n) r[(int64_t)blockIdx.z * P.ldr + i] = (why[i] - 0.0) - mu;
 }
 
+// -------------------------------------------------------- skipped SNPs --
+
+// z'z of every skipped SNP (xpx = dot_product(X(:,j), X(:,j)), models.f90:
+// 100-104), z from the position's standardisation table; one workgroup per
+// skipped SNP.
+__global__ __launch_bounds__(NT) void k_xpx(Par P, const uint32_t *__restrict__ geno,
+                                            const double *__restrict__ zlut, const int32_t *list, double *xpx)
+{
+    __shared__ double lds[NT / WAVE];
+    const int pos = list[blockIdx.x];
+    const double *zl = zlut + 4 * (int64_t)pos;
+    const uint32_t *gw = geno + (int64_t)pos * P.ldw;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < P.n; i += NT) {
+        const double z = zl[(gw[i >> 4] >> (2 * (i & 15))) & 3u];
+        s = fma(z, z, s);
+    }
+    s = block_sum<NT>(s, lds);
+    if (threadIdx.x == 0) xpx[blockIdx.x] = s;
+}
+
+// SNPs skipped by the pivoted-Cholesky fallback, after the block loop in
+// i_skipped order (models.f90:174-190 BayesA, 276-320 BayesB, 433-495
+// BayesC/Cpi/R/Rc): rhs = z'r with zz = z'z and no augmented part, the
+// method's draw, then r += z (g_old - g_new) at once, so each SNP sees the
+// previous one's update.  One workgroup: the products are block reductions
+// over the individuals and the draw runs on thread 0 (stream 0 in REPLAY
+// mode, the SNP's own Philox counter in COUNTER mode).  The reference reads
+// log_p / vare_gp in the BayesB and mixture loops without setting them
+// there; the values of the block sweep are used.
+constexpr int SKT = 1024;
+template <int NM>
+__global__ __launch_bounds__(SKT) void k_skipped(Par P, Scal *sc, const uint32_t *__restrict__ geno,
+                                                 const double *__restrict__ zlut, const int32_t *list, int nsk,
+                                                 const double *xpx, const int32_t *grp,
+                                                 const int32_t *snp_of_pos, int rep, double *r, double *beta,
+                                                 int32_t *cls, double *vara_s, double *delta, uint8_t *chg)
+{
+    __shared__ double lds[SKT / WAVE];
+    __shared__ double dsh;
+    const bool replay = P.rng_mode == SB_RNG_REPLAY;
+    XoroSrc xs{{sc->rng[0], sc->rng[1]}};
+    for (int k = 0; k < nsk; k++) {
+        const int pos = list[k];
+        const double *zl = zlut + 4 * (int64_t)pos;
+        const double z0 = zl[0], z1 = zl[1], z2 = zl[2];
+        const uint32_t *gw = geno + (int64_t)pos * P.ldw;
+        double s = 0.0;
+        for (int w = threadIdx.x; w * 16 < P.n; w += SKT) {
+            const uint32_t g = gw[w];
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int i = w * 16 + e;
+                const uint32_t x = (g >> (2 * e)) & 3u;
+                if (i < P.n) s = fma(x == 0 ? z0 : (x == 1 ? z1 : z2), r[i], s);
+            }
+        }
+        s = block_sum<SKT>(s, lds);
+        if (threadIdx.x == 0) {
+            const double zz = xpx[k];
+            if (replay) {
+                draw_snp<XoroSrc, NM>(P, sc, xs, s, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta,
+                                      chg);
+            } else {
+                PhiloxSrc src(chain_key(P, 0), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
+                draw_snp<PhiloxSrc, NM>(P, sc, src, s, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s,
+                                        delta, chg);
+            }
+            dsh = delta[pos];
+        }
+        __syncthreads();
+        const double d = dsh;
+        for (int w = threadIdx.x; w * 16 < P.n; w += SKT) {
+            const uint32_t g = gw[w];
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int i = w * 16 + e;
+                const uint32_t x = (g >> (2 * e)) & 3u;
+                if (i < P.n) r[i] = r[i] + (x == 0 ? z0 : (x == 1 ? z1 : z2)) * d;
+            }
+        }
+        __syncthreads();
+    }
+    if (replay && threadIdx.x == 0) {
+        sc->rng[0] = xs.st.s1;
+        sc->rng[1] = xs.st.s2;
+    }
+}
+
 }  // namespace
 
 // X_s'r + U_s'r_a of block position jk for the FP64 path (diagnostics)
@@ -701,6 +810,12 @@ __global__ void k_rhs_out(Par P, const double *part, const double *partA, const 
 }
 
 // ------------------------------------------------------------- host side --
+
+// dense factor of block b (pivoted-Cholesky fallback), or nullptr
+static const double *dense_block(const sb_handle *h, int b)
+{
+    return (b < (int)h->dense_off.size() && h->dense_off[b] >= 0) ? h->Ud + h->dense_off[b] : nullptr;
+}
 
 int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
 {
@@ -801,6 +916,8 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
     std::vector<double> vs(p * Cs, vara);
     hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p * Cs, hipMemcpyHostToDevice, h->stream);
     k_init_r<<<dim3((n + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->why, sc.mu, h->r);
+    if (h->nskipped > 0)
+        k_xpx<<<h->nskipped, NT, 0, h->stream>>>(P, h->geno, h->zlut, h->d_skip_list, h->xpx);
     h->rstat_nwg = 0;
     k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal, yty);
     if (P.rng_mode == SB_RNG_REPLAY)
@@ -833,8 +950,8 @@ int block_rhs(sb_handle *h, int b, double *out_host)
         int spw = (int)std::max<int64_t>(2, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 2048));
         int ngs = (nb + 4 * spw - 1) / (4 * spw);
         int nxg = ngs * P.nchunk;
-        k_rhs<<<nxg + (nb + 3) / 4, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, h->tri_off[b], p0, nb, spw, ngs,
-                                                 nxg, h->sc.part, h->sc.partA, h->sc.partR);
+        k_rhs<<<nxg + (nb + 3) / 4, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, h->tri_off[b], dense_block(h, b), p0,
+                                                 nb, spw, ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
         k_rhs_out<<<(nb + NT - 1) / NT, NT, 0, st>>>(P, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, p0,
                                                      nb, dout);
     }
@@ -894,8 +1011,8 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 if (rc) return rc;
             } else {
                 kc.begin(st);
-                k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, p0, nb, spw, ngs, nxg,
-                                                h->sc.part, h->sc.partA, h->sc.partR);
+                k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, dense_block(h, b), p0, nb, spw,
+                                                ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
                 kc.end(st, KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
             }
             if (mixed) {
@@ -907,12 +1024,13 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                     k_draw_replay<<<1, NT, 0, st>>>(P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2,
                                                     h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp,
                                                     h->sc.uu, h->sc.chg, h->beta, h->cls, h->vara_s, delta,
-                                                    chg);
+                                                    chg, h->skip);
                 else
                     with_nmix(P.nmix, [&](auto nm) {
                         k_draw_counter<decltype(nm)::value><<<(nb + NT - 1) / NT, NT, 0, st>>>(
                             P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
-                            h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg);
+                            h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg,
+                            h->skip);
                     });
                 kc.end(st, KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
             }
@@ -927,11 +1045,20 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                                                      h->u_cl, h->u_cnt, h->u_kc);
                 kc.end(st, KS_LIST, 25.0 * nb);
                 kc.begin(st);
-                k_update<<<dim3(nrw + nra, 1, C), UW * WAVE, 0, st>>>(P, h->geno, h->U, tri0, p0, nb, nrw,
+                k_update<<<dim3(nrw + nra, 1, C), UW * WAVE, 0, st>>>(P, h->geno, h->U, tri0, dense_block(h, b),
+                                                                      p0, nb, nrw,
                                                                       h->u_lst, h->u_dl, h->u_cl, h->u_cnt,
                                                                       h->u_kc, h->r, h->ra);
                 kc.end(st, KS_UPDATE, 16.0 * P.n + 16.0 * nb);
             }
+        }
+        if (h->nskipped > 0) {
+            // the FP64 path only: setup refuses the fallback for the mixed path
+            with_nmix(P.nmix, [&](auto nm) {
+                k_skipped<decltype(nm)::value><<<1, SKT, 0, st>>>(P, h->scal, h->geno, h->zlut, h->d_skip_list,
+                                                                  h->nskipped, h->xpx, h->grp, h->d_pos_by_id + P.p,
+                                                                  rep, h->r, h->beta, h->cls, h->vara_s, delta, chg);
+            });
         }
         k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);
         k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);

@@ -18,7 +18,7 @@ module soda_gpu
         SB_Q_INDISTSTORE = 9, SB_Q_VARA_SSTORE = 10, SB_Q_STORES = 11, &
         SB_Q_SNPSTORE = 12, SB_Q_VARSTORE = 13, SB_Q_PSTORE = 14, SB_Q_XA = 15, &
         SB_Q_XXD = 16, SB_Q_GRAM = 17, SB_Q_SNPINDIST = 18, SB_Q_VARINDIST = 19, &
-        SB_Q_KTIMES = 20
+        SB_Q_KTIMES = 20, SB_Q_SKIPPED = 21, SB_Q_NAIND = 22
 
     type, bind(C) :: sb_dims
         integer(c_int32_t) :: method, nind, nloci, nmix, ngroup

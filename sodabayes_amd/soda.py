@@ -230,12 +230,18 @@ class SodaSampler:
         xxd = np.zeros(self.nblocks)
         fb, fc = C.c_int32(0), C.c_int32(0)
         rc = L.sb_build_aug(h, xxd.ctypes.data_as(C.POINTER(C.c_double)), C.byref(fb), C.byref(fc))
-        if rc == _lib.SB_ERR_NOT_PD:
-            raise _lib.SbError(rc, f"D-XX is not positive definite (block {fb.value}, column {fc.value}); "
-                                   "the pivoted-Cholesky fallback of soda.f90:492-510 is not on the GPU path")
         _lib.check(rc)
         self.xxd = xxd
+        na = self.get("NAIND", 2)
+        self.naind, self.nskipped = int(na[0]), int(na[1])
         return xxd
+
+    def skipped(self) -> np.ndarray:
+        """1-based ids of the SNPs the pivoted-Cholesky fallback skipped, in
+        the order of their updates (i_skipped, soda.f90:505-507)."""
+        if not self.nskipped:
+            return np.zeros(0, dtype=np.int64)
+        return self.get("SKIPPED", self.nskipped).astype(np.int64)
 
     # -- soda_bayes --------------------------------------------------------
     def init(self) -> None:
@@ -308,7 +314,7 @@ class SodaSampler:
         sc = get("SCALARS", 4)
         out = {
             "nblocks": self.nblocks, "blocksize": self.blocksize, "blockt": self.blockt,
-            "naind": self.blocksize, "freq": self.freq, "permvec": self.permvec + 1,
+            "naind": self.naind, "nskipped": self.nskipped, "freq": self.freq, "permvec": self.permvec + 1,
             "xxd": self.get("XXD", self.nblocks),
             "mu": sc[0], "vare": sc[1], "vara": sc[2],
             "mu_store": st[1] / cnt, "vare_store": st[2] / cnt,

@@ -6,6 +6,9 @@ chains: every discrete quantity (classes, counts, mixture proportions) equal
 to the reference after the full fixture run, continuous state within 1e-9
 relative to its largest entry.
 """
+import contextlib
+import os
+
 import numpy as np
 import pytest
 
@@ -26,6 +29,30 @@ def make_sampler(c, **extra):
     s = soda.SodaSampler(o, int(c["nind"]), int(c["nloci"]), c["why"], bed=c["bed"],
                          grp=None if grp is None else [str(g) for g in grp])
     return s
+
+
+@contextlib.contextmanager
+def dshift_env(c):
+    """Cases whose fixture lowered one block's d (to run the reference's
+    pivoted-Cholesky fallback) set the library's test hook the same way."""
+    o = c["opts"]
+    if "dshift_block" not in o:
+        yield
+        return
+    old = os.environ.get("SODABAYES_TEST_D_SHIFT")
+    os.environ["SODABAYES_TEST_D_SHIFT"] = f"{o['dshift_block']}:{o['dshift']!r}"
+    try:
+        yield
+    finally:
+        if old is None:
+            del os.environ["SODABAYES_TEST_D_SHIFT"]
+        else:
+            os.environ["SODABAYES_TEST_D_SHIFT"] = old
+
+
+def setup(s, c):
+    with dshift_env(c):
+        return s.setup()
 
 
 @pytest.mark.parametrize("case", ["bayesr", "ragged", "nblocks", "rblocks", "ldblock"])
@@ -49,18 +76,26 @@ def test_gram_bit_exact(case):
 def test_setup_matches_reference(case):
     c = load_case(case)
     s = make_sampler(c)
-    xxd = s.setup()
+    xxd = setup(s, c)
     gold = c["out"]
     assert rel_err(xxd, gold["xxd"]) < 1e-12
     bs, p = s.blocksize, int(c["nloci"])
     XA = s.get("XA", bs * p).reshape(bs, p, order="F")
     XAg = gold["XA"].reshape(bs, p, order="F")
     assert np.linalg.norm(XA - XAg) / np.linalg.norm(XAg) < 1e-10
-    # invariant U'U + G = dI per block
+    # skip lists of the pivoted-Cholesky fallback (soda.f90:501-514)
+    assert s.naind == int(gold["naind"][0])
+    assert s.nskipped == int(gold["nskipped"][0])
+    want = gold["i_skipped"].astype(np.int64) if "i_skipped" in gold else np.zeros(0, dtype=np.int64)
+    assert np.array_equal(s.skipped(), want)
+    skipped = set(int(v) - 1 for v in s.skipped())
+    # invariant U'U + G = dI per block (blocks without skipped SNPs)
     n = int(c["nind"])
     f = orc.freq(c["bed"], n)
     for b, (a, e) in enumerate(soda.block_bounds(p, bs, s.nblocks, s.blockt)):
         nb = e - a
+        if any(int(s.permvec[k]) in skipped for k in range(a, e)):
+            continue
         U = XA[:nb, a:e]
         G = orc.gram(c["bed"], n, list(s.permvec[a:e]), f)
         R = U.T @ U + G - xxd[b] * np.eye(nb)
@@ -72,7 +107,7 @@ def test_setup_matches_reference(case):
 def test_replay_chain_matches_reference(case):
     c = load_case(case)
     s = make_sampler(c)
-    s.setup()
+    setup(s, c)
     s.init()
     hyp = s.run()
     res = s.results()
@@ -87,7 +122,7 @@ def test_replay_chain_matches_reference(case):
     s.close()
 
 
-@pytest.mark.parametrize("case", ["bayesr", "bayesb"])
+@pytest.mark.parametrize("case", ["bayesr", "bayesb", "psd_bayesa"])
 def test_replay_matches_oracle_trace(case):
     """Per-iteration mu / vara / vare of the replayed chain against the
     oracle's full-precision trace."""
@@ -99,7 +134,7 @@ def test_replay_matches_oracle_trace(case):
     ref = orc.run(oracle_options(c["opts"], int(c["nind"]), int(c["nloci"]), ngroup),
                   c["bed"], c["why"], grp)
     s = make_sampler(c)
-    s.setup()
+    setup(s, c)
     s.init()
     hyp = s.run()
     assert rel_err(hyp[:, 2], ref["hyp_mu"]) < 1e-11
