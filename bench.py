@@ -35,11 +35,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # kernel timing slots (sodabayes_amd._lib.KT_SLOTS) -> kernel names
 KERNEL_NAMES = {
     "rhs": "k_rhs (X_s'r + U_s'r_a, FP64)", "draw": "k_draw_counter", "list": "k_list",
-    "update": "k_update (r += Z delta, r_a += U delta)", "rquant": "k_quant (+ k_rstat per iteration; r limbs)",
-    "xq": "k_prod (X_s'Q int8 MFMA | U_s'R_a FP64 MFMA)", "ut0": "k_utile<0> (U_s'R_a, FP64 MFMA)",
+    "update": "k_update (r += Z delta, r_a += U delta)", "rquant": "k_rstat + k_quant (r limbs, first block)",
+    "xq": "k_prod (X_s'Q int8 MFMA | U_s'R_a FP32 U, FP64 MFMA)", "ut0": "k_utile<0> (U_s'R_a, FP64 MFMA)",
     "drawb": "k_draw_b (limb sums + draw)", "dquant": "k_dquant (delta limbs)",
-    "xupd": "k_upd (R += X_s C int8 MFMA | U_s D FP64 MFMA)", "ut1": "k_utile<1> (U_s D, FP64 MFMA)",
-    "rared": "k_ra_reduce",
+    "xupd": "k_upd (R += X_s C int8 MFMA | U_s D FP32 U, FP64 MFMA)", "ut1": "k_utile<1> (U_s D, FP64 MFMA)",
+    "rared": "k_rared_quant (r_a reduction + next block's r limbs)",
 }
 
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2, the factor a 1 GiB
@@ -238,7 +238,7 @@ def main():
         "metric": "SNP-effect updates/s (BayesR n=50k,p=500k): mixed+batched vs FP64 GPU, same acc.",
         "value": mixed["value"], "unit": "SNP-effect updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": mixed["ms"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int8 MFMA limbs (int32 sums) + f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "int8 MFMA limbs (int32 sums) + f64 (U stored f32)", "data": "synthetic",
         "config": {"workload": f"SODA BayesR n={n} p={p} n_s={bs}, 4-class mixture, {a.chains} chains "
                                "batched, counter-based RNG, mixed-precision products",
                    "nind": n, "nloci": p, "blocksize": bs, "chains": a.chains, "limbs": 3,

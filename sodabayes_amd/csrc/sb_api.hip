@@ -286,8 +286,8 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         A((void **)&h->b_kc, sizeof(double) * C);
         A((void **)&h->b_dmax, sizeof(uint64_t) * C * nwg_d);
         A((void **)&h->b_dk, sizeof(double) * C * nwg_d);
-        A((void **)&h->ra_sw, sizeof(double) * rows16 * 16 * ntc);
-        A((void **)&h->dl_sw, sizeof(double) * rows16 * 16 * ntc);
+        A((void **)&h->ra_sw, sizeof(float) * rows16 * 16 * ntc);
+        A((void **)&h->dl_sw, sizeof(float) * rows16 * 16 * ntc);
         h->gt_off.assign(d.nblocks + 1, 0);
         h->gs_off.assign(d.nblocks + 1, 0);
         h->tt_off.assign(d.nblocks + 1, 0);
@@ -299,8 +299,8 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         }
         A((void **)&h->geno_t, (size_t)16 * h->gt_off[d.nblocks]);
         A((void **)&h->geno_sw, (size_t)16 * h->gs_off[d.nblocks]);
-        A((void **)&h->Ut0, sizeof(double) * 256 * (size_t)h->tt_off[d.nblocks]);
-        A((void **)&h->Ut1, sizeof(double) * 256 * (size_t)h->tt_off[d.nblocks]);
+        A((void **)&h->Ut0, sizeof(float) * 256 * (size_t)h->tt_off[d.nblocks]);
+        A((void **)&h->Ut1, sizeof(float) * 256 * (size_t)h->tt_off[d.nblocks]);
     }
     if (rc) {
         sb_destroy(h);
@@ -346,8 +346,8 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         hipMemsetAsync(h->b_Q, 0, (size_t)16 * h->b_ntile * h->b_ldq, st);
         hipMemsetAsync(h->b_Dq, 0, (size_t)16 * h->b_ntile * h->b_ldd, st);
         const int64_t rows16 = 16 * ((bs + 15) / 16), ntc = (C + 15) / 16;
-        hipMemsetAsync(h->ra_sw, 0, sizeof(double) * rows16 * 16 * ntc, st);
-        hipMemsetAsync(h->dl_sw, 0, sizeof(double) * rows16 * 16 * ntc, st);
+        hipMemsetAsync(h->ra_sw, 0, sizeof(float) * rows16 * 16 * ntc, st);
+        hipMemsetAsync(h->dl_sw, 0, sizeof(float) * rows16 * 16 * ntc, st);
     }
     hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
     hipMemsetAsync(h->skip, 0, (size_t)p + 64, st);

@@ -118,13 +118,13 @@ __device__ __forceinline__ int fixed_exponent(unsigned long long m, int L)
 // for lane = l16 + 16 g, the 16 slots (order of unpack_word) of row
 // 16 t + l16 at word 16 S + 4 g + s.  Every workgroup reduces the nwg
 // partials (max |r|, sum r) of its chain; workgroup 0 stores e, sum r.
-__global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const double *r, const double *rsum,
-                                              const uint64_t *rmax, int nwg, uint4 *Q, int *expo, double *rtot)
+__device__ __forceinline__ void quant_wg(const Par &P, int L, int ntile, const double *r, const double *rsum,
+                                         const uint64_t *rmax, int nwg, uint4 *Q, int *expo, double *rtot,
+                                         int bx, int c)
 {
     __shared__ int e_s;
     __shared__ double sl[NT / WAVE];
     __shared__ unsigned long long ml[NT / WAVE];
-    const int c = blockIdx.z;
     unsigned long long m = 0;
     double sr = 0.0;
     for (int w0 = 0; w0 < nwg; w0 += 4 * NT) {
@@ -159,14 +159,14 @@ __global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const dou
             m = ml[w] > m ? ml[w] : m;
         }
         e_s = fixed_exponent(m, L);
-        if (blockIdx.x == 0) {
+        if (bx == 0) {
             expo[c] = e_s;
             rtot[c] = sr;
         }
     }
     __syncthreads();
     const int e = e_s;
-    const int w = blockIdx.x * NT + threadIdx.x;
+    const int w = bx * NT + threadIdx.x;
     if (w >= P.ldw) return;
     const double *rc = r + (int64_t)c * P.ldr + (int64_t)w * 16;
     int32_t q[16];
@@ -192,6 +192,12 @@ __global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const dou
         Q[(((int64_t)S * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
             make_uint4(out[0], out[1], out[2], out[3]);
     }
+}
+
+__global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const double *r, const double *rsum,
+                                              const uint64_t *rmax, int nwg, uint4 *Q, int *expo, double *rtot)
+{
+    quant_wg(P, L, ntile, r, rsum, rmax, nwg, Q, expo, rtot, blockIdx.x, blockIdx.z);
 }
 
 // Tp[chunk][col][jk] = sum over the chunk's individuals of x_(p0+jk, i) * Q[col][i]
@@ -261,17 +267,19 @@ __device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ 
 }
 
 // Products with a block's upper factor U on FP64 MFMA (v_mfma_f64_16x16x4f64),
-// U held as 16 x 16 tiles (I <= J at tile J(J+1)/2 + I) in the operand order
-// of the mode, so that a wave reads a tile as two contiguous 1 KB loads:
+// U held in FP32 as 16 x 16 tiles (I <= J at tile J(J+1)/2 + I) in the operand
+// order of the mode, so that a wave reads a tile as one contiguous 1 KB load
+// (the operands are widened to FP64 in registers; sums are FP64):
 //   mode 0 (rhs):    part[g][c][jk] = sum_{row tiles I of group g, I <= J} U(i, jk) R(i, c)
 //   mode 1 (update): part[g][c][i]  = sum_{col tiles J of group g, J >= I} U(i, j) R(j, c)
-// R (r_a for mode 0, delta for mode 1) is a swizzled panel (rsw_index).  A
-// wave owns one 16-row/column output tile; a workgroup four of them over one
-// group of `spant` tiles; waves with no tile of the group on their side of
-// the diagonal write nothing.
+// R (r_a for mode 0, delta for mode 1) is an FP32 swizzled panel
+// (rsw_index).  A wave owns one 16-row/column output tile; a workgroup four
+// of them over one group of `spant` tiles; waves with no tile of the group on
+// their side of the diagonal write nothing.
+constexpr int UKU = 4;  // U tiles in flight per wave
 template <int MODE, int NTC>
-__device__ __forceinline__ void ut_tile(const Par &P, const double2 *__restrict__ Ut, int nbt, int nb,
-                                        const double2 *__restrict__ Rsw, int spant, double *__restrict__ part,
+__device__ __forceinline__ void ut_tile(const Par &P, const float4 *__restrict__ Ut, int nbt, int nb,
+                                        const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
                                         int bx, int by)
 {
     const int C = P.nchains;
@@ -290,31 +298,25 @@ __device__ __forceinline__ void ut_tile(const Par &P, const double2 *__restrict_
     v4d acc[NTC];
 #pragma unroll
     for (int t = 0; t < NTC; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int K = K0; K < K1; K += 2) {
-        double2 a[2][2], r[2][NTC][2];
+    for (int K = K0; K < K1; K += UKU) {
+        float4 a[UKU], r[UKU][NTC];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
+        for (int u = 0; u < UKU; u++) {
             const int KK = min(K + u, K1 - 1);
             const int I = MODE == 0 ? KK : O, J = MODE == 0 ? O : KK;
-            const double2 *tp = Ut + ((int64_t)J * (J + 1) / 2 + I) * 128 + lane;
-            a[u][0] = tp[0];
-            a[u][1] = tp[64];
-            if (K + u >= K1) a[u][0] = a[u][1] = make_double2(0.0, 0.0);
+            a[u] = Ut[((int64_t)J * (J + 1) / 2 + I) * 64 + lane];
+            if (K + u >= K1) a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-            for (int t = 0; t < NTC; t++) {
-                const double2 *rp = Rsw + ((int64_t)KK * NTC + t) * 128 + lane;
-                r[u][t][0] = rp[0];
-                r[u][t][1] = rp[64];
-            }
+            for (int t = 0; t < NTC; t++) r[u][t] = Rsw[((int64_t)KK * NTC + t) * 64 + lane];
         }
 #pragma unroll
-        for (int u = 0; u < 2; u++)
+        for (int u = 0; u < UKU; u++)
 #pragma unroll
             for (int t = 0; t < NTC; t++) {
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][0].x, r[u][t][0].x, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][0].y, r[u][t][0].y, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][1].x, r[u][t][1].x, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][1].y, r[u][t][1].y, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].x, (double)r[u][t].x, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].y, (double)r[u][t].y, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].z, (double)r[u][t].z, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].w, (double)r[u][t].w, acc[t], 0, 0, 0);
             }
     }
     // D[m][n] of the f64 16x16x4 MFMA: m = lg + 4*v (rows interleaved across
@@ -328,12 +330,12 @@ __device__ __forceinline__ void ut_tile(const Par &P, const double2 *__restrict_
         }
 }
 
-// U tiles of every block from the packed triangles: tile (I, J), element
-// th = (h * 64 + lane) * 2 + s2 (s = 2 h + s2, lane = l16 + 16 lg) holds
+// FP32 U tiles of every block from the packed FP64 triangles: tile (I, J),
+// element th = lane * 4 + s (lane = l16 + 16 lg) holds
 //   Ut0: U(16 I + 4 lg + s, 16 J + l16)     Ut1: U(16 I + l16, 16 J + 4 lg + s)
 // (zero below the diagonal and beyond the block).
 __global__ __launch_bounds__(256) void k_pack_tiles(const double *U, const int64_t *tri_off, const int32_t *ls,
-                                                    const int64_t *tt_off, double *Ut0, double *Ut1)
+                                                    const int64_t *tt_off, float *Ut0, float *Ut1)
 {
     const int b = blockIdx.y, nb = ls[b + 1] - ls[b], nbt = (nb + 15) / 16;
     const int64_t T = blockIdx.x;
@@ -342,23 +344,23 @@ __global__ __launch_bounds__(256) void k_pack_tiles(const double *U, const int64
     while ((int64_t)(J + 1) * (J + 2) / 2 <= T) J++;
     while ((int64_t)J * (J + 1) / 2 > T) J--;
     const int I = (int)(T - (int64_t)J * (J + 1) / 2);
-    const int th = threadIdx.x, s = 2 * (th >> 7) + (th & 1), lane = (th >> 1) & 63;
+    const int th = threadIdx.x, s = th & 3, lane = th >> 2;
     const int l16 = lane & 15, lg = lane >> 4;
     const double *Ub = U + tri_off[b];
     auto u = [&](int i, int j) -> double {
         return (i <= j && j < nb) ? Ub[(int64_t)j * (j + 1) / 2 + i] : 0.0;
     };
     const int64_t o = (tt_off[b] + T) * 256 + th;
-    Ut0[o] = u(16 * I + 4 * lg + s, 16 * J + l16);
-    Ut1[o] = u(16 * I + l16, 16 * J + 4 * lg + s);
+    Ut0[o] = (float)u(16 * I + 4 * lg + s, 16 * J + l16);
+    Ut1[o] = (float)u(16 * I + l16, 16 * J + 4 * lg + s);
 }
 
 // r_a(i, c) += sum of the mode-1 partials of the column groups that reach row
 // i's tile (groups left of the diagonal wrote nothing); plain and swizzled.
-__global__ void k_ra_reduce(Par P, const double *part, int spant, int ngroups, int nb, double *ra, double *ra_sw)
+__device__ __forceinline__ void ra_reduce_wg(const Par &P, const double *part, int spant, int ngroups, int nb,
+                                             double *ra, float *ra_sw, int bx, int c)
 {
-    const int i = blockIdx.x * NT + threadIdx.x;
-    const int c = blockIdx.y;
+    const int i = bx * NT + threadIdx.x;
     if (i >= nb) return;
     const int C = P.nchains;
     const int64_t gstride = (int64_t)C * P.blocksize;
@@ -373,7 +375,28 @@ __global__ void k_ra_reduce(Par P, const double *part, int spant, int ngroups, i
     }
     const double v = ra[(int64_t)c * P.blocksize + i] + s;
     ra[(int64_t)c * P.blocksize + i] = v;
-    ra_sw[rsw_index(i, c, (C + 15) / 16)] = v;
+    ra_sw[rsw_index(i, c, (C + 15) / 16)] = (float)v;
+}
+
+__global__ __launch_bounds__(NT) void k_ra_reduce(Par P, const double *part, int spant, int ngroups, int nb,
+                                                  double *ra, float *ra_sw)
+{
+    ra_reduce_wg(P, part, spant, ngroups, nb, ra, ra_sw, blockIdx.x, blockIdx.y);
+}
+
+// The r_a reduction of this block beside the r limbs of the next block (both
+// need only the update just done): workgroups [0, nq * C) quantise r,
+// the rest reduce r_a.
+__global__ __launch_bounds__(NT) void k_rared_quant(Par P, const double *part, int spant, int ngroups, int nb,
+                                                    double *ra, float *ra_sw, int nra, int L, int ntile,
+                                                    const double *r, const double *rsum, const uint64_t *rmax,
+                                                    int nwg, uint4 *Q, int *expo, double *rtot, int nq)
+{
+    const int id = blockIdx.x, C = P.nchains;
+    if (id < nq * C)
+        quant_wg(P, L, ntile, r, rsum, rmax, nwg, Q, expo, rtot, id % nq, id / nq);
+    else
+        ra_reduce_wg(P, part, spant, ngroups, nb, ra, ra_sw, (id - nq * C) % nra, (id - nq * C) / nra);
 }
 
 // SNP-major genotype tiles of one block for k_xq: element ((tile * nS + S) *
@@ -420,19 +443,21 @@ __global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, 
     }
 }
 
-// One thread per (16-SNP word, chain): c_j = delta_j / s_j quantised to L
-// balanced int8 limbs (slot order of unpack_word), written as B-operand
-// blocks like Q: Dq[((ss * NTILE + t) * 4 + s) * 64 + l16 + 16 g] for limb row
-// 16 t + l16 and SNP word 16 ss + 4 g + s (zero beyond the block).  The
-// exponent comes from the per-workgroup maxima of |c_j| that k_draw_b left;
-// the thread of word 0 stores it and K_c = sum_j 2 f_j c_j.
+// One thread per (block position jk, chain): c_j = delta_j / s_j quantised to
+// L balanced int8 limbs, written as B-operand blocks like Q:
+// Dq[((ss * NTILE + t) * 4 + s) * 64 + l16 + 16 g] for limb row 16 t + l16 and
+// SNP word sw = 16 ss + 4 g + s (zero beyond the block).  The 16 positions
+// of a word sit in 16 consecutive lanes; dword m of the word's operand holds
+// the digits of positions m, m+4, m+8, m+12 (slot order of unpack_word), so
+// lane m of the group gathers them by shuffles.  The exponent comes from the
+// per-workgroup maxima of |c_j| that k_draw_b left; position 0 stores it and
+// K_c = sum_j 2 f_j c_j.
 __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const double *delta, const double *sdv,
-                                               int p0, int nb, int nsw, const uint64_t *dmax, const double *dk,
+                                               int p0, int nb, const uint64_t *dmax, const double *dk,
                                                int nwg_d, uint4 *Dq, int *expo, double *kconst)
 {
     const int c = blockIdx.y;
-    const int sw = blockIdx.x * NT + threadIdx.x;
-    if (sw >= nsw) return;
+    const int jk = blockIdx.x * NT + threadIdx.x;   // the grid covers whole words
     unsigned long long m = 0;
     double kc = 0.0;
     for (int w0 = 0; w0 < nwg_d; w0 += 8) {
@@ -441,7 +466,7 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             v[u] = w0 + u < nwg_d ? dmax[(int64_t)c * nwg_d + w0 + u] : 0ull;
-            kv[u] = (sw == 0 && w0 + u < nwg_d) ? dk[(int64_t)c * nwg_d + w0 + u] : 0.0;
+            kv[u] = (jk == 0 && w0 + u < nwg_d) ? dk[(int64_t)c * nwg_d + w0 + u] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -450,40 +475,30 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
         }
     }
     const int e = fixed_exponent(m, L);
-    if (sw == 0) {
+    if (jk == 0) {
         expo[c] = e;
         kconst[c] = kc;
     }
-    const double *dc = delta + (int64_t)c * P.p + p0;
-    int32_t q[16];
-#pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const int j = 16 * sw + t;
-        double cj = 0.0;
-        if (j < nb) {
-            const double s = sdv[p0 + j];
-            cj = s > 0.0 ? dc[j] / s : 0.0;
-        }
-        q[slot_of(t)] = (int32_t)rint(ldexp(cj, e));
+    double cj = 0.0;
+    if (jk < nb) {
+        const double s = sdv[p0 + jk];
+        cj = s > 0.0 ? delta[(int64_t)c * P.p + p0 + jk] / s : 0.0;
     }
+    int32_t q = (int32_t)rint(ldexp(cj, e));
+    const int t = jk & 15, sw = jk >> 4;
     const int ss = sw >> 4, g = (sw >> 2) & 3, s = sw & 3;
+    const int src = (threadIdx.x & (WAVE - 1) & ~15) + (t & 3);
     for (int k = 0; k < L; k++) {
-        uint32_t out[4];
-#pragma unroll
-        for (int m4 = 0; m4 < 4; m4++) {
-            uint32_t v = 0;
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int8_t d = (int8_t)(q[4 * m4 + b] & 0xFF);
-                v |= ((uint32_t)(uint8_t)d) << (8 * b);
-            }
-            out[m4] = v;
+        const int32_t d = (int32_t)(int8_t)(q & 0xFF);   // balanced digit in [-128, 127]
+        q = (q - d) >> 8;
+        const uint32_t byte = (uint32_t)(uint8_t)d;
+        const uint32_t v = __shfl(byte, src, WAVE) | (__shfl(byte, src + 4, WAVE) << 8) |
+                           (__shfl(byte, src + 8, WAVE) << 16) | (__shfl(byte, src + 12, WAVE) << 24);
+        if (t < 4) {
+            const int row = c * L + k;
+            reinterpret_cast<uint32_t *>(Dq + (((int64_t)ss * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) +
+                                         16 * g)[t] = v;
         }
-#pragma unroll
-        for (int t = 0; t < 16; t++) q[t] = (q[t] - (int32_t)(int8_t)(q[t] & 0xFF)) >> 8;
-        const int row = c * L + k;
-        Dq[(((int64_t)ss * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
-            make_uint4(out[0], out[1], out[2], out[3]);
     }
 }
 
@@ -596,8 +611,8 @@ __device__ __forceinline__ void split_ids(int id, int n0, int n1, int &kind, int
 template <int NTILE, int IT, int NTC>
 __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
                                              const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp,
-                                             int64_t tstride, int g0x, int n0, const double2 *__restrict__ Ut,
-                                             const double2 *__restrict__ Rsw, int spant, double *__restrict__ part,
+                                             int64_t tstride, int g0x, int n0, const float4 *__restrict__ Ut,
+                                             const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
                                              int g1x, int n1)
 {
     int kind, local;
@@ -615,8 +630,8 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
                                             const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
                                             const double *__restrict__ kconst, double *__restrict__ r,
                                             double *__restrict__ rsum, uint64_t *__restrict__ rmax, int n0,
-                                            const double2 *__restrict__ Ut, int nbt, int nb,
-                                            const double2 *__restrict__ Dsw, int spant, double *__restrict__ part,
+                                            const float4 *__restrict__ Ut, int nbt, int nb,
+                                            const float4 *__restrict__ Dsw, int spant, double *__restrict__ part,
                                             int g1x, int n1)
 {
     int kind, local;
@@ -639,7 +654,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
                                                const int32_t *snp_of_pos, int p0, int nb, double zz,
                                                int rep, double *beta, int32_t *cls, double *vara_s,
                                                double *delta, uint8_t *chg, double *rhs_out, uint64_t *dmax,
-                                               double *dk, double *dl_sw)
+                                               double *dk, float *dl_sw)
 {
     __shared__ double lds[NT / WAVE];
     __shared__ unsigned long long mx[NT / WAVE];
@@ -649,22 +664,25 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
     const int C = P.nchains;
     double cj = 0.0, kc = 0.0;
     if (jk < nb) {
-        // chunk partials and r_a partials: 8 independent loads in flight per
-        // round (runtime-bounded loops would wait on each load in turn)
+        // chunk partials and r_a partials: 8 loads in flight per round.  The
+        // loads are unconditional (clamped index) and masked afterwards: a
+        // guarded load compiles to a branch and a wait per load.
         const int *tcol = Tp + (int64_t)c * L * P.blocksize + jk;
         long long t0 = 0, t1 = 0, t2 = 0;
         for (int ch0 = 0; ch0 < nch; ch0 += 8) {
             int v[8][3];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
+            for (int u = 0; u < 8; u++) {
+                const int64_t ch = min(ch0 + u, nch - 1);
 #pragma unroll
-                for (int k = 0; k < 3; k++)
-                    v[u][k] = ch0 + u < nch ? tcol[(int64_t)(ch0 + u) * tstride + (int64_t)k * P.blocksize] : 0;
+                for (int k = 0; k < 3; k++) v[u][k] = tcol[ch * tstride + (int64_t)k * P.blocksize];
+            }
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                t0 += v[u][0];
-                t1 += v[u][1];
-                t2 += v[u][2];
+                const int keep = ch0 + u < nch;
+                t0 += keep ? v[u][0] : 0;
+                t1 += keep ? v[u][1] : 0;
+                t2 += keep ? v[u][2] : 0;
             }
         }
         const long long S = (t2 * 256 + t1) * 256 + t0;
@@ -679,9 +697,9 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
         for (int g0 = 0; g0 < ng; g0 += 8) {
             double v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = g0 + u < ng ? rp[(g0 + u) * gstride] : 0.0;
+            for (int u = 0; u < 8; u++) v[u] = rp[(int64_t)min(g0 + u, ng - 1) * gstride];
 #pragma unroll
-            for (int u = 0; u < 8; u++) rhs += v[u];
+            for (int u = 0; u < 8; u++) rhs += g0 + u < ng ? v[u] : 0.0;
         }
         if (rhs_out) {  // diagnostics: the products only, no draw
             rhs_out[(int64_t)c * nb + jk] = rhs;
@@ -692,7 +710,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
             draw_snp<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, pos, grp[pos], beta + o, cls + o,
                                     vara_s + o, delta + o, chg + o);
             const double dlt = delta[o + pos];
-            dl_sw[rsw_index(jk, c, (C + 15) / 16)] = dlt;
+            dl_sw[rsw_index(jk, c, (C + 15) / 16)] = (float)dlt;
             cj = s > 0.0 ? dlt / s : 0.0;
             kc = f2[pos] * cj;
         }
@@ -731,31 +749,35 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
     KClock &kc = h->kc;
     const double n = P.n, cl = (double)C * L, gbytes = (double)nb * ((n + 3.0) / 4.0);
-    const double tbytes = 8.0 * 256.0 * nbt * (nbt + 1) / 2.0;   // U tiles of the block
-    kc.begin(st);
-    if (h->rstat_nwg == 0) {   // r changed since the last batched update
-        k_rstat<<<dim3(nwg_r, 1, C), NT, 0, st>>>(P, h->r, h->b_rsum, h->b_rmax);
-        h->rstat_nwg = nwg_r;
+    const double tbytes = 4.0 * 256.0 * nbt * (nbt + 1) / 2.0;   // FP32 U tiles of the block
+    if (h->q_block != b || h->rstat_nwg == 0) {
+        // r changed outside the batched update (first block of a sweep)
+        kc.begin(st);
+        if (h->rstat_nwg == 0) {
+            k_rstat<<<dim3(nwg_r, 1, C), NT, 0, st>>>(P, h->r, h->b_rsum, h->b_rmax);
+            h->rstat_nwg = nwg_r;
+        }
+        k_quant<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax,
+                                                                 h->rstat_nwg, Q, h->b_expo, h->b_rtot);
+        kc.end(st, KS_RQUANT, 16.0 * C * n + cl * n);
     }
-    k_quant<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax,
-                                                             h->rstat_nwg, Q, h->b_expo, h->b_rtot);
-    kc.end(st, KS_RQUANT, 16.0 * C * n + cl * n);
+    h->q_block = -1;
     const int kss = h->kss, nS = P.ldw / 16;
     const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
     const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
     const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
-    const double2 *ut = reinterpret_cast<const double2 *>(h->Ut0) + h->tt_off[b] * 128;
+    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut0) + h->tt_off[b] * 64;
     kc.begin(st);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
         const int g0x = (nbt + IT - 1) / IT, n0 = g0x * ((nS + kss - 1) / kss);
         const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
         k_prod<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, tstride, g0x, n0, ut,
-                                                     reinterpret_cast<const double2 *>(h->ra_sw), spant,
+                                                     reinterpret_cast<const float4 *>(h->ra_sw), spant,
                                                      h->b_rapart, g1x, n1);
     });
     if (rc) return fail(rc, "unsupported chain count");
-    kc.end(st, KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 16.0 * C * nb);
+    kc.end(st, KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 12.0 * C * nb);
     return hip_check(hipGetLastError(), "batch products");
 }
 
@@ -769,14 +791,14 @@ int batch_update(sb_handle *h, int b, const double *delta)
     uint4 *Dq = reinterpret_cast<uint4 *>(h->b_Dq);
     KClock &kc = h->kc;
     const double n = P.n, cl = (double)C * L, gbytes = (double)nb * ((n + 3.0) / 4.0);
-    const double tbytes = 8.0 * 256.0 * nbt * (nbt + 1) / 2.0;
+    const double tbytes = 4.0 * 256.0 * nbt * (nbt + 1) / 2.0;
     kc.begin(st);
-    k_dquant<<<dim3((nsw + NT - 1) / NT, C), NT, 0, st>>>(P, L, h->b_ntile, delta, h->sdv, p0, nb, nsw, h->b_dmax,
-                                                           h->b_dk, nwg_d, Dq, h->b_expo2, h->b_kc);
+    k_dquant<<<dim3(nsw * 16 / NT, C), NT, 0, st>>>(P, L, h->b_ntile, delta, h->sdv, p0, nb, h->b_dmax, h->b_dk,
+                                                     nwg_d, Dq, h->b_expo2, h->b_kc);
     kc.end(st, KS_DQUANT, 16.0 * C * nb + cl * nb);
     const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
-    const double2 *ut = reinterpret_cast<const double2 *>(h->Ut1) + h->tt_off[b] * 128;
+    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut1) + h->tt_off[b] * 64;
     kc.begin(st);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
@@ -784,15 +806,27 @@ int batch_update(sb_handle *h, int b, const double *delta)
         const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
         k_upd<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r, h->b_rsum,
                                                     h->b_rmax, n0, ut, nbt,
-                                                    nb, reinterpret_cast<const double2 *>(h->dl_sw), spant,
+                                                    nb, reinterpret_cast<const float4 *>(h->dl_sw), spant,
                                                     h->b_uapart, g1x, n1);
     });
     if (rc) return fail(rc, "unsupported chain count");
     h->rstat_nwg = (int)(P.ldr / (16 * rows_tiles(h->b_ntile)));  // partials of the new r for the next block
-    kc.end(st, KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 16.0 * C * nb);
+    kc.end(st, KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
+    const int nra = (nb + NT - 1) / NT;
     kc.begin(st);
-    k_ra_reduce<<<dim3((nb + NT - 1) / NT, C), NT, 0, st>>>(P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
-    kc.end(st, KS_RARED, 8.0 * C * nb * (ncg + 3.0));
+    if (b + 1 < P.nblocks) {
+        // the next block's r limbs in the same launch
+        const int nq = (P.ldw + NT - 1) / NT;
+        uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
+        k_rared_quant<<<(nq + nra) * C, NT, 0, st>>>(P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw, nra, L,
+                                                     h->b_ntile, h->r, h->b_rsum, h->b_rmax, h->rstat_nwg, Q,
+                                                     h->b_expo, h->b_rtot, nq);
+        h->q_block = b + 1;
+        kc.end(st, KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 16.0 * C * n + cl * n);
+    } else {
+        k_ra_reduce<<<dim3(nra, C), NT, 0, st>>>(P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
+        kc.end(st, KS_RARED, 8.0 * C * nb * (ncg + 3.0));
+    }
     return hip_check(hipGetLastError(), "batch update");
 }
 

@@ -19,8 +19,12 @@ __device__ __forceinline__ uint64_t chain_key(const Par &P, int c)
 
 // Class probabilities (models.f90:363-390) for BayesC/Cpi/R/Rc.  NM is a
 // compile-time bound on the number of classes (>= P.nmix), so that the
-// per-class terms stay in registers when it is small.
-template <int NM>
+// per-class terms stay in registers when it is small.  EXACT follows the
+// reference's pairwise form 1 / (1 + sum_j exp(s_j - s_k)) with its +-700
+// cut-offs (nmix^2 exponentials; REPLAY reproduces the reference's doubles);
+// otherwise the same probabilities as a max-shifted softmax (nmix
+// exponentials), for the counter-based sampler.
+template <int NM, bool EXACT = true>
 __device__ inline void mix_probs(const Par &P, const Scal *sc, int g, double rhs, double zz,
                                  double zz_vare, double *stemp)
 {
@@ -34,6 +38,22 @@ __device__ inline void mix_probs(const Par &P, const Scal *sc, int g, double rhs
             double uhat = rhs / (zz + sc->vare_gp[k]);
             s[k] = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[bin_of(P, g, k)];
         }
+    }
+    if (!EXACT) {
+        double mx = s[0];
+#pragma unroll
+        for (int k = 1; k < NM; k++)
+            if (k < P.nmix) mx = fmax(mx, s[k]);
+        double tot = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; k++) {
+            stemp[k] = k < P.nmix ? exp(s[k] - mx) : 0.0;
+            tot += stemp[k];
+        }
+        const double inv = 1.0 / tot;
+#pragma unroll
+        for (int k = 0; k < NM; k++) stemp[k] = stemp[k] * inv;
+        return;
     }
 #pragma unroll
     for (int kk = 0; kk < NM; kk++) {
@@ -118,7 +138,7 @@ __device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double r
         return;
     }
     double stemp[NM];
-    mix_probs<NM>(P, sc, g, rhs, zz, zz_vare, stemp);
+    mix_probs<NM, std::is_same<Src, XoroSrc>::value>(P, sc, g, rhs, zz, zz_vare, stemp);
     double u = src.next(), ss = 0.0;
     int flag = 1;
 #pragma unroll

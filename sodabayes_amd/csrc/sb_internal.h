@@ -174,6 +174,7 @@ struct sb_handle {
     int limbs = 3;                // int8 limbs per residual (8L-2 fixed-point bits)
     int nwg_r = 0;                // workgroups of the residual statistics (k_rstat)
     int rstat_nwg = 0;            // valid partials of (max |r|, sum r) in b_rsum / b_rmax (0: stale)
+    int q_block = -1;             // block whose r limbs the last update already wrote to b_Q (-1: none)
     double *b_rsum = nullptr;     // nchains * nwg_r partial sums of r
     uint64_t *b_rmax = nullptr;   // nchains * nwg_r partial max |r| (bit patterns)
     int8_t *b_Q = nullptr;        // (16 * b_ntile) x ldq int8 limbs, limb-major
@@ -197,11 +198,11 @@ struct sb_handle {
     std::vector<int64_t> gt_off;  // 16-byte element offset of each block in geno_t
     uint32_t *geno_sw = nullptr;  // SNP-major genotypes per block, MFMA operand tiles (product GEMM)
     std::vector<int64_t> gs_off;  // 16-byte element offset of each block in geno_sw
-    double *Ut0 = nullptr;        // U as 16x16 tiles in the operand order of U' R (mode 0)
-    double *Ut1 = nullptr;        // U as 16x16 tiles in the operand order of U D (mode 1)
-    std::vector<int64_t> tt_off;  // tile offset of each block in Ut0 / Ut1 (256 doubles per tile)
-    double *ra_sw = nullptr;      // r_a of every chain as a swizzled panel (rsw_index)
-    double *dl_sw = nullptr;      // delta of the current block, swizzled panel
+    float *Ut0 = nullptr;         // U (FP32) as 16x16 tiles in the operand order of U' R (mode 0)
+    float *Ut1 = nullptr;         // U (FP32) as 16x16 tiles in the operand order of U D (mode 1)
+    std::vector<int64_t> tt_off;  // tile offset of each block in Ut0 / Ut1 (256 values per tile)
+    float *ra_sw = nullptr;       // r_a of every chain as an FP32 swizzled panel (rsw_index)
+    float *dl_sw = nullptr;       // delta of the current block, FP32 swizzled panel
 
     std::vector<void *> allocs;
 };

@@ -582,7 +582,7 @@ __global__ void k_ra_replay(Par P, Scal *sc, double *ra)
 }
 
 // (ra_sw, when given, receives the same values as a swizzled panel)
-__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag, double *ra_sw)
+__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag, float *ra_sw)
 {
     int i = blockIdx.x * NT + threadIdx.x;
     const int c = blockIdx.z;
@@ -590,7 +590,7 @@ __global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag
     PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)i, tag);
     const double v = draw_normal(src, 0.0, sqrt(sc[c].vare));
     ra[(int64_t)c * P.blocksize + i] = v;
-    if (ra_sw) ra_sw[rsw_index(i, c, (P.nchains + 15) / 16)] = v;
+    if (ra_sw) ra_sw[rsw_index(i, c, (P.nchains + 15) / 16)] = (float)v;
 }
 
 // posterior sums per SNP (models.f90:528-544)
