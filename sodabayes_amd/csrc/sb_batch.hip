@@ -266,6 +266,89 @@ __device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ 
     }
 }
 
+// X_s'Q with the limb operand shared through LDS: the four waves split the
+// rows (IT row tiles each; one workgroup = 64 IT SNPs) and walk the chunk's
+// super-steps together; each super-step's limb blocks are fetched once per
+// workgroup by direct global-to-LDS loads, one super-step ahead, into a ring
+// of three buffers.  Chunk partials as in xq_tile.
+constexpr int prod_rows(int ntile) { return ntile <= 4 ? 2 : 1; }   // row tiles per wave (LDS path)
+constexpr bool prod_lds(int ntile) { return ntile <= 6; }
+template <int NTILE, int IT>
+__device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ gsw, int nbt, int nb,
+                                       const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp, int64_t tstride,
+                                       int bx, int by)
+{
+    constexpr int W = NT / WAVE;
+    constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
+    constexpr int SLAB = NTILE * 4, NB = SLAB / W, NBUF = 3;
+    constexpr int RED4 = (ROWS * (COLS + 1) + 3) / 4;
+    constexpr int LDS4 = NBUF * SLAB * 64 > RED4 ? NBUF * SLAB * 64 : RED4;
+    static_assert(SLAB % W == 0, "limb blocks per super-step split evenly over the waves");
+    __shared__ uint4 lds[LDS4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int r0 = bx * ROWS;
+    const int nS = P.ldw / 16;
+    const int S0 = by * kss, S1 = min(nS, S0 + kss);
+    const int tile0 = (bx * W + wave) * IT;
+    bool valid[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) valid[it] = tile0 + it < nbt;
+    const uint4 *abase = gsw + (int64_t)tile0 * nS * 64 + lane;
+    auto slab = [&](int S) {
+        uint4 *dst = lds + (S % NBUF) * SLAB * 64;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const int q = wave + W * j;
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(Q + ((int64_t)S * SLAB + q) * 64 + lane),
+                                             (__attribute__((address_space(3))) void *)(dst + q * 64), 16, 0, 0);
+        }
+    };
+    v4i acc[IT][NTILE];
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
+    slab(S0);
+    for (int S = S0; S < S1; S++) {
+        if (S + 1 < S1) slab(S + 1);
+        uint4 a[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++)
+            a[it] = valid[it] ? abase[((int64_t)it * nS + S) * 64] : make_uint4(0, 0, 0, 0);
+        __syncthreads();   // waits for this wave's loads; then every wave's part of slabs S, S+1 is in LDS
+        const uint4 *B = lds + (S % NBUF) * SLAB * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            uint4 b[NTILE];
+#pragma unroll
+            for (int t = 0; t < NTILE; t++) b[t] = B[(t * 4 + s) * 64];
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const v4i av = unpack_word(word_of(a[it], s));
+#pragma unroll
+                for (int t = 0; t < NTILE; t++)
+                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[t]), acc[it][t], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();   // the ring is reused for the transpose of the partials
+    int(*red)[COLS + 1] = reinterpret_cast<int(*)[COLS + 1]>(lds);
+    // acc[it][t][e]: SNP row 16 (IT wave + it) + 4 g + e, column 16 t + l16
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) red[16 * (IT * wave + it) + 4 * g + e][16 * t + l16] = acc[it][t][e];
+    __syncthreads();
+    int *out = Tp + (int64_t)by * tstride;
+    for (int k = threadIdx.x; k < ROWS * COLS; k += NT) {
+        const int col = k / ROWS, row = k - col * ROWS;
+        if (r0 + row < nb) out[(int64_t)col * P.blocksize + r0 + row] = red[row][col];
+    }
+}
+
 // Products with a block's upper factor U on FP64 MFMA (v_mfma_f64_16x16x4f64),
 // U held in FP32 as 16 x 16 tiles (I <= J at tile J(J+1)/2 + I) in the operand
 // order of the mode, so that a wave reads a tile as one contiguous 1 KB load
@@ -585,6 +668,109 @@ __device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict_
     }
 }
 
+// The same update with the limb operand shared through LDS: the four waves
+// split the rows (IT row tiles each, one workgroup = 64 IT individuals) and
+// walk every super-step together.  Each super-step's limb blocks (NTILE * 4
+// KB) are fetched once per workgroup by direct global-to-LDS loads, one
+// super-step ahead, into a ring of three buffers (a buffer is refilled two
+// steps after it was read, past one barrier); the genotype tiles come
+// straight to registers.  The limb bytes read from L2 drop by the number of
+// waves.  Statistics of the new r go out per 64 rows.
+constexpr int upd_rows(int ntile) { return ntile <= 4 ? 2 : 1; }   // row tiles per wave (LDS path)
+constexpr bool upd_lds(int ntile) { return ntile <= 6; }            // ring of 3 slabs fits in LDS
+template <int NTILE, int IT>
+__device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__ gt, int nss,
+                                         const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
+                                         const double *__restrict__ kconst, double *__restrict__ r,
+                                         double *__restrict__ rsum, uint64_t *__restrict__ rmax, int nwg, int bx)
+{
+    constexpr int W = NT / WAVE;
+    constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
+    constexpr int SLAB = NTILE * 4, NB = SLAB / W, NBUF = 3;
+    constexpr int RED4 = (ROWS * (COLS + 1) + 3) / 4;
+    constexpr int LDS4 = NBUF * SLAB * 64 > RED4 ? NBUF * SLAB * 64 : RED4;
+    static_assert(SLAB % W == 0, "limb blocks per super-step split evenly over the waves");
+    __shared__ uint4 lds[LDS4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int i0 = bx * ROWS;
+    const uint4 *abase = gt + (int64_t)(bx * W + wave) * IT * nss * 64 + lane;
+    auto slab = [&](int ss) {
+        uint4 *dst = lds + (ss % NBUF) * SLAB * 64;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const int q = wave + W * j;
+            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(Dq + ((int64_t)ss * SLAB + q) * 64 + lane),
+                                             (__attribute__((address_space(3))) void *)(dst + q * 64),
+                                             16, 0, 0);
+        }
+    };
+    v4i acc[IT][NTILE];
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
+    slab(0);
+    for (int ss = 0; ss < nss; ss++) {
+        if (ss + 1 < nss) slab(ss + 1);
+        uint4 a[IT];
+#pragma unroll
+        for (int it = 0; it < IT; it++) a[it] = abase[((int64_t)it * nss + ss) * 64];
+        __syncthreads();   // waits for this wave's loads; then every wave's part of slabs ss, ss+1 is in LDS
+        const uint4 *B = lds + (ss % NBUF) * SLAB * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            uint4 b[NTILE];
+#pragma unroll
+            for (int t = 0; t < NTILE; t++) b[t] = B[(t * 4 + s) * 64];
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const v4i av = unpack_word(word_of(a[it], s));
+#pragma unroll
+                for (int t = 0; t < NTILE; t++)
+                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[t]), acc[it][t], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();   // the ring is reused for the row sums
+    int(*red)[COLS + 1] = reinterpret_cast<int(*)[COLS + 1]>(lds);
+    // acc[it][t][e]: individual 16 (IT wave + it) + 4 g + e, column 16 t + l16
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) red[16 * (IT * wave + it) + 4 * g + e][16 * t + l16] = acc[it][t][e];
+    __syncthreads();
+    // r update, then per (64 rows, chain) max |r| and sum r of the new values
+    // (one wave = one segment of one chain) for the next block's quantisation
+    const int C = P.nchains;
+    for (int pr = threadIdx.x; pr < ROWS * C; pr += NT) {
+        const int c = pr / ROWS, li = pr - c * ROWS;
+        const int i = i0 + li;
+        double nv = 0.0;
+        if (i < P.n) {
+            long long S = 0;
+            for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
+            double *rc = r + (int64_t)c * P.ldr;
+            nv = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
+            rc[i] = nv;
+        }
+        double sm = nv;
+        unsigned long long mx = (unsigned long long)__double_as_longlong(fabs(nv));
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            sm += __shfl_xor(sm, off, WAVE);
+            const unsigned long long o = __shfl_xor(mx, off, WAVE);
+            mx = o > mx ? o : mx;
+        }
+        if (lane == 0) {
+            rsum[(int64_t)c * nwg + i / WAVE] = sm;
+            rmax[(int64_t)c * nwg + i / WAVE] = mx;
+        }
+    }
+}
+
 // Workgroup id -> (kind, local id) for two horizontally fused grids of n0
 // and n1 workgroups: interleaved in runs of 8 (one run per XCD, which take
 // workgroups round-robin), then the rest of the larger grid.
@@ -617,10 +803,14 @@ __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gs
 {
     int kind, local;
     split_ids(blockIdx.x, n0, n1, kind, local);
-    if (kind == 0)
-        xq_tile<NTILE, IT>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
-    else
+    if (kind == 0) {
+        if constexpr (prod_lds(NTILE))
+            xq_lds<NTILE, prod_rows(NTILE)>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
+        else
+            xq_tile<NTILE, IT>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
+    } else {
         ut_tile<0, NTC>(P, Ut, nbt, nb, Rsw, spant, part, local % g1x, local / g1x);
+    }
 }
 
 // Updates of one block in one launch: r += X_s C - K (n0 workgroups) beside
@@ -636,10 +826,15 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
 {
     int kind, local;
     split_ids(blockIdx.x, n0, n1, kind, local);
-    if (kind == 0)
-        xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, n0, local);
-    else
+    if (kind == 0) {
+        if constexpr (upd_lds(NTILE))
+            xupd_lds<NTILE, upd_rows(NTILE)>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, (int)(P.ldr / WAVE),
+                                             local);
+        else
+            xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, n0, local);
+    } else {
         ut_tile<1, NTC>(P, Ut, nbt, nb, Dsw, spant, part, local % g1x, local / g1x);
+    }
 }
 
 // One thread per (SNP, chain): exact limb combination of the chunk
@@ -737,6 +932,10 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
 // SNP words of a block rounded up to whole super-steps (16 words)
 int batch_words(int nb) { return ((nb + 255) / 256) * 16; }
 
+// super-steps per X_s'Q chunk: shorter chunks for the LDS path, whose
+// workgroups cover 4x the rows (measured best at 4,096 SNPs x 50,000)
+int batch_kss(int ntile) { return prod_lds(ntile) ? 16 : 32; }
+
 int batch_block(sb_handle *h, int b, double zz, int rep)
 {
     (void)zz;
@@ -770,7 +969,8 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     kc.begin(st);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
-        const int g0x = (nbt + IT - 1) / IT, n0 = g0x * ((nS + kss - 1) / kss);
+        const int rt = prod_lds(NTL) ? 4 * prod_rows(NTL) : IT;   // row tiles per workgroup
+        const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
         const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
         k_prod<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, tstride, g0x, n0, ut,
                                                      reinterpret_cast<const float4 *>(h->ra_sw), spant,
@@ -802,7 +1002,7 @@ int batch_update(sb_handle *h, int b, const double *delta)
     kc.begin(st);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
-        const int n0 = (int)(P.ldr / (16 * IT));
+        const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * upd_rows(NTL) : 16 * IT));
         const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
         k_upd<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r, h->b_rsum,
                                                     h->b_rmax, n0, ut, nbt,
@@ -810,7 +1010,8 @@ int batch_update(sb_handle *h, int b, const double *delta)
                                                     h->b_uapart, g1x, n1);
     });
     if (rc) return fail(rc, "unsupported chain count");
-    h->rstat_nwg = (int)(P.ldr / (16 * rows_tiles(h->b_ntile)));  // partials of the new r for the next block
+    // partials of the new r for the next block: per 64 rows (LDS path) or per workgroup
+    h->rstat_nwg = (int)(P.ldr / (upd_lds(h->b_ntile) ? WAVE : 16 * rows_tiles(h->b_ntile)));
     kc.end(st, KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
     const int nra = (nb + NT - 1) / NT;
     kc.begin(st);

@@ -181,7 +181,7 @@ struct sb_handle {
     int64_t b_ldq = 0;
     int *b_expo = nullptr;        // fixed-point exponent per chain
     double *b_rtot = nullptr;     // sum of r per chain
-    int kss = 32;                 // super-steps (16 genotype words) per MFMA workgroup chunk
+    int kss = 32;                 // super-steps (16 genotype words) per MFMA workgroup chunk (batch_kss)
     int b_ntile = 1;              // 16-column MFMA tiles covering nchains * limbs
     int *b_T = nullptr;           // chunks x blocksize x b_ldt int32 limb products per individual chunk
     int b_ldt = 16;
@@ -229,6 +229,7 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
 int batch_update(sb_handle *h, int b, const double *delta);
 int batch_transpose(sb_handle *h);
 int batch_words(int nb);
+int batch_kss(int ntile);
 int batch_tiles(sb_handle *h);
 int block_rhs(sb_handle *h, int b, double *out_host);
 }  // namespace sb
