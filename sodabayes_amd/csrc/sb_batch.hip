@@ -271,7 +271,7 @@ __device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ 
 // super-steps together; each super-step's limb blocks are fetched once per
 // workgroup by direct global-to-LDS loads, one super-step ahead, into a ring
 // of three buffers.  Chunk partials as in xq_tile.
-constexpr int prod_rows(int ntile) { return ntile <= 4 ? 2 : 1; }   // row tiles per wave (LDS path)
+constexpr int prod_rows(int ntile) { return ntile <= 6 ? 2 : 1; }   // row tiles per wave (LDS path)
 constexpr bool prod_lds(int ntile) { return ntile <= 6; }
 template <int NTILE, int IT>
 __device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ gsw, int nbt, int nb,
@@ -676,7 +676,7 @@ __device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict_
 // steps after it was read, past one barrier); the genotype tiles come
 // straight to registers.  The limb bytes read from L2 drop by the number of
 // waves.  Statistics of the new r go out per 64 rows.
-constexpr int upd_rows(int ntile) { return ntile <= 4 ? 2 : 1; }   // row tiles per wave (LDS path)
+constexpr int upd_rows(int ntile) { return ntile <= 6 ? 2 : 1; }   // row tiles per wave (LDS path)
 constexpr bool upd_lds(int ntile) { return ntile <= 6; }            // ring of 3 slabs fits in LDS
 template <int NTILE, int IT>
 __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__ gt, int nss,
@@ -841,6 +841,8 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
 // partials, rhs, draw (Philox, counter (iteration, SNP id, tag)); delta
 // also into the swizzled panel for U delta; per workgroup, max |delta / s|
 // and sum 2 f delta / s for the update.
+constexpr int DCH = 16;  // chunk partials unrolled in k_draw_b
+constexpr int DRG = 16;  // r_a row-group partials unrolled in k_draw_b
 template <int NM>
 __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int *Tp,
                                                int64_t tstride, int nch, const int *expo, const double *rtot,
@@ -859,55 +861,67 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
     const int C = P.nchains;
     double cj = 0.0, kc = 0.0;
     if (jk < nb) {
-        // chunk partials and r_a partials: 8 loads in flight per round.  The
-        // loads are unconditional (clamped index) and masked afterwards: a
-        // guarded load compiles to a branch and a wait per load.
-        const int *tcol = Tp + (int64_t)c * L * P.blocksize + jk;
-        long long t0 = 0, t1 = 0, t2 = 0;
-        for (int ch0 = 0; ch0 < nch; ch0 += 8) {
-            int v[8][3];
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int64_t ch = min(ch0 + u, nch - 1);
-#pragma unroll
-                for (int k = 0; k < 3; k++) v[u][k] = tcol[ch * tstride + (int64_t)k * P.blocksize];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int keep = ch0 + u < nch;
-                t0 += keep ? v[u][0] : 0;
-                t1 += keep ? v[u][1] : 0;
-                t2 += keep ? v[u][2] : 0;
-            }
-        }
-        const long long S = (t2 * 256 + t1) * 256 + t0;
-        const double sx = ldexp((double)S, -expo[c]);
+        // Every load of the thread is issued before the first use: the SNP
+        // state, the chunk partials (up to DCH chunks unrolled) and the r_a
+        // partials (up to DRG row groups).  Loads are unconditional (clamped
+        // index) and masked afterwards: a guarded load compiles to a branch
+        // and a wait per load.
         const int pos = p0 + jk;
-        const double s = sdv[pos];
-        double rhs = s > 0.0 ? (sx - f2[pos] * rtot[c]) / s : 0.0;
+        const int64_t o = (int64_t)c * P.p;
+        const bool ab = P.method == SB_BAYES_A || P.method == SB_BAYES_B;
+        const double s = sdv[pos], f2p = f2[pos];
+        const double gk = beta[o + pos];
+        const int cold = P.method == SB_BAYES_A ? 2 : cls[o + pos];
+        const double vs = ab ? vara_s[o + pos] : 0.0;
+        const int gp = grp[pos], sid = snp_of_pos[pos];
+        const int *tcol = Tp + (int64_t)c * L * P.blocksize + jk;
+        int v[DCH][3];
+#pragma unroll
+        for (int u = 0; u < DCH; u++) {
+            const int64_t ch = min(u, nch - 1);
+#pragma unroll
+            for (int k = 0; k < 3; k++) v[u][k] = tcol[ch * tstride + (int64_t)k * P.blocksize];
+        }
         // row groups entirely below this column's diagonal tile hold no partial
         const int ng = (jk >> 4) / spant + 1;
         const double *rp = rapart + (int64_t)c * P.blocksize + jk;
         const int64_t gstride = (int64_t)C * P.blocksize;
-        for (int g0 = 0; g0 < ng; g0 += 8) {
-            double v[8];
+        double ra[DRG];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = rp[(int64_t)min(g0 + u, ng - 1) * gstride];
+        for (int u = 0; u < DRG; u++) ra[u] = rp[(int64_t)min(u, ng - 1) * gstride];
+        long long t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
-            for (int u = 0; u < 8; u++) rhs += g0 + u < ng ? v[u] : 0.0;
+        for (int u = 0; u < DCH; u++) {
+            const int keep = u < nch;
+            t0 += keep ? v[u][0] : 0;
+            t1 += keep ? v[u][1] : 0;
+            t2 += keep ? v[u][2] : 0;
         }
+        for (int ch = DCH; ch < nch; ch++) {   // more chunks than unrolled
+            t0 += tcol[(int64_t)ch * tstride];
+            t1 += tcol[(int64_t)ch * tstride + P.blocksize];
+            t2 += tcol[(int64_t)ch * tstride + 2 * (int64_t)P.blocksize];
+        }
+        const long long S = (t2 * 256 + t1) * 256 + t0;
+        const double sx = ldexp((double)S, -expo[c]);
+        double rhs = s > 0.0 ? (sx - f2p * rtot[c]) / s : 0.0;
+#pragma unroll
+        for (int u = 0; u < DRG; u++) rhs += u < ng ? ra[u] : 0.0;
+        for (int g = DRG; g < ng; g++) rhs += rp[(int64_t)g * gstride];
         if (rhs_out) {  // diagnostics: the products only, no draw
             rhs_out[(int64_t)c * nb + jk] = rhs;
         } else {
             const Scal *scc = sc + c;
-            PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)snp_of_pos[pos], 2u);
-            const int64_t o = (int64_t)c * P.p;
-            draw_snp<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, pos, grp[pos], beta + o, cls + o,
-                                    vara_s + o, delta + o, chg + o);
-            const double dlt = delta[o + pos];
-            dl_sw[rsw_index(jk, c, (C + 15) / 16)] = (float)dlt;
-            cj = s > 0.0 ? dlt / s : 0.0;
-            kc = f2[pos] * cj;
+            PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)sid, 2u);
+            const SnpDraw d = draw_core<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, gp, gk, cold, vs);
+            beta[o + pos] = d.beta;
+            if (P.method != SB_BAYES_A) cls[o + pos] = d.cls;
+            if (ab) vara_s[o + pos] = d.vs;
+            delta[o + pos] = d.delta;
+            chg[o + pos] = d.chg;
+            dl_sw[rsw_index(jk, c, (C + 15) / 16)] = (float)d.delta;
+            cj = s > 0.0 ? d.delta / s : 0.0;
+            kc = f2p * cj;
         }
     }
     if (rhs_out) return;

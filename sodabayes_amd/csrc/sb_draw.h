@@ -91,32 +91,39 @@ __device__ __forceinline__ double b_prob(const Scal *sc, double rhs, double zz, 
     return 1.0 / (1.0 + exp(clike));
 }
 
-// Per-SNP draw of the chosen method from uniform source `src`; returns the
-// new effect and sets class / variance.  Mirrors models.f90:145-161 (A),
-// 210-255 (B), 349-412 (C/Cpi/R/Rc).
+// New state of one SNP after its draw.
+struct SnpDraw {
+    double beta;   // new effect
+    double vs;     // new per-SNP variance (BayesA/B)
+    double delta;  // old minus new effect
+    int cls;       // new class (not BayesA)
+    bool chg;      // the residuals change (models.f90:416-422: gflag /= 1)
+};
+
+// Per-SNP draw of the chosen method from uniform source `src`, from the old
+// effect gk, class cold (BayesA: 2) and variance vs (BayesA/B).  Mirrors
+// models.f90:145-161 (A), 210-255 (B), 349-412 (C/Cpi/R/Rc).
 template <class Src, int NM = MAXMIX>
-__device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
-                         int pos, int g, double *beta, int32_t *cls, double *vara_s, double *delta,
-                         uint8_t *chg)
+__device__ inline SnpDraw draw_core(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
+                                   int g, double gk, int cold, double vs)
 {
-    double gk = beta[pos];
-    int cold = P.method == SB_BAYES_A ? 2 : cls[pos];
+    SnpDraw d;
     double vare = sc->vare;
     if (P.method == SB_BAYES_A) {
         double rhs = rhs0 + zz * gk;
-        double v1 = zz + vare / vara_s[pos];
+        double v1 = zz + vare / vs;
         double gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
-        beta[pos] = gn;
         double scale = (gn * gn + sc->scale_va * P.df_va) / (P.df_va + 1.0);
-        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
-        delta[pos] = gk - gn;
-        chg[pos] = 1;
-        return;
+        d.beta = gn;
+        d.vs = draw_sinvchi2(src, P.df_va + 1.0, scale);
+        d.delta = gk - gn;
+        d.cls = 2;
+        d.chg = true;
+        return d;
     }
     double rhs = rhs0;
     if (cold > 1) rhs = rhs + zz * gk;
     if (P.method == SB_BAYES_B) {
-        double vs = vara_s[pos];
         double sk = b_prob(sc, rhs, zz, zz_vare, vs);
         double u = src.next();
         int flag = u < sk ? 2 : 1;
@@ -129,13 +136,13 @@ __device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double r
             gv = draw_normal(src, rhs / v1, sqrt(vare / v1));
             gn = gv;
         }
-        beta[pos] = gn;
-        cls[pos] = flag;
         double scale = (gv * gv + sc->scale_va * P.df_va) / (P.df_va + 1.0);
-        vara_s[pos] = draw_sinvchi2(src, P.df_va + 1.0, scale);
-        delta[pos] = gk - gn;
-        chg[pos] = !(flag == 1 && cold == 1);
-        return;
+        d.beta = gn;
+        d.cls = flag;
+        d.vs = draw_sinvchi2(src, P.df_va + 1.0, scale);
+        d.delta = gk - gn;
+        d.chg = !(flag == 1 && cold == 1);
+        return d;
     }
     double stemp[NM];
     mix_probs<NM, std::is_same<Src, XoroSrc>::value>(P, sc, g, rhs, zz, zz_vare, stemp);
@@ -155,10 +162,28 @@ __device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double r
         double v1 = zz + vare / sc->gp[flag - 1];
         gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
     }
-    beta[pos] = gn;
-    cls[pos] = flag;
-    delta[pos] = gk - gn;
-    chg[pos] = !(flag == 1 && cold == 1);
+    d.beta = gn;
+    d.vs = vs;
+    d.cls = flag;
+    d.delta = gk - gn;
+    d.chg = !(flag == 1 && cold == 1);
+    return d;
+}
+
+// draw_core on the state arrays at position pos (loads, draw, stores).
+template <class Src, int NM = MAXMIX>
+__device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
+                                int pos, int g, double *beta, int32_t *cls, double *vara_s, double *delta,
+                                uint8_t *chg)
+{
+    const bool ab = P.method == SB_BAYES_A || P.method == SB_BAYES_B;
+    const SnpDraw d = draw_core<Src, NM>(P, sc, src, rhs0, zz, zz_vare, g, beta[pos],
+                                         P.method == SB_BAYES_A ? 2 : cls[pos], ab ? vara_s[pos] : 0.0);
+    beta[pos] = d.beta;
+    if (P.method != SB_BAYES_A) cls[pos] = d.cls;
+    if (ab) vara_s[pos] = d.vs;
+    delta[pos] = d.delta;
+    chg[pos] = d.chg;
 }
 
 // Host-side dispatch on a compile-time bound NM >= nmix of the class count

@@ -401,6 +401,10 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
     } else {
         ra += c * P.blocksize;
         const int i = (blockIdx.x - nrw) * WAVE + lane;
+        // column jk of the factor: packed triangle (rows <= jk) or dense n_b
+        // column; chosen by select, so that the loads below stay unbranched
+        const bool dense = Ud != nullptr;
+        const double *Ub = dense ? Ud : U + tri0;
         double acc = 0.0;
         if (i < nb)
             for (int q0 = wave * UNR; q0 < m; q0 += UW * UNR) {
@@ -413,10 +417,8 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
                     if (q < m) {
                         const int jk = lst[q];
                         dv[u] = dl[q];
-                        if (Ud)
-                            uv[u] = Ud[(int64_t)jk * nb + i];
-                        else if (i <= jk)
-                            uv[u] = U[tri0 + (int64_t)jk * (jk + 1) / 2 + i];
+                        const int64_t off = dense ? (int64_t)jk * nb : (int64_t)jk * (jk + 1) / 2;
+                        if (dense || i <= jk) uv[u] = Ub[off + i];
                     }
                 }
 #pragma unroll
