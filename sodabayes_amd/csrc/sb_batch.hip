@@ -855,18 +855,22 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
 {
     __shared__ double lds[NT / WAVE];
     __shared__ unsigned long long mx[NT / WAVE];
+    __shared__ MixConst mc;
     constexpr int L = LIMBS;
     const int jk = blockIdx.x * NT + threadIdx.x;
     const int c = blockIdx.y;
     const int C = P.nchains;
+    const bool mixm = P.method != SB_BAYES_A && P.method != SB_BAYES_B;
     double cj = 0.0, kc = 0.0;
-    if (jk < nb) {
+    {
         // Every load of the thread is issued before the first use: the SNP
         // state, the chunk partials (up to DCH chunks unrolled) and the r_a
         // partials (up to DRG row groups).  Loads are unconditional (clamped
-        // index) and masked afterwards: a guarded load compiles to a branch
-        // and a wait per load.
-        const int pos = p0 + jk;
+        // index, also past the block) and masked afterwards: a guarded load
+        // compiles to a branch and a wait per load.  The per-chain mixture
+        // constants are computed while they are in flight.
+        const int jkc = min(jk, nb - 1);
+        const int pos = p0 + jkc;
         const int64_t o = (int64_t)c * P.p;
         const bool ab = P.method == SB_BAYES_A || P.method == SB_BAYES_B;
         const double s = sdv[pos], f2p = f2[pos];
@@ -874,7 +878,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
         const int cold = P.method == SB_BAYES_A ? 2 : cls[o + pos];
         const double vs = ab ? vara_s[o + pos] : 0.0;
         const int gp = grp[pos], sid = snp_of_pos[pos];
-        const int *tcol = Tp + (int64_t)c * L * P.blocksize + jk;
+        const int *tcol = Tp + (int64_t)c * L * P.blocksize + jkc;
         int v[DCH][3];
 #pragma unroll
         for (int u = 0; u < DCH; u++) {
@@ -883,12 +887,14 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
             for (int k = 0; k < 3; k++) v[u][k] = tcol[ch * tstride + (int64_t)k * P.blocksize];
         }
         // row groups entirely below this column's diagonal tile hold no partial
-        const int ng = (jk >> 4) / spant + 1;
-        const double *rp = rapart + (int64_t)c * P.blocksize + jk;
+        const int ng = (jkc >> 4) / spant + 1;
+        const double *rp = rapart + (int64_t)c * P.blocksize + jkc;
         const int64_t gstride = (int64_t)C * P.blocksize;
         double ra[DRG];
 #pragma unroll
         for (int u = 0; u < DRG; u++) ra[u] = rp[(int64_t)min(u, ng - 1) * gstride];
+        if (mixm && !rhs_out) mix_const(P, sc + c, zz, mc);   // workgroup-uniform
+        if (jk >= nb) goto reduce;
         long long t0 = 0, t1 = 0, t2 = 0;
 #pragma unroll
         for (int u = 0; u < DCH; u++) {
@@ -913,7 +919,8 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
         } else {
             const Scal *scc = sc + c;
             PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)sid, 2u);
-            const SnpDraw d = draw_core<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, gp, gk, cold, vs);
+            const SnpDraw d = mixm ? draw_mix_fast<NM>(P, scc, src, mc, rhs, zz, gp, gk, cold)
+                                   : draw_core<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, gp, gk, cold, vs);
             beta[o + pos] = d.beta;
             if (P.method != SB_BAYES_A) cls[o + pos] = d.cls;
             if (ab) vara_s[o + pos] = d.vs;
@@ -924,6 +931,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
             kc = f2p * cj;
         }
     }
+reduce:
     if (rhs_out) return;
     kc = block_sum<NT>(kc, lds);
     unsigned long long m = (unsigned long long)__double_as_longlong(fabs(cj));

@@ -48,6 +48,13 @@ __device__ __forceinline__ double box_muller(double mean, double sd, double u1, 
     return __dadd_rn(mean, __dmul_rn(__dmul_rn(sd, r), sin(theta)));
 }
 
+// Box-Muller with the exact pi (sinpi): the counter-based sampler's normal
+// variates (REPLAY keeps box_muller and the reference's f32 pi).
+__device__ __forceinline__ double box_muller_pi(double mean, double sd, double u1, double u2)
+{
+    return mean + sd * (sqrt(-2.0 * log(u1)) * sinpi(2.0 * u2));
+}
+
 // Uniform source concept: struct with `double next()`.
 struct XoroSrc {
     Xoro st;
@@ -72,23 +79,26 @@ __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t 
 
 // Counter-based uniform source: one SNP's private stream
 // (key = seed ^ chain, counter = (iteration, SNP, tag, sequence)).
+// Each Philox block gives two uniforms, words (0, 1) then (2, 3); they are
+// kept as two 64-bit values (no indexed array, which would leave registers).
 struct PhiloxSrc {
     uint32_t k0, k1, c0, c1, c2, seq;
-    uint32_t buf[4];
+    uint64_t x0, x1;
     int avail;
     __device__ PhiloxSrc(uint64_t key, uint32_t iter, uint32_t snp, uint32_t tag)
-        : k0((uint32_t)key), k1((uint32_t)(key >> 32)), c0(iter), c1(snp), c2(tag), seq(0),
+        : k0((uint32_t)key), k1((uint32_t)(key >> 32)), c0(iter), c1(snp), c2(tag), seq(0), x0(0), x1(0),
           avail(0) {}
     __device__ double next()
     {
         if (avail == 0) {
-            buf[0] = c0; buf[1] = c1; buf[2] = c2; buf[3] = seq++;
-            philox4x32(buf, k0, k1);
+            uint32_t b[4] = {c0, c1, c2, seq++};
+            philox4x32(b, k0, k1);
+            x0 = ((uint64_t)b[0] << 32) | b[1];
+            x1 = ((uint64_t)b[2] << 32) | b[3];
             avail = 2;
         }
-        int i = 2 - avail;
+        const uint64_t x = avail == 2 ? x0 : x1;
         avail--;
-        uint64_t x = ((uint64_t)buf[2 * i] << 32) | buf[2 * i + 1];
         return bits_to_unit(x);
     }
 };

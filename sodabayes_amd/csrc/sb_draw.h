@@ -186,6 +186,84 @@ __device__ inline void draw_snp(const Par &P, const Scal *sc, Src &src, double r
     chg[pos] = d.chg;
 }
 
+// Mixture conditionals of the counter-based sampler (BayesC/Cpi/R/Rc).  The
+// terms that depend only on the chain, the class and the block are computed
+// once per workgroup: with v1_k = zz + vare / gp_k,
+//   s_k = log_p(g, k) + a_k + b_k rhs^2,  a_k = -log(gp_k zz / vare + 1) / 2,
+//   b_k = 1 / (2 vare v1_k)                 (models.f90:363-371, uhat = rhs / v1_k)
+// and the class-k effect ~ N(m_k rhs, sd_k^2), m_k = 1 / v1_k,
+// sd_k = sqrt(vare / v1_k) (models.f90:402-405).  The same probabilities and
+// effects as draw_core up to rounding; REPLAY keeps draw_core.
+struct MixConst {
+    double a[MAXMIX], b[MAXMIX], m[MAXMIX], sd[MAXMIX];
+    double log_p[MAXBIN];   // copy of the chain's log mixture proportions
+};
+
+// Workgroup-wide: threads 1 .. nmix-1 fill class k of mc (LDS), every thread
+// copies part of log_p, then sync.
+__device__ inline void mix_const(const Par &P, const Scal *sc, double zz, MixConst &mc)
+{
+    for (int t = threadIdx.x; t < P.nbins; t += blockDim.x) mc.log_p[t] = sc->log_p[t];
+    const int k = threadIdx.x;
+    if (k >= 1 && k < P.nmix) {
+        const double vare = sc->vare;
+        const double v1 = zz + sc->vare_gp[k];
+        mc.a[k] = -0.5 * log(sc->gp[k] * (zz / vare) + 1.0);
+        mc.m[k] = 1.0 / v1;
+        mc.b[k] = 0.5 * mc.m[k] / vare;
+        mc.sd[k] = sqrt(vare * mc.m[k]);
+    }
+    __syncthreads();
+}
+
+template <int NM>
+__device__ inline SnpDraw draw_mix_fast(const Par &P, const Scal *sc, PhiloxSrc &src, const MixConst &mc,
+                                        double rhs0, double zz, int g, double gk, int cold)
+{
+    double rhs = rhs0;
+    if (cold > 1) rhs = rhs + zz * gk;
+    const double r2 = rhs * rhs;
+    double s[NM];
+    s[0] = mc.log_p[bin_of(P, g, 0)];
+    double mx = s[0];
+#pragma unroll
+    for (int k = 1; k < NM; k++)
+        if (k < P.nmix) {
+            s[k] = mc.log_p[bin_of(P, g, k)] + mc.a[k] + mc.b[k] * r2;
+            mx = fmax(mx, s[k]);
+        }
+    double tot = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        s[k] = k < P.nmix ? exp(s[k] - mx) : 0.0;
+        tot += s[k];
+    }
+    const double u = src.next() * tot;   // u < cumulative sum / tot
+    double ss = 0.0;
+    int flag = 1;
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        if (k >= P.nmix) break;
+        ss = ss + s[k];
+        if (u < ss) {
+            flag = k + 1;
+            break;
+        }
+    }
+    SnpDraw d;
+    double gn = 0.0;
+    if (flag > 1) {
+        const double u1 = src.next(), u2 = src.next();
+        gn = box_muller_pi(rhs * mc.m[flag - 1], mc.sd[flag - 1], u1, u2);
+    }
+    d.beta = gn;
+    d.vs = 0.0;
+    d.cls = flag;
+    d.delta = gk - gn;
+    d.chg = !(flag == 1 && cold == 1);
+    return d;
+}
+
 // Host-side dispatch on a compile-time bound NM >= nmix of the class count
 // (4, 8 or MAXMIX) for the draw kernels.
 template <class F>

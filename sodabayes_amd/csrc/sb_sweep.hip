@@ -190,6 +190,9 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
                                                      int32_t *cls, double *vara_s, double *delta,
                                                      uint8_t *chg, const uint8_t *skip)
 {
+    __shared__ MixConst mc;
+    const bool mixm = P.method != SB_BAYES_A && P.method != SB_BAYES_B;
+    if (mixm) mix_const(P, sc, zz, mc);   // workgroup-uniform
     int jk = blockIdx.x * NT + threadIdx.x;
     if (jk >= nb) return;
     int pos = p0 + jk;
@@ -199,7 +202,15 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
     }
     double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
     PhiloxSrc src(chain_key(P, 0), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
-    draw_snp<PhiloxSrc, NM>(P, sc, src, rhs, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta, chg);
+    if (mixm) {
+        const SnpDraw d = draw_mix_fast<NM>(P, sc, src, mc, rhs, zz, grp[pos], beta[pos], cls[pos]);
+        beta[pos] = d.beta;
+        cls[pos] = d.cls;
+        delta[pos] = d.delta;
+        chg[pos] = d.chg;
+    } else {
+        draw_snp<PhiloxSrc, NM>(P, sc, src, rhs, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta, chg);
+    }
 }
 
 // REPLAY mode: rhs in parallel, then every draw from stream 0 in block
