@@ -154,11 +154,24 @@ __global__ __launch_bounds__(NT) void k_rhs(Par P, const uint32_t *__restrict__ 
     } else {
         const int jk = (t - nxg) * 4 + wave;
         if (jk >= nb) return;
-        // packed triangle, or (pivoted-Cholesky blocks) a dense n_b column
+        // packed triangle, or (pivoted-Cholesky blocks) a dense n_b column;
+        // 8 rows per lane in flight (a long column would otherwise wait on
+        // each load in turn)
         const double *col = Ud ? Ud + (int64_t)jk * nb : U + tri0 + (int64_t)jk * (jk + 1) / 2;
         const int len = Ud ? nb : jk + 1;
         double acc = 0.0;
-        for (int i = lane; i < len; i += 64) acc = fma(col[i], ra[i], acc);
+        for (int i0 = lane; i0 < len; i0 += 8 * WAVE) {
+            double cv[8], rv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = min(i0 + u * WAVE, len - 1);
+                cv[u] = col[i];
+                rv[u] = ra[i];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (i0 + u * WAVE < len) acc = fma(cv[u], rv[u], acc);
+        }
         acc = wave_sum(acc);
         if (lane == 0) partA[jk] = acc;
     }
@@ -315,7 +328,28 @@ __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const do
     cl += c * P.blocksize;
     const int R = (nb + NT - 1) / NT;
     int a = threadIdx.x * R, e = min(nb, a + R), k = 0;
-    for (int j = a; j < e; j++) k += chg[p0 + j] ? 1 : 0;
+    // up to LR positions per thread: flags and values in one batch of loads
+    // (clamped index, masked), so that no load waits on another
+    constexpr int LR = 16;
+    uint8_t fl[LR];
+    double dv[LR], sv[LR], fv[LR];
+    if (R <= LR) {
+#pragma unroll
+        for (int u = 0; u < LR; u++) {
+            const int pos = p0 + min(a + u, nb - 1);
+            fl[u] = chg[pos];
+            dv[u] = delta[pos];
+            sv[u] = sdv[pos];
+            fv[u] = f2[pos];
+        }
+#pragma unroll
+        for (int u = 0; u < LR; u++) {
+            if (a + u >= e) fl[u] = 0;
+            k += fl[u] ? 1 : 0;
+        }
+    } else {
+        for (int j = a; j < e; j++) k += chg[p0 + j] ? 1 : 0;
+    }
     // exclusive scan of the per-thread counts: wave scan, then wave offsets
     const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
     int incl = k;
@@ -333,6 +367,18 @@ __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const do
     __syncthreads();
     int o = cnt[threadIdx.x];
     double kc = 0.0;
+    if (R <= LR) {
+#pragma unroll
+        for (int u = 0; u < LR; u++)
+            if (fl[u]) {
+                const double cj = sv[u] > 0.0 ? dv[u] / sv[u] : 0.0;
+                lst[o] = a + u;
+                dl[o] = dv[u];
+                cl[o] = cj;
+                kc = fma(fv[u], cj, kc);
+                o++;
+            }
+    } else
     for (int j = a; j < e; j++)
         if (chg[p0 + j]) {
             const int pos = p0 + j;
@@ -354,15 +400,19 @@ __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const do
 // Workgroups [0, nrw): 256 individuals each; lane l of every wave owns
 // individuals 4l..4l+3; the UW waves take interleaved runs of the list
 // (UNR loads in flight per wave) and their partial sums are added in wave
-// order through LDS.  Workgroups after that: 64 rows of r_a each.
+// order through LDS.  Workgroups after that: (64 rows of r_a) x (RCH list
+// entries), so that the rows of the first tiles, which meet every changed
+// column of the triangle, spread over many workgroups; each writes its
+// partial sums (r_a is updated by k_ra_sum, in chunk order).
 constexpr int UW = 16;   // waves per update workgroup
 constexpr int UNR = 8;   // list entries in flight per wave
+constexpr int RCH = UW * UNR * 2;   // list entries per r_a workgroup
 __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__restrict__ geno,
                                                       const double *__restrict__ U, int64_t tri0,
                                                       const double *__restrict__ Ud, int p0,
                                                       int nb, int nrw, const int *lst, const double *dl,
                                                       const double *cl, const int *count,
-                                                      const double *kconst, double *r, double *ra)
+                                                      const double *kconst, double *r, double *rapart)
 {
     __shared__ double red[UW][WAVE][4];
     const int64_t c = blockIdx.z;
@@ -377,25 +427,32 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
         const int i0 = blockIdx.x * 256 + 4 * lane;       // first of 4 individuals
         const int64_t byte = i0 >> 2;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int q0 = wave * UNR; q0 < m; q0 += UW * UNR) {
-            uint32_t gb[UNR];
-            double cj[UNR];
+        // each wave walks its own run of the list; 64 entries at a time are
+        // read as one vector load and taken out lane by lane (no load waits
+        // on another inside the run; entries past the run carry c = 0)
+        const int qa = (int)((int64_t)m * wave / UW), qz = (int)((int64_t)m * (wave + 1) / UW);
+        for (int qb = qa; qb < qz; qb += WAVE) {
+            const int ql = min(qb + lane, qz - 1);
+            const int pos_l = p0 + lst[ql];
+            const double cj_l = qb + lane < qz ? cl[ql] : 0.0;
+            const int cnt = min(WAVE, qz - qb);
+            for (int u0 = 0; u0 < cnt; u0 += UNR) {
+                uint32_t gb[UNR];
+                double cj[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const int q = q0 + u;
-                gb[u] = 0;
-                cj[u] = 0.0;
-                if (q < m) {
-                    const int pos = p0 + lst[q];
-                    cj[u] = cl[q];
+                for (int u = 0; u < UNR; u++) {
+                    const int src = min(u0 + u, WAVE - 1);
+                    const int pos = __builtin_amdgcn_readlane(pos_l, src);
+                    cj[u] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(cj_l), src),
+                                             __builtin_amdgcn_readlane(__double2loint(cj_l), src));
                     gb[u] = reinterpret_cast<const uint8_t *>(geno + (int64_t)pos * P.ldw)[byte];
                 }
+#pragma unroll
+                for (int u = 0; u < UNR; u++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++)
+                        acc[e] = fma((double)((gb[u] >> (2 * e)) & 3u), cj[u], acc[e]);
             }
-#pragma unroll
-            for (int u = 0; u < UNR; u++)
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-                    acc[e] = fma((double)((gb[u] >> (2 * e)) & 3u), cj[u], acc[e]);
         }
 #pragma unroll
         for (int e = 0; e < 4; e++) red[wave][lane][e] = acc[e];
@@ -410,22 +467,26 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
             }
         }
     } else {
-        ra += c * P.blocksize;
-        const int i = (blockIdx.x - nrw) * WAVE + lane;
+        const int nrt = (nb + WAVE - 1) / WAVE;
+        const int rt = (blockIdx.x - nrw) % nrt, cg = (blockIdx.x - nrw) / nrt;
+        const int qb = cg * RCH, qe = min(m, qb + RCH);
+        if (qb >= m) return;   // chunk past the list (not summed)
+        rapart += c * P.blocksize * ((P.blocksize + RCH - 1) / RCH) + (int64_t)cg * P.blocksize;
+        const int i = rt * WAVE + lane;
         // column jk of the factor: packed triangle (rows <= jk) or dense n_b
         // column; chosen by select, so that the loads below stay unbranched
         const bool dense = Ud != nullptr;
         const double *Ub = dense ? Ud : U + tri0;
         double acc = 0.0;
         if (i < nb)
-            for (int q0 = wave * UNR; q0 < m; q0 += UW * UNR) {
+            for (int q0 = qb + wave * UNR; q0 < qe; q0 += UW * UNR) {
                 double uv[UNR], dv[UNR];
 #pragma unroll
                 for (int u = 0; u < UNR; u++) {
                     const int q = q0 + u;
                     uv[u] = 0.0;
                     dv[u] = 0.0;
-                    if (q < m) {
+                    if (q < qe) {
                         const int jk = lst[q];
                         dv[u] = dl[q];
                         const int64_t off = dense ? (int64_t)jk * nb : (int64_t)jk * (jk + 1) / 2;
@@ -440,9 +501,22 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
         if (wave == 0 && i < nb) {
             double s = red[0][lane][0];
             for (int w = 1; w < UW; w++) s += red[w][lane][0];
-            ra[i] = ra[i] + s;
+            rapart[i] = s;
         }
     }
+}
+
+// r_a += the list chunks' partial sums, in chunk order.
+__global__ __launch_bounds__(NT) void k_ra_sum(Par P, const double *rapart, const int *count, int nb, double *ra)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int64_t c = blockIdx.z;
+    if (i >= nb) return;
+    const int ncg = (count[c] + RCH - 1) / RCH;
+    rapart += c * P.blocksize * ((P.blocksize + RCH - 1) / RCH) + i;
+    double s = 0.0;
+    for (int g = 0; g < ncg; g++) s += rapart[(int64_t)g * P.blocksize];
+    if (ncg > 0) ra[c * P.blocksize + i] += s;
 }
 
 // ------------------------------------------------ end-of-iteration stats --
@@ -1052,7 +1126,7 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 if (rc) return rc;
             } else {
                 const int nrw = (P.n + 255) / 256;
-                const int nra = (nb + WAVE - 1) / WAVE;
+                const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
                 kc.begin(st);
                 k_list<<<dim3(1, 1, C), NT, 0, st>>>(P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
                                                      h->u_cl, h->u_cnt, h->u_kc);
@@ -1061,7 +1135,8 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 k_update<<<dim3(nrw + nra, 1, C), UW * WAVE, 0, st>>>(P, h->geno, h->U, tri0, dense_block(h, b),
                                                                       p0, nb, nrw,
                                                                       h->u_lst, h->u_dl, h->u_cl, h->u_cnt,
-                                                                      h->u_kc, h->r, h->ra);
+                                                                      h->u_kc, h->r, h->u_rapart);
+                k_ra_sum<<<dim3((nb + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->u_rapart, h->u_cnt, nb, h->ra);
                 kc.end(st, KS_UPDATE, 16.0 * P.n + 16.0 * nb);
             }
         }
