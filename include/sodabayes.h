@@ -177,6 +177,21 @@ int sb_block_rhs(sb_handle *h, int32_t block, double *out);
  * count of sb_build_aug]. */
 int sb_get_info(sb_handle *h, double *out, int32_t n);
 
+/* ---- prediction (replaces compute_dgv, utilities.f90:616-622, after
+ * xscale's prediction branch, utilities.f90:540-553; -predict,
+ * SODABayes.f90:145-148) ---- */
+
+/* Genomic values pred_i = mu + sum_j z_ij g_j of nind individuals over nloci
+ * SNPs, z_ij = (x_ij - 2 freq_j) / sqrt(2 freq_j (1 - freq_j)), z = 0 where
+ * freq_j < eps or 1 - freq_j < eps.  bed: SNP-major PLINK payload without
+ * the magic bytes (nloci x ceil(nind/4) bytes, missing read as 1), on the
+ * host or (bed_on_device != 0) in device memory; freq, effects: nloci
+ * doubles in .bim order (the .frq / .eff columns setup_prediction reads);
+ * pred: nind doubles out.  No handle: the call allocates and frees its own
+ * device buffers. */
+int sb_predict(int32_t nind, int32_t nloci, const uint8_t *bed, int32_t bed_on_device, const double *freq,
+               const double *effects, double mu, double *pred);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *sb_last_error(void);
 

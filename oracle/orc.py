@@ -67,6 +67,7 @@ def lib():
         L.orc_pack_genbits.argtypes = [u8p, C.c_int, C.c_int, i64p]
         L.orc_freq.argtypes = [u8p, C.c_int, C.c_int, dp]
         L.orc_standardize.argtypes = [u8p, C.c_int, C.c_int, dp, dp]
+        L.orc_predict.argtypes = [u8p, C.c_int, C.c_int, dp, dp, C.c_double, dp]
         L.orc_xxproduct.argtypes = [i64p, i64p, C.c_int64, C.c_int, C.c_double, C.c_double]
         L.orc_xxproduct.restype = C.c_double
         L.orc_gram_count.argtypes = [i64p, i64p, C.c_int64]
@@ -147,6 +148,17 @@ def standardize(bed: np.ndarray, nind: int, f: np.ndarray) -> np.ndarray:
     lib().orc_standardize(_ptr(bed, C.c_uint8), nind, bed.shape[0], _ptr(f, C.c_double),
                           _ptr(X, C.c_double))
     return X.T
+
+
+def predict(bed: np.ndarray, nind: int, f: np.ndarray, eff: np.ndarray, mu: float) -> np.ndarray:
+    """compute_dgv after xscale's prediction branch (utilities.f90:540-553, 616-622)."""
+    bed = np.ascontiguousarray(bed, dtype=np.uint8)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    eff = np.ascontiguousarray(eff, dtype=np.float64)
+    out = np.zeros(nind)
+    lib().orc_predict(_ptr(bed, C.c_uint8), nind, bed.shape[0], _ptr(f, C.c_double), _ptr(eff, C.c_double),
+                      float(mu), _ptr(out, C.c_double))
+    return out
 
 
 def xxproduct(gb: np.ndarray, j: int, k: int, nind: int, f: np.ndarray) -> float:

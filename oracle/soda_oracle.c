@@ -200,6 +200,31 @@ void orc_standardize(const uint8_t *bed, int nind, int nloci, const double *freq
     }
 }
 
+/* Genomic values of the prediction run (-predict): xscale's prediction branch
+ * standardises the genotypes with the frequencies read from the .frq file
+ * (utilities.f90:540-553), compute_dgv adds mu to the row dot products with
+ * the effects (utilities.f90:616-622): pred_i = mu + sum_j X(i,j) g_j, the
+ * sum over j in order. */
+void orc_predict(const uint8_t *bed, int nind, int nloci, const double *freq, const double *eff, double mu,
+                 double *pred)
+{
+    const int64_t nb = (nind + 3) / 4;
+    double *sd = malloc(sizeof(double) * (size_t)nloci);
+    for (int j = 0; j < nloci; j++) {
+        double p = freq[j];
+        sd[j] = (p < DEPS || (1.0 - p) < DEPS) ? 0.0 : sqrt(2.0 * p * (1.0 - p));
+    }
+    for (int i = 0; i < nind; i++) {
+        double s = 0.0;
+        for (int j = 0; j < nloci; j++) {
+            double x = sd[j] > 0.0 ? ((double)bed_count(bed + (int64_t)j * nb, i) - 2.0 * freq[j]) / sd[j] : 0.0;
+            s += x * eff[j];
+        }
+        pred[i] = mu + s;
+    }
+    free(sd);
+}
+
 /* Integer part of xxproduct, soda.f90:541-553: 3 nd + 2 ng + nc = sum x_a x_b */
 int64_t orc_gram_count(const int64_t *a, const int64_t *b, int64_t nbatch)
 {

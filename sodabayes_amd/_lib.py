@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsodabayes.so")
 
 SB_OK = 0
+SB_ERR_ARG = -1
 SB_ERR_NOT_PD = -5
 METHODS = {"BayesA": 1, "BayesB": 2, "BayesC": 3, "BayesCpi": 4, "BayesR": 5, "BayesRc": 6}
 RNG_REPLAY, RNG_COUNTER = 0, 1
@@ -27,7 +28,7 @@ KT_SLOTS = ["rhs", "draw", "list", "update", "rquant", "xq", "ut0", "drawb", "dq
 # every function include/sodabayes.h declares
 EXPORTS = ["sb_rng_seed", "sb_permutate", "sb_create", "sb_destroy", "sb_rng_set", "sb_rng_get",
            "sb_build_aug", "sb_mcmc_init", "sb_mcmc_run", "sb_hyp_width", "sb_get", "sb_block_rhs",
-           "sb_get_info", "sb_last_error"]
+           "sb_get_info", "sb_predict", "sb_last_error"]
 
 
 class SbDims(C.Structure):
@@ -77,6 +78,7 @@ def load():
     L.sb_get.argtypes = [vp, C.c_int32, C.c_int32, dp, C.c_int64]
     L.sb_get_info.argtypes = [vp, dp, C.c_int32]
     L.sb_block_rhs.argtypes = [vp, C.c_int32, dp]
+    L.sb_predict.argtypes = [C.c_int32, C.c_int32, vp, C.c_int32, dp, dp, C.c_double, dp]
     L.sb_last_error.restype = C.c_char_p
     for f in EXPORTS:
         if f not in ("sb_hyp_width", "sb_last_error"):
