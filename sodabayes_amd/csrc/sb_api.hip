@@ -268,6 +268,8 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         h->b_ldt = 16 * h->b_ntile;
         const int nS = P.ldw / 16;
         h->kss = sb::batch_kss(h->b_ntile);
+        if (const char *e = std::getenv("SODABAYES_KSS")) h->kss = std::max(1, std::atoi(e));
+        if (const char *e = std::getenv("SODABAYES_ROWS_PER_WAVE")) h->b_rw = std::atoi(e) == 4 ? 4 : 2;
         const int nch = (nS + h->kss - 1) / h->kss;
         const int nwg_d = (int)((bs + 255) / 256);
         const int ntc = (int)((C + 15) / 16);

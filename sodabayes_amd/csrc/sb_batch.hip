@@ -792,9 +792,25 @@ __device__ __forceinline__ void split_ids(int id, int n0, int n1, int &kind, int
     }
 }
 
+// Row tiles per wave of the LDS-shared products: 2 or 4 for up to 3 limb
+// tiles (the limb slab of a super-step is then shared by 128 or 256 rows),
+// 2 for 4 to 6 tiles (register budget); the non-LDS path ignores it.
+template <int NTL, class F>
+void with_rows(int rw, F &&f)
+{
+    using std::integral_constant;
+    if constexpr (NTL <= 3) {
+        if (rw == 4) {
+            f(integral_constant<int, 4>{});
+            return;
+        }
+    }
+    f(integral_constant<int, 2>{});
+}
+
 // Block products of one block in one launch: the X_s'Q workgroups (n0, grid
 // g0x x .) and the U_s'R_a workgroups (n1, grid g1x x .) run side by side.
-template <int NTILE, int IT, int NTC>
+template <int NTILE, int IT, int NTC, int RW>
 __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
                                              const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp,
                                              int64_t tstride, int g0x, int n0, const float4 *__restrict__ Ut,
@@ -805,7 +821,7 @@ __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gs
     split_ids(blockIdx.x, n0, n1, kind, local);
     if (kind == 0) {
         if constexpr (prod_lds(NTILE))
-            xq_lds<NTILE, prod_rows(NTILE)>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
+            xq_lds<NTILE, RW>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
         else
             xq_tile<NTILE, IT>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
     } else {
@@ -815,7 +831,7 @@ __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gs
 
 // Updates of one block in one launch: r += X_s C - K (n0 workgroups) beside
 // U_s D (n1 workgroups, grid g1x x .).
-template <int NTILE, int IT, int NTC>
+template <int NTILE, int IT, int NTC, int RW>
 __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss,
                                             const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
                                             const double *__restrict__ kconst, double *__restrict__ r,
@@ -828,8 +844,7 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
     split_ids(blockIdx.x, n0, n1, kind, local);
     if (kind == 0) {
         if constexpr (upd_lds(NTILE))
-            xupd_lds<NTILE, upd_rows(NTILE)>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, (int)(P.ldr / WAVE),
-                                             local);
+            xupd_lds<NTILE, RW>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, (int)(P.ldr / WAVE), local);
         else
             xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, n0, local);
     } else {
@@ -991,12 +1006,15 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     kc.begin(st);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
-        const int rt = prod_lds(NTL) ? 4 * prod_rows(NTL) : IT;   // row tiles per workgroup
-        const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
-        const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
-        k_prod<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, tstride, g0x, n0, ut,
-                                                     reinterpret_cast<const float4 *>(h->ra_sw), spant,
-                                                     h->b_rapart, g1x, n1);
+        with_rows<NTL>(h->b_rw, [&](auto rw) {
+            constexpr int RW = decltype(rw)::value;
+            const int rt = prod_lds(NTL) ? 4 * RW : IT;   // row tiles per workgroup
+            const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
+            const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
+            k_prod<NTL, IT, NTC, RW><<<n0 + n1, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, tstride, g0x, n0,
+                                                             ut, reinterpret_cast<const float4 *>(h->ra_sw), spant,
+                                                             h->b_rapart, g1x, n1);
+        });
     });
     if (rc) return fail(rc, "unsupported chain count");
     kc.end(st, KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 12.0 * C * nb);
@@ -1024,12 +1042,15 @@ int batch_update(sb_handle *h, int b, const double *delta)
     kc.begin(st);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
-        const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * upd_rows(NTL) : 16 * IT));
-        const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
-        k_upd<NTL, IT, NTC><<<n0 + n1, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r, h->b_rsum,
-                                                    h->b_rmax, n0, ut, nbt,
-                                                    nb, reinterpret_cast<const float4 *>(h->dl_sw), spant,
-                                                    h->b_uapart, g1x, n1);
+        with_rows<NTL>(h->b_rw, [&](auto rw) {
+            constexpr int RW = decltype(rw)::value;
+            const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * RW : 16 * IT));
+            const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
+            k_upd<NTL, IT, NTC, RW><<<n0 + n1, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r,
+                                                            h->b_rsum, h->b_rmax, n0, ut, nbt, nb,
+                                                            reinterpret_cast<const float4 *>(h->dl_sw), spant,
+                                                            h->b_uapart, g1x, n1);
+        });
     });
     if (rc) return fail(rc, "unsupported chain count");
     // partials of the new r for the next block: per 64 rows (LDS path) or per workgroup

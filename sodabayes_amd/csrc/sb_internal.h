@@ -75,6 +75,21 @@ struct KClock {
     std::vector<double> bytes;    // algorithmic bytes of each recorded pair
     size_t used = 0;
     double ms[KS_N] = {}, launches[KS_N] = {}, tbytes[KS_N] = {};
+    // algorithmic bytes that only the device knows (the changed-SNP columns
+    // of the FP64 update), added by kernels per slot while timing is on
+    double *dextra = nullptr;
+    double *extra(int s)
+    {
+        if (!on) return nullptr;
+        if (!dextra) {
+            if (hipMalloc(&dextra, sizeof(double) * KS_N) != hipSuccess) {
+                dextra = nullptr;
+                return nullptr;
+            }
+            hipMemset(dextra, 0, sizeof(double) * KS_N);
+        }
+        return dextra + s;
+    }
     void begin(hipStream_t st)
     {
         if (!on) return;
@@ -105,10 +120,17 @@ struct KClock {
         slot.clear();
         bytes.clear();
         used = 0;
+        if (dextra) {
+            double x[KS_N];
+            if (hipMemcpy(x, dextra, sizeof x, hipMemcpyDeviceToHost) == hipSuccess)
+                for (int s = 0; s < KS_N; s++) tbytes[s] += x[s];
+            hipMemset(dextra, 0, sizeof(double) * KS_N);
+        }
     }
     ~KClock()
     {
         for (auto e : ev) hipEventDestroy(e);
+        if (dextra) hipFree(dextra);
     }
 };
 
@@ -183,6 +205,7 @@ struct sb_handle {
     int *b_expo = nullptr;        // fixed-point exponent per chain
     double *b_rtot = nullptr;     // sum of r per chain
     int kss = 32;                 // super-steps (16 genotype words) per MFMA workgroup chunk (batch_kss)
+    int b_rw = 2;                 // row tiles per wave of the LDS-shared products (2 or 4)
     int b_ntile = 1;              // 16-column MFMA tiles covering nchains * limbs
     int *b_T = nullptr;           // chunks x blocksize x b_ldt int32 limb products per individual chunk
     int b_ldt = 16;

@@ -315,7 +315,8 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
 // and the centring constant sum_j 2 f_j c_j.
 __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const double *delta,
                                              const double *f2, const double *sdv, int p0, int nb, int *lst,
-                                             double *dl, double *cl, int *count, double *kconst)
+                                             double *dl, double *cl, int *count, double *kconst,
+                                             double *ubytes)
 {
     __shared__ int cnt[NT + 1];
     __shared__ int wsum[NT / WAVE];
@@ -392,6 +393,17 @@ __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const do
         }
     kc = block_sum<NT>(kc, lds);
     if (threadIdx.x == 0) kconst[c] = kc;
+    if (ubytes) {
+        // kernel timing: the update's algorithmic bytes, i.e. each changed
+        // SNP's two-bit genotype column and its column of the packed upper
+        // factor (block position j holds j + 1 doubles)
+        double ub = 0.0;
+        for (int q = cnt[threadIdx.x]; q < o; q++)
+            ub += (double)((P.n + 3) / 4) + 8.0 * (double)(lst[q] + 1);
+        __syncthreads();
+        ub = block_sum<NT>(ub, lds);
+        if (threadIdx.x == 0) atomicAdd(ubytes, ub);
+    }
 }
 
 // r += sum_j z_j delta_j and r_a += sum_j A_j delta_j over the changed list
@@ -1129,7 +1141,7 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
                 kc.begin(st);
                 k_list<<<dim3(1, 1, C), NT, 0, st>>>(P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
-                                                     h->u_cl, h->u_cnt, h->u_kc);
+                                                     h->u_cl, h->u_cnt, h->u_kc, kc.extra(KS_UPDATE));
                 kc.end(st, KS_LIST, 25.0 * nb);
                 kc.begin(st);
                 k_update<<<dim3(nrw + nra, 1, C), UW * WAVE, 0, st>>>(P, h->geno, h->U, tri0, dense_block(h, b),
