@@ -60,6 +60,44 @@ def build_fortran() -> str | None:
     return exe
 
 
+REF_SRC = os.environ.get("SODABAYES_REF_SRC", "/root/reference/src")
+# the reference's host modules the program keeps (its SODA module soda.f90 is
+# replaced by soda_gpu_host and not linked); dependency order
+REF_HOST_MODULES = ["globals", "parallelrng", "utilities", "useroptions", "models"]
+
+
+def build_fortran_host() -> str | None:
+    """Builds fortran/build/sodabayes_gpu: the reference program with its
+    SODA branch on the GPU.  The reference's host modules (option parsing,
+    readers, writers, the -SODAOff sampler) are compiled from its sources
+    where they lie (REF_SRC); the binding, the glue module and the main
+    program are this package's.  Returns None where the reference sources
+    or the Fortran compiler are absent."""
+    if not (os.path.exists(FORTRAN) and os.path.isdir(REF_SRC)):
+        return None
+    hdir = os.path.join(FBUILD, "host")
+    os.makedirs(hdir, exist_ok=True)
+    srcs = [os.path.join(REF_SRC, m + ".f90") for m in REF_HOST_MODULES]
+    srcs += [os.path.join(FDIR, f) for f in ("soda_gpu.f90", "soda_gpu_host.f90", "sodabayes_gpu.f90")]
+    objs = []
+    for src in srcs:
+        obj = os.path.join(hdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+        r = subprocess.run([FORTRAN, "-O2", "-fopenmp", "-J", hdir, "-c", src, "-o", obj],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"Fortran host failed to compile {os.path.basename(src)}")
+        objs.append(obj)
+    exe = os.path.join(FBUILD, "sodabayes_gpu")
+    r = subprocess.run([FORTRAN, "-fopenmp", "-o", exe, *objs, "-L", HERE, "-lsodabayes",
+                        "-Wl,-rpath,$ORIGIN/../.."], capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("Fortran host failed to link")
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_fortran())
+    print(build_fortran_host())

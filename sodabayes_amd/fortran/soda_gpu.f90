@@ -104,6 +104,15 @@ module soda_gpu
             real(c_double) :: out(*)
             integer(c_int32_t), value :: n
         end function
+        integer(c_int) function sb_predict(nind, nloci, bed, bed_on_device, freq, effects, mu, pred) &
+                bind(C, name='sb_predict')
+            import :: c_int, c_int32_t, c_double, c_ptr
+            integer(c_int32_t), value :: nind, nloci, bed_on_device
+            type(c_ptr), value :: bed
+            real(c_double), intent(in) :: freq(*), effects(*)
+            real(c_double), value :: mu
+            real(c_double) :: pred(*)
+        end function
         type(c_ptr) function sb_last_error() bind(C, name='sb_last_error')
             import :: c_ptr
         end function
