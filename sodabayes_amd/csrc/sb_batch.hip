@@ -990,12 +990,12 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
         // r changed outside the batched update (first block of a sweep)
         kc.begin(st);
         if (h->rstat_nwg == 0) {
-            k_rstat<<<dim3(nwg_r, 1, C), NT, 0, st>>>(P, h->r, h->b_rsum, h->b_rmax);
+            launch(kc, k_rstat, dim3(nwg_r, 1, C), dim3(NT), 0, st, P, h->r, h->b_rsum, h->b_rmax);
             h->rstat_nwg = nwg_r;
         }
-        k_quant<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax,
-                                                                 h->rstat_nwg, Q, h->b_expo, h->b_rtot);
-        kc.end(st, KS_RQUANT, 16.0 * C * n + cl * n);
+        kc.last(KS_RQUANT, 16.0 * C * n + cl * n);
+        launch(kc, k_quant, dim3((P.ldw + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, L, h->b_ntile, h->r, h->b_rsum,
+               h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_rtot);
     }
     h->q_block = -1;
     const int kss = h->kss, nS = P.ldw / 16;
@@ -1004,6 +1004,7 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
     const float4 *ut = reinterpret_cast<const float4 *>(h->Ut0) + h->tt_off[b] * 64;
     kc.begin(st);
+    kc.last(KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 12.0 * C * nb);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
         with_rows<NTL>(h->b_rw, [&](auto rw) {
@@ -1011,13 +1012,11 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
             const int rt = prod_lds(NTL) ? 4 * RW : IT;   // row tiles per workgroup
             const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
             const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
-            k_prod<NTL, IT, NTC, RW><<<n0 + n1, NT, 0, st>>>(P, gsw, nbt, nb, Q, kss, h->b_T, tstride, g0x, n0,
-                                                             ut, reinterpret_cast<const float4 *>(h->ra_sw), spant,
-                                                             h->b_rapart, g1x, n1);
+            launch(kc, k_prod<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss, h->b_T,
+                   tstride, g0x, n0, ut, reinterpret_cast<const float4 *>(h->ra_sw), spant, h->b_rapart, g1x, n1);
         });
     });
     if (rc) return fail(rc, "unsupported chain count");
-    kc.end(st, KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 12.0 * C * nb);
     return hip_check(hipGetLastError(), "batch products");
 }
 
@@ -1033,43 +1032,42 @@ int batch_update(sb_handle *h, int b, const double *delta)
     const double n = P.n, cl = (double)C * L, gbytes = (double)nb * ((n + 3.0) / 4.0);
     const double tbytes = 4.0 * 256.0 * nbt * (nbt + 1) / 2.0;
     kc.begin(st);
-    k_dquant<<<dim3(nsw * 16 / NT, C), NT, 0, st>>>(P, L, h->b_ntile, delta, h->sdv, p0, nb, h->b_dmax, h->b_dk,
-                                                     nwg_d, Dq, h->b_expo2, h->b_kc);
-    kc.end(st, KS_DQUANT, 16.0 * C * nb + cl * nb);
+    kc.last(KS_DQUANT, 16.0 * C * nb + cl * nb);
+    launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, L, h->b_ntile, delta, h->sdv, p0, nb,
+           h->b_dmax, h->b_dk, nwg_d, Dq, h->b_expo2, h->b_kc);
     const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
     const float4 *ut = reinterpret_cast<const float4 *>(h->Ut1) + h->tt_off[b] * 64;
     kc.begin(st);
+    kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
     int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
         constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
         with_rows<NTL>(h->b_rw, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
             const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * RW : 16 * IT));
             const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
-            k_upd<NTL, IT, NTC, RW><<<n0 + n1, NT, 0, st>>>(P, gt, nsw / 16, Dq, L, h->b_expo2, h->b_kc, h->r,
-                                                            h->b_rsum, h->b_rmax, n0, ut, nbt, nb,
-                                                            reinterpret_cast<const float4 *>(h->dl_sw), spant,
-                                                            h->b_uapart, g1x, n1);
+            launch(kc, k_upd<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gt, nsw / 16, Dq, L, h->b_expo2,
+                   h->b_kc, h->r, h->b_rsum, h->b_rmax, n0, ut, nbt, nb,
+                   reinterpret_cast<const float4 *>(h->dl_sw), spant, h->b_uapart, g1x, n1);
         });
     });
     if (rc) return fail(rc, "unsupported chain count");
     // partials of the new r for the next block: per 64 rows (LDS path) or per workgroup
     h->rstat_nwg = (int)(P.ldr / (upd_lds(h->b_ntile) ? WAVE : 16 * rows_tiles(h->b_ntile)));
-    kc.end(st, KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
     const int nra = (nb + NT - 1) / NT;
     kc.begin(st);
     if (b + 1 < P.nblocks) {
         // the next block's r limbs in the same launch
         const int nq = (P.ldw + NT - 1) / NT;
         uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
-        k_rared_quant<<<(nq + nra) * C, NT, 0, st>>>(P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw, nra, L,
-                                                     h->b_ntile, h->r, h->b_rsum, h->b_rmax, h->rstat_nwg, Q,
-                                                     h->b_expo, h->b_rtot, nq);
+        kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 16.0 * C * n + cl * n);
+        launch(kc, k_rared_quant, dim3((nq + nra) * C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra,
+               h->ra_sw, nra, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_rtot,
+               nq);
         h->q_block = b + 1;
-        kc.end(st, KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 16.0 * C * n + cl * n);
     } else {
-        k_ra_reduce<<<dim3(nra, C), NT, 0, st>>>(P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
-        kc.end(st, KS_RARED, 8.0 * C * nb * (ncg + 3.0));
+        kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0));
+        launch(kc, k_ra_reduce, dim3(nra, C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
     }
     return hip_check(hipGetLastError(), "batch update");
 }
@@ -1117,15 +1115,15 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
     const int nch = (nS + h->kss - 1) / h->kss;
     const double C = P.nchains;
     h->kc.begin(h->stream);
-    with_nmix(P.nmix, [&](auto nm) {
-        k_draw_b<decltype(nm)::value><<<dim3((nb + NT - 1) / NT, P.nchains), NT, 0, h->stream>>>(
-            P, h->scal, h->b_T, tstride, nch, h->b_expo, h->b_rtot, h->b_rapart,
-            h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta,
-            h->cls, h->vara_s, delta, chg, rhs_out, h->b_dmax, h->b_dk, h->dl_sw);
-    });
     // limb partials, r_a partials, effect state read and written
-    h->kc.end(h->stream, KS_DRAWB,
-              4.0 * nch * C * h->limbs * nb + 8.0 * C * nb * ((nb + 255) / 256 / 2.0 + 1.0) + 60.0 * C * nb);
+    h->kc.last(KS_DRAWB,
+               4.0 * nch * C * h->limbs * nb + 8.0 * C * nb * ((nb + 255) / 256 / 2.0 + 1.0) + 60.0 * C * nb);
+    with_nmix(P.nmix, [&](auto nm) {
+        launch(h->kc, k_draw_b<decltype(nm)::value>, dim3((nb + NT - 1) / NT, P.nchains), dim3(NT), 0, h->stream,
+               P, h->scal, h->b_T, tstride, nch, h->b_expo, h->b_rtot, h->b_rapart, h->rows_per_group / 16, h->f2,
+               h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls, h->vara_s, delta, chg,
+               rhs_out, h->b_dmax, h->b_dk, h->dl_sw);
+    });
     return hip_check(hipGetLastError(), "batch draw");
 }
 

@@ -1,5 +1,6 @@
 // Internal state of a SODA sampler handle (device buffers and launchers).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -58,10 +59,12 @@ struct Scratch {
     double *red;       // generic reduction partials
 };
 
-// Per-kernel HIP-event timing on a handle's stream (enabled for a run of
-// sb_mcmc_run by the environment variable SODABAYES_KERNEL_TIMING=1).  Each
-// slot sums the elapsed ms, the launches and the algorithmic bytes of its
-// timed launches.
+// Per-kernel timing on a handle's stream (enabled for a run of sb_mcmc_run by
+// the environment variable SODABAYES_KERNEL_TIMING=1).  Each slot sums the
+// elapsed ms, the launches and the algorithmic bytes of its timed launches.
+// The events are taken by the kernels themselves (hipExtLaunchKernelGGL's
+// start / stop events), so a slot's time is kernel execution from the start
+// of its first kernel to the end of its last, without the dispatch.
 enum KSlot {
     KS_RHS, KS_DRAW, KS_LIST, KS_UPDATE,                          // FP64 path
     KS_RQUANT, KS_XQ, KS_UT0, KS_DRAWB, KS_DQUANT, KS_XUPD, KS_UT1, KS_RARED,  // batched path
@@ -71,10 +74,18 @@ enum KSlot {
 struct KClock {
     bool on = false;
     std::vector<hipEvent_t> ev;   // event pool, reused across runs
-    std::vector<int> slot;        // slot of each recorded pair
-    std::vector<double> bytes;    // algorithmic bytes of each recorded pair
     size_t used = 0;
+    struct Rec {
+        hipEvent_t a, b;
+        int slot;
+        double bytes;
+    };
+    std::vector<Rec> recs;        // timed (start, stop) pairs of this run
     double ms[KS_N] = {}, launches[KS_N] = {}, tbytes[KS_N] = {};
+    bool want_start = false, want_stop = false;
+    hipEvent_t cur_start = nullptr;
+    int stop_slot = 0;
+    double stop_bytes = 0.0;
     // algorithmic bytes that only the device knows (the changed-SNP columns
     // of the FP64 update), added by kernels per slot while timing is on
     double *dextra = nullptr;
@@ -90,35 +101,50 @@ struct KClock {
         }
         return dextra + s;
     }
-    void begin(hipStream_t st)
+    hipEvent_t get()
     {
-        if (!on) return;
         if (used == ev.size()) {
             hipEvent_t e;
             hipEventCreate(&e);
             ev.push_back(e);
         }
-        hipEventRecord(ev[used++], st);
+        return ev[used++];
     }
-    void end(hipStream_t st, int s, double b)
+    // the next launch starts a timed span
+    void begin(hipStream_t) { want_start = on; }
+    // the next launch ends the span, which counts for `slot` with `b` bytes
+    void last(int slot, double b)
     {
-        if (!on) return;
-        begin(st);
-        slot.push_back(s);
-        bytes.push_back(b);
+        want_stop = on;
+        stop_slot = slot;
+        stop_bytes = b;
+    }
+    hipEvent_t take_start()
+    {
+        if (!want_start) return nullptr;
+        want_start = false;
+        cur_start = get();
+        return cur_start;
+    }
+    hipEvent_t take_stop()
+    {
+        if (!want_stop) return nullptr;
+        want_stop = false;
+        hipEvent_t e = get();
+        recs.push_back({cur_start, e, stop_slot, stop_bytes});
+        return e;
     }
     // after the stream has drained
     void collect()
     {
-        for (size_t k = 0; k < slot.size(); k++) {
+        for (const Rec &r : recs) {
             float t = 0.f;
-            hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]);
-            ms[slot[k]] += t;
-            launches[slot[k]] += 1.0;
-            tbytes[slot[k]] += bytes[k];
+            hipEventElapsedTime(&t, r.a, r.b);
+            ms[r.slot] += t;
+            launches[r.slot] += 1.0;
+            tbytes[r.slot] += r.bytes;
         }
-        slot.clear();
-        bytes.clear();
+        recs.clear();
         used = 0;
         if (dextra) {
             double x[KS_N];
@@ -133,6 +159,22 @@ struct KClock {
         if (dextra) hipFree(dextra);
     }
 };
+
+// Launches kernel k; while the clock is on, through hipExtLaunchKernelGGL
+// with the clock's pending start / stop events.  Every argument is converted
+// to the kernel's own parameter type first.
+template <typename... P, typename... A>
+void launch(KClock &kc, void (*k)(P...), dim3 grid, dim3 block, uint32_t shm, hipStream_t st, A &&...a)
+{
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    if (kc.on) {
+        hipEvent_t e0 = kc.take_start();
+        hipEvent_t e1 = kc.take_stop();
+        hipExtLaunchKernelGGL(k, grid, block, shm, st, e0, e1, 0u, static_cast<P>(a)...);
+    } else {
+        k<<<grid, block, shm, st>>>(static_cast<P>(a)...);
+    }
+}
 
 }  // namespace sb
 

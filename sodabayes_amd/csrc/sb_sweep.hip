@@ -1110,28 +1110,27 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 if (rc) return rc;
             } else {
                 kc.begin(st);
-                k_rhs<<<nxg + nag, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, tri0, dense_block(h, b), p0, nb, spw,
-                                                ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
-                kc.end(st, KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
+                kc.last(KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
+                launch(kc, k_rhs, dim3(nxg + nag), dim3(NT), 0, st, P, h->geno, h->r, h->ra, h->U, tri0,
+                       dense_block(h, b), p0, nb, spw, ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
             }
             if (mixed) {
                 int rc = batch_draw(h, b, xxd[b], rep, delta, chg);
                 if (rc) return rc;
             } else {
                 kc.begin(st);
+                kc.last(KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
                 if (P.rng_mode == SB_RNG_REPLAY)
-                    k_draw_replay<<<1, NT, 0, st>>>(P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2,
-                                                    h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp,
-                                                    h->sc.uu, h->sc.chg, h->beta, h->cls, h->vara_s, delta,
-                                                    chg, h->skip);
+                    launch(kc, k_draw_replay, dim3(1), dim3(NT), 0, st, P, h->scal, h->sc.part, h->sc.partA,
+                           h->sc.partR, h->f2, h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp, h->sc.uu,
+                           h->sc.chg, h->beta, h->cls, h->vara_s, delta, chg, h->skip);
                 else
                     with_nmix(P.nmix, [&](auto nm) {
-                        k_draw_counter<decltype(nm)::value><<<(nb + NT - 1) / NT, NT, 0, st>>>(
-                            P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
-                            h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg,
-                            h->skip);
+                        launch(kc, k_draw_counter<decltype(nm)::value>, dim3((nb + NT - 1) / NT), dim3(NT), 0, st,
+                               P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
+                               h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg,
+                               h->skip);
                     });
-                kc.end(st, KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
             }
             if (mixed) {
                 int rc = batch_update(h, b, delta);
@@ -1140,16 +1139,16 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 const int nrw = (P.n + 255) / 256;
                 const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
                 kc.begin(st);
-                k_list<<<dim3(1, 1, C), NT, 0, st>>>(P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
-                                                     h->u_cl, h->u_cnt, h->u_kc, kc.extra(KS_UPDATE));
-                kc.end(st, KS_LIST, 25.0 * nb);
+                kc.last(KS_LIST, 25.0 * nb);
+                launch(kc, k_list, dim3(1, 1, C), dim3(NT), 0, st, P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst,
+                       h->u_dl, h->u_cl, h->u_cnt, h->u_kc, kc.extra(KS_UPDATE));
                 kc.begin(st);
-                k_update<<<dim3(nrw + nra, 1, C), UW * WAVE, 0, st>>>(P, h->geno, h->U, tri0, dense_block(h, b),
-                                                                      p0, nb, nrw,
-                                                                      h->u_lst, h->u_dl, h->u_cl, h->u_cnt,
-                                                                      h->u_kc, h->r, h->u_rapart);
-                k_ra_sum<<<dim3((nb + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->u_rapart, h->u_cnt, nb, h->ra);
-                kc.end(st, KS_UPDATE, 16.0 * P.n + 16.0 * nb);
+                launch(kc, k_update, dim3(nrw + nra, 1, C), dim3(UW * WAVE), 0, st, P, h->geno, h->U, tri0,
+                       dense_block(h, b), p0, nb, nrw, h->u_lst, h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->r,
+                       h->u_rapart);
+                kc.last(KS_UPDATE, 16.0 * P.n + 16.0 * nb);
+                launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt, nb,
+                       h->ra);
             }
         }
         if (h->nskipped > 0) {
