@@ -171,6 +171,17 @@ __device__ __forceinline__ int wave_sum_i(int v)
     return v;
 }
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup L on XCD
+// L % 8), and each XCD has its own L2.  Renumbers the first 8 * (n / 8) of
+// n workgroups so that every XCD takes one contiguous range of logical ids:
+// neighbouring tiles that share cache lines or operand slices then meet in
+// one L2 instead of being fetched by every XCD.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_contiguous(int L, int n)
+{
+    const int m = (n / 8) * 8;
+    return L < m ? (L % 8) * (m / 8) + L / 8 : L;
+}
+
 // Fixed-order block reduction (deterministic for a fixed blockDim).
 template <int NT>
 __device__ double block_sum(double v, double *lds)

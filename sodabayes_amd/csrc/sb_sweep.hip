@@ -436,7 +436,10 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if ((int)blockIdx.x < nrw) {
         r += c * P.ldr;
-        const int i0 = blockIdx.x * 256 + 4 * lane;       // first of 4 individuals
+        // neighbouring 256-individual ranges share 128-byte genotype lines:
+        // keep them on one XCD (one L2)
+        const int bx = xcd_contiguous((int)blockIdx.x, nrw);
+        const int i0 = bx * 256 + 4 * lane;       // first of 4 individuals
         const int64_t byte = i0 >> 2;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         // each wave walks its own run of the list; 64 entries at a time are
