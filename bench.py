@@ -43,15 +43,17 @@ KERNEL_NAMES = {
 }
 
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2, the factor a 1 GiB
-# device copy shows on this GPU) + WRITE_SIZE (x1), per kernel slot, measured
-# on 4,096-SNP blocks at n = 50,000 with 16 chains (the per-launch shape of this
-# workload; profiles/soda_bayesr_n50k_bs4096_x16_pmc_traffic.md); None where
-# not measured
+# device copy shows on this GPU) + WRITE_SIZE (x1), per (path, kernel slot),
+# measured on 4,096-SNP blocks at n = 50,000 (16 chains for the batched path),
+# the per-launch shape of this workload
+# (profiles/soda_bayesr_n50k_bs4096_x16_pmc_traffic.md); None where not measured
 TRAFFIC_CHAINS = 16
 TRAFFIC = {
-    "xq": int((52670 * 2 + 14336) * 1024),     # k_prod<3,4,1>
-    "xupd": int((46982 * 2 + 11002) * 1024),   # k_upd<3,4,1>
-    "drawb": int((7794 * 2 + 1616) * 1024),    # k_draw_b<4>
+    ("mixed", "xq"): int((52665 * 2 + 14336) * 1024),     # k_prod<3,4,1,2>
+    ("mixed", "xupd"): int((46985 * 2 + 11002) * 1024),   # k_upd<3,4,1,2>
+    ("mixed", "drawb"): int((7754 * 2 + 1616) * 1024),    # k_draw_b<4>
+    ("fp64", "rhs"): int((60273 * 2 + 891) * 1024),       # k_rhs
+    ("fp64", "update"): int((50600 * 2 + 807) * 1024),    # k_update (k_ra_sum not counted)
 }
 
 
@@ -205,8 +207,8 @@ def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int):
         out["roofline"] = {
             "bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": TRAFFIC.get(dom) if (precision == "mixed" and chains == TRAFFIC_CHAINS
-                                            and a.nind == 50000 and a.blocksize == 4096) else None,
+            "traffic": TRAFFIC.get((precision, dom)) if (chains == (TRAFFIC_CHAINS if precision == "mixed" else 1)
+                                                         and a.nind == 50000 and a.blocksize == 4096) else None,
             "kernel": d["name"], "avg_launch_us": d["avg_us"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
         for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["share"]):
             log(f"[{precision} x{chains}]   {v['name']:<34s} {v['avg_us']:8.2f} us/launch  {v['gbs']:7.0f} GB/s"
