@@ -113,6 +113,20 @@ int sb_build_aug(sb_handle *h, double *xxd, int32_t *fail_block, int32_t *fail_c
 
 /* Initial state, models.f90:56-128: why has nind values. */
 int sb_mcmc_init(sb_handle *h, const sb_priors *pr, const double *why);
+/* Batched traits and cross-validation folds (no counterpart in the reference,
+ * whose jobs are one phenotype on all individuals): the initial state of
+ * every chain from its own phenotype, chain c's nind values at why + c*ldy
+ * (ldy = 0: one column for all chains, as sb_mcmc_init).  heldout (may be
+ * NULL): chain c's nind flags at heldout + c*ldh; a nonzero flag marks an
+ * individual whose phenotype chain c does not use (a validation row).  Such
+ * a row's phenotype is treated as missing: it starts at the chain's training
+ * mean and is redrawn every iteration from its full conditional
+ * y ~ N(mu + x'g, vare), i.e. its residual becomes a fresh N(0, vare) draw
+ * after vare's update, so the chain samples the posterior given the
+ * training rows while the augmenting matrices stay shared.  Held-out rows
+ * need SB_RNG_COUNTER. */
+int sb_mcmc_init_chains(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
+                        const uint8_t *heldout, int64_t ldh);
 /* Runs iterations rep = first .. first+count-1 (models.f90:129-574): the
  * sweep, hyperparameter draws, posterior accumulation and the residual
  * resync every 1000 iterations.  hyp (optional, host) receives one record
@@ -148,7 +162,8 @@ enum {
                             SB_KT_N x (summed ms, launches, algorithmic bytes), slot order below */
     SB_Q_SKIPPED,        /* SNPs skipped by the pivoted-Cholesky fallback, 1-based ids in update
                             order (i_skipped, soda.f90:505-507) (nskipped) */
-    SB_Q_NAIND           /* naind, nskipped (2) (soda.f90:440, 505, 514) */
+    SB_Q_NAIND,          /* naind, nskipped (2) (soda.f90:440, 505, 514) */
+    SB_Q_WHY             /* the chain's phenotype (nind), held-out rows at their current draw */
 };
 
 /* Kernel timing slots of SB_Q_KTIMES (diagnostics; no counterpart in the

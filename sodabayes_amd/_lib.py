@@ -20,15 +20,15 @@ PREC_FP64, PREC_MIXED = 0, 1
 
 Q = dict(G=1, CLASS=2, VARA_S=3, YADJ=4, YAADJ=5, P=6, SCALARS=7, GSTORE=8, INDISTSTORE=9,
          VARA_SSTORE=10, STORES=11, SNPSTORE=12, VARSTORE=13, PSTORE=14, XA=15, XXD=16, GRAM=17,
-         SNPINDIST=18, VARINDIST=19, KTIMES=20, SKIPPED=21, NAIND=22)
+         SNPINDIST=18, VARINDIST=19, KTIMES=20, SKIPPED=21, NAIND=22, WHY=23)
 
 # kernel timing slots of the KTIMES query (order of SB_KT_* in the header)
 KT_SLOTS = ["rhs", "draw", "list", "update", "rquant", "xq", "ut0", "drawb", "dquant", "xupd", "ut1", "rared"]
 
 # every function include/sodabayes.h declares
 EXPORTS = ["sb_rng_seed", "sb_permutate", "sb_create", "sb_destroy", "sb_rng_set", "sb_rng_get",
-           "sb_build_aug", "sb_mcmc_init", "sb_mcmc_run", "sb_hyp_width", "sb_get", "sb_block_rhs",
-           "sb_get_info", "sb_predict", "sb_last_error"]
+           "sb_build_aug", "sb_mcmc_init", "sb_mcmc_init_chains", "sb_mcmc_run", "sb_hyp_width", "sb_get",
+           "sb_block_rhs", "sb_get_info", "sb_predict", "sb_last_error"]
 
 
 class SbDims(C.Structure):
@@ -72,6 +72,7 @@ def load():
     L.sb_rng_get.argtypes = [vp, u64p]
     L.sb_build_aug.argtypes = [vp, dp, i32p, i32p]
     L.sb_mcmc_init.argtypes = [vp, C.POINTER(SbPriors), dp]
+    L.sb_mcmc_init_chains.argtypes = [vp, C.POINTER(SbPriors), dp, C.c_int64, C.POINTER(C.c_uint8), C.c_int64]
     L.sb_mcmc_run.argtypes = [vp, C.c_int32, C.c_int32, dp]
     L.sb_hyp_width.argtypes = [vp]
     L.sb_hyp_width.restype = C.c_int32

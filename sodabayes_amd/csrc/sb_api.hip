@@ -228,7 +228,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->d_ls, sizeof(int32_t) * (d.nblocks + 1));
     A((void **)&h->r, sizeof(double) * ldr * C);
     A((void **)&h->ra, sizeof(double) * bs * C);
-    A((void **)&h->why, sizeof(double) * ldr);
+    A((void **)&h->why, sizeof(double) * ldr * C);
     A((void **)&h->beta, sizeof(double) * p * C);
     A((void **)&h->cls, sizeof(int32_t) * p * C);
     A((void **)&h->vara_s, sizeof(double) * p * C);
@@ -403,6 +403,15 @@ int sb_mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
     return mcmc_init(h, pr, why);
 }
 
+int sb_mcmc_init_chains(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
+                        const uint8_t *heldout, int64_t ldh)
+{
+    if (!h || !pr || !why) return fail(SB_ERR_ARG, "sb_mcmc_init_chains: null argument");
+    if (ldy < 0 || (ldy > 0 && ldy < h->par.n) || (heldout && ldh < 0) || (heldout && ldh > 0 && ldh < h->par.n))
+        return fail(SB_ERR_ARG, "sb_mcmc_init_chains: column stride below nind");
+    return mcmc_init(h, pr, why, ldy, heldout, ldh);
+}
+
 int sb_mcmc_run(sb_handle *h, int32_t first, int32_t count, double *hyp)
 {
     if (!h) return fail(SB_ERR_ARG, "sb_mcmc_run: null handle");
@@ -454,6 +463,11 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
         if (need(P.n)) return SB_ERR_ARG;
         return hip_check(hipMemcpy(out, h->r + (int64_t)chain * P.ldr, sizeof(double) * P.n, hipMemcpyDeviceToHost),
                          "get r");
+    case SB_Q_WHY:
+        if (need(P.n)) return SB_ERR_ARG;
+        return hip_check(hipMemcpy(out, h->why + (int64_t)chain * P.ldr, sizeof(double) * P.n,
+                                   hipMemcpyDeviceToHost),
+                         "get why");
     case SB_Q_YAADJ:
         if (need(P.blocksize)) return SB_ERR_ARG;
         return hip_check(hipMemcpy(out, h->ra + (int64_t)chain * P.blocksize, sizeof(double) * P.blocksize,

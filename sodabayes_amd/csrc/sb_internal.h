@@ -217,7 +217,9 @@ struct sb_handle {
     // chain state (nchains copies)
     double *r = nullptr;          // ldw*16 per chain
     double *ra = nullptr;         // blocksize per chain
-    double *why = nullptr;        // ldw*16
+    double *why = nullptr;        // ldr per chain: phenotype (held-out rows: current draw)
+    uint8_t *heldout = nullptr;   // ldr per chain, nonzero = validation row (imputed), or null
+    bool use_heldout = false;     // the current chains have held-out rows
     double *beta = nullptr;       // nloci per chain (position order)
     int32_t *cls = nullptr;       // nloci per chain
     double *vara_s = nullptr;     // nloci per chain
@@ -285,7 +287,8 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
 int setup_gram_block(sb_handle *h, int block, double *G_host);
 
 // MCMC (sb_sweep.hip)
-int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why);
+int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy = 0,
+              const uint8_t *heldout = nullptr, int64_t ldh = 0);
 int mcmc_run(sb_handle *h, int first, int count, double *hyp);
 
 // chain-batched mixed-precision block products and draws (sb_batch.hip)

@@ -11,6 +11,7 @@
 // z = (x - 2f)/sqrt(2f(1-f)) (utilities.f90:531) from a 3-entry table.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -25,7 +26,7 @@ namespace sb {
 namespace {
 
 constexpr int NT = 256;
-enum { TAG_MU = 1, TAG_SNP = 2, TAG_HYP = 3, TAG_RA = 4, TAG_INIT = 5 };
+enum { TAG_MU = 1, TAG_SNP = 2, TAG_HYP = 3, TAG_RA = 4, TAG_INIT = 5, TAG_IMPUTE = 6 };
 
 
 
@@ -756,6 +757,7 @@ __global__ __launch_bounds__(NT) void k_resync(Par P, const uint32_t *geno, cons
     int w = blockIdx.x * NT + threadIdx.x;
     if (w >= P.ldw || w * 16 >= P.n) return;
     const int64_t ch = blockIdx.z;
+    why += ch * P.ldr;
     beta += ch * P.p;
     list += ch * P.p;
     count += ch;
@@ -788,13 +790,14 @@ __global__ __launch_bounds__(NT) void k_resync(Par P, const uint32_t *geno, cons
 
 // Draws of the initial state (models.f90:115-120): sigma2_e from y'y of the
 // initial residuals, then r_a.
-__global__ void k_init_draws(Par P, Scal *sc, double yty)
+// (each chain's y'y of its initial residuals is in its sc->rtr)
+__global__ void k_init_draws(Par P, Scal *sc)
 {
     const int c = threadIdx.x;
     if (c >= P.nchains) return;
     sc += c;
     double nn = (double)P.n;
-    double scale = (yty + P.scale_ve * P.df_ve) / (nn + P.df_ve);
+    double scale = (sc->rtr + P.scale_ve * P.df_ve) / (nn + P.df_ve);
     if (P.rng_mode == SB_RNG_REPLAY) {
         XoroSrc src{{sc->rng[0], sc->rng[1]}};
         sc->vare = draw_sinvchi2(src, nn + P.df_ve, scale);
@@ -806,10 +809,29 @@ __global__ void k_init_draws(Par P, Scal *sc, double yty)
     }
 }
 
-__global__ void k_init_r(Par P, const double *why, double mu, double *r)
+__global__ void k_init_r(Par P, const double *why, const Scal *sc, double *r)
 {
     int i = blockIdx.x * NT + threadIdx.x;
-    if (i < P.n) r[(int64_t)blockIdx.z * P.ldr + i] = (why[i] - 0.0) - mu;
+    const int64_t c = blockIdx.z;
+    if (i < P.n) r[c * P.ldr + i] = (why[c * P.ldr + i] - 0.0) - sc[c].mu;
+}
+
+// Held-out rows (validation individuals of a fold) are missing phenotypes:
+// given mu, g and vare their full conditional is y_i ~ N(mu + z_i'g, vare),
+// so after vare's update every held-out y_i is redrawn, which makes its
+// residual a fresh N(0, vare) draw.  The phenotype keeps the drawn value
+// (y = y - r + e), so that the resync recomputes the same residual.
+__global__ void k_impute(Par P, const Scal *sc, const uint8_t *heldout, double *why, double *r, int rep)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int64_t c = blockIdx.z;
+    if (i >= P.n) return;
+    const int64_t o = c * P.ldr + i;
+    if (!heldout[o]) return;
+    PhiloxSrc src(chain_key(P, (int)c), (uint32_t)rep, (uint32_t)i, TAG_IMPUTE);
+    const double e = draw_normal(src, 0.0, sqrt(sc[c].vare));
+    why[o] = (why[o] - r[o]) + e;
+    r[o] = e;
 }
 
 // -------------------------------------------------------- skipped SNPs --
@@ -919,10 +941,13 @@ static const double *dense_block(const sb_handle *h, int b)
     return (b < (int)h->dense_off.size() && h->dense_off[b] >= 0) ? h->Ud + h->dense_off[b] : nullptr;
 }
 
-int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
+int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy, const uint8_t *heldout,
+              int64_t ldh)
 {
     if (!h->aug_built) return fail(SB_ERR_STATE, "sb_mcmc_init before sb_build_aug");
     Par &P = h->par;
+    if (heldout && P.rng_mode != SB_RNG_COUNTER)
+        return fail(SB_ERR_ARG, "held-out rows need SB_RNG_COUNTER (their draws have no reference order)");
     P.pizero = pr->pizero;
     P.scale_va0 = pr->scale_va;
     P.scale_ve = pr->scale_ve;
@@ -942,71 +967,104 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
     P.thin = pr->thin > 0 ? pr->thin : 1;
     if (P.rng_mode == SB_RNG_COUNTER) P.seed = (uint64_t)pr->seed;
     const int n = P.n, p = P.p, nmix = P.nmix, ngroup = P.ngroup;
+    const int C = P.nchains;
+    const size_t Cs = (size_t)C;
 
-    // Host part of models.f90:56-114 (no draws): sums in the reference's
-    // sequential order so the initial scalars are the reference's doubles.
-    Scal sc;
-    std::memset(&sc, 0, sizeof sc);
-    if (hip_check(hipMemcpy(&sc.rng, &h->scal->rng, sizeof sc.rng, hipMemcpyDeviceToHost), "rng state"))
+    // Host part of models.f90:56-114 (no draws), per chain: sums in the
+    // reference's sequential order so the initial scalars are the
+    // reference's doubles.  A chain's held-out rows enter at the mean of its
+    // training rows (residual 0) and are redrawn every iteration.
+    Scal base;
+    std::memset(&base, 0, sizeof base);
+    if (hip_check(hipMemcpy(&base.rng, &h->scal->rng, sizeof base.rng, hipMemcpyDeviceToHost), "rng state"))
         return SB_ERR_HIP;
     auto IDX = [&](int g, int k) { return k * ngroup + g; };
     if (P.method != SB_BAYES_A) {
         if (P.method == SB_BAYES_B || P.method == SB_BAYES_C) {
             for (int g = 0; g < ngroup; g++) {
-                sc.P[IDX(g, 0)] = P.pizero;
-                sc.P[IDX(g, 1)] = 1.0 - P.pizero;
+                base.P[IDX(g, 0)] = P.pizero;
+                base.P[IDX(g, 1)] = 1.0 - P.pizero;
             }
         } else {
             for (int g = 0; g < ngroup; g++) {
-                sc.P[IDX(g, 0)] = 0.5;
+                base.P[IDX(g, 0)] = 0.5;
                 double s = 0.0;
-                for (int k = 1; k < nmix; k++) sc.P[IDX(g, k)] = 1.0 / P.mix[k];
-                for (int k = 1; k < nmix; k++) s += sc.P[IDX(g, k)];
-                for (int k = 1; k < nmix; k++) sc.P[IDX(g, k)] = 0.5 * sc.P[IDX(g, k)] / s;
+                for (int k = 1; k < nmix; k++) base.P[IDX(g, k)] = 1.0 / P.mix[k];
+                for (int k = 1; k < nmix; k++) s += base.P[IDX(g, k)];
+                for (int k = 1; k < nmix; k++) base.P[IDX(g, k)] = 0.5 * base.P[IDX(g, k)] / s;
             }
         }
     }
-    double nn = (double)n, sw = 0.0;
-    for (int i = 0; i < n; i++) sw += why[i];
-    sc.mu = sw / nn;
-    double vara = P.vara0, scale_va = P.scale_va0;
-    if (P.df_va <= 0.0) {
-        double yhat = sw / nn, vary = 0.0;
-        for (int i = 0; i < n; i++) vary += (why[i] - yhat) * (why[i] - yhat);
-        vary = vary / (nn - 1.0);
-        if (P.method == SB_BAYES_A) {
-            vara = scale_va * vary / p;
-        } else if (P.method == SB_BAYES_B) {
-            vara = scale_va * vary / p / sc.P[IDX(0, 1)];
+    std::vector<Scal> scs(Cs, base);
+    std::vector<double> ys((size_t)P.ldr * Cs, 0.0);
+    std::vector<uint8_t> held;
+    if (heldout) held.assign((size_t)P.ldr * Cs, 0);
+    const double nn = (double)n;
+    for (int c = 0; c < C; c++) {
+        const double *yc = why + (int64_t)c * ldy;
+        double *y = ys.data() + (size_t)c * P.ldr;
+        if (heldout) {
+            const uint8_t *hc = heldout + (int64_t)c * ldh;
+            double st = 0.0;
+            int nt = 0;
+            for (int i = 0; i < n; i++)
+                if (!hc[i]) {
+                    st += yc[i];
+                    nt++;
+                }
+            if (nt == 0) return fail(SB_ERR_ARG, "a chain has no training rows");
+            const double tmean = st / nt;
+            for (int i = 0; i < n; i++) {
+                held[(size_t)c * P.ldr + i] = hc[i] ? 1 : 0;
+                y[i] = hc[i] ? tmean : yc[i];
+            }
         } else {
-            for (int g = 0; g < ngroup; g++) {
-                double d = 0.0;
-                for (int k = 0; k < nmix; k++) d += sc.P[IDX(g, k)] * P.mix[k];
-                vara = vara + d;
-            }
-            vara = scale_va * vary / p / vara;
+            for (int i = 0; i < n; i++) y[i] = yc[i];
         }
-        scale_va = 0.0;
-    } else {
-        vara = scale_va * P.df_va / (P.df_va + 2.0);
+        Scal &sc = scs[c];
+        double sw = 0.0;
+        for (int i = 0; i < n; i++) sw += y[i];
+        sc.mu = sw / nn;
+        double vara = P.vara0, scale_va = P.scale_va0;
+        if (P.df_va <= 0.0) {
+            double yhat = sw / nn, vary = 0.0;
+            for (int i = 0; i < n; i++) vary += (y[i] - yhat) * (y[i] - yhat);
+            vary = vary / (nn - 1.0);
+            if (P.method == SB_BAYES_A) {
+                vara = scale_va * vary / p;
+            } else if (P.method == SB_BAYES_B) {
+                vara = scale_va * vary / p / sc.P[IDX(0, 1)];
+            } else {
+                for (int g = 0; g < ngroup; g++) {
+                    double d = 0.0;
+                    for (int k = 0; k < nmix; k++) d += sc.P[IDX(g, k)] * P.mix[k];
+                    vara = vara + d;
+                }
+                vara = scale_va * vary / p / vara;
+            }
+            scale_va = 0.0;
+        } else {
+            vara = scale_va * P.df_va / (P.df_va + 2.0);
+        }
+        sc.vara = vara;
+        sc.scale_va = scale_va;
+        for (int k = 0; k < nmix; k++) sc.gp[k] = P.mix[k] * vara;
+        double yty = 0.0;
+        for (int i = 0; i < n; i++) {
+            double r = (y[i] - 0.0) - sc.mu;
+            yty += r * r;
+        }
+        sc.rtr = yty;   // k_init_draws' input
     }
-    sc.vara = vara;
-    sc.scale_va = scale_va;
-    for (int k = 0; k < nmix; k++) sc.gp[k] = P.mix[k] * vara;
-    double yty = 0.0;
-    for (int i = 0; i < n; i++) {
-        double r = (why[i] - 0.0) - sc.mu;
-        yty += r * r;
+    if (hip_check(hipMemcpyAsync(h->scal, scs.data(), sizeof(Scal) * Cs, hipMemcpyHostToDevice, h->stream),
+                  "init scal"))
+        return SB_ERR_HIP;
+    hipMemcpyAsync(h->why, ys.data(), sizeof(double) * ys.size(), hipMemcpyHostToDevice, h->stream);
+    h->use_heldout = heldout != nullptr;
+    if (heldout) {
+        if (!h->heldout && dev_alloc(h, (void **)&h->heldout, held.size())) return SB_ERR_NOMEM;
+        hipMemcpyAsync(h->heldout, held.data(), held.size(), hipMemcpyHostToDevice, h->stream);
     }
-    // every chain starts from the same state (models.f90:56-114); the draws
-    // that follow differ per chain
-    const int C = P.nchains;
-    const size_t Cs = (size_t)C;
-    for (int c = 0; c < C; c++)
-        if (hip_check(hipMemcpyAsync(h->scal + c, &sc, sizeof sc, hipMemcpyHostToDevice, h->stream), "init scal"))
-            return SB_ERR_HIP;
-    hipMemsetAsync(h->why, 0, sizeof(double) * (size_t)P.ldr, h->stream);
-    hipMemcpyAsync(h->why, why, sizeof(double) * n, hipMemcpyHostToDevice, h->stream);
     hipMemsetAsync(h->r, 0, sizeof(double) * (size_t)P.ldr * Cs, h->stream);
     hipMemsetAsync(h->ra, 0, sizeof(double) * P.blocksize * Cs, h->stream);
     hipMemsetAsync(h->beta, 0, sizeof(double) * p * Cs, h->stream);
@@ -1015,13 +1073,14 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)
     hipMemsetAsync(h->indiststore, 0, sizeof(double) * p * nmix * Cs, h->stream);
     std::vector<int32_t> two(p * Cs, 2);
     hipMemcpyAsync(h->cls, two.data(), sizeof(int32_t) * p * Cs, hipMemcpyHostToDevice, h->stream);
-    std::vector<double> vs(p * Cs, vara);
+    std::vector<double> vs(p * Cs);
+    for (int c = 0; c < C; c++) std::fill(vs.begin() + (size_t)c * p, vs.begin() + (size_t)(c + 1) * p, scs[c].vara);
     hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p * Cs, hipMemcpyHostToDevice, h->stream);
-    k_init_r<<<dim3((n + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->why, sc.mu, h->r);
+    k_init_r<<<dim3((n + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->why, h->scal, h->r);
     if (h->nskipped > 0)
         k_xpx<<<h->nskipped, NT, 0, h->stream>>>(P, h->geno, h->zlut, h->d_skip_list, h->xpx);
     h->rstat_nwg = 0;
-    k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal, yty);
+    k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal);
     if (P.rng_mode == SB_RNG_REPLAY)
         k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
     else
@@ -1172,6 +1231,10 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
         else
             k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA,
                                                                              h->ra_sw);
+        if (h->use_heldout) {
+            k_impute<<<dim3((P.n + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->heldout, h->why, h->r, rep);
+            h->rstat_nwg = 0;
+        }
         if (store)
             k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->vara_s, h->gstore,
                                                                    h->indiststore, h->vara_sstore);

@@ -18,7 +18,7 @@ module soda_gpu
         SB_Q_INDISTSTORE = 9, SB_Q_VARA_SSTORE = 10, SB_Q_STORES = 11, &
         SB_Q_SNPSTORE = 12, SB_Q_VARSTORE = 13, SB_Q_PSTORE = 14, SB_Q_XA = 15, &
         SB_Q_XXD = 16, SB_Q_GRAM = 17, SB_Q_SNPINDIST = 18, SB_Q_VARINDIST = 19, &
-        SB_Q_KTIMES = 20, SB_Q_SKIPPED = 21, SB_Q_NAIND = 22
+        SB_Q_KTIMES = 20, SB_Q_SKIPPED = 21, SB_Q_NAIND = 22, SB_Q_WHY = 23
 
     type, bind(C) :: sb_dims
         integer(c_int32_t) :: method, nind, nloci, nmix, ngroup
@@ -80,6 +80,15 @@ module soda_gpu
             type(c_ptr), value :: h
             type(sb_priors), intent(in) :: pr
             real(c_double), intent(in) :: why(*)
+        end function
+        integer(c_int) function sb_mcmc_init_chains(h, pr, why, ldy, heldout, ldh) &
+                bind(C, name='sb_mcmc_init_chains')
+            import :: c_int, c_int64_t, c_double, c_ptr, sb_priors
+            type(c_ptr), value :: h
+            type(sb_priors), intent(in) :: pr
+            real(c_double), intent(in) :: why(*)
+            integer(c_int64_t), value :: ldy, ldh
+            type(c_ptr), value :: heldout
         end function
         integer(c_int) function sb_mcmc_run(h, first, count, hyp) bind(C, name='sb_mcmc_run')
             import :: c_int, c_int32_t, c_double, c_ptr

@@ -163,6 +163,11 @@ class SodaSampler:
     ``bed`` is the SNP-major PLINK payload (nloci x ceil(nind/4)); ``grp`` the
     SNP group labels for BayesRc.  ``bed_device_ptr`` (optional) passes a
     device pointer to the same payload instead of host memory.
+
+    Batched traits and folds: ``why`` may be (chains, nind), one phenotype
+    per chain, and ``heldout`` (nind) or (chains, nind) flags validation
+    individuals whose phenotypes a chain does not use (imputed every
+    iteration; sb_mcmc_init_chains).
     """
     opts: SodaOptions
     nind: int
@@ -172,6 +177,7 @@ class SodaSampler:
     grp: object = None
     freq: np.ndarray | None = None
     bed_device_ptr: int | None = None
+    heldout: np.ndarray | None = None
     handle: C.c_void_p = field(default=None, init=False)
 
     def __post_init__(self):
@@ -255,8 +261,24 @@ class SodaSampler:
             pr.alpha[k] = a
         pr.numit, pr.burnin, pr.thin, pr.seed = o.numit, o.burnin, o.thin, int(o.seed)
         self._why = np.ascontiguousarray(self.why, dtype=np.float64)
-        _lib.check(_lib.load().sb_mcmc_init(self.handle, C.byref(pr),
-                                            self._why.ctypes.data_as(C.POINTER(C.c_double))))
+        chains = int(o.chains)
+        if self._why.ndim == 1 and self.heldout is None:
+            _lib.check(_lib.load().sb_mcmc_init(self.handle, C.byref(pr),
+                                                self._why.ctypes.data_as(C.POINTER(C.c_double))))
+        else:
+            if self._why.shape not in ((self.nind,), (chains, self.nind)):
+                raise ValueError(f"why must be ({self.nind},) or ({chains}, {self.nind})")
+            ldy = self.nind if self._why.ndim == 2 else 0
+            hp, ldh = None, 0
+            if self.heldout is not None:
+                self._held = np.ascontiguousarray(self.heldout, dtype=np.uint8)
+                if self._held.shape not in ((self.nind,), (chains, self.nind)):
+                    raise ValueError(f"heldout must be ({self.nind},) or ({chains}, {self.nind})")
+                hp = self._held.ctypes.data_as(C.POINTER(C.c_uint8))
+                ldh = self.nind if self._held.ndim == 2 else 0
+            _lib.check(_lib.load().sb_mcmc_init_chains(self.handle, C.byref(pr),
+                                                       self._why.ctypes.data_as(C.POINTER(C.c_double)), ldy,
+                                                       hp, ldh))
         self.rep = 0
 
     def run(self, count: int | None = None, trace: bool = True) -> np.ndarray:
