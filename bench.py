@@ -181,6 +181,11 @@ def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int):
     out = {"setup_s": setup_s, "power_iters": int(info[11]), "dt": dt, "ms": dt * 1e3 / a.steps,
            "value": p * chains * a.steps / dt}
     log(f"[{precision} x{chains}] {out['ms']:.2f} ms/iteration, {out['value']:.4g} SNP-updates/s")
+    if precision == "mixed":
+        inf = s.info()
+        out["tile_shape"] = {"chunk_supersteps": inf["chunk_supersteps"], "rows_per_wave": inf["rows_per_wave"]}
+        log(f"[{precision} x{chains}] tile shape: {inf['chunk_supersteps']} super-steps per X_s'Q chunk, "
+            f"{inf['rows_per_wave']} row tiles per wave (tuned: {inf['tuned_block_us']:.1f} us for block 0)")
     if troof:
         # separate pass with HIP events around every block kernel
         os.environ["SODABAYES_KERNEL_TIMING"] = "1"
@@ -261,6 +266,7 @@ def main():
         "setup_seconds": round(mixed["setup_s"], 2),
         "power_method_iterations": mixed["power_iters"],
         "iterations_per_s": a.steps / mixed["dt"],
+        "tile_shape": mixed.get("tile_shape"),
         "roofline": mixed.get("roofline"),
         "cpu_baseline": cpu,
     }

@@ -189,7 +189,10 @@ int sb_block_rhs(sb_handle *h, int32_t block, double *out);
  * HIP-event milliseconds and launch counts of the block kernels:
  * rhs ms, rhs launches, rhs algorithmic bytes, draw ms, draw launches,
  * update ms, update launches; then the largest power-method iteration
- * count of sb_build_aug]. */
+ * count of sb_build_aug; then the batched path's tile shape: super-steps per
+ * X_s'Q chunk, row tiles per wave, and the tuned block-0 products + update
+ * microseconds (0 when the shape was fixed by SODABAYES_KSS /
+ * SODABAYES_ROWS_PER_WAVE)]. */
 int sb_get_info(sb_handle *h, double *out, int32_t n);
 
 /* ---- prediction (replaces compute_dgv, utilities.f90:616-622, after

@@ -268,9 +268,18 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         h->b_ldt = 16 * h->b_ntile;
         const int nS = P.ldw / 16;
         h->kss = sb::batch_kss(h->b_ntile);
-        if (const char *e = std::getenv("SODABAYES_KSS")) h->kss = std::max(1, std::atoi(e));
-        if (const char *e = std::getenv("SODABAYES_ROWS_PER_WAVE")) h->b_rw = std::atoi(e) == 4 ? 4 : 2;
-        const int nch = (nS + h->kss - 1) / h->kss;
+        // the chunk partials are sized for the shortest chunk the tuner tries
+        int kss_min = sb::batch_min_kss(h->b_ntile);
+        if (const char *e = std::getenv("SODABAYES_KSS")) {
+            h->kss = std::max(1, std::atoi(e));
+            kss_min = std::min(kss_min, h->kss);
+            h->tuned = true;
+        }
+        if (const char *e = std::getenv("SODABAYES_ROWS_PER_WAVE")) {
+            h->b_rw = std::atoi(e) == 4 ? 4 : 2;
+            h->tuned = true;
+        }
+        const int nch = (nS + kss_min - 1) / kss_min;
         const int nwg_d = (int)((bs + 255) / 256);
         const int ntc = (int)((C + 15) / 16);
         const int64_t rows16 = 16 * ((bs + 15) / 16);
@@ -582,9 +591,10 @@ int sb_get_info(sb_handle *h, double *out, int32_t n)
     const double upd_ms = k.ms[sb::KS_LIST] + k.ms[sb::KS_UPDATE] + k.ms[sb::KS_DQUANT] + k.ms[sb::KS_XUPD] +
                           k.ms[sb::KS_UT1] + k.ms[sb::KS_RARED];
     const double upd_n = k.launches[sb::KS_UPDATE] + k.launches[sb::KS_XUPD];
-    double v[12] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes,
-                    prod_ms, prod_n, prod_b, draw_ms, draw_n, upd_ms, upd_n, h->pm_iters_max};
-    for (int i = 0; i < n && i < 12; i++) out[i] = v[i];
+    double v[15] = {h->setup_seconds, (double)h->iterations, h->last_sweep_ms, (double)h->bytes,
+                    prod_ms, prod_n, prod_b, draw_ms, draw_n, upd_ms, upd_n, h->pm_iters_max,
+                    (double)h->kss, (double)h->b_rw, h->tune_us};
+    for (int i = 0; i < n && i < 15; i++) out[i] = v[i];
     return SB_OK;
 }
 

@@ -973,6 +973,60 @@ int batch_words(int nb) { return ((nb + 255) / 256) * 16; }
 // workgroups cover 4x the rows (measured best at 4,096 SNPs x 50,000)
 int batch_kss(int ntile) { return prod_lds(ntile) ? 16 : 32; }
 
+// X_s'Q beside U_s'R_a of block b with row tiles per wave rw and chunks of
+// kss super-steps (one launch, through the handle's kernel clock)
+int prod_launch(sb_handle *h, int b, int rw_sel, int kss)
+{
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
+    const int C = P.nchains, nS = P.ldw / 16;
+    const uint4 *Q = reinterpret_cast<const uint4 *>(h->b_Q);
+    const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
+    const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
+    const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
+    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut0) + h->tt_off[b] * 64;
+    KClock &kc = h->kc;
+    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
+        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
+        with_rows<NTL>(rw_sel, [&](auto rw) {
+            constexpr int RW = decltype(rw)::value;
+            const int rt = prod_lds(NTL) ? 4 * RW : IT;   // row tiles per workgroup
+            const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
+            const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
+            launch(kc, k_prod<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss, h->b_T,
+                   tstride, g0x, n0, ut, reinterpret_cast<const float4 *>(h->ra_sw), spant, h->b_rapart, g1x, n1);
+        });
+    });
+}
+
+// r += X_s C - K beside U_s D of block b with row tiles per wave rw, on the
+// residuals r (the handle's, or a scratch copy while tuning)
+int upd_launch(sb_handle *h, int b, int rw_sel, double *r)
+{
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
+    const int C = P.nchains, L = h->limbs;
+    const int nsw = batch_words(nb);
+    const uint4 *Dq = reinterpret_cast<const uint4 *>(h->b_Dq);
+    const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
+    const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
+    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut1) + h->tt_off[b] * 64;
+    KClock &kc = h->kc;
+    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
+        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
+        with_rows<NTL>(rw_sel, [&](auto rw) {
+            constexpr int RW = decltype(rw)::value;
+            const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * RW : 16 * IT));
+            const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
+            launch(kc, k_upd<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gt, nsw / 16, Dq, L, h->b_expo2,
+                   h->b_kc, r, h->b_rsum, h->b_rmax, n0, ut, nbt, nb, reinterpret_cast<const float4 *>(h->dl_sw),
+                   spant, h->b_uapart, g1x, n1);
+        });
+    });
+}
+
 int batch_block(sb_handle *h, int b, double zz, int rep)
 {
     (void)zz;
@@ -998,24 +1052,9 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
                h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_rtot);
     }
     h->q_block = -1;
-    const int kss = h->kss, nS = P.ldw / 16;
-    const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
-    const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
-    const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
-    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut0) + h->tt_off[b] * 64;
     kc.begin(st);
     kc.last(KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 12.0 * C * nb);
-    int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
-        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
-        with_rows<NTL>(h->b_rw, [&](auto rw) {
-            constexpr int RW = decltype(rw)::value;
-            const int rt = prod_lds(NTL) ? 4 * RW : IT;   // row tiles per workgroup
-            const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
-            const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
-            launch(kc, k_prod<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss, h->b_T,
-                   tstride, g0x, n0, ut, reinterpret_cast<const float4 *>(h->ra_sw), spant, h->b_rapart, g1x, n1);
-        });
-    });
+    int rc = prod_launch(h, b, h->b_rw, h->kss);
     if (rc) return fail(rc, "unsupported chain count");
     return hip_check(hipGetLastError(), "batch products");
 }
@@ -1035,22 +1074,10 @@ int batch_update(sb_handle *h, int b, const double *delta)
     kc.last(KS_DQUANT, 16.0 * C * nb + cl * nb);
     launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, L, h->b_ntile, delta, h->sdv, p0, nb,
            h->b_dmax, h->b_dk, nwg_d, Dq, h->b_expo2, h->b_kc);
-    const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
-    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut1) + h->tt_off[b] * 64;
     kc.begin(st);
     kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
-    int rc = with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
-        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
-        with_rows<NTL>(h->b_rw, [&](auto rw) {
-            constexpr int RW = decltype(rw)::value;
-            const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * RW : 16 * IT));
-            const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
-            launch(kc, k_upd<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gt, nsw / 16, Dq, L, h->b_expo2,
-                   h->b_kc, h->r, h->b_rsum, h->b_rmax, n0, ut, nbt, nb,
-                   reinterpret_cast<const float4 *>(h->dl_sw), spant, h->b_uapart, g1x, n1);
-        });
-    });
+    int rc = upd_launch(h, b, h->b_rw, h->r);
     if (rc) return fail(rc, "unsupported chain count");
     // partials of the new r for the next block: per 64 rows (LDS path) or per workgroup
     h->rstat_nwg = (int)(P.ldr / (upd_lds(h->b_ntile) ? WAVE : 16 * rows_tiles(h->b_ntile)));
@@ -1070,6 +1097,81 @@ int batch_update(sb_handle *h, int b, const double *delta)
         launch(kc, k_ra_reduce, dim3(nra, C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
     }
     return hip_check(hipGetLastError(), "batch update");
+}
+
+// Tile shape of the batched products: the chunk length of X_s'Q (kss
+// super-steps of 256 individuals) and the row tiles per wave of the
+// LDS-shared X products (2 or 4).  Every shape gives bit-identical results
+// (the limb products are exact int32 sums and the chunk partials are added
+// as 64-bit integers), so the choice affects speed only.  Picked once per
+// handle, before its first sweep, by timing block 0's products (k_prod) and
+// update (k_upd, on a scratch copy of the residuals) for every candidate;
+// SODABAYES_KSS / SODABAYES_ROWS_PER_WAVE fix the shape instead.
+void tune_candidates(int ntile, std::vector<int> &rws, std::vector<int> &ksss)
+{
+    rws.assign({2});
+    if (ntile <= 3) rws.push_back(4);
+    if (prod_lds(ntile))
+        ksss.assign({8, 12, 16, 25});
+    else
+        ksss.assign({16, 24, 32, 48});
+}
+
+int batch_min_kss(int ntile)
+{
+    std::vector<int> rws, ksss;
+    tune_candidates(ntile, rws, ksss);
+    return ksss[0];
+}
+
+int batch_autotune(sb_handle *h)
+{
+    h->tuned = true;
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    if (!h->tune_r && dev_alloc(h, (void **)&h->tune_r, sizeof(double) * (size_t)P.ldr * P.nchains))
+        return SB_ERR_NOMEM;
+    std::vector<int> rws, ksss;
+    tune_candidates(h->b_ntile, rws, ksss);
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    const bool timing = h->kc.on;
+    h->kc.on = false;   // the trial launches are not kernel-clock spans
+    constexpr int REPS = 3;
+    auto time_it = [&](auto &&fn) {
+        fn();   // warm-up
+        hipEventRecord(e0, st);
+        for (int k = 0; k < REPS; k++) fn();
+        hipEventRecord(e1, st);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        return ms / REPS;
+    };
+    double best = 1e30;
+    int brw = h->b_rw, bkss = h->kss, rc = 0;
+    for (int rw : rws) {
+        const float tu = time_it([&] { rc |= upd_launch(h, 0, rw, h->tune_r); });
+        for (int kss : ksss) {
+            const float tp = time_it([&] { rc |= prod_launch(h, 0, rw, kss); });
+            if (tp + tu < best) {
+                best = tp + tu;
+                brw = rw;
+                bkss = kss;
+            }
+        }
+    }
+    h->kc.on = timing;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return fail(rc, "unsupported chain count");
+    h->b_rw = brw;
+    h->kss = bkss;
+    h->tune_us = best * 1e3;
+    h->rstat_nwg = 0;   // the trial updates overwrote the residual statistics
+    h->q_block = -1;
+    return hip_check(hipGetLastError(), "tile-shape tuning");
 }
 
 // Genotype operand layouts of the batched path, built once at sb_create.

@@ -250,6 +250,9 @@ struct sb_handle {
     double *b_rtot = nullptr;     // sum of r per chain
     int kss = 32;                 // super-steps (16 genotype words) per MFMA workgroup chunk (batch_kss)
     int b_rw = 2;                 // row tiles per wave of the LDS-shared products (2 or 4)
+    bool tuned = false;           // tile shape chosen (batch_autotune, or fixed by the environment)
+    double tune_us = 0.0;         // block-0 products + update time of the chosen shape
+    double *tune_r = nullptr;     // scratch residuals of the tuning updates
     int b_ntile = 1;              // 16-column MFMA tiles covering nchains * limbs
     int *b_T = nullptr;           // chunks x blocksize x b_ldt int32 limb products per individual chunk
     int b_ldt = 16;
@@ -299,6 +302,8 @@ int batch_update(sb_handle *h, int b, const double *delta);
 int batch_transpose(sb_handle *h);
 int batch_words(int nb);
 int batch_kss(int ntile);
+int batch_min_kss(int ntile);
+int batch_autotune(sb_handle *h);
 int batch_tiles(sb_handle *h);
 int block_rhs(sb_handle *h, int b, double *out_host);
 }  // namespace sb

@@ -1146,7 +1146,14 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     double *delta = h->sc.delta;
     std::vector<double> xxd(P.nblocks);
     hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
-    // optional per-kernel HIP events on this stream
+    if (mixed && !h->tuned) {
+        int rc = batch_autotune(h);
+        if (rc) {
+            if (dhyp) hipFree(dhyp);
+            return rc;
+        }
+    }
+    // optional per-kernel timing on this stream
     const char *kt = std::getenv("SODABAYES_KERNEL_TIMING");
     h->kc.on = kt && kt[0] == '1';
     KClock &kc = h->kc;

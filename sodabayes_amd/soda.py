@@ -304,8 +304,8 @@ class SodaSampler:
         return out
 
     def info_raw(self) -> np.ndarray:
-        v = np.zeros(12)
-        _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 12))
+        v = np.zeros(15)
+        _lib.check(_lib.load().sb_get_info(self.handle, v.ctypes.data_as(C.POINTER(C.c_double)), 15))
         return v
 
     def kernel_times(self) -> dict:
@@ -323,7 +323,8 @@ class SodaSampler:
 
     def info(self) -> dict:
         v = self.info_raw()
-        return dict(setup_seconds=v[0], iterations=v[1], sweep_ms=v[2], device_bytes=v[3])
+        return dict(setup_seconds=v[0], iterations=v[1], sweep_ms=v[2], device_bytes=v[3],
+                    chunk_supersteps=int(v[12]), rows_per_wave=int(v[13]), tuned_block_us=v[14])
 
     def results(self, chain: int = 0) -> dict:
         """Final state and posterior means under the reference's names

@@ -208,3 +208,40 @@ def test_kernel_timing_slots(monkeypatch, precision, slots, absent):
     for k in absent:
         assert kt[k][1] == 0, k
     s.close()
+
+
+def _chain_states(s, chains):
+    p, n = s.nloci, s.nind
+    return [(s.get("G", p, c), s.get("CLASS", p, c), s.get("YADJ", n, c)) for c in range(chains)]
+
+
+@pytest.mark.parametrize("chains", [4, 40])
+def test_tile_shapes_give_identical_chains(monkeypatch, chains):
+    """The X_s'Q chunk length and the row tiles per wave only regroup exact
+    int32 limb sums, so every shape - fixed ones and the one the tuner picks
+    before the first sweep - gives bit-identical chains; the tuner reports a
+    shape it tried."""
+    states, infos = [], []
+    for kss, rw in ((8, 4), (25, 2), (None, None)):
+        for k, v in (("SODABAYES_KSS", kss), ("SODABAYES_ROWS_PER_WAVE", rw)):
+            if v is None:
+                monkeypatch.delenv(k, raising=False)
+            else:
+                monkeypatch.setenv(k, str(v))
+        s, _, _ = sampler("BayesR", "mixed", chains, 13)
+        s.setup()
+        s.init()
+        s.run(8, trace=False)
+        states.append(_chain_states(s, chains))
+        infos.append(s.info())
+        s.close()
+    for other in states[1:]:
+        for a, b in zip(states[0], other):
+            assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    tuned = infos[-1]
+    print(f"x{chains}: tuned shape kss={tuned['chunk_supersteps']} rows/wave={tuned['rows_per_wave']} "
+          f"({tuned['tuned_block_us']:.1f} us for block 0)")
+    assert tuned["tuned_block_us"] > 0
+    assert tuned["chunk_supersteps"] in ((8, 12, 16, 25) if chains <= 32 else (16, 24, 32, 48))
+    assert tuned["rows_per_wave"] in ((2, 4) if chains <= 5 else (2,))
+    assert infos[0]["chunk_supersteps"] == 8 and infos[0]["tuned_block_us"] == 0
