@@ -70,6 +70,7 @@ def parse_args():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
+    ap.add_argument("--no-folds", action="store_true", help="skip the cross-validation folds pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (a known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -222,6 +223,48 @@ def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int):
     return out
 
 
+def run_folds(soda, torch, a) -> dict:
+    """Cross-validation folds in one launch: BayesCpi, n = 40,000, p = 100,000,
+    64 folds, each holding out a random 5-40% of the individuals (imputed
+    every iteration, sb_mcmc_init_chains), on the mixed batched path.  The
+    SODA subset size is chosen from {1024, 2048, 4096} by timing a few sweeps
+    of each (soda.autotune_blocksize); then ``steps`` iterations are timed."""
+    import numpy as np
+
+    n, p, C = 40000, 100000, 64
+    bed, freq, why = synth_on_device(torch, n, p, a.seed + 1, nqtl=1000, h2=0.3)
+    rs = np.random.default_rng(a.seed)
+    held = np.zeros((C, n), bool)
+    for c in range(C):
+        held[c, rs.choice(n, size=int(rs.uniform(0.05, 0.4) * n), replace=False)] = True
+    opts = soda.SodaOptions(method="BayesCpi", seed=a.seed, rng="counter", precision="mixed", chains=C)
+    kw = dict(freq=freq, bed_device_ptr=bed.data_ptr(), heldout=held)
+    best, timings = soda.autotune_blocksize(opts, n, p, why, candidates=(1024, 2048, 4096), iters=3, warmup=2,
+                                            **kw)
+    log(f"[folds x{C}] subset size {best} (ms per iteration: "
+        + ", ".join(f"{k}: {v:.1f}" for k, v in timings.items()) + ")")
+    warm = 5
+    o = soda.SodaOptions(method="BayesCpi", seed=a.seed, rng="counter", precision="mixed", chains=C,
+                         blocksize=best, numit=warm + a.steps, burnin=0)
+    s = soda.SodaSampler(o, n, p, why, **kw)
+    s.setup()
+    s.init()
+    s.run(warm, trace=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(a.steps, trace=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s.close()
+    del bed
+    out = {"workload": f"SODA BayesCpi n={n} p={p}, {C} cross-validation folds (5-40% of the individuals held "
+                       "out and imputed each iteration), mixed batched path, counter-based RNG",
+           "value": p * C * a.steps / dt, "unit": "SNP-effect updates/s", "ms_per_step": dt * 1e3 / a.steps,
+           "chains": C, "blocksize": best, "blocksize_ms_per_step": timings}
+    log(f"[folds x{C}] {out['ms_per_step']:.2f} ms/iteration, {out['value']:.4g} SNP-updates/s")
+    return out
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -248,6 +291,8 @@ def main():
     mixed = run_path(soda, torch, a, why, freq, bed.data_ptr(), "mixed", a.chains)
     del bed
     torch.cuda.empty_cache()
+    folds = None if a.no_folds else run_folds(soda, torch, a)
+    torch.cuda.empty_cache()
 
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.seed)
     out = {
@@ -267,6 +312,7 @@ def main():
         "power_method_iterations": mixed["power_iters"],
         "iterations_per_s": a.steps / mixed["dt"],
         "tile_shape": mixed.get("tile_shape"),
+        "folds": folds,
         "roofline": mixed.get("roofline"),
         "cpu_baseline": cpu,
     }

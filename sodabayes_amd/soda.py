@@ -372,3 +372,39 @@ class SodaSampler:
             self.close()
         except Exception:
             pass
+
+
+def autotune_blocksize(opts: SodaOptions, nind: int, nloci: int, why, candidates=(1024, 2048, 4096),
+                       iters: int = 5, warmup: int = 2, **sampler_kw) -> tuple[int, dict]:
+    """SODA subset size with the fastest sweep for this job's shape.
+
+    Every subset size gives a valid SODA sampler of the same posterior (the
+    blocking only decides which SNPs share an augmenting matrix), so the
+    choice is one of speed.  For each candidate the job is set up (Gram,
+    bound, factorisation), initialised and swept ``warmup`` + ``iters``
+    times; returns the subset size with the lowest milliseconds per
+    iteration and {size: ms per iteration}.  ``sampler_kw`` goes to
+    SodaSampler (bed / bed_device_ptr, freq, grp, heldout).
+    """
+    import time
+    from dataclasses import replace
+
+    timings = {}
+    for bs in candidates:
+        if bs > nloci:
+            continue
+        o = replace(opts, blocksize=int(bs), nblocks=0, numit=warmup + iters, burnin=0)
+        s = SodaSampler(o, nind, nloci, why, **sampler_kw)
+        try:
+            s.setup()
+            s.init()
+            s.run(warmup, trace=False)
+            t0 = time.perf_counter()
+            s.run(iters, trace=False)   # returns after the stream has drained
+            timings[int(bs)] = (time.perf_counter() - t0) * 1e3 / iters
+        finally:
+            s.close()
+    if not timings:
+        raise ValueError("no candidate subset size fits nloci")
+    best = min(timings, key=timings.get)
+    return best, timings
