@@ -71,6 +71,7 @@ def parse_args():
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
     ap.add_argument("--no-folds", action="store_true", help="skip the cross-validation folds pass")
+    ap.add_argument("--no-traits", action="store_true", help="skip the batched traits pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (a known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -123,6 +124,58 @@ def synth_on_device(torch, n, p, seed, fmin=0.01, fmax=0.5, nqtl=5000, h2=0.4, c
     gv *= np.sqrt(h2 / gv.var())
     y = 100.0 + gv + rs.standard_normal(n) * np.sqrt(1.0 - h2)
     return bed, freq, y
+
+
+def traits_on_device(torch, n, p, seed, ntraits=16, nqtl=3000, shared=0.5, chunk=2048):
+    """SNP-major PLINK payload (p x ceil(n/4) bytes) generated in HBM, with
+    ``ntraits`` phenotypes: each trait has ``nqtl`` QTL with N(0, 1)
+    effects, a fraction ``shared`` of them common to all traits (genetic
+    correlation about ``shared``), and h2 drawn from U(0.1, 0.6)."""
+    import numpy as np
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    f = torch.rand(p, generator=g, device=dev, dtype=torch.float64) * 0.49 + 0.01
+    nb = (n + 3) // 4
+    npad = nb * 4
+    bed = torch.empty((p, nb), dtype=torch.uint8, device=dev)
+    counts = torch.empty(p, dtype=torch.float64, device=dev)
+    rs = np.random.default_rng(seed)
+    ns = int(nqtl * shared)
+    common = rs.choice(p, size=ns, replace=False)
+    eff = np.zeros((p, ntraits))
+    eff[common] = rs.standard_normal((ns, 1)) * np.ones((1, ntraits))   # shared QTL, same effect
+    for t in range(ntraits):
+        own = rs.choice(np.setdiff1d(np.arange(p), common), size=nqtl - ns, replace=False)
+        eff[own, t] = rs.standard_normal(nqtl - ns)
+    qtl = np.flatnonzero(np.abs(eff).sum(1) > 0)
+    qtl_t = torch.from_numpy(qtl).to(dev)
+    eff_t = torch.from_numpy(eff[qtl]).to(dev)
+    gval = torch.zeros((n, ntraits), dtype=torch.float64, device=dev)
+    for j0 in range(0, p, chunk):
+        m = min(chunk, p - j0)
+        fc = f[j0:j0 + m].float()
+        u = torch.rand((2, npad, m), generator=g, device=dev)
+        x = (u[0] < fc).to(torch.uint8) + (u[1] < fc).to(torch.uint8)
+        del u
+        if npad > n:
+            x[n:] = 0
+        counts[j0:j0 + m] = x.sum(0, dtype=torch.int64).to(torch.float64)
+        code = x + (x > 0).to(torch.uint8)
+        c4 = code.view(nb, 4, m)
+        bed[j0:j0 + m] = (c4[:, 0] | (c4[:, 1] << 2) | (c4[:, 2] << 4) | (c4[:, 3] << 6)).t()
+        sel = (qtl_t >= j0) & (qtl_t < j0 + m)
+        if bool(sel.any()):
+            cols = qtl_t[sel] - j0
+            gval += (x[:n, cols].to(torch.float64) - 2.0 * f[j0 + cols]) @ eff_t[sel]
+        del x, code, c4
+    freq = (counts / (2.0 * n)).cpu().numpy()
+    gv = gval.cpu().numpy()
+    h2 = rs.uniform(0.1, 0.6, size=ntraits)
+    gv *= np.sqrt(h2 / gv.var(0))
+    y = 100.0 + gv + rs.standard_normal((n, ntraits)) * np.sqrt(1.0 - h2)
+    return bed, freq, np.ascontiguousarray(y.T)
 
 
 def cpu_baseline(seed: int) -> dict | None:
@@ -265,6 +318,41 @@ def run_folds(soda, torch, a) -> dict:
     return out
 
 
+def run_traits(soda, torch, a) -> dict:
+    """Batched traits in one launch: BayesRc, n = 100,000, p = 600,000, SNPs in
+    4 annotation classes (proportions 0.6 / 0.2 / 0.15 / 0.05) with their own
+    mixture proportions, 16 traits (traits_on_device), subsets of 4,096 SNPs,
+    mixed batched path; ``steps`` iterations timed."""
+    import numpy as np
+
+    n, p, T = 100000, 600000, 16
+    bed, freq, why = traits_on_device(torch, n, p, a.seed + 2, ntraits=T)
+    grp = np.random.default_rng(a.seed).choice(4, size=p, p=[0.6, 0.2, 0.15, 0.05]) + 1
+    warm = 5
+    o = soda.SodaOptions(method="BayesRc", nmix=4, mix="0.0,0.0001,0.001,0.01", seed=a.seed, rng="counter",
+                         precision="mixed", chains=T, blocksize=4096, numit=warm + a.steps, burnin=0)
+    s = soda.SodaSampler(o, n, p, why, freq=freq, bed_device_ptr=bed.data_ptr(), grp=grp)
+    t0 = time.time()
+    s.setup()
+    setup_s = time.time() - t0
+    s.init()
+    s.run(warm, trace=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.run(a.steps, trace=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    s.close()
+    del bed
+    out = {"workload": f"SODA BayesRc n={n} p={p} n_s=4096, 4 annotation classes, {T} traits (3,000 QTL each, "
+                       "half shared, h2 ~ U(0.1, 0.6)), mixed batched path, counter-based RNG",
+           "value": p * T * a.steps / dt, "unit": "SNP-effect updates/s", "ms_per_step": dt * 1e3 / a.steps,
+           "chains": T, "setup_seconds": round(setup_s, 2)}
+    log(f"[traits x{T}] setup {setup_s:.1f} s, {out['ms_per_step']:.2f} ms/iteration, "
+        f"{out['value']:.4g} SNP-updates/s")
+    return out
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -293,6 +381,8 @@ def main():
     torch.cuda.empty_cache()
     folds = None if a.no_folds else run_folds(soda, torch, a)
     torch.cuda.empty_cache()
+    traits = None if a.no_traits else run_traits(soda, torch, a)
+    torch.cuda.empty_cache()
 
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.seed)
     out = {
@@ -313,6 +403,7 @@ def main():
         "iterations_per_s": a.steps / mixed["dt"],
         "tile_shape": mixed.get("tile_shape"),
         "folds": folds,
+        "traits": traits,
         "roofline": mixed.get("roofline"),
         "cpu_baseline": cpu,
     }
