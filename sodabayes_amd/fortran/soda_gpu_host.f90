@@ -21,11 +21,12 @@ module soda_gpu_host
     use iso_c_binding
     use globals
     use ParallelRNG, only: ss
-    use utilities, only: output_model, write_dgv
+    use utilities, only: output_model, write_dgv, load_snp_grp
     use soda_gpu
     implicit none
     private
     public :: soda_setup_gpu, soda_bayes_gpu, soda_predict_gpu, soda_gpu_release
+    public :: soda_gpu_packed_input, soda_allocate_packed, soda_load_packed
 
     type(c_ptr), save :: h = c_null_ptr
     integer(c_int8_t), allocatable, target, save :: bed(:)
@@ -64,7 +65,7 @@ contains
     end subroutine read_env
 
     ! SNP-major .bed payload (nloci * ceiling(nind/4) bytes after the three
-    ! magic bytes, which load_geno has already checked).
+    ! magic bytes, checked as load_geno checks them, utilities.f90:423-438).
     subroutine read_bed()
         integer :: u, ios
         integer(c_int8_t) :: magic(3)
@@ -75,9 +76,129 @@ contains
         open(newunit=u, file=trim(genfil), access='stream', form='unformatted', status='old', &
              action='read', iostat=ios)
         if (ios /= 0) stop 'cannot open the .bed file'
-        read(u) magic, bed
+        read(u) magic
+        if (magic(1) /= int(z'6c', c_int8_t) .or. magic(2) /= int(z'1b', c_int8_t)) then
+            write(*, *) 'Binary genotype file may not be a plink file'
+            write(21, '(a)') 'Binary genotype file may not be a plink file'
+            call flush(21)
+            stop
+        end if
+        if (magic(3) /= 1_c_int8_t) then
+            write(*, '(a)') 'should not be individual mode - check >>>'
+            write(21, '(a)') 'SNP file should not be in individual mode'
+            call flush(21)
+            stop
+        end if
+        read(u) bed
         close(u)
     end subroutine read_bed
+
+    ! Whether the job runs the SODA branch on the GPU, i.e. needs no dense
+    ! genotype matrix on the host: prediction, or a SODA run whose partition
+    ! (soda.f90:407-421) gives blocks of more than one SNP.  Setting
+    ! SODABAYES_HOST_DENSE_X keeps the reference's dense loaders.
+    logical function soda_gpu_packed_input()
+        integer :: bs, ln, st
+        character(len=8) :: v
+        call get_environment_variable('SODABAYES_HOST_DENSE_X', v, ln, st)
+        soda_gpu_packed_input = .false.
+        if (st == 0) return
+        if (.not. mcmc) then
+            soda_gpu_packed_input = .true.
+            return
+        end if
+        if (SODAOFF) return
+        if (setblocksize) then
+            bs = blocksize
+        else if (setblocks) then
+            bs = ceiling(dble(nloci) / nblocks)
+        else
+            bs = nthreads
+        end if
+        soda_gpu_packed_input = bs > 1
+    end function soda_gpu_packed_input
+
+    ! allocate_data (utilities.f90:558-586) without the dense genotype matrix
+    ! (8 * nind * nloci bytes), which the GPU path never reads: X gets a 1 x 1
+    ! placeholder so that clean_up's deallocation stays valid.
+    subroutine soda_allocate_packed()
+        integer :: ios, ios2
+        ios = 0
+        ios2 = 0
+        if (mcmc) then
+            allocate(animalID(nind), snpID(nloci), why(nind), pred(nind), X(1, 1), g(nloci), yadj(nind), &
+                     xpx(nloci), permvec(nloci), varindist(ngroup, nmix), varstore(ngroup, nmix), gstore(nloci), &
+                     freqstore(nloci), stat=ios)
+            if (BayesMethod == BayesA) then
+                allocate(vara_s(nloci), vara_sstore(nloci), stat=ios2)
+            else if (BayesMethod == BayesB) then
+                allocate(vara_s(nloci), vara_sstore(nloci), p(ngroup, nmix), log_p(ngroup, nmix), &
+                         snpindist(ngroup, nmix), snptracker(nloci, 2), snpstore(ngroup, nmix), &
+                         indiststore(nloci, nmix), stat=ios2)
+            else if (BayesMethod == BayesC) then
+                allocate(p(ngroup, nmix), log_p(ngroup, nmix), snpindist(ngroup, nmix), snptracker(nloci, 2), &
+                         snpstore(ngroup, nmix), indiststore(nloci, nmix), vare_gp(nmix), gp(nmix), stat=ios2)
+            else
+                allocate(p(ngroup, nmix), log_p(ngroup, nmix), snpindist(ngroup, nmix), snptracker(nloci, 2), &
+                         snpstore(ngroup, nmix), indiststore(nloci, nmix), vare_gp(nmix), gp(nmix), &
+                         pstore(ngroup, nmix), dirx(ngroup, nmix), stat=ios2)
+            end if
+        else
+            allocate(animalID(nind), snpID(nloci), X(1, 1), freqstore(nloci), gstore(nloci), pred(nind), stat=ios)
+        end if
+        if (ios /= 0 .or. ios2 /= 0) then
+            print *, 'ERROR :: Unable to allocate required storage'
+            stop 'Unable to allocate required storage for data'
+        end if
+    end subroutine soda_allocate_packed
+
+    ! load_geno + xscale (utilities.f90:402-460, 519-556) without the dense
+    ! matrix: SNP ids from the .bim, the packed .bed, and the allele
+    ! frequencies from its two-bit counts (missing read as 1, as load_geno
+    ! reads it).  The counts are the integers xscale sums as doubles, so the
+    ! frequencies are the reference's to the last bit; the .frq file is
+    ! written in its format.  Prediction reads the frequencies from -freq.
+    subroutine soda_load_packed()
+        integer :: u, j, k, ios, nb, last
+        character(len=80) :: dum, id
+        integer(int64) :: cnt, o
+        integer :: b, w
+        open(newunit=u, file=trim(bimfil), status='old', form='formatted')
+        do j = 1, nloci
+            read(u, *, iostat=ios) dum, id
+            if (ios /= 0) write(*, *) 'error: reading snpID from: ' // trim(bimfil)
+            snpID(j) = trim(id)
+        end do
+        close(u)
+        call read_bed()
+        if (mcmc) then
+            nb = (nind + 3) / 4
+            last = nind - 4 * (nb - 1)   ! genotypes in the last byte of a SNP
+            do j = 1, nloci
+                o = int(j - 1, int64) * nb
+                cnt = 0
+                do k = 1, nb
+                    b = iand(int(bed(o + k)), 255)
+                    w = b - iand(ishft(b, -1), 85)          ! two-bit codes -> counts 0 / 1 / 1 / 2
+                    if (k == nb .and. last < 4) w = iand(w, ishft(1, 2 * last) - 1)
+                    cnt = cnt + iand(w, 3) + iand(ishft(w, -2), 3) + iand(ishft(w, -4), 3) + ishft(w, -6)
+                end do
+                freqstore(j) = dble(cnt) / (2.0d0 * nind)
+            end do
+            open(newunit=u, file=trim(freqfil), status='unknown')
+            do j = 1, nloci
+                write(u, '(A10,1X,F10.6)') snpID(j), freqstore(j)
+            end do
+            close(u, status='keep')
+            if (BayesMethod == BayesRc) call load_snp_grp
+        else
+            open(newunit=u, file=trim(freqfil), status='old')
+            do j = 1, nloci
+                read(u, *) dum, freqstore(j)
+            end do
+            close(u, status='keep')
+        end if
+    end subroutine soda_load_packed
 
     ! setup_soda (soda.f90:403-443): block partition, permutation, packed
     ! genotypes on the device and the augmenting matrices.  Leaves SODAOFF

@@ -7,6 +7,11 @@
 ! (SODABayes.f90:139-148) with the gfx950 library.  -SODAOff keeps the
 ! reference's regular sampler (models.f90:607-962) on the host.
 !
+! Jobs on the GPU never build the reference's dense FP64 genotype matrix
+! (8 * nind * nloci bytes on the host): the .bed is read packed, and the
+! allele frequencies come from its counts (soda_load_packed).  -SODAOff
+! keeps the reference's loaders.
+!
 ! Deliberate deviations: the .bim file is closed after load_geno, so that
 ! a .par file may hold several jobs (the reference stops at the second one).
 ! With neither -blocksize nor -nblocks the reference
@@ -137,13 +142,19 @@ contains
         write(21, '(a)') 'Running program SODABayes: ' // trim(job_name)
         write(21, 101) 'Run started at', cdate(1:4), cdate(5:6), cdate(7:8), ctime(1:2), ctime(3:4), ctime(5:6)
         call get_size
-        call allocate_data
-        call load_geno
-        ! load_geno leaves the .bim connected to unit 40 (utilities.f90:414),
-        ! which stops the reference at the next job's get_size
-        inquire(unit=40, opened=bim_open)
-        if (bim_open) close(40)
-        call xscale
+        if (soda_gpu_packed_input()) then
+            ! the GPU reads the packed .bed: no dense genotype matrix on the host
+            call soda_allocate_packed()
+            call soda_load_packed()
+        else
+            call allocate_data
+            call load_geno
+            ! load_geno leaves the .bim connected to unit 40 (utilities.f90:414),
+            ! which stops the reference at the next job's get_size
+            inquire(unit=40, opened=bim_open)
+            if (bim_open) close(40)
+            call xscale
+        end if
         if (mcmc) then
             call load_pheno
             call set_random_seed

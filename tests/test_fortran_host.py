@@ -108,13 +108,18 @@ def test_help_lists_the_reference_options():
 
 @pytest.mark.gpu
 @needs_exe
-@pytest.mark.parametrize("case", ["bayesr_cmdline", "bayesa_rblocks", "bayesrc_groups", "bayescpi_nblocks",
-                                  "par_two_jobs"])
-def test_gpu_job_files_match_the_reference(tmp_path, case):
+@pytest.mark.parametrize("case,dense_x", [("bayesr_cmdline", False), ("bayesa_rblocks", False),
+                                          ("bayesrc_groups", False), ("bayescpi_nblocks", False),
+                                          ("par_two_jobs", False), ("bayesr_cmdline", True)])
+def test_gpu_job_files_match_the_reference(tmp_path, case, dense_x):
+    """By default the program reads the .bed packed and computes the allele
+    frequencies from its counts (no dense genotype matrix on the host);
+    SODABAYES_HOST_DENSE_X keeps the reference's loaders.  Both give the
+    reference's files."""
     z, manifest = load_fixture()
     m = manifest[case]
     write_inputs(z, case, m["inputs"], str(tmp_path))
-    r = run(str(tmp_path), m["args"])
+    r = run(str(tmp_path), m["args"], {"SODABAYES_HOST_DENSE_X": "1"} if dense_x else None)
     assert "SODA on the GPU: rng replay , precision fp64 , chains 1" in r.stdout
     for f in m["outputs"]:
         path = tmp_path / f
