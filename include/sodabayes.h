@@ -210,6 +210,23 @@ int sb_get_info(sb_handle *h, double *out, int32_t n);
 int sb_predict(int32_t nind, int32_t nloci, const uint8_t *bed, int32_t bed_on_device, const double *freq,
                const double *effects, double mu, double *pred);
 
+/* ---- device-state checkpoints (no counterpart in the reference; for long
+ * chains that must survive a job's end) ---- */
+
+/* Bytes of a checkpoint of this handle's chains (after sb_mcmc_init). */
+int sb_state_size(sb_handle *h, int64_t *bytes);
+/* Copies every chain's state after its last completed iteration into buf
+ * (size >= sb_state_size): priors, scalars and the replay generator state,
+ * r, r_a, phenotypes and held-out flags, effects, classes, per-SNP
+ * variances and the posterior sums, with the iteration count. */
+int sb_state_save(sb_handle *h, void *buf, int64_t size);
+/* Restores a checkpoint into a handle built on the same inputs (sb_create +
+ * sb_build_aug with the same dims and genotypes; checked through the dims,
+ * naind and the bounds d, which must match bit for bit) in place of
+ * sb_mcmc_init.  sb_mcmc_run then continues from iteration count + 1 and
+ * gives the chain the uninterrupted run would have given. */
+int sb_state_load(sb_handle *h, const void *buf, int64_t size);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *sb_last_error(void);
 

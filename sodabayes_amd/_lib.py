@@ -28,7 +28,8 @@ KT_SLOTS = ["rhs", "draw", "list", "update", "rquant", "xq", "ut0", "drawb", "dq
 # every function include/sodabayes.h declares
 EXPORTS = ["sb_rng_seed", "sb_permutate", "sb_create", "sb_destroy", "sb_rng_set", "sb_rng_get",
            "sb_build_aug", "sb_mcmc_init", "sb_mcmc_init_chains", "sb_mcmc_run", "sb_hyp_width", "sb_get",
-           "sb_block_rhs", "sb_get_info", "sb_predict", "sb_last_error"]
+           "sb_block_rhs", "sb_get_info", "sb_predict", "sb_state_size", "sb_state_save", "sb_state_load",
+           "sb_last_error"]
 
 
 class SbDims(C.Structure):
@@ -80,6 +81,9 @@ def load():
     L.sb_get_info.argtypes = [vp, dp, C.c_int32]
     L.sb_block_rhs.argtypes = [vp, C.c_int32, dp]
     L.sb_predict.argtypes = [C.c_int32, C.c_int32, vp, C.c_int32, dp, dp, C.c_double, dp]
+    L.sb_state_size.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.sb_state_save.argtypes = [vp, vp, C.c_int64]
+    L.sb_state_load.argtypes = [vp, vp, C.c_int64]
     L.sb_last_error.restype = C.c_char_p
     for f in EXPORTS:
         if f not in ("sb_hyp_width", "sb_last_error"):

@@ -82,6 +82,47 @@ int dev_alloc(sb_handle *h, void **p, size_t bytes)
     return SB_OK;
 }
 
+// ---- device-state checkpoints ----
+struct StateHeader {
+    char magic[8];             // "SBSTATE1"
+    int64_t bytes;             // whole checkpoint
+    sb_dims dims;
+    int32_t naind, nskipped, use_heldout, nsec;
+    int64_t iterations;
+    int64_t sec[16];           // bytes of each device section, in state_sections order
+};
+constexpr char STATE_MAGIC[8] = {'S', 'B', 'S', 'T', 'A', 'T', 'E', '1'};
+
+struct Section {
+    void *dev;
+    size_t bytes;
+};
+
+// Device sections of a handle's chain state, in checkpoint order.
+std::vector<Section> state_sections(sb_handle *h, bool heldout)
+{
+    const Par &P = h->par;
+    const size_t C = (size_t)P.nchains, p = (size_t)P.p, ldr = (size_t)P.ldr, bs = (size_t)P.blocksize;
+    std::vector<Section> v = {
+        {h->scal, sizeof(Scal) * C},     {h->r, sizeof(double) * ldr * C},
+        {h->ra, sizeof(double) * bs * C},    {h->why, sizeof(double) * ldr * C},
+        {h->beta, sizeof(double) * p * C},   {h->cls, sizeof(int32_t) * p * C},
+        {h->vara_s, sizeof(double) * p * C}, {h->gstore, sizeof(double) * p * C},
+        {h->vara_sstore, sizeof(double) * p * C},
+        {h->indiststore, sizeof(double) * p * (size_t)P.nmix * C},
+    };
+    if (heldout) v.push_back({h->heldout, ldr * C});
+    if (h->ra_sw) v.push_back({h->ra_sw, sizeof(float) * 16 * ((bs + 15) / 16) * 16 * ((C + 15) / 16)});
+    return v;
+}
+
+int64_t state_bytes(sb_handle *h, bool heldout)
+{
+    int64_t b = sizeof(StateHeader) + sizeof(Par) + sizeof(double) * h->par.nblocks;
+    for (const Section &s : state_sections(h, heldout)) b += (int64_t)s.bytes;
+    return b;
+}
+
 }  // namespace sb
 
 using namespace sb;
@@ -576,6 +617,95 @@ int sb_block_rhs(sb_handle *h, int32_t block, double *out)
 {
     if (!h || !out) return fail(SB_ERR_ARG, "sb_block_rhs: null argument");
     return block_rhs(h, block, out);
+}
+
+
+int sb_state_size(sb_handle *h, int64_t *bytes)
+{
+    if (!h || !bytes) return fail(SB_ERR_ARG, "sb_state_size: null argument");
+    if (!h->initialised) return fail(SB_ERR_STATE, "sb_state_size before sb_mcmc_init");
+    *bytes = sb::state_bytes(h, h->use_heldout);
+    return SB_OK;
+}
+
+int sb_state_save(sb_handle *h, void *buf, int64_t size)
+{
+    if (!h || !buf) return fail(SB_ERR_ARG, "sb_state_save: null argument");
+    if (!h->initialised) return fail(SB_ERR_STATE, "sb_state_save before sb_mcmc_init");
+    const int64_t need = sb::state_bytes(h, h->use_heldout);
+    if (size < need) return fail(SB_ERR_ARG, "sb_state_save: buffer smaller than sb_state_size");
+    if (hip_check(hipStreamSynchronize(h->stream), "state save")) return SB_ERR_HIP;
+    const std::vector<sb::Section> secs = sb::state_sections(h, h->use_heldout);
+    sb::StateHeader hd;
+    std::memset(&hd, 0, sizeof hd);
+    std::memcpy(hd.magic, sb::STATE_MAGIC, 8);
+    hd.bytes = need;
+    hd.dims = h->d;
+    hd.naind = h->par.naind;
+    hd.nskipped = h->nskipped;
+    hd.use_heldout = h->use_heldout ? 1 : 0;
+    hd.nsec = (int32_t)secs.size();
+    hd.iterations = h->iterations;
+    for (size_t k = 0; k < secs.size(); k++) hd.sec[k] = (int64_t)secs[k].bytes;
+    char *o = static_cast<char *>(buf);
+    std::memcpy(o, &hd, sizeof hd);
+    o += sizeof hd;
+    std::memcpy(o, &h->par, sizeof(sb::Par));
+    o += sizeof(sb::Par);
+    if (hip_check(hipMemcpy(o, h->xxd, sizeof(double) * h->par.nblocks, hipMemcpyDeviceToHost), "state save"))
+        return SB_ERR_HIP;
+    o += sizeof(double) * h->par.nblocks;
+    for (const sb::Section &s : secs) {
+        if (hip_check(hipMemcpy(o, s.dev, s.bytes, hipMemcpyDeviceToHost), "state save")) return SB_ERR_HIP;
+        o += s.bytes;
+    }
+    return SB_OK;
+}
+
+int sb_state_load(sb_handle *h, const void *buf, int64_t size)
+{
+    if (!h || !buf) return fail(SB_ERR_ARG, "sb_state_load: null argument");
+    if (!h->aug_built) return fail(SB_ERR_STATE, "sb_state_load before sb_build_aug");
+    sb::StateHeader hd;
+    if (size < (int64_t)sizeof hd) return fail(SB_ERR_ARG, "sb_state_load: not a checkpoint");
+    std::memcpy(&hd, buf, sizeof hd);
+    if (std::memcmp(hd.magic, sb::STATE_MAGIC, 8) != 0) return fail(SB_ERR_ARG, "sb_state_load: not a checkpoint");
+    if (hd.bytes != size || std::memcmp(&hd.dims, &h->d, sizeof(sb_dims)) != 0 || hd.naind != h->par.naind ||
+        hd.nskipped != h->nskipped)
+        return fail(SB_ERR_ARG, "sb_state_load: checkpoint of another job shape");
+    const bool heldout = hd.use_heldout != 0;
+    if (heldout && !h->heldout &&
+        sb::dev_alloc(h, (void **)&h->heldout, (size_t)h->par.ldr * h->par.nchains))
+        return SB_ERR_NOMEM;
+    const std::vector<sb::Section> secs = sb::state_sections(h, heldout);
+    if (hd.nsec != (int32_t)secs.size() || sb::state_bytes(h, heldout) != size)
+        return fail(SB_ERR_ARG, "sb_state_load: checkpoint layout differs");
+    for (size_t k = 0; k < secs.size(); k++)
+        if (hd.sec[k] != (int64_t)secs[k].bytes) return fail(SB_ERR_ARG, "sb_state_load: checkpoint layout differs");
+    const char *o = static_cast<const char *>(buf) + sizeof hd;
+    sb::Par par;
+    std::memcpy(&par, o, sizeof par);
+    o += sizeof par;
+    std::vector<double> xxd(h->par.nblocks);
+    if (hip_check(hipMemcpy(xxd.data(), h->xxd, sizeof(double) * xxd.size(), hipMemcpyDeviceToHost), "state load"))
+        return SB_ERR_HIP;
+    if (std::memcmp(xxd.data(), o, sizeof(double) * xxd.size()) != 0)
+        return fail(SB_ERR_ARG, "sb_state_load: the augmenting matrices differ (other genotypes or setup)");
+    o += sizeof(double) * xxd.size();
+    for (const sb::Section &s : secs) {
+        if (hip_check(hipMemcpy(s.dev, o, s.bytes, hipMemcpyHostToDevice), "state load")) return SB_ERR_HIP;
+        o += s.bytes;
+    }
+    h->par = par;   // the structural fields are equal (dims checked); the priors come from the checkpoint
+    h->use_heldout = heldout;
+    h->iterations = hd.iterations;
+    h->rstat_nwg = 0;
+    h->q_block = -1;
+    int rc = sb::init_xpx(h);
+    if (rc) return rc;
+    if (hip_check(hipStreamSynchronize(h->stream), "state load")) return SB_ERR_HIP;
+    h->initialised = true;
+    return SB_OK;
 }
 
 int sb_get_info(sb_handle *h, double *out, int32_t n)

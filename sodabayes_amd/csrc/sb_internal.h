@@ -304,6 +304,7 @@ int batch_words(int nb);
 int batch_kss(int ntile);
 int batch_min_kss(int ntile);
 int batch_autotune(sb_handle *h);
+int init_xpx(sb_handle *h);
 int batch_tiles(sb_handle *h);
 int block_rhs(sb_handle *h, int b, double *out_host);
 }  // namespace sb

@@ -1077,8 +1077,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
     for (int c = 0; c < C; c++) std::fill(vs.begin() + (size_t)c * p, vs.begin() + (size_t)(c + 1) * p, scs[c].vara);
     hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p * Cs, hipMemcpyHostToDevice, h->stream);
     k_init_r<<<dim3((n + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->why, h->scal, h->r);
-    if (h->nskipped > 0)
-        k_xpx<<<h->nskipped, NT, 0, h->stream>>>(P, h->geno, h->zlut, h->d_skip_list, h->xpx);
+    if (int rc = init_xpx(h)) return rc;
     h->rstat_nwg = 0;
     k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal);
     if (P.rng_mode == SB_RNG_REPLAY)
@@ -1091,6 +1090,14 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
     h->initialised = true;
     h->iterations = 0;
     return SB_OK;
+}
+
+// z'z of the SNPs the pivoted-Cholesky fallback skipped (models.f90:82-85)
+int init_xpx(sb_handle *h)
+{
+    if (h->nskipped > 0)
+        k_xpx<<<h->nskipped, NT, 0, h->stream>>>(h->par, h->geno, h->zlut, h->d_skip_list, h->xpx);
+    return hip_check(hipGetLastError(), "xpx");
 }
 
 int block_rhs(sb_handle *h, int b, double *out_host)

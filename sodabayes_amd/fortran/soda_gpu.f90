@@ -122,6 +122,21 @@ module soda_gpu
             real(c_double), value :: mu
             real(c_double) :: pred(*)
         end function
+        integer(c_int) function sb_state_size(h, bytes) bind(C, name='sb_state_size')
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int64_t) :: bytes
+        end function
+        integer(c_int) function sb_state_save(h, buf, size) bind(C, name='sb_state_save')
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: h, buf
+            integer(c_int64_t), value :: size
+        end function
+        integer(c_int) function sb_state_load(h, buf, size) bind(C, name='sb_state_load')
+            import :: c_int, c_int64_t, c_ptr
+            type(c_ptr), value :: h, buf
+            integer(c_int64_t), value :: size
+        end function
         type(c_ptr) function sb_last_error() bind(C, name='sb_last_error')
             import :: c_ptr
         end function

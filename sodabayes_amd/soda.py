@@ -297,6 +297,23 @@ class SodaSampler:
         self.rep += count
         return hyp[:nrec]
 
+    def save_state(self) -> bytes:
+        """Checkpoint of every chain after its last completed iteration
+        (sb_state_save)."""
+        L = _lib.load()
+        size = C.c_int64()
+        _lib.check(L.sb_state_size(self.handle, C.byref(size)))
+        buf = np.zeros(size.value, dtype=np.uint8)
+        _lib.check(L.sb_state_save(self.handle, buf.ctypes.data_as(C.c_void_p), size.value))
+        return buf.tobytes()
+
+    def load_state(self, blob: bytes) -> None:
+        """Continues the chains of a checkpoint (after setup(), in place of
+        init(); sb_state_load)."""
+        buf = np.frombuffer(blob, dtype=np.uint8).copy()
+        _lib.check(_lib.load().sb_state_load(self.handle, buf.ctypes.data_as(C.c_void_p), buf.size))
+        self.rep = int(self.info_raw()[1])
+
     def get(self, name: str, n: int, chain: int = 0) -> np.ndarray:
         out = np.zeros(n)
         _lib.check(_lib.load().sb_get(self.handle, _lib.Q[name], chain,
