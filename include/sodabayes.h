@@ -50,7 +50,10 @@ typedef struct {
     int32_t method;             /* SB_BAYES_* */
     int32_t nind, nloci;        /* n individuals, p SNPs */
     int32_t nmix, ngroup;       /* mixture classes (1 for BayesA), SNP groups */
-    int32_t nblocks, blocksize, blockt; /* resolved partition, soda.f90:407-417 */
+    int32_t nblocks, blocksize, blockt; /* resolved partition, soda.f90:407-417;
+                                         * all 0: -SODAOff, the single-site
+                                         * sampler regular_bayes
+                                         * (models.f90:607-962), FP64 only */
     int32_t nchains;            /* independent chains on the same genotypes */
     int32_t rng_mode;           /* SB_RNG_* (REPLAY needs nchains == 1) */
     int32_t precision;          /* SB_PREC_* */
@@ -106,7 +109,9 @@ int sb_rng_get(sb_handle *h, uint64_t state[2]);
  * (1-based) set, for any failing block under SB_PREC_MIXED: the fallback
  * runs on the FP64 path only.  The environment variable
  * SODABAYES_TEST_D_SHIFT="<block>:<amount>" (0-based block) lowers that
- * block's d before the factorisation, to exercise the fallback in tests. */
+ * block's d before the factorisation, to exercise the fallback in tests.
+ * For the single-site sampler (nblocks = 0) it builds no matrices and
+ * leaves xxd untouched: every SNP is updated one by one, in permvec order. */
 int sb_build_aug(sb_handle *h, double *xxd, int32_t *fail_block, int32_t *fail_col);
 
 /* ---- MCMC (replaces soda_bayes, models.f90:9-605) ---- */

@@ -177,13 +177,19 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     int want_ngroup = d.method == SB_BAYES_RC ? d.ngroup : 1;
     if (d.ngroup != want_ngroup || d.ngroup < 1 || d.ngroup > MAXGRP) return fail(SB_ERR_ARG, "bad ngroup");
     if (d.method == SB_BAYES_RC && !snpgrp) return fail(SB_ERR_ARG, "BayesRc needs snpgrp");
-    if (d.blocksize < 2 || d.blocksize > 4096) return fail(SB_ERR_ARG, "blocksize must be in [2, 4096]");
-    if (d.nblocks < 1 || d.blockt < 0 || d.blockt > d.nblocks) return fail(SB_ERR_ARG, "bad block partition");
+    // nblocks = blocksize = 0: -SODAOff, the reference's single-site sampler
+    const bool regular = d.nblocks == 0 && d.blocksize == 0;
+    if (!regular) {
+        if (d.blocksize < 2 || d.blocksize > 4096) return fail(SB_ERR_ARG, "blocksize must be in [2, 4096]");
+        if (d.nblocks < 1 || d.blockt < 0 || d.blockt > d.nblocks) return fail(SB_ERR_ARG, "bad block partition");
+    }
     if (d.rng_mode != SB_RNG_REPLAY && d.rng_mode != SB_RNG_COUNTER) return fail(SB_ERR_ARG, "bad rng_mode");
     if (d.precision != SB_PREC_FP64 && d.precision != SB_PREC_MIXED) return fail(SB_ERR_ARG, "bad precision");
     if (d.nchains < 1 || d.nchains > 64) return fail(SB_ERR_ARG, "nchains must be in [1, 64]");
-    if (d.nchains > 1 && d.precision != SB_PREC_MIXED)
-        return fail(SB_ERR_ARG, "several chains per handle need the mixed-precision batched path");
+    if (regular && d.precision != SB_PREC_FP64)
+        return fail(SB_ERR_ARG, "the single-site sampler (-SODAOff) runs in FP64 (no block products)");
+    if (!regular && d.nchains > 1 && d.precision != SB_PREC_MIXED)
+        return fail(SB_ERR_ARG, "several SODA chains per handle need the mixed-precision batched path");
     if (d.rng_mode == SB_RNG_REPLAY && (d.nchains != 1 || d.precision != SB_PREC_FP64))
         return fail(SB_ERR_ARG, "replay runs one FP64 chain (the reference's -nthreads 1 stream)");
 
@@ -204,7 +210,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         pos = s0 + sz;
     }
     ls[d.nblocks] = pos;
-    if (pos != d.nloci) return fail(SB_ERR_ARG, "block partition does not cover nloci");
+    if (!regular && pos != d.nloci) return fail(SB_ERR_ARG, "block partition does not cover nloci");
 
     sb_handle *h = new sb_handle();
     h->d = d;
@@ -225,6 +231,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     P.blocksize = d.blocksize;
     P.blockt = d.blockt;
     P.naind = d.blocksize;  // naind - naless with every block positive definite
+    P.regular = regular ? 1 : 0;
     int words = (d.nind + 15) / 16;
     P.ldw = ((words + CHUNK_WORDS - 1) / CHUNK_WORDS) * CHUNK_WORDS;
     P.nchunk = P.ldw / CHUNK_WORDS;

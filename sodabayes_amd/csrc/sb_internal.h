@@ -25,6 +25,7 @@ struct Par {
     int nchunk;        // ldw / CHUNK_WORDS
     int64_t ldr;       // residual stride per chain (ldw * 16)
     int rng_mode, nchains, precision;
+    int regular;       // -SODAOff: the single-site sampler (nblocks = blocksize = 0)
     int numit, burnin, thin;
     double pizero, scale_va0, scale_ve, df_va, df_ve, vara0;
     double mix[MAXMIX], alpha[MAXMIX];

@@ -649,7 +649,10 @@ __global__ void k_hyper(Par P, Scal *sc, const double *red_rtr, int nrtr, const 
     if (store) {
         sc->counter += 1.0;
         sc->mu_store += sc->mu;
-        for (int b = 0; b < nb; b++) sc->varstore[b] += sc->varindist[b];
+        // the single-site sampler sums varindist for BayesA only in SODA runs
+        // (models.f90:530 against 888-899)
+        if (!(P.regular && P.method == SB_BAYES_A))
+            for (int b = 0; b < nb; b++) sc->varstore[b] += sc->varindist[b];
         sc->vare_store += sc->vare;
         if (P.method != SB_BAYES_A) {
             sc->included_store += sc->included;
@@ -875,6 +878,14 @@ __global__ __launch_bounds__(SKT) void k_skipped(Par P, Scal *sc, const uint32_t
     __shared__ double lds[SKT / WAVE];
     __shared__ double dsh;
     const bool replay = P.rng_mode == SB_RNG_REPLAY;
+    const int c = blockIdx.x;   // one workgroup per chain
+    sc += c;
+    r += (int64_t)c * P.ldr;
+    beta += (int64_t)c * P.p;
+    cls += (int64_t)c * P.p;
+    vara_s += (int64_t)c * P.p;
+    delta += (int64_t)c * P.p;
+    chg += (int64_t)c * P.p;
     XoroSrc xs{{sc->rng[0], sc->rng[1]}};
     for (int k = 0; k < nsk; k++) {
         const int pos = list[k];
@@ -898,7 +909,7 @@ __global__ __launch_bounds__(SKT) void k_skipped(Par P, Scal *sc, const uint32_t
                 draw_snp<XoroSrc, NM>(P, sc, xs, s, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s, delta,
                                       chg);
             } else {
-                PhiloxSrc src(chain_key(P, 0), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
+                PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
                 draw_snp<PhiloxSrc, NM>(P, sc, src, s, zz, zz / sc->vare, pos, grp[pos], beta, cls, vara_s,
                                         delta, chg);
             }
@@ -1080,11 +1091,13 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
     if (int rc = init_xpx(h)) return rc;
     h->rstat_nwg = 0;
     k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal);
-    if (P.rng_mode == SB_RNG_REPLAY)
-        k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
-    else
-        k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->scal, h->ra,
-                                                                               0x7FFFFFFF, TAG_INIT, h->ra_sw);
+    if (P.naind > 0) {   // no augmented pseudo-data with -SODAOff
+        if (P.rng_mode == SB_RNG_REPLAY)
+            k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
+        else
+            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->scal, h->ra,
+                                                                                   0x7FFFFFFF, TAG_INIT, h->ra_sw);
+    }
     if (hip_check(hipGetLastError(), "init kernels")) return SB_ERR_HIP;
     if (hip_check(hipStreamSynchronize(h->stream), "init")) return SB_ERR_HIP;
     h->initialised = true;
@@ -1152,7 +1165,7 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     uint8_t *chg = reinterpret_cast<uint8_t *>(h->sc.flag);
     double *delta = h->sc.delta;
     std::vector<double> xxd(P.nblocks);
-    hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
+    if (P.nblocks > 0) hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
     if (mixed && !h->tuned) {
         int rc = batch_autotune(h);
         if (rc) {
@@ -1228,9 +1241,10 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             }
         }
         if (h->nskipped > 0) {
-            // the FP64 path only: setup refuses the fallback for the mixed path
+            // the FP64 path only (setup refuses the fallback for the mixed
+            // path); with -SODAOff every SNP, one workgroup per chain
             with_nmix(P.nmix, [&](auto nm) {
-                k_skipped<decltype(nm)::value><<<1, SKT, 0, st>>>(P, h->scal, h->geno, h->zlut, h->d_skip_list,
+                k_skipped<decltype(nm)::value><<<C, SKT, 0, st>>>(P, h->scal, h->geno, h->zlut, h->d_skip_list,
                                                                   h->nskipped, h->xpx, h->grp, h->d_pos_by_id + P.p,
                                                                   rep, h->r, h->beta, h->cls, h->vara_s, delta, chg);
             });
@@ -1240,11 +1254,13 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
         const int store = (rep > P.burnin) && (rep % P.thin == 0);
         const int hidx = (dhyp && rep % P.thin == 0) ? rec++ : -1;
         k_hyper<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, rep, store, dhyp, hidx);
-        if (P.rng_mode == SB_RNG_REPLAY)
-            k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
-        else
-            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA,
-                                                                             h->ra_sw);
+        if (P.naind > 0) {
+            if (P.rng_mode == SB_RNG_REPLAY)
+                k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
+            else
+                k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA,
+                                                                                 h->ra_sw);
+        }
         if (h->use_heldout) {
             k_impute<<<dim3((P.n + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->heldout, h->why, h->r, rep);
             h->rstat_nwg = 0;

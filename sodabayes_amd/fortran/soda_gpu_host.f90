@@ -34,8 +34,9 @@ module soda_gpu_host
 
 contains
 
-    ! Stops the job the way the reference does on a fatal error: message on
-    ! stdout and in the log (unit 21), then stop.
+    ! Stops the job on a library failure: message on stdout and in the log
+    ! (unit 21), as the reference reports its fatal errors, then a nonzero
+    ! exit status.
     subroutine fail(what, rc)
         character(len=*), intent(in) :: what
         integer(c_int), intent(in) :: rc
@@ -44,7 +45,7 @@ contains
         write(*, '(a)') trim(msg)
         write(21, '(a)') trim(msg)
         call flush(21)
-        stop 'SODA on the GPU failed'
+        error stop 'SODA on the GPU failed'
     end subroutine fail
 
     subroutine read_env()
@@ -93,21 +94,22 @@ contains
         close(u)
     end subroutine read_bed
 
-    ! Whether the job runs the SODA branch on the GPU, i.e. needs no dense
-    ! genotype matrix on the host: prediction, or a SODA run whose partition
-    ! (soda.f90:407-421) gives blocks of more than one SNP.  Setting
-    ! SODABAYES_HOST_DENSE_X keeps the reference's dense loaders.
-    logical function soda_gpu_packed_input()
-        integer :: bs, ln, st
+    ! The single-site sampler (-SODAOff, or a partition into blocks of one
+    ! SNP, soda.f90:407-421) stays the reference's host code when
+    ! SODABAYES_REGULAR_HOST is set; otherwise it runs on the GPU.
+    logical function regular_on_host()
+        integer :: ln, st
         character(len=8) :: v
-        call get_environment_variable('SODABAYES_HOST_DENSE_X', v, ln, st)
-        soda_gpu_packed_input = .false.
-        if (st == 0) return
-        if (.not. mcmc) then
-            soda_gpu_packed_input = .true.
+        call get_environment_variable('SODABAYES_REGULAR_HOST', v, ln, st)
+        regular_on_host = st == 0
+    end function regular_on_host
+
+    logical function single_site()
+        integer :: bs
+        if (SODAOFF) then
+            single_site = .true.
             return
         end if
-        if (SODAOFF) return
         if (setblocksize) then
             bs = blocksize
         else if (setblocks) then
@@ -115,7 +117,19 @@ contains
         else
             bs = nthreads
         end if
-        soda_gpu_packed_input = bs > 1
+        single_site = bs <= 1
+    end function single_site
+
+    ! Whether the job needs no dense genotype matrix on the host: prediction,
+    ! or a chain on the GPU.  SODABAYES_HOST_DENSE_X keeps the reference's
+    ! dense loaders.
+    logical function soda_gpu_packed_input()
+        integer :: ln, st
+        character(len=8) :: v
+        call get_environment_variable('SODABAYES_HOST_DENSE_X', v, ln, st)
+        soda_gpu_packed_input = .false.
+        if (st == 0) return
+        soda_gpu_packed_input = .not. mcmc .or. .not. (single_site() .and. regular_on_host())
     end function soda_gpu_packed_input
 
     ! allocate_data (utilities.f90:558-586) without the dense genotype matrix
@@ -201,9 +215,13 @@ contains
     end subroutine soda_load_packed
 
     ! setup_soda (soda.f90:403-443): block partition, permutation, packed
-    ! genotypes on the device and the augmenting matrices.  Leaves SODAOFF
-    ! set when the partition gives blocks of one SNP (soda.f90:418-421).
-    subroutine soda_setup_gpu()
+    ! genotypes on the device and the augmenting matrices.  Sets SODAOFF
+    ! when the partition gives blocks of one SNP (soda.f90:418-421).  With
+    ! SODAOFF the chain is the single-site sampler (models.f90:607-962), on
+    ! the GPU (nblocks = blocksize = 0 at the boundary) unless
+    ! SODABAYES_REGULAR_HOST keeps it on the host.  Returns whether the chain
+    ! runs on the GPU (soda_bayes_gpu) or on the host (regular_bayes).
+    logical function soda_setup_gpu()
         type(sb_dims) :: d
         integer(c_int64_t) :: st(2)
         integer(c_int32_t), allocatable, target :: perm0(:), grp0(:)
@@ -214,7 +232,14 @@ contains
         type(c_ptr) :: gptr
         integer :: k
 
-        if (SODAOFF) return
+        soda_setup_gpu = .false.
+        if (single_site()) then
+            SODAOFF = .true.
+            if (regular_on_host()) return
+            call setup_single_site()
+            soda_setup_gpu = .true.
+            return
+        end if
         ! the partition of setup_soda (soda.f90:407-417)
         if (setblocksize) then
             nblocks = ceiling(dble(nloci) / blocksize)
@@ -226,10 +251,6 @@ contains
             blocksize = nthreads
             nblocks = ceiling(dble(nloci) / blocksize)
             blockt = nblocks
-        end if
-        if (blocksize <= 1) then
-            SODAOFF = .true.
-            return
         end if
         naind = blocksize
         call read_env()
@@ -292,7 +313,51 @@ contains
         rc = sb_rng_get(h, st)
         ss(0)%s1 = st(1)
         ss(0)%s2 = st(2)
-    end subroutine soda_setup_gpu
+        soda_setup_gpu = .true.
+    end function soda_setup_gpu
+
+    ! The single-site sampler's handle: SNPs in id order (permvec = identity,
+    ! models.f90:672-675), no blocks and no augmenting matrices.
+    subroutine setup_single_site()
+        type(sb_dims) :: d
+        integer(c_int64_t) :: st(2)
+        integer(c_int32_t), allocatable, target :: perm0(:), grp0(:)
+        integer(c_int32_t) :: fb, fc
+        integer(c_int) :: rc
+        real(c_double) :: xxd0(1)
+        type(c_ptr) :: gptr
+        integer :: k
+        call read_env()
+        if (precision /= SB_PREC_FP64) then
+            write(*, '(a)') 'SODA off: the single-site sampler runs in FP64'
+            precision = SB_PREC_FP64
+        end if
+        if (nchains > 1 .and. rng_mode /= SB_RNG_COUNTER) &
+            stop 'SODABAYES_CHAINS > 1 needs SODABAYES_RNG=counter'
+        write(*, '(a,a,a,i0)') 'single-site sampler on the GPU: rng ', &
+            merge('replay ', 'counter', rng_mode == SB_RNG_REPLAY), ', chains ', nchains
+        allocate(perm0(nloci))
+        perm0 = [(k - 1, k = 1, nloci)]
+        permvec = perm0 + 1
+        call read_bed()
+        gptr = c_null_ptr
+        if (BayesMethod == BayesRc) then
+            allocate(grp0(nloci))
+            grp0 = snptracker(:, 1) - 1
+            gptr = c_loc(grp0)
+        end if
+        d = sb_dims(int(BayesMethod, c_int32_t), int(nind, c_int32_t), int(nloci, c_int32_t), &
+                    int(nmix, c_int32_t), int(ngroup, c_int32_t), 0_c_int32_t, 0_c_int32_t, 0_c_int32_t, &
+                    nchains, rng_mode, precision, 0_c_int32_t)
+        rc = sb_create(d, c_loc(bed), freqstore, perm0, gptr, h)
+        if (rc /= SB_OK) call fail('sb_create', rc)
+        st(1) = ss(0)%s1
+        st(2) = ss(0)%s2
+        rc = sb_rng_set(h, st)
+        if (rc /= SB_OK) call fail('sb_rng_set', rc)
+        rc = sb_build_aug(h, xxd0, fb, fc)
+        if (rc /= SB_OK) call fail('sb_build_aug', rc)
+    end subroutine setup_single_site
 
     ! soda_bayes (models.f90:9-605) on the device: the .hyp header and lines
     ! (models.f90:24-53, 549-569), the posterior means (models.f90:575-594),
@@ -493,7 +558,7 @@ contains
         vara = sc(3)
         call get_q(SB_Q_G, 0, g)
         call get_q(SB_Q_YADJ, 0, yadj)
-        call get_q(SB_Q_YAADJ, 0, yAadj)
+        if (allocated(yAadj)) call get_q(SB_Q_YAADJ, 0, yAadj)
         allocate(v(ngroup * nmix))
         call get_q(SB_Q_VARINDIST, 0, v)
         varindist = reshape(v, [ngroup, nmix])

@@ -4,13 +4,14 @@
 ! (SODABayes.f90).  The reference's own modules do the option parsing, the
 ! PLINK / phenotype readers, write_log, output_model and write_dgv; the
 ! soda_gpu_host module replaces setup_soda, soda_bayes and compute_dgv
-! (SODABayes.f90:139-148) with the gfx950 library.  -SODAOff keeps the
-! reference's regular sampler (models.f90:607-962) on the host.
+! (SODABayes.f90:139-148) with the gfx950 library, and -SODAOff's
+! single-site sampler (models.f90:607-962) too, unless SODABAYES_REGULAR_HOST
+! keeps the reference's host version.
 !
 ! Jobs on the GPU never build the reference's dense FP64 genotype matrix
 ! (8 * nind * nloci bytes on the host): the .bed is read packed, and the
-! allele frequencies come from its counts (soda_load_packed).  -SODAOff
-! keeps the reference's loaders.
+! allele frequencies come from its counts (soda_load_packed).  A chain on
+! the host keeps the reference's loaders.
 !
 ! Deliberate deviations: the .bim file is closed after load_geno, so that
 ! a .par file may hold several jobs (the reference stops at the second one).
@@ -18,7 +19,7 @@
 ! partitions into blocks of nthreads SNPs, and with -nthreads 1 (its
 ! default) setup_soda switches SODA off but soda_bayes still runs and
 ! crashes (soda.f90:418-421, SODABayes.f90:134-142); here that job runs the
-! regular sampler instead.
+! single-site sampler instead.
 program sodabayes_gpu
     use globals
     use utilities
@@ -130,7 +131,7 @@ contains
         integer, intent(inout) :: status
         real(real64) :: t0, tsetup, t1
         character(len=100) :: s1, s2, s3
-        logical :: bim_open
+        logical :: bim_open, on_gpu
 
         status = 1
         call date_and_time(date=cdate, time=ctime)
@@ -158,13 +159,13 @@ contains
         if (mcmc) then
             call load_pheno
             call set_random_seed
-            if (.not. SODAOFF) call soda_setup_gpu()   ! sets SODAOFF for blocks of one SNP
+            on_gpu = soda_setup_gpu()   ! sets SODAOFF for blocks of one SNP
             tsetup = omp_get_wtime()
             call write_log
-            if (SODAOFF) then
-                call regular_bayes
-            else
+            if (on_gpu) then
                 call soda_bayes_gpu()
+            else
+                call regular_bayes
             end if
         else
             call setup_prediction

@@ -51,6 +51,7 @@ class SodaOptions:
     blocksize: int = 0
     nblocks: int = 0
     rblocks: bool = False
+    sodaoff: bool = False         # -SODAOff: the single-site sampler (models.f90:607-962)
     vara0: float = 0.0            # module vara before the job (models.f90:99-101)
     # build-side choices (no reference counterpart)
     rng: str = "replay"           # "replay" (xoroshiro stream 0) or "counter" (Philox)
@@ -191,9 +192,12 @@ class SodaSampler:
         else:
             self.grp_idx, self.ngroup = None, 1
         bs, nb, bt = partition(self.nloci, o.blocksize, o.nblocks, o.nthreads)
-        if bs <= 1:
-            raise ValueError("SODA needs -blocksize or -nblocks (blocksize <= 1 switches SODA off, "
-                             "soda.f90:418-421)")
+        # -SODAOff, or a partition into blocks of one SNP, which switches SODA
+        # off (soda.f90:418-421; the reference then crashes in soda_bayes):
+        # the single-site sampler, nblocks = blocksize = 0 at the boundary
+        self.regular = bool(o.sodaoff) or bs <= 1
+        if self.regular:
+            bs, nb, bt = 0, 0, 0
         self.blocksize, self.nblocks, self.blockt = bs, nb, bt
         if self.freq is None:
             if self.bed is None:
@@ -202,7 +206,7 @@ class SodaSampler:
         if o.seed == 0 and o.rng == "replay":
             raise ValueError("replay needs a nonzero seed (seed 0 reads the system clock)")
         self.state = rng_seed(o.seed)
-        if o.rblocks:
+        if o.rblocks and not self.regular:
             self.permvec = permutate(self.nloci, bs, self.state)
         else:
             self.permvec = np.arange(self.nloci, dtype=np.int32)

@@ -3,7 +3,8 @@
 Runs the reference program (oracle/_ref/sodabayes_ref, built from
 /root/reference/src by ``make -C oracle ref``) with its default -nthreads 1
 on small seeded synthetic PLINK data, from the command line and from a .par
-file with two jobs, plus a -predict job on the first job's model files.  For
+file with two jobs, plus a -predict job on the first job's model files, and
+-SODAOff jobs (the single-site sampler) for every method.  For
 each job the fixture keeps the input files, the arguments and every output
 file (.hyp, .eff, .mod, .frq, .gv, .log).  The GPU test runs the Fortran
 program with its SODA branch on the GPU (sodabayes_amd/fortran) on the same
@@ -73,6 +74,20 @@ CASES = {
     "bayescpi_nblocks": (170, 233, 44, 0, ["-bfile", "d", "-method", "BayesCpi", "-nblocks", "7",
                                            "-numit", "15", "-burnin", "5", "-seed", "17", "-out", "cp"], ["cp"]),
     "par_two_jobs": (160, 200, 45, 0, None, ["c", "b"]),
+    # -SODAOff: the single-site sampler (models.f90:607-962)
+    "bayesr_sodaoff": (150, 130, 46, 0, ["-bfile", "d", "-method", "BayesR", *R4, "-SODAOff", "-numit", "20",
+                                         "-burnin", "5", "-seed", "21", "-out", "ro"], ["ro"]),
+    "bayesa_sodaoff": (140, 120, 47, 0, ["-bfile", "d", "-method", "BayesA", "-SODAOff", "-df_va", "4",
+                                         "-scale_va", "0.002", "-numit", "15", "-burnin", "3", "-seed", "22",
+                                         "-out", "ao"], ["ao"]),
+    "bayesb_sodaoff": (130, 110, 48, 0, ["-bfile", "d", "-method", "BayesB", "-SODAOff", "-df_va", "4",
+                                         "-scale_va", "0.002", "-numit", "15", "-burnin", "3", "-seed", "23",
+                                         "-out", "bo"], ["bo"]),
+    "bayesrc_sodaoff": (140, 150, 49, 3, ["-bfile", "d", "-method", "BayesRc", *R4, "-grpfile", "d.grp",
+                                          "-SODAOff", "-numit", "15", "-burnin", "3", "-seed", "24",
+                                          "-out", "rco"], ["rco"]),
+    "bayescpi_sodaoff": (150, 140, 50, 0, ["-bfile", "d", "-method", "BayesCpi", "-SODAOff", "-numit", "20",
+                                           "-burnin", "5", "-thin", "2", "-seed", "25", "-out", "cpo"], ["cpo"]),
 }
 PREDICT_ARGS = ["-predict", "-bfile", "d", "-model", "r.mod", "-freq", "r.frq", "-effects", "r.eff",
                 "-out", "p"]
