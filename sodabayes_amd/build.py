@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsodabayes.so")
-SOURCES = ["sb_api.hip", "sb_setup.hip", "sb_sweep.hip", "sb_batch.hip", "sb_predict.hip"]
+SOURCES = ["sb_api.hip", "sb_setup.hip", "sb_sweep.hip", "sb_batch.hip", "sb_predict.hip", "sb_single.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

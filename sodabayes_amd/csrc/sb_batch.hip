@@ -933,7 +933,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
             rhs_out[(int64_t)c * nb + jk] = rhs;
         } else {
             const Scal *scc = sc + c;
-            PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)sid, 2u);
+            PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)sid, TAG_SNP);
             const SnpDraw d = mixm ? draw_mix_fast<NM>(P, scc, src, mc, rhs, zz, gp, gk, cold)
                                    : draw_core<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, gp, gk, cold, vs);
             beta[o + pos] = d.beta;

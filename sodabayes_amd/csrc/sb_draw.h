@@ -11,6 +11,9 @@ namespace sb {
 
 __device__ __forceinline__ int bin_of(const Par &P, int g, int k) { return k * P.ngroup + g; }
 
+// Philox counter word c2 of each kind of draw (COUNTER mode)
+enum { TAG_MU = 1, TAG_SNP = 2, TAG_HYP = 3, TAG_RA = 4, TAG_INIT = 5, TAG_IMPUTE = 6 };
+
 // Philox key of chain c (COUNTER mode): chains draw independent streams.
 __device__ __forceinline__ uint64_t chain_key(const Par &P, int c)
 {
@@ -24,8 +27,8 @@ __device__ __forceinline__ uint64_t chain_key(const Par &P, int c)
 // cut-offs (nmix^2 exponentials; REPLAY reproduces the reference's doubles);
 // otherwise the same probabilities as a max-shifted softmax (nmix
 // exponentials), for the counter-based sampler.
-template <int NM, bool EXACT = true>
-__device__ inline void mix_probs(const Par &P, const Scal *sc, int g, double rhs, double zz,
+template <int NM, bool EXACT = true, class S = Scal>
+__device__ inline void mix_probs(const Par &P, const S *sc, int g, double rhs, double zz,
                                  double zz_vare, double *stemp)
 {
     double s[NM];
@@ -77,7 +80,8 @@ __device__ inline void mix_probs(const Par &P, const Scal *sc, int g, double rhs
 }
 
 // BayesB two-class probability of class 2 (models.f90:223-234)
-__device__ __forceinline__ double b_prob(const Scal *sc, double rhs, double zz, double zz_vare,
+template <class S>
+__device__ __forceinline__ double b_prob(const S *sc, double rhs, double zz, double zz_vare,
                                          double vs)
 {
     double vare = sc->vare;
@@ -102,9 +106,10 @@ struct SnpDraw {
 
 // Per-SNP draw of the chosen method from uniform source `src`, from the old
 // effect gk, class cold (BayesA: 2) and variance vs (BayesA/B).  Mirrors
-// models.f90:145-161 (A), 210-255 (B), 349-412 (C/Cpi/R/Rc).
-template <class Src, int NM = MAXMIX>
-__device__ inline SnpDraw draw_core(const Par &P, const Scal *sc, Src &src, double rhs0, double zz, double zz_vare,
+// models.f90:145-161 (A), 210-255 (B), 349-412 (C/Cpi/R/Rc).  S is Scal or
+// any type with the fields read here (vare, scale_va, gp, vare_gp, log_p).
+template <class Src, int NM = MAXMIX, class S = Scal>
+__device__ inline SnpDraw draw_core(const Par &P, const S *sc, Src &src, double rhs0, double zz, double zz_vare,
                                    int g, double gk, int cold, double vs)
 {
     SnpDraw d;
@@ -164,6 +169,94 @@ __device__ inline SnpDraw draw_core(const Par &P, const Scal *sc, Src &src, doub
     }
     d.beta = gn;
     d.vs = vs;
+    d.cls = flag;
+    d.delta = gk - gn;
+    d.chg = !(flag == 1 && cold == 1);
+    return d;
+}
+
+// draw_core's mixture branch (BayesC/Cpi/R/Rc) spread over the lanes of one
+// wave, for a caller whose draws are serial (the single-site sweep): lane k
+// forms the class term s_k and then class k's probability, each with the
+// operations and in the order of mix_probs, so the doubles are draw_core's;
+// lane 0 then takes the uniform and the effect from `src`.  Every lane of the
+// wave calls it with the same arguments; the result is lane 0's.
+template <class Src, int NM, class S>
+__device__ SnpDraw draw_mix_wave(const Par &P, const S *sc, Src &src, double rhs0, double zz, double zz_vare, int g,
+                                 double gk, int cold, int lane)
+{
+    constexpr bool EXACT = std::is_same<Src, XoroSrc>::value;
+    const double vare = sc->vare;
+    double rhs = rhs0;
+    if (cold > 1) rhs = rhs + zz * gk;
+    double sv = 0.0;
+    if (lane == 0) {
+        sv = sc->log_p[bin_of(P, g, 0)];
+    } else if (lane < P.nmix) {
+        double logdetV = log(sc->gp[lane] * zz_vare + 1.0);
+        double uhat = rhs / (zz + sc->vare_gp[lane]);
+        sv = -0.5 * (logdetV - (rhs * uhat / vare)) + sc->log_p[bin_of(P, g, lane)];
+    }
+    double s[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) s[k] = __shfl(sv, k, WAVE);
+    double st = 0.0;
+    if (EXACT) {
+        if (lane < P.nmix) {
+            double sk = 0.0;
+            bool overflow = false;
+#pragma unroll
+            for (int j = 0; j < NM; j++) {
+                if (j >= P.nmix) break;
+                if (j == lane) continue;
+                double clike = s[j] - sv;
+                if (clike < -700) continue;
+                if (clike > 700) {
+                    overflow = true;
+                    break;
+                }
+                sk = sk + exp(clike);
+            }
+            st = overflow ? 0.0 : 1.0 / (1.0 + sk);
+        }
+    } else {
+        double mx = s[0];
+#pragma unroll
+        for (int k = 1; k < NM; k++)
+            if (k < P.nmix) mx = fmax(mx, s[k]);
+        st = lane < P.nmix ? exp(sv - mx) : 0.0;
+    }
+    double stemp[NM];
+#pragma unroll
+    for (int k = 0; k < NM; k++) stemp[k] = __shfl(st, k, WAVE);
+    SnpDraw d{};
+    if (lane != 0) return d;
+    if (!EXACT) {
+        double tot = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; k++) tot += stemp[k];
+        const double inv = 1.0 / tot;
+#pragma unroll
+        for (int k = 0; k < NM; k++) stemp[k] = stemp[k] * inv;
+    }
+    double u = src.next(), ss = 0.0;
+    int flag = 1;
+#pragma unroll
+    for (int k = 0; k < NM; k++) {
+        if (k >= P.nmix) break;
+        ss = ss + stemp[k];
+        if (u < ss) {
+            flag = k + 1;
+            break;
+        }
+    }
+    double gn = 0.0;
+    if (flag > 1) {
+        double v1 = zz + vare / sc->gp[flag - 1];
+        gn = draw_normal(src, rhs / v1, sqrt(vare / v1));
+    }
+    d.beta = gn;
+    d.vs = 0.0;
     d.cls = flag;
     d.delta = gk - gn;
     d.chg = !(flag == 1 && cold == 1);

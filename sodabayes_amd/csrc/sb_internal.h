@@ -214,6 +214,11 @@ struct sb_handle {
     int32_t *d_skip_list = nullptr;
     double *xpx = nullptr;         // per skipped SNP: z'z
     int nskipped = 0;
+    // single-site sampler (-SODAOff): windows of 16 SNPs in update order
+    uint32_t *win = nullptr;       // per window: ldr words, SNP j's code in bits 2j..2j+1
+    double *wz = nullptr;          // per window: z of codes 0..2 of its 16 SNPs (4 per SNP)
+    double *wgram = nullptr;       // per window: packed lower triangle of z_j'z_l
+    int nwin = 0;
     int32_t *d_ls = nullptr;
     // chain state (nchains copies)
     double *r = nullptr;          // ldw*16 per chain
@@ -306,6 +311,9 @@ int batch_kss(int ntile);
 int batch_min_kss(int ntile);
 int batch_autotune(sb_handle *h);
 int init_xpx(sb_handle *h);
+// single-site sampler (sb_single.hip)
+int single_setup(sb_handle *h);
+int single_sweep(sb_handle *h, int rep);
 int batch_tiles(sb_handle *h);
 int block_rhs(sb_handle *h, int b, double *out_host);
 }  // namespace sb

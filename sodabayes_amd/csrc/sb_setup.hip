@@ -856,17 +856,16 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     if (P.regular) {
         // -SODAOff (models.f90:607-962): no augmenting matrices; every SNP is
         // updated one by one in id order (permvec = identity, models.f90:672-675)
-        // with zz = z'z (computed at sb_mcmc_init), the step the fallback's
-        // skipped SNPs take after the block sweep
+        // by the single-site sweep, which gets its windows' layouts here
         h->skip_list.resize(P.p);
         for (int k = 0; k < P.p; k++) h->skip_list[k] = h->pos_of[k];
         h->nskipped = P.p;
         int rc = dev_alloc(h, (void **)&h->d_skip_list, sizeof(int32_t) * P.p);
-        if (!rc) rc = dev_alloc(h, (void **)&h->xpx, sizeof(double) * P.p);
         if (!rc)
             rc = hip_check(hipMemcpy(h->d_skip_list, h->skip_list.data(), sizeof(int32_t) * P.p,
                                      hipMemcpyHostToDevice),
                            "upload SNP order");
+        if (!rc) rc = single_setup(h);
         if (rc) return rc;
         h->aug_built = true;
         h->setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -26,7 +26,6 @@ namespace sb {
 namespace {
 
 constexpr int NT = 256;
-enum { TAG_MU = 1, TAG_SNP = 2, TAG_HYP = 3, TAG_RA = 4, TAG_INIT = 5, TAG_IMPUTE = 6 };
 
 
 
@@ -1108,7 +1107,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
 // z'z of the SNPs the pivoted-Cholesky fallback skipped (models.f90:82-85)
 int init_xpx(sb_handle *h)
 {
-    if (h->nskipped > 0)
+    if (h->nskipped > 0 && !h->par.regular)   // the single-site sweep has its windows' Grams
         k_xpx<<<h->nskipped, NT, 0, h->stream>>>(h->par, h->geno, h->zlut, h->d_skip_list, h->xpx);
     return hip_check(hipGetLastError(), "xpx");
 }
@@ -1240,9 +1239,15 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                        h->ra);
             }
         }
-        if (h->nskipped > 0) {
-            // the FP64 path only (setup refuses the fallback for the mixed
-            // path); with -SODAOff every SNP, one workgroup per chain
+        if (P.regular) {
+            // -SODAOff: the single-site sweep over every SNP
+            int rc = single_sweep(h, rep);
+            if (rc) {
+                if (dhyp) hipFree(dhyp);
+                return rc;
+            }
+        } else if (h->nskipped > 0) {
+            // the FP64 path only (setup refuses the fallback for the mixed path)
             with_nmix(P.nmix, [&](auto nm) {
                 k_skipped<decltype(nm)::value><<<C, SKT, 0, st>>>(P, h->scal, h->geno, h->zlut, h->d_skip_list,
                                                                   h->nskipped, h->xpx, h->grp, h->d_pos_by_id + P.p,
