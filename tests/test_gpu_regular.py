@@ -8,7 +8,10 @@ draw (tests/test_fortran_host.py).  Here, on the counter-based generator:
   posterior means of mu, vare, vara and model size over 16 chains of each
   agree within 4 combined Monte Carlo standard errors, and the posterior
   effect means correlate;
-* r = y - Z g - mu holds at the end of a run.
+* r = y - Z g - mu holds at the end of a run;
+* a checkpoint resumes the chain bit for bit;
+* with held-out rows (cross-validation folds) the held-out phenotypes never
+  enter and the identity holds with their current draws.
 """
 import numpy as np
 import pytest
@@ -83,3 +86,62 @@ def test_single_site_and_soda_samplers_share_the_posterior():
     corr = np.corrcoef(ga, gb)[0, 1]
     print(f"correlation of posterior effect means {corr:.4f}")
     assert corr > 0.95
+
+
+@pytest.mark.gpu
+def test_single_site_checkpoint_resumes_bit_for_bit():
+    n, p = 400, 150
+    bed, y, _ = synth.dataset(n, p, seed=83, nqtl=10, h2=0.5)
+    o = soda.SodaOptions(method="BayesR", nmix=4, mix=R4, numit=30, burnin=2, seed=41, sodaoff=True)
+
+    def make():
+        s = soda.SodaSampler(o, n, p, y, bed=bed)
+        s.setup()
+        return s
+
+    a = make()
+    a.init()
+    a.run(9, trace=False)
+    blob = a.save_state()
+    hyp_a = a.run(7)
+    want = {q: a.get(q, k, 0) for q, k in (("G", p), ("CLASS", p), ("YADJ", n), ("SCALARS", 4), ("GSTORE", p))}
+    a.close()
+    b = make()
+    b.load_state(blob)
+    hyp_b = b.run(7)
+    assert np.array_equal(hyp_a, hyp_b)
+    for q, k in (("G", p), ("CLASS", p), ("YADJ", n), ("SCALARS", 4), ("GSTORE", p)):
+        assert np.array_equal(want[q], b.get(q, k, 0)), q
+    b.close()
+
+
+@pytest.mark.gpu
+def test_single_site_folds_keep_the_identity_and_never_read_heldout_phenotypes():
+    n, p, C = 500, 120, 4
+    bed, y, _ = synth.dataset(n, p, seed=84, nqtl=10, h2=0.5)
+    held = np.zeros((C, n), bool)
+    rs = np.random.default_rng(85)
+    for c in range(C):
+        held[c, rs.choice(n, size=n // 5, replace=False)] = True
+    ys = np.tile(y, (C, 1))
+    junk = ys.copy()
+    junk[held] = 1.0e6
+
+    def run(why):
+        o = soda.SodaOptions(method="BayesCpi", numit=12, burnin=0, seed=43, rng="counter", chains=C, sodaoff=True)
+        s = soda.SodaSampler(o, n, p, why, bed=bed, heldout=held)
+        s.setup()
+        s.init()
+        s.run(trace=False)
+        return s
+
+    a, b = run(ys), run(junk)
+    Z = orc.standardize(bed, n, orc.freq(bed, n))
+    for c in range(C):
+        ga, gb = a.get("G", p, c), b.get("G", p, c)
+        assert np.array_equal(ga, gb), c
+        r, mu = a.get("YADJ", n, c), a.get("SCALARS", 4, c)[0]
+        want = a.get("WHY", n, c) - Z @ ga - mu
+        assert np.linalg.norm(r - want) < 1e-9 * np.linalg.norm(want), c
+    a.close()
+    b.close()
