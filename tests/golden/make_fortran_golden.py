@@ -4,7 +4,8 @@ Runs the reference program (oracle/_ref/sodabayes_ref, built from
 /root/reference/src by ``make -C oracle ref``) with its default -nthreads 1
 on small seeded synthetic PLINK data, from the command line and from a .par
 file with two jobs, plus a -predict job on the first job's model files, and
--SODAOff jobs (the single-site sampler) for every method.  For
+-SODAOff jobs (the single-site sampler) for every method, and mixtures of 5
+and 9 classes.  For
 each job the fixture keeps the input files, the arguments and every output
 file (.hyp, .eff, .mod, .frq, .gv, .log).  The GPU test runs the Fortran
 program with its SODA branch on the GPU (sodabayes_amd/fortran) on the same
@@ -29,6 +30,8 @@ from sodabayes_amd import plink, synth  # noqa: E402
 REF = os.path.join(ROOT, "oracle", "_ref", "sodabayes_ref")
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran_jobs.npz")
 R4 = ["-nmix", "4", "-mix", "0.0,0.0001,0.001,0.01"]
+R5 = ["-nmix", "5", "-mix", "0.0,0.0001,0.001,0.01,0.1"]
+R9 = ["-nmix", "9", "-mix", "0.0,0.00001,0.0001,0.0003,0.001,0.003,0.01,0.03,0.1"]
 
 JOB_C = """begin jobC
 -bfile d
@@ -88,6 +91,15 @@ CASES = {
                                           "-out", "rco"], ["rco"]),
     "bayescpi_sodaoff": (150, 140, 50, 0, ["-bfile", "d", "-method", "BayesCpi", "-SODAOff", "-numit", "20",
                                            "-burnin", "5", "-thin", "2", "-seed", "25", "-out", "cpo"], ["cpo"]),
+    # mixtures of more than four classes (5 and 9), with SODA blocks and single-site
+    "bayesr5_blocks": (160, 150, 51, 0, ["-bfile", "d", "-method", "BayesR", *R5, "-blocksize", "50", "-numit", "20",
+                                         "-burnin", "5", "-seed", "26", "-out", "r5"], ["r5"]),
+    "bayesr5_sodaoff": (140, 120, 52, 0, ["-bfile", "d", "-method", "BayesR", *R5, "-SODAOff", "-numit", "20",
+                                          "-burnin", "5", "-seed", "27", "-out", "r5o"], ["r5o"]),
+    "bayesr9_blocks": (150, 140, 53, 0, ["-bfile", "d", "-method", "BayesR", *R9, "-blocksize", "40", "-numit", "15",
+                                         "-burnin", "3", "-seed", "28", "-out", "r9"], ["r9"]),
+    "bayesr9_sodaoff": (130, 100, 54, 0, ["-bfile", "d", "-method", "BayesR", *R9, "-SODAOff", "-numit", "15",
+                                          "-burnin", "3", "-seed", "29", "-out", "r9o"], ["r9o"]),
 }
 PREDICT_ARGS = ["-predict", "-bfile", "d", "-model", "r.mod", "-freq", "r.frq", "-effects", "r.eff",
                 "-out", "p"]

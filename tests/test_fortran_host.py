@@ -27,7 +27,9 @@ FIXTURE = os.path.join(ROOT, "tests", "golden", "fortran_jobs.npz")
 needs_exe = pytest.mark.skipif(not os.path.exists(EXE),
                                reason="the Fortran host is built only where the reference's Fortran sources are")
 CLOCK_LINES = ("Run started at", "Run ended at", "Setup:")
-SODAOFF_CASES = ("bayesr_sodaoff", "bayesa_sodaoff", "bayesb_sodaoff", "bayesrc_sodaoff", "bayescpi_sodaoff")
+SODAOFF_CASES = ("bayesr_sodaoff", "bayesa_sodaoff", "bayesb_sodaoff", "bayesrc_sodaoff", "bayescpi_sodaoff",
+                 "bayesr5_sodaoff", "bayesr9_sodaoff")
+WIDE_MIX_CASES = ("bayesr5_blocks", "bayesr9_blocks")   # 5 and 9 classes, SODA blocks
 # BayesA with -SODAOff: the .mod "Va" line prints varstore, which
 # regular_bayes neither zeroes nor sums for BayesA (models.f90:650-655,
 # 888-899), so the reference prints whatever its allocation held; the GPU
@@ -86,7 +88,7 @@ def run(d: str, args, extra_env=None) -> subprocess.CompletedProcess:
 def test_fixture_lists_every_job():
     z, manifest = load_fixture()
     assert set(manifest) == {"bayesr_cmdline", "bayesa_rblocks", "bayesrc_groups", "bayescpi_nblocks",
-                             "par_two_jobs", *SODAOFF_CASES}
+                             "par_two_jobs", *SODAOFF_CASES, *WIDE_MIX_CASES}
     for case, m in manifest.items():
         for f in m["outputs"]:
             assert f"{case}/out/{f}" in z.files
@@ -122,7 +124,8 @@ def test_help_lists_the_reference_options():
 @needs_exe
 @pytest.mark.parametrize("case,dense_x", [("bayesr_cmdline", False), ("bayesa_rblocks", False),
                                           ("bayesrc_groups", False), ("bayescpi_nblocks", False),
-                                          ("par_two_jobs", False), ("bayesr_cmdline", True)])
+                                          ("par_two_jobs", False), ("bayesr_cmdline", True),
+                                          *((c, False) for c in WIDE_MIX_CASES)])
 def test_gpu_job_files_match_the_reference(tmp_path, case, dense_x):
     """By default the program reads the .bed packed and computes the allele
     frequencies from its counts (no dense genotype matrix on the host);
