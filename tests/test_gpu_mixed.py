@@ -94,11 +94,11 @@ POST_OPTS = dict(method="BayesR", nmix=4, mix=R4, blocksize=100, df_va=4.0, scal
                  df_ve=4.0, scale_ve=0.5)
 
 
-def posterior_trace(rng, precision, chains, numit, burnin, seed):
+def posterior_trace(rng, precision, chains, numit, burnin, seed, opts=POST_OPTS):
     d = POST_DATA
     bed, why, _ = synth.dataset(d["n"], d["p"], seed=d["seed"], nqtl=15, h2=0.5)
     o = soda.SodaOptions(numit=numit, burnin=burnin, seed=seed, rng=rng, precision=precision,
-                         chains=chains, **POST_OPTS)
+                         chains=chains, **opts)
     s = soda.SodaSampler(o, d["n"], d["p"], why, bed=bed)
     s.setup()
     s.init()
@@ -123,8 +123,12 @@ def test_counter_and_mixed_posteriors_match_replay(replay_reference, precision, 
     production samplers (Philox draws; FP64 or mixed-precision batched
     products) agree with the reference algorithm within 4 combined Monte
     Carlo standard errors, and their effect means correlate with it."""
-    ref_hyp, ref_res = replay_reference
     hyp, res = posterior_trace("counter", precision, chains, numit, burnin, 29)
+    compare_posteriors(replay_reference, hyp, res, chains, burnin, f"{precision} x{chains}")
+
+
+def compare_posteriors(reference, hyp, res, chains, burnin, label):
+    ref_hyp, ref_res = reference
     cols = {"mu": 2, "vara": 3, "vare": 4, "included": 1}
     for name, col in cols.items():
         m_ref, se_ref = ref_hyp[:, col].mean(), batch_means_se(ref_hyp[:, col])
@@ -137,12 +141,39 @@ def test_counter_and_mixed_posteriors_match_replay(replay_reference, precision, 
             cm = np.array([float(np.ravel(r[key])[0]) for r in res])
             m_gpu, se_gpu = cm.mean(), cm.std(ddof=1) / np.sqrt(chains)
         z = abs(m_gpu - m_ref) / np.sqrt(se_gpu ** 2 + se_ref ** 2)
-        print(f"{precision} x{chains} {name}: {m_gpu:.6g} +- {se_gpu:.2g} vs reference "
+        print(f"{label} {name}: {m_gpu:.6g} +- {se_gpu:.2g} vs reference "
               f"{m_ref:.6g} +- {se_ref:.2g}  z={z:.2f}")
         assert z < 4.0, name
     g = np.mean([r["gstore"] for r in res], axis=0)
     corr = np.corrcoef(g, ref_res["gstore"])[0, 1]
-    print(f"{precision} x{chains}: correlation of posterior effect means {corr:.4f}")
+    print(f"{label}: correlation of posterior effect means {corr:.4f}")
+    assert corr > 0.95
+
+
+# Six classes: the draw code compiled for up to eight.  REPLAY of that code
+# is pinned by the reference's 5- and 9-class jobs (tests/test_fortran_host.py);
+# here the two Philox samplers, the batched SODA path (class probabilities
+# from per-workgroup constants, draw_mix_fast) and the single-site sampler
+# (draw_mix_wave), 16 chains each, must agree on the posterior.  Their
+# standard errors come from the spread of the chain means: a single long
+# chain mixes too slowly over six classes to serve as the reference.
+WIDE_OPTS = dict(POST_OPTS, nmix=6, mix="0.0,0.00003,0.0001,0.0003,0.001,0.01")
+
+
+def test_wide_mixture_samplers_agree():
+    C = 16
+    _, soda_res = posterior_trace("counter", "mixed", C, 1500, 500, 31, WIDE_OPTS)
+    _, site_res = posterior_trace("counter", "fp64", C, 1500, 500, 37, dict(WIDE_OPTS, sodaoff=True))
+    for key in ("mu_store", "vara_store", "vare_store", "included_store"):
+        a = np.array([float(np.ravel(r[key])[0]) for r in soda_res])
+        b = np.array([float(np.ravel(r[key])[0]) for r in site_res])
+        z = abs(a.mean() - b.mean()) / np.sqrt(a.var(ddof=1) / C + b.var(ddof=1) / C)
+        print(f"6 classes {key}: SODA mixed x{C} {a.mean():.6g} vs single-site x{C} {b.mean():.6g}  z={z:.2f}")
+        assert z < 4.0, key
+    ga = np.mean([r["gstore"] for r in soda_res], axis=0)
+    gb = np.mean([r["gstore"] for r in site_res], axis=0)
+    corr = np.corrcoef(ga, gb)[0, 1]
+    print(f"6 classes: correlation of posterior effect means {corr:.4f}")
     assert corr > 0.95
 
 
