@@ -72,6 +72,7 @@ def parse_args():
     ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
     ap.add_argument("--no-folds", action="store_true", help="skip the cross-validation folds pass")
     ap.add_argument("--no-traits", action="store_true", help="skip the batched traits pass")
+    ap.add_argument("--no-single-site", action="store_true", help="skip the single-site (-SODAOff) pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (a known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -353,6 +354,38 @@ def run_traits(soda, torch, a) -> dict:
     return out
 
 
+def run_single_site(soda, torch, a) -> dict:
+    """The single-site sampler (-SODAOff, regular_bayes) on the reference's
+    example shape: BayesR, n = 2,326, p = 9,346 (examples/example2), 4-class
+    mixture, synthetic genotypes of that shape.  REPLAY, one chain (the
+    reference's -nthreads 1 stream), then 64 counter-based chains, one
+    workgroup each; iterations timed after a short warm-up.  The chain is
+    serial in SNPs, so this is a latency figure, not a bandwidth one."""
+    n, p = 2326, 9346
+    bed, freq, why = synth_on_device(torch, n, p, a.seed + 3, nqtl=200, h2=0.5)
+    out = {"workload": f"single-site sampler (-SODAOff) BayesR n={n} p={p}, 4-class mixture, FP64",
+           "unit": "SNP-effect updates/s"}
+    for key, kw, its in (("replay", dict(rng="replay"), 20), ("counter_64_chains", dict(rng="counter", chains=64), 10)):
+        warm = 3
+        o = soda.SodaOptions(method="BayesR", nmix=4, mix="0.0,0.0001,0.001,0.01", seed=a.seed, sodaoff=True,
+                             numit=warm + its, burnin=0, **kw)
+        s = soda.SodaSampler(o, n, p, why, freq=freq, bed_device_ptr=bed.data_ptr())
+        s.setup()
+        s.init()
+        s.run(warm, trace=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(its, trace=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        s.close()
+        C = kw.get("chains", 1)
+        out[key] = {"value": p * C * its / dt, "ms_per_step": dt * 1e3 / its, "chains": C}
+        log(f"[single-site {key}] {out[key]['ms_per_step']:.2f} ms/iteration, {out[key]['value']:.4g} SNP-updates/s")
+    del bed
+    return out
+
+
 def main():
     a = parse_args()
     if a.gpus != 1:
@@ -383,6 +416,8 @@ def main():
     torch.cuda.empty_cache()
     traits = None if a.no_traits else run_traits(soda, torch, a)
     torch.cuda.empty_cache()
+    single = None if a.no_single_site else run_single_site(soda, torch, a)
+    torch.cuda.empty_cache()
 
     cpu = None if a.no_cpu_baseline else cpu_baseline(a.seed)
     out = {
@@ -404,6 +439,7 @@ def main():
         "tile_shape": mixed.get("tile_shape"),
         "folds": folds,
         "traits": traits,
+        "single_site": single,
         "roofline": mixed.get("roofline"),
         "cpu_baseline": cpu,
     }
