@@ -516,8 +516,10 @@ int single_setup(sb_handle *h)
 int single_sweep(sb_handle *h, int rep)
 {
     const Par &P = h->par;
-    int E = 1;
-    while (E < 8 && (int64_t)E * SST < P.n) E *= 2;
+    // individuals per thread and pass: the smallest count that covers n in
+    // one pass, up to 8 (then passes of 8 * SST)
+    const int64_t need = (P.n + SST - 1) / SST;
+    const int E = need <= 6 ? (int)std::max<int64_t>(need, 1) : 8;
     const int nchunk = (int)((P.n + (int64_t)SST * E - 1) / ((int64_t)SST * E));
     const int64_t ldn = P.ldr;
     with_nmix(P.nmix, [&](auto nm) {
@@ -525,7 +527,10 @@ int single_sweep(sb_handle *h, int rep)
         switch (E) {
             case 1: launch_single<NM, 1>(h, rep, ldn, nchunk); break;
             case 2: launch_single<NM, 2>(h, rep, ldn, nchunk); break;
+            case 3: launch_single<NM, 3>(h, rep, ldn, nchunk); break;
             case 4: launch_single<NM, 4>(h, rep, ldn, nchunk); break;
+            case 5: launch_single<NM, 5>(h, rep, ldn, nchunk); break;
+            case 6: launch_single<NM, 6>(h, rep, ldn, nchunk); break;
             default: launch_single<NM, 8>(h, rep, ldn, nchunk); break;
         }
     });
