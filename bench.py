@@ -73,6 +73,7 @@ def parse_args():
     ap.add_argument("--no-folds", action="store_true", help="skip the cross-validation folds pass")
     ap.add_argument("--no-traits", action="store_true", help="skip the batched traits pass")
     ap.add_argument("--no-single-site", action="store_true", help="skip the single-site (-SODAOff) pass")
+    ap.add_argument("--no-c3", action="store_true", help="skip the BayesA / BayesB pass on the headline genotypes")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 1 GiB device-to-device copy (a known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -354,6 +355,46 @@ def run_traits(soda, torch, a) -> dict:
     return out
 
 
+def run_c3(soda, torch, a, why, freq, bed_ptr) -> dict:
+    """BayesA and BayesB on the headline genotypes (n = 50,000, p = 500,000,
+    subsets of 4,096): -pi 0.99 -df_va 4 -df_ve 4, prior scales set from
+    h2 = 0.4 of the phenotype's variance, mixed batched path, ``--chains``
+    counter-based chains; a few warm-up iterations, then ``steps`` timed."""
+    import numpy as np
+
+    n, p, bs, C = a.nind, a.nloci, a.blocksize, a.chains
+    vary = float(np.var(why))
+    df = 4.0
+    out = {"workload": f"SODA BayesA and BayesB n={n} p={p} n_s={bs}, -pi 0.99 -df_va 4 -df_ve 4 (scales from "
+                       f"h2 = 0.4), {C} chains batched, mixed batched path, counter-based RNG",
+           "unit": "SNP-effect updates/s"}
+    for method in ("BayesA", "BayesB"):
+        # the prior modes of the per-SNP and residual variances: h2 vary spread
+        # over the SNPs (over the 1 - pi included ones for BayesB), (1 - h2) vary
+        share = p * (1.0 if method == "BayesA" else 0.01)
+        kw = dict(df_va=df, df_ve=df, scale_va=0.4 * vary / share * (df + 2.0) / df,
+                  scale_ve=0.6 * vary * (df + 2.0) / df)
+        if method == "BayesB":
+            kw["pi"] = 0.99
+        warm = 5
+        o = soda.SodaOptions(method=method, blocksize=bs, numit=warm + a.steps, burnin=0, seed=a.seed,
+                             rng="counter", precision="mixed", chains=C, **kw)
+        s = soda.SodaSampler(o, n, p, why, freq=freq, bed_device_ptr=bed_ptr)
+        s.setup()
+        s.init()
+        s.run(warm, trace=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(a.steps, trace=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        s.close()
+        out[method] = {"value": p * C * a.steps / dt, "ms_per_step": dt * 1e3 / a.steps, "chains": C}
+        log(f"[c3 {method} x{C}] {out[method]['ms_per_step']:.2f} ms/iteration, "
+            f"{out[method]['value']:.4g} SNP-updates/s")
+    return out
+
+
 def run_single_site(soda, torch, a) -> dict:
     """The single-site sampler (-SODAOff, regular_bayes) on the reference's
     example shape: BayesR, n = 2,326, p = 9,346 (examples/example2), 4-class
@@ -410,6 +451,7 @@ def main():
     log(f"genotypes generated in HBM: n={n} p={p} ({bed.numel() / 1e9:.2f} GB) in {time.time() - t0:.1f} s")
     fp64 = None if a.chains_only else run_path(soda, torch, a, why, freq, bed.data_ptr(), "fp64", 1)
     mixed = run_path(soda, torch, a, why, freq, bed.data_ptr(), "mixed", a.chains)
+    c3 = None if a.no_c3 else run_c3(soda, torch, a, why, freq, bed.data_ptr())
     del bed
     torch.cuda.empty_cache()
     folds = None if a.no_folds else run_folds(soda, torch, a)
@@ -439,6 +481,7 @@ def main():
         "tile_shape": mixed.get("tile_shape"),
         "folds": folds,
         "traits": traits,
+        "bayesa_bayesb": c3,
         "single_site": single,
         "roofline": mixed.get("roofline"),
         "cpu_baseline": cpu,
