@@ -24,22 +24,6 @@ namespace sb {
 
 namespace {
 constexpr int NT = 256;
-
-// Loads of operands a launch reads once (genotype tiles, U tiles): marked
-// non-temporal, so that they do not displace the limb slabs that every
-// workgroup re-reads.
-__device__ __forceinline__ uint4 ld_once(const uint4 *p)
-{
-    typedef unsigned int v4u __attribute__((ext_vector_type(4)));
-    const v4u v = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(p));
-    return make_uint4(v.x, v.y, v.z, v.w);
-}
-__device__ __forceinline__ float4 ld_once(const float4 *p)
-{
-    typedef float v4f __attribute__((ext_vector_type(4)));
-    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
-    return make_float4(v.x, v.y, v.z, v.w);
-}
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr uint32_t M2 = 0x03030303u;
@@ -331,7 +315,7 @@ __device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ g
         uint4 a[IT];
 #pragma unroll
         for (int it = 0; it < IT; it++)
-            a[it] = valid[it] ? ld_once(&abase[((int64_t)it * nS + S) * 64]) : make_uint4(0, 0, 0, 0);
+            a[it] = valid[it] ? abase[((int64_t)it * nS + S) * 64] : make_uint4(0, 0, 0, 0);
         __syncthreads();   // waits for this wave's loads; then every wave's part of slabs S, S+1 is in LDS
         const uint4 *B = lds + (S % NBUF) * SLAB * 64 + lane;
 #pragma unroll
@@ -403,7 +387,7 @@ __device__ __forceinline__ void ut_tile(const Par &P, const float4 *__restrict__
         for (int u = 0; u < UKU; u++) {
             const int KK = min(K + u, K1 - 1);
             const int I = MODE == 0 ? KK : O, J = MODE == 0 ? O : KK;
-            a[u] = ld_once(&Ut[((int64_t)J * (J + 1) / 2 + I) * 64 + lane]);
+            a[u] = Ut[((int64_t)J * (J + 1) / 2 + I) * 64 + lane];
             if (K + u >= K1) a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
             for (int t = 0; t < NTC; t++) r[u][t] = Rsw[((int64_t)KK * NTC + t) * 64 + lane];
@@ -731,7 +715,7 @@ __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__
         if (ss + 1 < nss) slab(ss + 1);
         uint4 a[IT];
 #pragma unroll
-        for (int it = 0; it < IT; it++) a[it] = ld_once(&abase[((int64_t)it * nss + ss) * 64]);
+        for (int it = 0; it < IT; it++) a[it] = abase[((int64_t)it * nss + ss) * 64];
         __syncthreads();   // waits for this wave's loads; then every wave's part of slabs ss, ss+1 is in LDS
         const uint4 *B = lds + (ss % NBUF) * SLAB * 64 + lane;
 #pragma unroll
