@@ -29,6 +29,15 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr uint32_t M2 = 0x03030303u;
 constexpr int LIMBS = 3;  // int8 limbs per fixed-point value (22 bits)
 
+// Exact value of a chain's limb sums t_0 .. t_{LIMBS-1} (limb k weighs 256^k)
+__device__ __forceinline__ long long limb_sum(const int *t)
+{
+    long long S = 0;
+#pragma unroll
+    for (int k = LIMBS - 1; k >= 0; k--) S = S * 256 + (long long)t[k];
+    return S;
+}
+
 // One 32-bit word of 2-bit counts as a 16-byte MFMA operand: dword m holds
 // the counts of positions m, m+4, m+8, m+12 (bytes 0..3).  Hence the limb
 // slot order {0,4,8,12, 1,5,9,13, 2,6,10,14, 3,7,11,15} of the other operand.
@@ -200,15 +209,17 @@ __global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const dou
     quant_wg(P, L, ntile, r, rsum, rmax, nwg, Q, expo, rtot, blockIdx.x, blockIdx.z);
 }
 
-// Tp[chunk][col][jk] = sum over the chunk's individuals of x_(p0+jk, i) * Q[col][i]
-// (columns of length blocksize).
+// Tp[chunk][c][jk] = sum over the chunk's individuals and the limbs k of
+// 256^k x_(p0+jk, i) * Q[c L + k][i]: one exact int64 per (chunk, chain, SNP),
+// the limb columns combined after the exact int32 sums (columns of length
+// blocksize).
 // Workgroup: 16*IT SNPs x all 16*NTILE columns over one chunk of `kss`
 // super-steps (16 words = 256 individuals).  The four waves split the
 // chunk's super-steps; every operand load is one contiguous 1 KB per wave
 // (A tiles of geno_sw, B blocks of Q); the waves are summed exactly in LDS.
 template <int NTILE, int IT>
 __device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                        const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp,
+                                        const uint4 *__restrict__ Q, int kss, long long *__restrict__ Tp,
                                         int64_t tstride, int bx, int by)
 {
     constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
@@ -259,10 +270,10 @@ __device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ 
             for (int e = 0; e < 4; e++)
                 if (acc[it][t][e] != 0) atomicAdd(&red[16 * it + 4 * g + e][16 * t + l16], acc[it][t][e]);
     __syncthreads();
-    int *out = Tp + (int64_t)by * tstride;
-    for (int k = threadIdx.x; k < ROWS * COLS; k += NT) {
-        const int col = k / ROWS, row = k - col * ROWS;
-        if (r0 + row < nb) out[(int64_t)col * P.blocksize + r0 + row] = red[row][col];
+    long long *out = Tp + (int64_t)by * tstride;
+    for (int k = threadIdx.x; k < ROWS * P.nchains; k += NT) {
+        const int c = k / ROWS, row = k - c * ROWS;
+        if (r0 + row < nb) out[(int64_t)c * P.blocksize + r0 + row] = limb_sum(red[row] + LIMBS * c);
     }
 }
 
@@ -275,7 +286,7 @@ constexpr int prod_rows(int ntile) { return ntile <= 6 ? 2 : 1; }   // row tiles
 constexpr bool prod_lds(int ntile) { return ntile <= 6; }
 template <int NTILE, int IT>
 __device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                       const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp, int64_t tstride,
+                                       const uint4 *__restrict__ Q, int kss, long long *__restrict__ Tp, int64_t tstride,
                                        int bx, int by)
 {
     constexpr int W = NT / WAVE;
@@ -342,10 +353,10 @@ __device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ g
 #pragma unroll
             for (int e = 0; e < 4; e++) red[16 * (IT * wave + it) + 4 * g + e][16 * t + l16] = acc[it][t][e];
     __syncthreads();
-    int *out = Tp + (int64_t)by * tstride;
-    for (int k = threadIdx.x; k < ROWS * COLS; k += NT) {
-        const int col = k / ROWS, row = k - col * ROWS;
-        if (r0 + row < nb) out[(int64_t)col * P.blocksize + r0 + row] = red[row][col];
+    long long *out = Tp + (int64_t)by * tstride;
+    for (int k = threadIdx.x; k < ROWS * P.nchains; k += NT) {
+        const int c = k / ROWS, row = k - c * ROWS;
+        if (r0 + row < nb) out[(int64_t)c * P.blocksize + r0 + row] = limb_sum(red[row] + LIMBS * c);
     }
 }
 
@@ -812,7 +823,7 @@ void with_rows(int rw, F &&f)
 // g0x x .) and the U_s'R_a workgroups (n1, grid g1x x .) run side by side.
 template <int NTILE, int IT, int NTC, int RW>
 __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                             const uint4 *__restrict__ Q, int kss, int *__restrict__ Tp,
+                                             const uint4 *__restrict__ Q, int kss, long long *__restrict__ Tp,
                                              int64_t tstride, int g0x, int n0, const float4 *__restrict__ Ut,
                                              const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
                                              int g1x, int n1)
@@ -859,7 +870,7 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
 constexpr int DCH = 16;  // chunk partials unrolled in k_draw_b
 constexpr int DRG = 16;  // r_a row-group partials unrolled in k_draw_b
 template <int NM>
-__global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int *Tp,
+__global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long long *Tp,
                                                int64_t tstride, int nch, const int *expo, const double *rtot,
                                                const double *rapart, int spant, const double *f2,
                                                const double *sdv, const int32_t *grp,
@@ -893,14 +904,10 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
         const int cold = P.method == SB_BAYES_A ? 2 : cls[o + pos];
         const double vs = ab ? vara_s[o + pos] : 0.0;
         const int gp = grp[pos], sid = snp_of_pos[pos];
-        const int *tcol = Tp + (int64_t)c * L * P.blocksize + jkc;
-        int v[DCH][3];
+        const long long *tcol = Tp + (int64_t)c * P.blocksize + jkc;
+        long long v[DCH];
 #pragma unroll
-        for (int u = 0; u < DCH; u++) {
-            const int64_t ch = min(u, nch - 1);
-#pragma unroll
-            for (int k = 0; k < 3; k++) v[u][k] = tcol[ch * tstride + (int64_t)k * P.blocksize];
-        }
+        for (int u = 0; u < DCH; u++) v[u] = tcol[(int64_t)min(u, nch - 1) * tstride];
         // row groups entirely below this column's diagonal tile hold no partial
         const int ng = (jkc >> 4) / spant + 1;
         const double *rp = rapart + (int64_t)c * P.blocksize + jkc;
@@ -910,20 +917,10 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const int 
         for (int u = 0; u < DRG; u++) ra[u] = rp[(int64_t)min(u, ng - 1) * gstride];
         if (mixm && !rhs_out) mix_const(P, sc + c, zz, mc);   // workgroup-uniform
         if (jk >= nb) goto reduce;
-        long long t0 = 0, t1 = 0, t2 = 0;
+        long long S = 0;
 #pragma unroll
-        for (int u = 0; u < DCH; u++) {
-            const int keep = u < nch;
-            t0 += keep ? v[u][0] : 0;
-            t1 += keep ? v[u][1] : 0;
-            t2 += keep ? v[u][2] : 0;
-        }
-        for (int ch = DCH; ch < nch; ch++) {   // more chunks than unrolled
-            t0 += tcol[(int64_t)ch * tstride];
-            t1 += tcol[(int64_t)ch * tstride + P.blocksize];
-            t2 += tcol[(int64_t)ch * tstride + 2 * (int64_t)P.blocksize];
-        }
-        const long long S = (t2 * 256 + t1) * 256 + t0;
+        for (int u = 0; u < DCH; u++) S += u < nch ? v[u] : 0;
+        for (int ch = DCH; ch < nch; ch++) S += tcol[(int64_t)ch * tstride];   // more chunks than unrolled
         const double sx = ldexp((double)S, -expo[c]);
         double rhs = s > 0.0 ? (sx - f2p * rtot[c]) / s : 0.0;
 #pragma unroll
@@ -982,7 +979,7 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int kss)
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
     const int C = P.nchains, nS = P.ldw / 16;
     const uint4 *Q = reinterpret_cast<const uint4 *>(h->b_Q);
-    const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
+    const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
     const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
     const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
     const float4 *ut = reinterpret_cast<const float4 *>(h->Ut0) + h->tt_off[b] * 64;
@@ -994,7 +991,8 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int kss)
             const int rt = prod_lds(NTL) ? 4 * RW : IT;   // row tiles per workgroup
             const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
             const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
-            launch(kc, k_prod<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss, h->b_T,
+            launch(kc, k_prod<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss,
+                   reinterpret_cast<long long *>(h->b_T),
                    tstride, g0x, n0, ut, reinterpret_cast<const float4 *>(h->ra_sw), spant, h->b_rapart, g1x, n1);
         });
     });
@@ -1053,7 +1051,7 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     }
     h->q_block = -1;
     kc.begin(st);
-    kc.last(KS_XQ, gbytes + cl * n + 4.0 * cl * nb + tbytes + 12.0 * C * nb);
+    kc.last(KS_XQ, gbytes + cl * n + 8.0 * C * nb + tbytes + 12.0 * C * nb);
     int rc = prod_launch(h, b, h->b_rw, h->kss);
     if (rc) return fail(rc, "unsupported chain count");
     return hip_check(hipGetLastError(), "batch products");
@@ -1212,18 +1210,17 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
 {
     const Par &P = h->par;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
-    const int64_t tstride = (int64_t)P.blocksize * h->b_ldt;
+    const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
     const int nS = P.ldw / 16;
     const int nch = (nS + h->kss - 1) / h->kss;
     const double C = P.nchains;
     h->kc.begin(h->stream);
-    // limb partials, r_a partials, effect state read and written
-    h->kc.last(KS_DRAWB,
-               4.0 * nch * C * h->limbs * nb + 8.0 * C * nb * ((nb + 255) / 256 / 2.0 + 1.0) + 60.0 * C * nb);
+    // chunk partials (one int64 each), r_a partials, effect state read and written
+    h->kc.last(KS_DRAWB, 8.0 * nch * C * nb + 8.0 * C * nb * ((nb + 255) / 256 / 2.0 + 1.0) + 60.0 * C * nb);
     with_nmix(P.nmix, [&](auto nm) {
         launch(h->kc, k_draw_b<decltype(nm)::value>, dim3((nb + NT - 1) / NT, P.nchains), dim3(NT), 0, h->stream,
-               P, h->scal, h->b_T, tstride, nch, h->b_expo, h->b_rtot, h->b_rapart, h->rows_per_group / 16, h->f2,
-               h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls, h->vara_s, delta, chg,
+               P, h->scal, reinterpret_cast<const long long *>(h->b_T), tstride, nch, h->b_expo, h->b_rtot,
+               h->b_rapart, h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls, h->vara_s, delta, chg,
                rhs_out, h->b_dmax, h->b_dk, h->dl_sw);
     });
     return hip_check(hipGetLastError(), "batch draw");
