@@ -49,9 +49,9 @@ KERNEL_NAMES = {
 # (profiles/soda_bayesr_n50k_bs4096_x16_pmc_traffic.md); None where not measured
 TRAFFIC_CHAINS = 16
 TRAFFIC = {
-    ("mixed", "xq"): int((52665 * 2 + 14336) * 1024),     # k_prod<3,4,1,2>
-    ("mixed", "xupd"): int((46985 * 2 + 11002) * 1024),   # k_upd<3,4,1,2>
-    ("mixed", "drawb"): int((7754 * 2 + 1616) * 1024),    # k_draw_b<4>
+    ("mixed", "xq"): int((52670 * 2 + 11008) * 1024),     # k_prod<3,4,1,2>
+    ("mixed", "xupd"): int((46982 * 2 + 11002) * 1024),   # k_upd<3,4,1,2>
+    ("mixed", "drawb"): int((7100 * 2 + 1616) * 1024),    # k_draw_b<4>
     ("fp64", "rhs"): int((60273 * 2 + 891) * 1024),       # k_rhs
     ("fp64", "update"): int((50600 * 2 + 807) * 1024),    # k_update (k_ra_sum not counted)
 }
