@@ -81,7 +81,7 @@ __global__ __launch_bounds__(GT) void k_win_pack(Par P, const uint32_t *__restri
 }
 
 // Packed lower triangle of each window's Gram z_j'z_l (l <= j); the diagonal
-// is xpx (models.f90:677).  Chunks of GT individuals are decoded into LDS,
+// is xpx (models.f90:676).  Chunks of GT individuals are decoded into LDS,
 // then thread q < SWT sums its pair over the chunk.
 __global__ __launch_bounds__(GT) void k_win_gram(Par P, const uint32_t *__restrict__ win, int64_t ldn,
                                                  const double *__restrict__ wz, double *wgram)
@@ -135,7 +135,7 @@ __device__ __forceinline__ double wave_sum16(const double (&v)[SW], int lane, in
     return x;
 }
 
-// The terms of a mixture draw (models.f90:805-813, 855-858) that do not
+// The terms of a mixture draw (models.f90:810-811, 852-853) that do not
 // depend on rhs, for SNP j of a window and class k >= 1, held by lane
 // 4j + k of wave 0: logdetV = log(gp_k zz/vare + 1), the denominator
 // zz + vare/gp_k of uhat, v1 = zz + vare/gp_k and sd = sqrt(vare/v1), each
