@@ -100,6 +100,12 @@ CASES = {
                                          "-burnin", "3", "-seed", "28", "-out", "r9"], ["r9"]),
     "bayesr9_sodaoff": (130, 100, 54, 0, ["-bfile", "d", "-method", "BayesR", *R9, "-SODAOff", "-numit", "15",
                                           "-burnin", "3", "-seed", "29", "-out", "r9o"], ["r9o"]),
+    # more individuals than one pass of the GPU single-site sweep covers with
+    # one per thread (3 per thread; 8 per thread over two passes)
+    "bayesr_sodaoff_n1100": (1100, 90, 55, 0, ["-bfile", "d", "-method", "BayesR", *R4, "-SODAOff", "-numit", "10",
+                                                "-burnin", "2", "-seed", "30", "-out", "rn"], ["rn"]),
+    "bayescpi_sodaoff_n4500": (4500, 70, 56, 0, ["-bfile", "d", "-method", "BayesCpi", "-SODAOff", "-numit", "8",
+                                                  "-burnin", "2", "-seed", "31", "-out", "cn"], ["cn"]),
 }
 PREDICT_ARGS = ["-predict", "-bfile", "d", "-model", "r.mod", "-freq", "r.frq", "-effects", "r.eff",
                 "-out", "p"]
