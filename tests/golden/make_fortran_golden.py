@@ -106,6 +106,9 @@ CASES = {
                                                 "-burnin", "2", "-seed", "30", "-out", "rn"], ["rn"]),
     "bayescpi_sodaoff_n4500": (4500, 70, 56, 0, ["-bfile", "d", "-method", "BayesCpi", "-SODAOff", "-numit", "8",
                                                   "-burnin", "2", "-seed", "31", "-out", "cn"], ["cn"]),
+    # SODA blocks over more individuals than one 2,048-individual genotype chunk
+    "bayesr_blocks_n4500": (4500, 240, 57, 0, ["-bfile", "d", "-method", "BayesR", *R4, "-blocksize", "80",
+                                               "-numit", "8", "-burnin", "2", "-seed", "32", "-out", "rb"], ["rb"]),
 }
 PREDICT_ARGS = ["-predict", "-bfile", "d", "-model", "r.mod", "-freq", "r.frq", "-effects", "r.eff",
                 "-out", "p"]

@@ -29,7 +29,8 @@ needs_exe = pytest.mark.skipif(not os.path.exists(EXE),
 CLOCK_LINES = ("Run started at", "Run ended at", "Setup:")
 SODAOFF_CASES = ("bayesr_sodaoff", "bayesa_sodaoff", "bayesb_sodaoff", "bayesrc_sodaoff", "bayescpi_sodaoff",
                  "bayesr5_sodaoff", "bayesr9_sodaoff", "bayesr_sodaoff_n1100", "bayescpi_sodaoff_n4500")
-WIDE_MIX_CASES = ("bayesr5_blocks", "bayesr9_blocks")   # 5 and 9 classes, SODA blocks
+WIDE_MIX_CASES = ("bayesr5_blocks", "bayesr9_blocks",   # 5 and 9 classes, SODA blocks
+                  "bayesr_blocks_n4500")                  # SODA blocks over three genotype chunks
 # BayesA with -SODAOff: the .mod "Va" line prints varstore, which
 # regular_bayes neither zeroes nor sums for BayesA (models.f90:650-655,
 # 888-899), so the reference prints whatever its allocation held; the GPU
