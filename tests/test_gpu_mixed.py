@@ -82,6 +82,27 @@ def test_block_products_against_fp64(method, precision, chains, bar):
     s.close()
 
 
+@pytest.mark.parametrize("precision,chains,bar", [("mixed", 16, 1e-6), ("fp64", 1, 1e-12)])
+def test_block_products_over_several_chunks(precision, chains, bar):
+    """n = 9,000 individuals: the products span five 2,048-individual
+    genotype chunks and several X_s'Q chunks of 16 super-steps, whose exact
+    partials are summed across workgroups; blocks of 512 SNPs."""
+    n, p = 9000, 1024
+    s, bed, why = sampler("BayesR", precision, chains, 6, blocksize=512, n=n, p=p)
+    s.setup()
+    s.init()
+    s.run(2, trace=False)
+    worst = 0.0
+    for b in range(s.nblocks):
+        got = s.block_rhs(b)
+        for c in range(chains):
+            ref = fp64_products(s, bed, c)[b]
+            worst = max(worst, np.linalg.norm(got[c] - ref) / np.linalg.norm(ref))
+    print(f"n={n} {precision} x{chains}: worst norm-wise relative error of X'r + A'r_a = {worst:.3e}")
+    assert worst <= bar
+    s.close()
+
+
 def batch_means_se(x, nb=20):
     x = np.asarray(x)
     m = len(x) // nb
