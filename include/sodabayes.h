@@ -68,7 +68,9 @@ typedef struct {
     double mix[20];             /* class variances as a fraction of Va */
     double alpha[20];           /* Dirichlet prior */
     int32_t numit, burnin, thin;
-    int32_t pad0;
+    int32_t resync;             /* iterations between residual resyncs r = y - X g - mu
+                                 * (compute_residuals, models.f90:571-573); 0: the
+                                 * reference's 1000 */
     int64_t seed;               /* COUNTER mode key (REPLAY uses sb_rng_set) */
 } sb_priors;
 
@@ -107,12 +109,18 @@ int sb_rng_get(sb_handle *h, uint64_t state[2]);
  * SB_Q_NAIND).  Returns SB_ERR_NOT_PD when the pivoted factorisation fails
  * too (the reference stops there), or, with *fail_block / *fail_col
  * (1-based) set, for any failing block under SB_PREC_MIXED: the fallback
- * runs on the FP64 path only.  The environment variable
- * SODABAYES_TEST_D_SHIFT="<block>:<amount>" (0-based block) lowers that
- * block's d before the factorisation, to exercise the fallback in tests.
+ * runs on the FP64 path only.
  * For the single-site sampler (nblocks = 0) it builds no matrices and
  * leaves xxd untouched: every SNP is updated one by one, in permvec order. */
 int sb_build_aug(sb_handle *h, double *xxd, int32_t *fail_block, int32_t *fail_col);
+
+/* Test hook (no counterpart in the reference; for tests that must reach the
+ * pivoted-Cholesky fallback on small data): the next sb_build_aug on this
+ * handle lowers block `block`'s (0-based) bound d by `amount` before the
+ * factorisation, so that dI - X_s'X_s is indefinite there.  block < 0
+ * clears it.  Nothing else sets it: a job never runs with a shifted d
+ * unless its caller asks for one. */
+int sb_test_shift_d(sb_handle *h, int32_t block, double amount);
 
 /* ---- MCMC (replaces soda_bayes, models.f90:9-605) ---- */
 
@@ -231,6 +239,14 @@ int sb_state_save(sb_handle *h, void *buf, int64_t size);
  * sb_mcmc_init.  sb_mcmc_run then continues from iteration count + 1 and
  * gives the chain the uninterrupted run would have given. */
 int sb_state_load(sb_handle *h, const void *buf, int64_t size);
+
+/* Diagnostics of the COUNTER generator (no counterpart in the reference):
+ * for each of n draw sites k, the first `per` uniforms in [0, 1) of the
+ * stream the sampler opens for chain `chain` of a handle seeded `seed`, at
+ * counter (iteration ctr[3k], site ctr[3k+1], tag ctr[3k+2]); out holds
+ * n x per doubles, site-major.  Runs on the current device. */
+int sb_counter_uniforms(int64_t seed, int32_t chain, const uint32_t *ctr, int32_t n, int32_t per,
+                        double *out);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *sb_last_error(void);

@@ -81,6 +81,12 @@ def lib():
         L.orc_permutate.argtypes = [ip, C.c_int, C.c_int, rp]
         L.orc_run.argtypes = [C.POINTER(OrcOpts), u8p, dp, C.POINTER(C.c_int32), C.c_char_p]
         L.orc_run.restype = C.c_int
+        u32p = C.POINTER(C.c_uint32)
+        L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.orc_chain_key.argtypes = [C.c_uint64, C.c_int]
+        L.orc_chain_key.restype = C.c_uint64
+        L.orc_counter_uniforms.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
+        L.orc_counter_first.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -123,6 +129,33 @@ class Rng:
         x = np.zeros_like(a)
         lib().orc_dirichlet(C.byref(self.st), len(a), _ptr(a, C.c_double), _ptr(x, C.c_double))
         return x
+
+
+def philox4x32_10(ctr, key) -> tuple:
+    """Philox4x32-10 block of counter ctr (4 words) under key (2 words)."""
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return tuple(o)
+
+
+def chain_key(seed: int, chain: int) -> int:
+    return int(lib().orc_chain_key(seed & (2 ** 64 - 1), chain))
+
+
+def counter_uniforms(seed: int, chain: int, iteration: int, site: int, tag: int, n: int) -> np.ndarray:
+    """The first n uniforms of one draw site's counter-based stream."""
+    out = np.zeros(n)
+    lib().orc_counter_uniforms(chain_key(seed, chain), iteration, site, tag, n, _ptr(out, C.c_double))
+    return out
+
+
+def counter_first(seed: int, chain: int, iteration: int, site0: int, tag: int, n: int) -> np.ndarray:
+    """First uniform of each of n consecutive sites site0, site0 + 1, ..."""
+    out = np.zeros(n)
+    lib().orc_counter_first(chain_key(seed, chain), iteration, site0, tag, n, _ptr(out, C.c_double))
+    return out
 
 
 def pack_genbits(bed: np.ndarray, nind: int) -> np.ndarray:

@@ -21,6 +21,71 @@ static const double DEPS = 2.220446049250313080847e-16; /* epsilon(1.0d0) */
 
 /* ------------------------------------------------------------------ RNG -- */
 
+/* Philox4x32-10: J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw,
+ * "Parallel random numbers: as easy as 1, 2, 3", SC'11, section 3.2.  Not a
+ * routine of the reference (whose generator is xoroshiro128+): it is the
+ * build's counter-based generator, restated here from the paper so that
+ * the device's uniforms can be checked bit for bit.  One round maps
+ * (c0, c1, c2, c3) with key (k0, k1) to
+ *   (hi(M1 c2) ^ c1 ^ k0, lo(M1 c2), hi(M0 c0) ^ c3 ^ k1, lo(M0 c0)),
+ * M0 = 0xD2511F53, M1 = 0xCD9E8D57; the key is bumped by the Weyl constants
+ * (0x9E3779B9, 0xBB67AE85) between rounds; 10 rounds.  Pinned by the
+ * known-answer vectors the authors publish with their Random123 library
+ * (tests/test_oracle_math.py). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+/* Key of chain c in the counter-based mode: seed ^ (golden-ratio odd
+ * constant * (c + 1)), split into (low, high) 32-bit key words. */
+uint64_t orc_chain_key(uint64_t seed, int chain)
+{
+    return seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(chain + 1));
+}
+
+static inline double int64_to_real64(uint64_t x);
+
+/* The counter-based stream of one draw site: block s = Philox(counter
+ * (iter, site, tag, s), key) gives two 64-bit words (w0 << 32 | w1) and
+ * (w2 << 32 | w3), each mapped to [0, 1) as the reference maps its 64-bit
+ * outputs (rng_int64_to_real64, parallelrng.f90:242-249). */
+void orc_counter_uniforms(uint64_t key, uint32_t iter, uint32_t site, uint32_t tag, int n, double *out)
+{
+    const uint32_t k[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    for (int i = 0, s = 0; i < n; s++) {
+        const uint32_t c[4] = {iter, site, tag, (uint32_t)s};
+        uint32_t b[4];
+        orc_philox4x32_10(c, k, b);
+        out[i++] = int64_to_real64(((uint64_t)b[0] << 32) | b[1]);
+        if (i < n) out[i++] = int64_to_real64(((uint64_t)b[2] << 32) | b[3]);
+    }
+}
+
+/* First uniform of each of n consecutive sites (site0, site0 + 1, ...) of
+ * one (key, iteration, tag): the per-SNP draws of one sweep. */
+void orc_counter_first(uint64_t key, uint32_t iter, uint32_t site0, uint32_t tag, int n, double *out)
+{
+    for (int j = 0; j < n; j++) orc_counter_uniforms(key, iter, site0 + (uint32_t)j, tag, 1, out + j);
+}
+
 static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
 /* rng_xoroshiro128plus, parallelrng.f90:100-123 */

@@ -26,6 +26,13 @@ extern "C" {
 
 typedef struct { uint64_t s1, s2; } orc_rng;
 
+/* Philox4x32-10 (Salmon et al., SC'11): the build's counter-based generator,
+ * not a reference routine; pinned by the Random123 known-answer vectors */
+void     orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+uint64_t orc_chain_key(uint64_t seed, int chain);
+void     orc_counter_uniforms(uint64_t key, uint32_t iter, uint32_t site, uint32_t tag, int n, double *out);
+void     orc_counter_first(uint64_t key, uint32_t iter, uint32_t site0, uint32_t tag, int n, double *out);
+
 /* parallelrng.f90 */
 void     orc_rng_seed(int64_t seed, int nstreams, orc_rng *streams);
 uint64_t orc_rng_next(orc_rng *s);

@@ -27,9 +27,9 @@ KT_SLOTS = ["rhs", "draw", "list", "update", "rquant", "xq", "ut0", "drawb", "dq
 
 # every function include/sodabayes.h declares
 EXPORTS = ["sb_rng_seed", "sb_permutate", "sb_create", "sb_destroy", "sb_rng_set", "sb_rng_get",
-           "sb_build_aug", "sb_mcmc_init", "sb_mcmc_init_chains", "sb_mcmc_run", "sb_hyp_width", "sb_get",
+           "sb_build_aug", "sb_test_shift_d", "sb_mcmc_init", "sb_mcmc_init_chains", "sb_mcmc_run", "sb_hyp_width", "sb_get",
            "sb_block_rhs", "sb_get_info", "sb_predict", "sb_state_size", "sb_state_save", "sb_state_load",
-           "sb_last_error"]
+           "sb_counter_uniforms", "sb_last_error"]
 
 
 class SbDims(C.Structure):
@@ -44,7 +44,7 @@ class SbPriors(C.Structure):
                 ("df_va", C.c_double), ("df_ve", C.c_double), ("vara0", C.c_double),
                 ("mix", C.c_double * 20), ("alpha", C.c_double * 20),
                 ("numit", C.c_int32), ("burnin", C.c_int32), ("thin", C.c_int32),
-                ("pad0", C.c_int32), ("seed", C.c_int64)]
+                ("resync", C.c_int32), ("seed", C.c_int64)]
 
 
 class SbError(RuntimeError):
@@ -72,6 +72,7 @@ def load():
     L.sb_rng_set.argtypes = [vp, u64p]
     L.sb_rng_get.argtypes = [vp, u64p]
     L.sb_build_aug.argtypes = [vp, dp, i32p, i32p]
+    L.sb_test_shift_d.argtypes = [vp, C.c_int32, C.c_double]
     L.sb_mcmc_init.argtypes = [vp, C.POINTER(SbPriors), dp]
     L.sb_mcmc_init_chains.argtypes = [vp, C.POINTER(SbPriors), dp, C.c_int64, C.POINTER(C.c_uint8), C.c_int64]
     L.sb_mcmc_run.argtypes = [vp, C.c_int32, C.c_int32, dp]
@@ -84,6 +85,7 @@ def load():
     L.sb_state_size.argtypes = [vp, C.POINTER(C.c_int64)]
     L.sb_state_save.argtypes = [vp, vp, C.c_int64]
     L.sb_state_load.argtypes = [vp, vp, C.c_int64]
+    L.sb_counter_uniforms.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_uint32), C.c_int32, C.c_int32, dp]
     L.sb_last_error.restype = C.c_char_p
     for f in EXPORTS:
         if f not in ("sb_hyp_width", "sb_last_error"):

@@ -113,6 +113,8 @@ std::vector<Section> state_sections(sb_handle *h, bool heldout)
     };
     if (heldout) v.push_back({h->heldout, ldr * C});
     if (h->ra_sw) v.push_back({h->ra_sw, sizeof(float) * 16 * ((bs + 15) / 16) * 16 * ((C + 15) / 16)});
+    // the mixed path's carried quantisation remainders: r is exact only with them
+    if (h->b_carry) v.push_back({h->b_carry, sizeof(double) * p * C});
     return v;
 }
 
@@ -344,9 +346,10 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         h->b_ldd = (int64_t)16 * sb::batch_words((int)bs);
         A((void **)&h->b_Dq, (size_t)16 * h->b_ntile * h->b_ldd);
         A((void **)&h->b_expo2, sizeof(int) * C);
-        A((void **)&h->b_kc, sizeof(double) * C);
         A((void **)&h->b_dmax, sizeof(uint64_t) * C * nwg_d);
-        A((void **)&h->b_dk, sizeof(double) * C * nwg_d);
+        A((void **)&h->b_kpart, sizeof(double) * C * nwg_d);
+        A((void **)&h->b_cj, sizeof(double) * C * bs);
+        A((void **)&h->b_carry, sizeof(double) * C * (size_t)d.nloci);
         A((void **)&h->ra_sw, sizeof(float) * rows16 * 16 * ntc);
         A((void **)&h->dl_sw, sizeof(float) * rows16 * 16 * ntc);
         h->gt_off.assign(d.nblocks + 1, 0);
@@ -452,6 +455,23 @@ int sb_build_aug(sb_handle *h, double *xxd, int32_t *fail_block, int32_t *fail_c
 {
     if (!h) return fail(SB_ERR_ARG, "sb_build_aug: null handle");
     return setup_build_aug(h, xxd, fail_block, fail_col);
+}
+
+int sb_counter_uniforms(int64_t seed, int32_t chain, const uint32_t *ctr, int32_t n, int32_t per, double *out)
+{
+    if (!ctr || !out || n < 0 || per < 0 || chain < 0) return fail(SB_ERR_ARG, "sb_counter_uniforms: bad argument");
+    if (n == 0 || per == 0) return SB_OK;
+    return counter_uniforms(seed, chain, ctr, n, per, out);
+}
+
+int sb_test_shift_d(sb_handle *h, int32_t block, double amount)
+{
+    if (!h) return fail(SB_ERR_ARG, "sb_test_shift_d: null handle");
+    if (block >= h->par.nblocks || !(amount >= 0.0))
+        return fail(SB_ERR_ARG, "sb_test_shift_d: block out of range or negative amount");
+    h->test_dshift_block = block < 0 ? -1 : block;
+    h->test_dshift = block < 0 ? 0.0 : amount;
+    return SB_OK;
 }
 
 int sb_mcmc_init(sb_handle *h, const sb_priors *pr, const double *why)

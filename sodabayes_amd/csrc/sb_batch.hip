@@ -537,48 +537,44 @@ __global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, 
     }
 }
 
-// One thread per (block position jk, chain): c_j = delta_j / s_j quantised to
-// L balanced int8 limbs, written as B-operand blocks like Q:
+// One thread per (block position jk, chain): c_j (delta_j / s_j plus the
+// SNP's carried remainder, from k_draw_b) quantised to L balanced int8
+// limbs, q_j = rint(c_j 2^e), written as B-operand blocks like Q:
 // Dq[((ss * NTILE + t) * 4 + s) * 64 + l16 + 16 g] for limb row 16 t + l16 and
 // SNP word sw = 16 ss + 4 g + s (zero beyond the block).  The 16 positions
 // of a word sit in 16 consecutive lanes; dword m of the word's operand holds
 // the digits of positions m, m+4, m+8, m+12 (slot order of unpack_word), so
 // lane m of the group gathers them by shuffles.  The exponent comes from the
-// per-workgroup maxima of |c_j| that k_draw_b left; position 0 stores it and
-// K_c = sum_j 2 f_j c_j.
-__global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const double *delta, const double *sdv,
-                                               int p0, int nb, const uint64_t *dmax, const double *dk,
-                                               int nwg_d, uint4 *Dq, int *expo, double *kconst)
+// per-workgroup maxima of |c_j| that k_draw_b left; position 0 stores it.
+// What the update applies is c~_j = q_j 2^-e: the remainder c_j - c~_j goes
+// back to carry_j for the SNP's next update, and each workgroup leaves its
+// part of the centring constant K = sum_j 2 f_j c~_j (kpart), formed from
+// the same c~_j, so that r moves by exactly sum_j (x_ij - 2 f_j) c~_j.
+__global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const double *cjv, const double *f2,
+                                               int p0, int nb, const uint64_t *dmax, int nwg_d, uint4 *Dq,
+                                               int *expo, double *carry, double *kpart)
 {
+    __shared__ double lds[NT / WAVE];
     const int c = blockIdx.y;
     const int jk = blockIdx.x * NT + threadIdx.x;   // the grid covers whole words
     unsigned long long m = 0;
-    double kc = 0.0;
     for (int w0 = 0; w0 < nwg_d; w0 += 8) {
         unsigned long long v[8];
-        double kv[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            v[u] = w0 + u < nwg_d ? dmax[(int64_t)c * nwg_d + w0 + u] : 0ull;
-            kv[u] = (jk == 0 && w0 + u < nwg_d) ? dk[(int64_t)c * nwg_d + w0 + u] : 0.0;
-        }
+        for (int u = 0; u < 8; u++) v[u] = w0 + u < nwg_d ? dmax[(int64_t)c * nwg_d + w0 + u] : 0ull;
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            m = v[u] > m ? v[u] : m;
-            kc += kv[u];
-        }
+        for (int u = 0; u < 8; u++) m = v[u] > m ? v[u] : m;
     }
     const int e = fixed_exponent(m, L);
-    if (jk == 0) {
-        expo[c] = e;
-        kconst[c] = kc;
-    }
-    double cj = 0.0;
-    if (jk < nb) {
-        const double s = sdv[p0 + jk];
-        cj = s > 0.0 ? delta[(int64_t)c * P.p + p0 + jk] / s : 0.0;
-    }
+    if (jk == 0) expo[c] = e;
+    const double cj = jk < nb ? cjv[(int64_t)c * P.blocksize + jk] : 0.0;
     int32_t q = (int32_t)rint(ldexp(cj, e));
+    double kq = 0.0;
+    if (jk < nb) {
+        const double cq = ldexp((double)q, -e);
+        carry[(int64_t)c * P.p + p0 + jk] = cj - cq;
+        kq = f2[p0 + jk] * cq;
+    }
     const int t = jk & 15, sw = jk >> 4;
     const int ss = sw >> 4, g = (sw >> 2) & 3, s = sw & 3;
     const int src = (threadIdx.x & (WAVE - 1) & ~15) + (t & 3);
@@ -594,6 +590,20 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
                                          16 * g)[t] = v;
         }
     }
+    kq = block_sum<NT>(kq, lds);
+    if (threadIdx.x == 0) kpart[(int64_t)c * gridDim.x + blockIdx.x] = kq;
+}
+
+// Centring constants K_c of the update: every chain's nkp workgroup parts
+// from k_dquant, summed in workgroup order into LDS (read after the first
+// barrier of the caller).
+__device__ __forceinline__ void load_kconst(const double *__restrict__ kpart, int nkp, int C, double *kc_s)
+{
+    if (threadIdx.x < C) {
+        double v = 0.0;
+        for (int w = 0; w < nkp; w++) v += kpart[(int64_t)threadIdx.x * nkp + w];
+        kc_s[threadIdx.x] = v;
+    }
 }
 
 // r_c += X_s c - K_c for every chain, as the int8 MFMA product of the
@@ -604,11 +614,13 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
 template <int NTILE, int IT>
 __device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict__ gt, int nss,
                                           const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
-                                          const double *__restrict__ kconst, double *__restrict__ r,
+                                          const double *__restrict__ kpart, int nkp, double *__restrict__ r,
                                           double *__restrict__ rsum, uint64_t *__restrict__ rmax, int nwg, int bx)
 {
     constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
     __shared__ int red[ROWS][COLS + 1];
+    __shared__ double kc_s[64];
+    load_kconst(kpart, nkp, P.nchains, kc_s);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
     const int i0 = bx * ROWS;
@@ -661,7 +673,7 @@ __device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict_
             long long S = 0;
             for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
             double *rc = r + (int64_t)c * P.ldr;
-            nv = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
+            nv = rc[i] + (ldexp((double)S, -expo[c]) - kc_s[c]);
             rc[i] = nv;
         }
         double sm = nv;
@@ -692,9 +704,11 @@ constexpr bool upd_lds(int ntile) { return ntile <= 6; }            // ring of 3
 template <int NTILE, int IT>
 __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__ gt, int nss,
                                          const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
-                                         const double *__restrict__ kconst, double *__restrict__ r,
+                                         const double *__restrict__ kpart, int nkp, double *__restrict__ r,
                                          double *__restrict__ rsum, uint64_t *__restrict__ rmax, int nwg, int bx)
 {
+    __shared__ double kc_s[64];
+    load_kconst(kpart, nkp, P.nchains, kc_s);
     constexpr int W = NT / WAVE;
     constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
     constexpr int SLAB = NTILE * 4, NB = SLAB / W, NBUF = 3;
@@ -764,7 +778,7 @@ __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__
             long long S = 0;
             for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
             double *rc = r + (int64_t)c * P.ldr;
-            nv = rc[i] + (ldexp((double)S, -expo[c]) - kconst[c]);
+            nv = rc[i] + (ldexp((double)S, -expo[c]) - kc_s[c]);
             rc[i] = nv;
         }
         double sm = nv;
@@ -845,7 +859,7 @@ __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gs
 template <int NTILE, int IT, int NTC, int RW>
 __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss,
                                             const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
-                                            const double *__restrict__ kconst, double *__restrict__ r,
+                                            const double *__restrict__ kpart, int nkp, double *__restrict__ r,
                                             double *__restrict__ rsum, uint64_t *__restrict__ rmax, int n0,
                                             const float4 *__restrict__ Ut, int nbt, int nb,
                                             const float4 *__restrict__ Dsw, int spant, double *__restrict__ part,
@@ -855,9 +869,9 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
     split_ids(blockIdx.x, n0, n1, kind, local);
     if (kind == 0) {
         if constexpr (upd_lds(NTILE))
-            xupd_lds<NTILE, RW>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, (int)(P.ldr / WAVE), local);
+            xupd_lds<NTILE, RW>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, (int)(P.ldr / WAVE), local);
         else
-            xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kconst, r, rsum, rmax, n0, local);
+            xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, n0, local);
     } else {
         ut_tile<1, NTC>(P, Ut, nbt, nb, Dsw, spant, part, local % g1x, local / g1x);
     }
@@ -865,8 +879,11 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
 
 // One thread per (SNP, chain): exact limb combination of the chunk
 // partials, rhs, draw (Philox, counter (iteration, SNP id, tag)); delta
-// also into the swizzled panel for U delta; per workgroup, max |delta / s|
-// and sum 2 f delta / s for the update.
+// also into the swizzled panel for U delta.  The residual update applies
+// c_j = delta_j / s_j + carry_j, where carry_j is what the last
+// quantisation of SNP j's update left unapplied (error feedback, so the
+// rounding of the fixed-point update never accumulates in r); c_j goes to
+// cj_out for k_dquant, and per workgroup max |c_j| sets its exponent.
 constexpr int DCH = 16;  // chunk partials unrolled in k_draw_b
 constexpr int DRG = 16;  // r_a row-group partials unrolled in k_draw_b
 template <int NM>
@@ -877,9 +894,8 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
                                                const int32_t *snp_of_pos, int p0, int nb, double zz,
                                                int rep, double *beta, int32_t *cls, double *vara_s,
                                                double *delta, uint8_t *chg, double *rhs_out, uint64_t *dmax,
-                                               double *dk, float *dl_sw)
+                                               const double *carry, double *cj_out, float *dl_sw)
 {
-    __shared__ double lds[NT / WAVE];
     __shared__ unsigned long long mx[NT / WAVE];
     __shared__ MixConst mc;
     constexpr int L = LIMBS;
@@ -887,7 +903,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
     const int c = blockIdx.y;
     const int C = P.nchains;
     const bool mixm = P.method != SB_BAYES_A && P.method != SB_BAYES_B;
-    double cj = 0.0, kc = 0.0;
+    double cj = 0.0;
     {
         // Every load of the thread is issued before the first use: the SNP
         // state, the chunk partials (up to DCH chunks unrolled) and the r_a
@@ -904,6 +920,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
         const int cold = P.method == SB_BAYES_A ? 2 : cls[o + pos];
         const double vs = ab ? vara_s[o + pos] : 0.0;
         const int gp = grp[pos], sid = snp_of_pos[pos];
+        const double cy = carry[o + pos];
         const long long *tcol = Tp + (int64_t)c * P.blocksize + jkc;
         long long v[DCH];
 #pragma unroll
@@ -939,13 +956,12 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
             delta[o + pos] = d.delta;
             chg[o + pos] = d.chg;
             dl_sw[rsw_index(jk, c, (C + 15) / 16)] = (float)d.delta;
-            cj = s > 0.0 ? d.delta / s : 0.0;
-            kc = f2p * cj;
+            cj = (s > 0.0 ? d.delta / s : 0.0) + cy;
+            cj_out[(int64_t)c * P.blocksize + jk] = cj;
         }
     }
 reduce:
     if (rhs_out) return;
-    kc = block_sum<NT>(kc, lds);
     unsigned long long m = (unsigned long long)__double_as_longlong(fabs(cj));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -957,7 +973,6 @@ reduce:
     if (threadIdx.x == 0) {
         for (int w = 1; w < NT / WAVE; w++) m = mx[w] > m ? mx[w] : m;
         dmax[(int64_t)c * gridDim.x + blockIdx.x] = m;
-        dk[(int64_t)c * gridDim.x + blockIdx.x] = kc;
     }
 }
 
@@ -1019,7 +1034,7 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r)
             const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * RW : 16 * IT));
             const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
             launch(kc, k_upd<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gt, nsw / 16, Dq, L, h->b_expo2,
-                   h->b_kc, r, h->b_rsum, h->b_rmax, n0, ut, nbt, nb, reinterpret_cast<const float4 *>(h->dl_sw),
+                   h->b_kpart, (nb + NT - 1) / NT, r, h->b_rsum, h->b_rmax, n0, ut, nbt, nb, reinterpret_cast<const float4 *>(h->dl_sw),
                    spant, h->b_uapart, g1x, n1);
         });
     });
@@ -1057,7 +1072,7 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     return hip_check(hipGetLastError(), "batch products");
 }
 
-int batch_update(sb_handle *h, int b, const double *delta)
+int batch_update(sb_handle *h, int b)
 {
     const Par &P = h->par;
     hipStream_t st = h->stream;
@@ -1070,8 +1085,8 @@ int batch_update(sb_handle *h, int b, const double *delta)
     const double tbytes = 4.0 * 256.0 * nbt * (nbt + 1) / 2.0;
     kc.begin(st);
     kc.last(KS_DQUANT, 16.0 * C * nb + cl * nb);
-    launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, L, h->b_ntile, delta, h->sdv, p0, nb,
-           h->b_dmax, h->b_dk, nwg_d, Dq, h->b_expo2, h->b_kc);
+    launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, L, h->b_ntile, h->b_cj, h->f2, p0, nb,
+           h->b_dmax, nwg_d, Dq, h->b_expo2, h->b_carry, h->b_kpart);
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
     kc.begin(st);
     kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
@@ -1172,6 +1187,23 @@ int batch_autotune(sb_handle *h)
     return hip_check(hipGetLastError(), "tile-shape tuning");
 }
 
+// Every chain's carried quantisation remainders to zero: when r is exact
+// (the initial state, a resync r = y - X beta - mu) nothing is left unapplied.
+// A fill kernel rather than a memset, so that the call can sit in a captured
+// graph.
+__global__ void k_zero(double *v, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * NT + threadIdx.x; k < n; k += (int64_t)gridDim.x * NT) v[k] = 0.0;
+}
+
+int batch_clear_carry(sb_handle *h)
+{
+    if (!h->b_carry) return SB_OK;
+    const int64_t n = (int64_t)h->par.nchains * h->par.p;
+    k_zero<<<(int)std::min<int64_t>((n + NT - 1) / NT, 4096), NT, 0, h->stream>>>(h->b_carry, n);
+    return hip_check(hipGetLastError(), "clear carry");
+}
+
 // Genotype operand layouts of the batched path, built once at sb_create.
 int batch_transpose(sb_handle *h)
 {
@@ -1221,7 +1253,7 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
         launch(h->kc, k_draw_b<decltype(nm)::value>, dim3((nb + NT - 1) / NT, P.nchains), dim3(NT), 0, h->stream,
                P, h->scal, reinterpret_cast<const long long *>(h->b_T), tstride, nch, h->b_expo, h->b_rtot,
                h->b_rapart, h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls, h->vara_s, delta, chg,
-               rhs_out, h->b_dmax, h->b_dk, h->dl_sw);
+               rhs_out, h->b_dmax, h->b_carry, h->b_cj, h->dl_sw);
     });
     return hip_check(hipGetLastError(), "batch draw");
 }

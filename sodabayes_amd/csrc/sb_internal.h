@@ -27,6 +27,7 @@ struct Par {
     int rng_mode, nchains, precision;
     int regular;       // -SODAOff: the single-site sampler (nblocks = blocksize = 0)
     int numit, burnin, thin;
+    int resync;        // iterations between residual resyncs (1000 in the reference)
     double pizero, scale_va0, scale_ve, df_va, df_ve, vara0;
     double mix[MAXMIX], alpha[MAXMIX];
     uint64_t seed;
@@ -194,6 +195,8 @@ struct sb_handle {
     int64_t iterations = 0;
     sb::KClock kc;                // per-kernel timing (SODABAYES_KERNEL_TIMING=1)
     double pm_iters_max = 0.0;        // largest power-method iteration count of the setup
+    int test_dshift_block = -1;       // sb_test_shift_d: block whose d is lowered (-1: none)
+    double test_dshift = 0.0;         // and by how much
 
     // genotype data
     uint32_t *geno = nullptr;     // nloci x ldw, position order, 2-bit counts
@@ -268,9 +271,10 @@ struct sb_handle {
     int8_t *b_Dq = nullptr;       // (16 * b_ntile) x b_ldd int8 limbs of delta / s
     int64_t b_ldd = 0;
     int *b_expo2 = nullptr;       // fixed-point exponent of delta / s per chain
-    double *b_kc = nullptr;       // centring constant per chain
-    uint64_t *b_dmax = nullptr;   // per chain, per draw workgroup: max |delta / s| (bit patterns)
-    double *b_dk = nullptr;       // per chain, per draw workgroup: sum 2 f delta / s
+    uint64_t *b_dmax = nullptr;   // per chain, per draw workgroup: max |c_j| (bit patterns)
+    double *b_kpart = nullptr;    // per chain, per quantisation workgroup: sum 2 f c~_j (centring constant)
+    double *b_cj = nullptr;       // blocksize per chain: c_j = delta_j / s_j + carry_j of the current block
+    double *b_carry = nullptr;    // nloci per chain (position order): remainder of the last quantised update
     uint32_t *geno_t = nullptr;   // individual-major genotypes per block, MFMA operand tiles (update GEMM)
     std::vector<int64_t> gt_off;  // 16-byte element offset of each block in geno_t
     uint32_t *geno_sw = nullptr;  // SNP-major genotypes per block, MFMA operand tiles (product GEMM)
@@ -290,6 +294,32 @@ int fail(int code, const std::string &msg);
 int hip_check(hipError_t e, const char *what);
 int dev_alloc(sb_handle *h, void **p, size_t bytes);
 
+// Device buffers that live for one call: every allocation is checked
+// (SB_ERR_NOMEM with the size and the buffer's name in sb_last_error) and
+// everything is freed when the object goes out of scope, on every return
+// path.  The caller synchronises its stream before returning.
+struct TmpDev {
+    std::vector<void *> ptrs;
+    template <class T>
+    int alloc(T **p, size_t bytes, const char *what)
+    {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes ? bytes : 8);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return fail(SB_ERR_NOMEM, std::string("hipMalloc(") + what + ", " + std::to_string(bytes) +
+                                          " B): " + hipGetErrorString(e));
+        }
+        ptrs.push_back(q);
+        *p = static_cast<T *>(q);
+        return SB_OK;
+    }
+    ~TmpDev()
+    {
+        for (void *q : ptrs) hipFree(q);
+    }
+};
+
 // setup (sb_setup.hip)
 int setup_pack(sb_handle *h, const uint8_t *bed, bool on_device);
 int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t *fail_col);
@@ -299,12 +329,14 @@ int setup_gram_block(sb_handle *h, int block, double *G_host);
 int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy = 0,
               const uint8_t *heldout = nullptr, int64_t ldh = 0);
 int mcmc_run(sb_handle *h, int first, int count, double *hyp);
+int counter_uniforms(int64_t seed, int chain, const uint32_t *ctr, int n, int per, double *out);
 
 // chain-batched mixed-precision block products and draws (sb_batch.hip)
 int batch_block(sb_handle *h, int b, double zz, int rep);
 int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *chg,
                double *rhs_out = nullptr);
-int batch_update(sb_handle *h, int b, const double *delta);
+int batch_update(sb_handle *h, int b);
+int batch_clear_carry(sb_handle *h);
 int batch_transpose(sb_handle *h);
 int batch_words(int nb);
 int batch_kss(int ntile);

@@ -503,26 +503,30 @@ int setup_pack(sb_handle *h, const uint8_t *bed, bool on_device)
     const Par &P = h->par;
     int64_t nbytes = ((int64_t)P.n + 3) / 4;
     const uint8_t *dbed = bed;
-    uint8_t *tmp = nullptr;
+    TmpDev tmp;   // staging copy of a host payload, freed on every return path
+    int rc = SB_OK;
     if (!on_device) {
-        if (hip_check(hipMalloc(&tmp, (size_t)(nbytes * P.p)), "hipMalloc(bed)")) return SB_ERR_NOMEM;
-        if (hip_check(hipMemcpyAsync(tmp, bed, (size_t)(nbytes * P.p), hipMemcpyHostToDevice, h->stream),
-                      "upload bed"))
+        uint8_t *t = nullptr;
+        if ((rc = tmp.alloc(&t, (size_t)(nbytes * P.p), "bed payload"))) return rc;
+        if (hip_check(hipMemcpyAsync(t, bed, (size_t)(nbytes * P.p), hipMemcpyHostToDevice, h->stream),
+                      "upload bed")) {
+            hipStreamSynchronize(h->stream);
             return SB_ERR_HIP;
-        dbed = tmp;
+        }
+        dbed = t;
     }
     int32_t *dperm = nullptr;
-    int rc = dev_alloc(h, (void **)&dperm, sizeof(int32_t) * P.p);
-    if (rc) return rc;
+    rc = dev_alloc(h, (void **)&dperm, sizeof(int32_t) * P.p);
+    if (rc) {
+        hipStreamSynchronize(h->stream);
+        return rc;
+    }
     hipMemcpyAsync(dperm, h->permvec.data(), sizeof(int32_t) * P.p, hipMemcpyHostToDevice, h->stream);
     dim3 grid(P.p, std::min(64, (P.ldw + 255) / 256));
     k_pack<<<grid, 256, 0, h->stream>>>(dbed, nbytes, dperm, P.p, P.ldw, h->geno);
-    if (hip_check(hipGetLastError(), "k_pack")) return SB_ERR_HIP;
-    if (tmp) {
-        hipStreamSynchronize(h->stream);
-        hipFree(tmp);
-    }
-    return hip_check(hipStreamSynchronize(h->stream), "pack");
+    rc = hip_check(hipGetLastError(), "k_pack");
+    const int rs = hip_check(hipStreamSynchronize(h->stream), "pack");
+    return rc ? rc : rs;
 }
 
 namespace {
@@ -548,9 +552,16 @@ int gram_alloc(sb_handle *h, GramBuf &gb, int only_block)
     int64_t total = only_block >= 0 ? (int64_t)(h->ls[only_block + 1] - h->ls[only_block]) *
                                           (h->ls[only_block + 1] - h->ls[only_block])
                                     : gb.hoff[nb];
-    if (hip_check(hipMalloc(&gb.G, sizeof(double) * (size_t)total), "hipMalloc(gram)"))
+    if (hip_check(hipMalloc(&gb.G, sizeof(double) * (size_t)total), "hipMalloc(gram)")) {
+        gb.G = nullptr;
         return SB_ERR_NOMEM;
-    if (hip_check(hipMalloc(&gb.goff, sizeof(int64_t) * (nb + 1)), "hipMalloc(goff)")) return SB_ERR_NOMEM;
+    }
+    if (hip_check(hipMalloc(&gb.goff, sizeof(int64_t) * (nb + 1)), "hipMalloc(goff)")) {
+        hipFree(gb.G);
+        gb.G = nullptr;
+        gb.goff = nullptr;
+        return SB_ERR_NOMEM;
+    }
     hipMemcpyAsync(gb.goff, gb.hoff.data(), sizeof(int64_t) * (nb + 1), hipMemcpyHostToDevice, h->stream);
     return SB_OK;
 }
@@ -838,17 +849,6 @@ static int psd_fallback(sb_handle *h, GramBuf &gb, const std::vector<int> &faile
     return rc;
 }
 
-// Test hook (no counterpart in the reference): SODABAYES_TEST_D_SHIFT =
-// "<block>:<amount>" lowers that block's d before the factorisation, so that
-// dI - G is indefinite there and the fallback above runs.
-static bool test_dshift(int nblocks, int &block, double &amount)
-{
-    const char *e = std::getenv("SODABAYES_TEST_D_SHIFT");
-    if (!e || !*e) return false;
-    if (std::sscanf(e, "%d:%lf", &block, &amount) != 2 || block < 0 || block >= nblocks) return false;
-    return true;
-}
-
 int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t *fail_col)
 {
     auto t0 = std::chrono::steady_clock::now();
@@ -895,9 +895,10 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     h->nskipped = 0;
     h->dense_off.assign(nb, -1);
     hipMemsetAsync(h->skip, 0, (size_t)P.p, h->stream);
-    int shift_block = -1;
-    double shift_amt = 0.0;
-    const bool dshift = test_dshift(nb, shift_block, shift_amt);
+    // sb_test_shift_d (tests only): lower one block's d before the factorisation
+    const int shift_block = h->test_dshift_block;
+    const double shift_amt = h->test_dshift;
+    const bool dshift = shift_block >= 0 && shift_block < nb;
     auto apply_dshift = [&]() {
         double v = 0.0;
         hipMemcpyAsync(&v, h->xxd + shift_block, sizeof v, hipMemcpyDeviceToHost, h->stream);
@@ -906,14 +907,19 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
         hipMemcpyAsync(h->xxd + shift_block, &v, sizeof v, hipMemcpyHostToDevice, h->stream);
         hipStreamSynchronize(h->stream);
     };
-    // power method
+    // power method; every buffer of this call is freed on every return path
+    TmpDev tmp;
     double *y = nullptr, *v = nullptr;
-    int32_t *active = nullptr, *iters = nullptr, *dfail = nullptr;
-    hipMalloc(&y, sizeof(double) * P.p);
-    hipMalloc(&v, sizeof(double) * P.p);
-    hipMalloc(&active, sizeof(int32_t) * nb);
-    hipMalloc(&iters, sizeof(int32_t) * nb);
-    hipMalloc(&dfail, sizeof(int32_t) * nb);
+    int32_t *active = nullptr, *iters = nullptr, *dfail = nullptr, *blist = nullptr;
+    if ((rc = tmp.alloc(&y, sizeof(double) * P.p, "power-method vector")) ||
+        (rc = tmp.alloc(&v, sizeof(double) * P.p, "power-method vector")) ||
+        (rc = tmp.alloc(&active, sizeof(int32_t) * nb, "block flags")) ||
+        (rc = tmp.alloc(&iters, sizeof(int32_t) * nb, "block flags")) ||
+        (rc = tmp.alloc(&dfail, sizeof(int32_t) * nb, "block flags")) ||
+        (rc = tmp.alloc(&blist, sizeof(int32_t) * nb, "block list"))) {
+        gram_free(gb);
+        return rc;
+    }
     std::vector<int32_t> ones(nb, 1);
     hipMemcpyAsync(active, ones.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, h->stream);
     hipMemsetAsync(iters, 0, sizeof(int32_t) * nb, h->stream);
@@ -935,10 +941,14 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     const int64_t maxit = lanczos ? 0 : 1000LL * P.blocksize;
     if (lanczos) {
         const int K = std::min(P.blocksize, LANCZOS_MAX), K1 = K + 1;
+        TmpDev lz;   // Lanczos basis: p x (K + 1) doubles (0.9 GB at 600,000 SNPs), freed after the bound
         double *Qv = nullptr, *al = nullptr, *be = nullptr;
-        hipMalloc(&Qv, sizeof(double) * (size_t)P.p * (K + 1));
-        hipMalloc(&al, sizeof(double) * nb * K1);
-        hipMalloc(&be, sizeof(double) * nb * K1);
+        if ((rc = lz.alloc(&Qv, sizeof(double) * (size_t)P.p * (K + 1), "Lanczos basis")) ||
+            (rc = lz.alloc(&al, sizeof(double) * nb * K1, "Lanczos coefficients")) ||
+            (rc = lz.alloc(&be, sizeof(double) * nb * K1, "Lanczos coefficients"))) {
+            gram_free(gb);
+            return rc;
+        }
         hipMemcpyAsync(Qv, v, sizeof(double) * P.p, hipMemcpyDeviceToDevice, h->stream);
         hipMemsetAsync(active, 0xFF, sizeof(int32_t) * nb, h->stream);
         for (int k = 0; k < K; k++) {
@@ -966,17 +976,17 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
             const double resid = exact ? 0.0 : std::fabs(bb[m - 1] * tridiag_last_component(ab, bb, m, theta));
             hx[b] = std::fabs(theta) + resid + D_EPS_ADD;
         }
-        hipMemcpyAsync(h->xxd, hx.data(), sizeof(double) * nb, hipMemcpyHostToDevice, h->stream);
+        if (!rc) rc = hip_check(hipMemcpyAsync(h->xxd, hx.data(), sizeof(double) * nb, hipMemcpyHostToDevice,
+                                               h->stream), "upload bounds");
         hipStreamSynchronize(h->stream);
-        hipFree(Qv);
-        hipFree(al);
-        hipFree(be);
+        if (rc) {
+            gram_free(gb);
+            return rc;
+        }
         progress("lanczos bound done", K, 0);
     }
     // launches cover the still-active blocks only (blist), refreshed at
     // every host check
-    int32_t *blist = nullptr;
-    hipMalloc(&blist, sizeof(int32_t) * nb);
     auto power_loop = [&](int64_t limit) {
         hipMemcpyAsync(hact.data(), active, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, h->stream);
         hipStreamSynchronize(h->stream);
@@ -1077,13 +1087,8 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
         }
     }
     if (!rc && h->Ut0) rc = batch_tiles(h);
-    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "build_aug");
-    hipFree(blist);
-    hipFree(y);
-    hipFree(v);
-    hipFree(active);
-    hipFree(iters);
-    hipFree(dfail);
+    const int rs = hip_check(hipStreamSynchronize(h->stream), "build_aug");
+    if (!rc) rc = rs;
     gram_free(gb);
     if (rc) return rc;
     if (xxd_out) std::copy(hxxd.begin(), hxxd.end(), xxd_out);

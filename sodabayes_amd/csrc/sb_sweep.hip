@@ -946,6 +946,40 @@ __global__ void k_rhs_out(Par P, const double *part, const double *partA, const 
 // ------------------------------------------------------------- host side --
 
 // dense factor of block b (pivoted-Cholesky fallback), or nullptr
+// Diagnostics of the counter-based generator: for site k, `per` uniforms of
+// the stream the sampler's kernels open as PhiloxSrc(chain_key(seed, chain),
+// ctr[3k], ctr[3k+1], ctr[3k+2]) (iteration, site, tag), in draw order.
+__global__ void k_counter_uniforms(uint64_t seed, int chain, const uint32_t *ctr, int n, int per, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    Par P{};
+    P.seed = seed;
+    PhiloxSrc src(chain_key(P, chain), ctr[3 * k], ctr[3 * k + 1], ctr[3 * k + 2]);
+    for (int u = 0; u < per; u++) out[(int64_t)k * per + u] = src.next();
+}
+
+int counter_uniforms(int64_t seed, int chain, const uint32_t *ctr, int n, int per, double *out)
+{
+    TmpDev tmp;
+    uint32_t *dc = nullptr;
+    double *dout = nullptr;
+    int rc;
+    if ((rc = tmp.alloc(&dc, sizeof(uint32_t) * 3 * (size_t)n, "counters")) ||
+        (rc = tmp.alloc(&dout, sizeof(double) * (size_t)n * per, "uniforms")))
+        return rc;
+    rc = hip_check(hipMemcpy(dc, ctr, sizeof(uint32_t) * 3 * (size_t)n, hipMemcpyHostToDevice), "upload counters");
+    if (!rc) {
+        k_counter_uniforms<<<(n + NT - 1) / NT, NT>>>((uint64_t)seed, chain, dc, n, per, dout);
+        rc = hip_check(hipGetLastError(), "k_counter_uniforms");
+    }
+    if (!rc) rc = hip_check(hipDeviceSynchronize(), "k_counter_uniforms");
+    if (!rc)
+        rc = hip_check(hipMemcpy(out, dout, sizeof(double) * (size_t)n * per, hipMemcpyDeviceToHost),
+                       "download uniforms");
+    return rc;
+}
+
 static const double *dense_block(const sb_handle *h, int b)
 {
     return (b < (int)h->dense_off.size() && h->dense_off[b] >= 0) ? h->Ud + h->dense_off[b] : nullptr;
@@ -975,6 +1009,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
     P.numit = pr->numit;
     P.burnin = pr->burnin;
     P.thin = pr->thin > 0 ? pr->thin : 1;
+    P.resync = pr->resync > 0 ? pr->resync : 1000;   // models.f90:571
     if (P.rng_mode == SB_RNG_COUNTER) P.seed = (uint64_t)pr->seed;
     const int n = P.n, p = P.p, nmix = P.nmix, ngroup = P.ngroup;
     const int C = P.nchains;
@@ -1088,6 +1123,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
     hipMemcpyAsync(h->vara_s, vs.data(), sizeof(double) * p * Cs, hipMemcpyHostToDevice, h->stream);
     k_init_r<<<dim3((n + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->why, h->scal, h->r);
     if (int rc = init_xpx(h)) return rc;
+    if (int rc = batch_clear_carry(h)) return rc;   // r is exact: nothing carried
     h->rstat_nwg = 0;
     k_init_draws<<<1, 64, 0, h->stream>>>(P, h->scal);
     if (P.naind > 0) {   // no augmented pseudo-data with -SODAOff
@@ -1165,12 +1201,11 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     double *delta = h->sc.delta;
     std::vector<double> xxd(P.nblocks);
     if (P.nblocks > 0) hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
-    if (mixed && !h->tuned) {
-        int rc = batch_autotune(h);
-        if (rc) {
-            if (dhyp) hipFree(dhyp);
-            return rc;
-        }
+    int rc = SB_OK;
+    if (mixed && !h->tuned) rc = batch_autotune(h);
+    if (rc) {
+        if (dhyp) hipFree(dhyp);
+        return rc;
     }
     // optional per-kernel timing on this stream
     const char *kt = std::getenv("SODABAYES_KERNEL_TIMING");
@@ -1178,12 +1213,14 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     KClock &kc = h->kc;
     auto t0 = std::chrono::steady_clock::now();
     int rec = 0;
-    for (int rep = first; rep < first + count; rep++) {
+    // every failure below leaves the loops with rc set and goes through the
+    // one exit at the end (stream drained, clock reset, hyp buffer freed)
+    for (int rep = first; rep < first + count && !rc; rep++) {
         k_mu_pre<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
         k_mu_draw<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, rep);
         k_mu_post<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal);
         h->rstat_nwg = 0;   // r changed: the batched path recomputes its statistics
-        for (int b = 0; b < P.nblocks; b++) {
+        for (int b = 0; b < P.nblocks && !rc; b++) {
             const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
             const int64_t tri0 = h->tri_off[b];
             // X part: ~2048 waves, each reusing its residual registers over spw SNPs
@@ -1194,8 +1231,8 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             const double gbytes = (double)nb * (((double)P.n + 3.0) / 4.0);   // 2-bit genotypes of the block
             const double ubytes = 8.0 * (double)nb * (nb + 1) / 2.0;           // packed upper factor
             if (mixed) {
-                int rc = batch_block(h, b, xxd[b], rep);
-                if (rc) return rc;
+                rc = batch_block(h, b, xxd[b], rep);
+                if (rc) break;
             } else {
                 kc.begin(st);
                 kc.last(KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
@@ -1203,8 +1240,8 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                        dense_block(h, b), p0, nb, spw, ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
             }
             if (mixed) {
-                int rc = batch_draw(h, b, xxd[b], rep, delta, chg);
-                if (rc) return rc;
+                rc = batch_draw(h, b, xxd[b], rep, delta, chg);
+                if (rc) break;
             } else {
                 kc.begin(st);
                 kc.last(KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
@@ -1221,8 +1258,8 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                     });
             }
             if (mixed) {
-                int rc = batch_update(h, b, delta);
-                if (rc) return rc;
+                rc = batch_update(h, b);
+                if (rc) break;
             } else {
                 const int nrw = (P.n + 255) / 256;
                 const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
@@ -1239,13 +1276,11 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                        h->ra);
             }
         }
+        if (rc) break;
         if (P.regular) {
             // -SODAOff: the single-site sweep over every SNP
-            int rc = single_sweep(h, rep);
-            if (rc) {
-                if (dhyp) hipFree(dhyp);
-                return rc;
-            }
+            rc = single_sweep(h, rep);
+            if (rc) break;
         } else if (h->nskipped > 0) {
             // the FP64 path only (setup refuses the fallback for the mixed path)
             with_nmix(P.nmix, [&](auto nm) {
@@ -1273,18 +1308,20 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
         if (store)
             k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->vara_s, h->gstore,
                                                                    h->indiststore, h->vara_sstore);
-        if (rep % 1000 == 0) {
+        if (rep % P.resync == 0) {
             k_nonzero<<<dim3(1, 1, C), NT, 0, st>>>(P, h->d_pos_by_id, h->beta, h->resync_list, h->d_count);
             k_resync<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->geno, h->zlut, h->beta,
                                                                       h->resync_list, h->d_count, h->why,
                                                                       h->scal, h->r);
             h->rstat_nwg = 0;
+            rc = batch_clear_carry(h);   // r is exact again
         }
-        if (hip_check(hipGetLastError(), "sweep launch")) return SB_ERR_HIP;
+        if (!rc) rc = hip_check(hipGetLastError(), "sweep launch");
     }
-    int rc = hip_check(hipStreamSynchronize(st), "sweep");
+    const int rs = hip_check(hipStreamSynchronize(st), "sweep");
+    if (!rc) rc = rs;
     auto t1 = std::chrono::steady_clock::now();
-    if (kc.on) kc.collect();
+    if (kc.on) kc.collect();   // also clears the records of a failed run
     kc.on = false;
     if (count > 0) h->last_sweep_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / count;
     h->iterations += count;

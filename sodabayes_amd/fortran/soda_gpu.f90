@@ -29,7 +29,7 @@ module soda_gpu
     type, bind(C) :: sb_priors
         real(c_double) :: pizero, scale_va, scale_ve, df_va, df_ve, vara0
         real(c_double) :: mix(20), alpha(20)
-        integer(c_int32_t) :: numit, burnin, thin, pad0
+        integer(c_int32_t) :: numit, burnin, thin, resync
         integer(c_int64_t) :: seed
     end type sb_priors
 

@@ -413,7 +413,7 @@ contains
         pr%numit = numit
         pr%burnin = burnin
         pr%thin = thin
-        pr%pad0 = 0
+        pr%resync = 0
         pr%seed = seed1
         rc = sb_mcmc_init(h, pr, why)
         if (rc /= SB_OK) call fail('sb_mcmc_init', rc)

@@ -57,6 +57,7 @@ class SodaOptions:
     rng: str = "replay"           # "replay" (xoroshiro stream 0) or "counter" (Philox)
     precision: str = "fp64"       # "fp64" or "mixed" (int8-limb MFMA products, chains batched)
     chains: int = 1               # independent chains sharing the genotypes (mixed only)
+    resync: int = 0               # iterations between residual resyncs; 0: the reference's 1000
 
     def method_code(self) -> int:
         for k, v in _lib.METHODS.items():
@@ -179,6 +180,9 @@ class SodaSampler:
     freq: np.ndarray | None = None
     bed_device_ptr: int | None = None
     heldout: np.ndarray | None = None
+    # tests only: (block, amount) lowers that block's d before the
+    # factorisation (sb_test_shift_d), to reach the pivoted-Cholesky fallback
+    test_dshift: tuple | None = None
     handle: C.c_void_p = field(default=None, init=False)
 
     def __post_init__(self):
@@ -237,6 +241,8 @@ class SodaSampler:
                                perm.ctypes.data_as(C.POINTER(C.c_int32)), grp, C.byref(h)))
         self.handle = h
         _lib.check(L.sb_rng_set(h, self.state.ctypes.data_as(C.POINTER(C.c_uint64))))
+        if self.test_dshift is not None:
+            _lib.check(L.sb_test_shift_d(h, int(self.test_dshift[0]), float(self.test_dshift[1])))
         xxd = np.zeros(self.nblocks)
         fb, fc = C.c_int32(0), C.c_int32(0)
         rc = L.sb_build_aug(h, xxd.ctypes.data_as(C.POINTER(C.c_double)), C.byref(fb), C.byref(fc))
@@ -264,6 +270,7 @@ class SodaSampler:
         for k, a in enumerate(o.alpha_values()):
             pr.alpha[k] = a
         pr.numit, pr.burnin, pr.thin, pr.seed = o.numit, o.burnin, o.thin, int(o.seed)
+        pr.resync = int(o.resync)
         self._why = np.ascontiguousarray(self.why, dtype=np.float64)
         chains = int(o.chains)
         if self._why.ndim == 1 and self.heldout is None:

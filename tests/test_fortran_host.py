@@ -26,6 +26,18 @@ EXE = os.path.join(ROOT, "sodabayes_amd", "fortran", "build", "sodabayes_gpu")
 FIXTURE = os.path.join(ROOT, "tests", "golden", "fortran_jobs.npz")
 needs_exe = pytest.mark.skipif(not os.path.exists(EXE),
                                reason="the Fortran host is built only where the reference's Fortran sources are")
+
+
+@pytest.fixture
+def gpu_exe():
+    """The GPU job tests are the drop-in boundary's end-to-end evidence: a
+    missing program fails them instead of skipping them.  The program is
+    built where the reference's Fortran sources are (build() of
+    __graft_entry__) and travels with the tree."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built: run __graft_entry__.build() where the reference's Fortran "
+                    "sources are, and ship the built program with the tree")
+    return EXE
 CLOCK_LINES = ("Run started at", "Run ended at", "Setup:")
 SODAOFF_CASES = ("bayesr_sodaoff", "bayesa_sodaoff", "bayesb_sodaoff", "bayesrc_sodaoff", "bayescpi_sodaoff",
                  "bayesr5_sodaoff", "bayesr9_sodaoff", "bayesr_sodaoff_n1100", "bayescpi_sodaoff_n4500")
@@ -122,7 +134,7 @@ def test_help_lists_the_reference_options():
 
 
 @pytest.mark.gpu
-@needs_exe
+@pytest.mark.usefixtures("gpu_exe")
 @pytest.mark.parametrize("case,dense_x", [("bayesr_cmdline", False), ("bayesa_rblocks", False),
                                           ("bayesrc_groups", False), ("bayescpi_nblocks", False),
                                           ("par_two_jobs", False), ("bayesr_cmdline", True),
@@ -144,7 +156,7 @@ def test_gpu_job_files_match_the_reference(tmp_path, case, dense_x):
 
 
 @pytest.mark.gpu
-@needs_exe
+@pytest.mark.usefixtures("gpu_exe")
 @pytest.mark.parametrize("case", SODAOFF_CASES)
 def test_gpu_single_site_job_files_match_the_reference(tmp_path, case):
     """-SODAOff: the single-site sampler (models.f90:607-962) on the GPU, one
@@ -176,7 +188,7 @@ def test_single_site_job_on_the_host_matches_the_reference(tmp_path):
 
 
 @pytest.mark.gpu
-@needs_exe
+@pytest.mark.usefixtures("gpu_exe")
 def test_gpu_default_partition_of_one_snp_runs_the_single_site_sampler(tmp_path):
     """Neither -blocksize nor -nblocks with -nthreads 1 partitions into
     blocks of one SNP: setup_soda switches SODA off and the reference then
@@ -194,7 +206,7 @@ def test_gpu_default_partition_of_one_snp_runs_the_single_site_sampler(tmp_path)
 
 
 @pytest.mark.gpu
-@needs_exe
+@pytest.mark.usefixtures("gpu_exe")
 def test_gpu_single_site_counter_chains(tmp_path):
     """-SODAOff with SODABAYES_RNG=counter and 3 chains: one set of files per
     chain, independent chains, each near the reference's posterior mean of
@@ -215,7 +227,7 @@ def test_gpu_single_site_counter_chains(tmp_path):
 
 
 @pytest.mark.gpu
-@needs_exe
+@pytest.mark.usefixtures("gpu_exe")
 def test_gpu_predict_job_matches_the_reference(tmp_path):
     """-predict on the reference's model files: genomic values through
     sb_predict in place of compute_dgv."""
@@ -230,7 +242,7 @@ def test_gpu_predict_job_matches_the_reference(tmp_path):
 
 
 @pytest.mark.gpu
-@needs_exe
+@pytest.mark.usefixtures("gpu_exe")
 def test_gpu_batched_chains_write_one_set_of_files_each(tmp_path):
     """SODABAYES_CHAINS=4 with the counter-based mixed path: the usual files
     for chain 0, <out>_chain<k>.eff/.mod/.gv for the others, each a complete

@@ -1,7 +1,7 @@
 """Pivoted-Cholesky fallback and skipped-SNP updates on the GPU (FP64 path).
 
 The fixture case psd_bayesa lowers block 1's d so that dI - G is indefinite
-there; the library's test hook SODABAYES_TEST_D_SHIFT applies the same shift.
+there; the library's test hook sb_test_shift_d applies the same shift.
 The reference fixture pins BayesA (setup and chain: test_gpu_parity); here
 the other methods are checked against the oracle in REPLAY mode (classes
 equal, continuous state within 1e-9 relative), COUNTER mode through the
@@ -35,15 +35,15 @@ def case_opts(method, **extra):
 
 
 @pytest.fixture
-def dshift(monkeypatch):
-    c = load_case("psd_bayesa")
-    o = c["opts"]
-    monkeypatch.setenv("SODABAYES_TEST_D_SHIFT", f"{o['dshift_block']}:{o['dshift']!r}")
+def dshift():
+    """The fixture's block and shift of d (sb_test_shift_d)."""
+    o = load_case("psd_bayesa")["opts"]
+    return int(o["dshift_block"]), float(o["dshift"])
 
 
-def make(c, o, **kw):
+def make(c, o, dshift, **kw):
     return soda.SodaSampler(sampler_options(o, **kw), int(c["nind"]), int(c["nloci"]), c["why"],
-                            bed=c["bed"])
+                            bed=c["bed"], test_dshift=dshift)
 
 
 @pytest.mark.parametrize("method", ["BayesB", "BayesC", "BayesR"])
@@ -51,7 +51,7 @@ def test_replay_fallback_matches_oracle(dshift, method):
     c, o = case_opts(method)
     n, p = int(c["nind"]), int(c["nloci"])
     ref = orc.run(oracle_options(o, n, p), c["bed"], c["why"], None)
-    s = make(c, o)
+    s = make(c, o, dshift)
     s.setup()
     assert s.nskipped == int(ref["nskipped"][0]) and s.nskipped > 0
     assert s.naind == int(ref["naind"][0])
@@ -71,7 +71,7 @@ def test_replay_fallback_matches_oracle(dshift, method):
 def test_counter_fallback_keeps_residuals(dshift, method):
     c, o = case_opts(method, numit=30)
     n, p = int(c["nind"]), int(c["nloci"])
-    s = make(c, o, rng="counter")
+    s = make(c, o, dshift, rng="counter")
     s.setup()
     assert s.nskipped > 0
     # every skipped SNP sits in the shifted block
@@ -100,7 +100,7 @@ def test_counter_fallback_keeps_residuals(dshift, method):
 
 def test_mixed_path_refuses_fallback(dshift):
     c, o = case_opts("BayesR")
-    s = make(c, o, rng="counter", precision="mixed", chains=2)
+    s = make(c, o, dshift, rng="counter", precision="mixed", chains=2)
     with pytest.raises(_lib.SbError) as ei:
         s.setup()
     assert ei.value.code == _lib.SB_ERR_NOT_PD

@@ -6,8 +6,6 @@ chains: every discrete quantity (classes, counts, mixture proportions) equal
 to the reference after the full fixture run, continuous state within 1e-9
 relative to its largest entry.
 """
-import contextlib
-import os
 
 import numpy as np
 import pytest
@@ -31,28 +29,14 @@ def make_sampler(c, **extra):
     return s
 
 
-@contextlib.contextmanager
-def dshift_env(c):
-    """Cases whose fixture lowered one block's d (to run the reference's
-    pivoted-Cholesky fallback) set the library's test hook the same way."""
-    o = c["opts"]
-    if "dshift_block" not in o:
-        yield
-        return
-    old = os.environ.get("SODABAYES_TEST_D_SHIFT")
-    os.environ["SODABAYES_TEST_D_SHIFT"] = f"{o['dshift_block']}:{o['dshift']!r}"
-    try:
-        yield
-    finally:
-        if old is None:
-            del os.environ["SODABAYES_TEST_D_SHIFT"]
-        else:
-            os.environ["SODABAYES_TEST_D_SHIFT"] = old
-
-
 def setup(s, c):
-    with dshift_env(c):
-        return s.setup()
+    """Cases whose fixture lowered one block's d (to run the reference's
+    pivoted-Cholesky fallback) lower the same block's d through the
+    library's test hook (sb_test_shift_d)."""
+    o = c["opts"]
+    if "dshift_block" in o:
+        s.test_dshift = (int(o["dshift_block"]), float(o["dshift"]))
+    return s.setup()
 
 
 @pytest.mark.parametrize("case", ["bayesr", "ragged", "nblocks", "rblocks", "ldblock"])
