@@ -84,100 +84,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def synth_on_device(torch, n, p, seed, fmin=0.01, fmax=0.5, nqtl=5000, h2=0.4, chunk=2048):
-    """SNP-major PLINK payload (p x ceil(n/4) bytes) generated in HBM."""
-    import numpy as np
+def synth_on_device(torch, n, p, seed, fmin=0.01, fmax=0.5, nqtl=5000, h2=0.4):
+    """SNP-major PLINK payload (p x ceil(n/4) bytes) generated in HBM, with a
+    phenotype from nqtl t(4) QTL at heritability h2 (synth.device_dataset)."""
+    from sodabayes_amd import synth
 
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    f = torch.rand(p, generator=g, device=dev, dtype=torch.float64) * (fmax - fmin) + fmin
-    nb = (n + 3) // 4
-    npad = nb * 4
-    bed = torch.empty((p, nb), dtype=torch.uint8, device=dev)
-    counts = torch.empty(p, dtype=torch.float64, device=dev)
-    rs = np.random.default_rng(seed)
-    qtl = np.sort(rs.choice(p, size=min(nqtl, p), replace=False))
-    eff = rs.standard_t(4, size=len(qtl))
-    qtl_t = torch.from_numpy(qtl).to(dev)
-    eff_t = torch.from_numpy(eff).to(dev)
-    gval = torch.zeros(n, dtype=torch.float64, device=dev)
-    for j0 in range(0, p, chunk):
-        m = min(chunk, p - j0)
-        fc = f[j0:j0 + m].float()
-        u = torch.rand((2, npad, m), generator=g, device=dev)
-        x = (u[0] < fc).to(torch.uint8) + (u[1] < fc).to(torch.uint8)
-        del u
-        if npad > n:
-            x[n:] = 0
-        counts[j0:j0 + m] = x.sum(0, dtype=torch.int64).to(torch.float64)
-        code = x + (x > 0).to(torch.uint8)          # 0 -> 00, 1 -> 10, 2 -> 11
-        c4 = code.view(nb, 4, m)
-        byte = c4[:, 0] | (c4[:, 1] << 2) | (c4[:, 2] << 4) | (c4[:, 3] << 6)
-        bed[j0:j0 + m] = byte.t()
-        sel = (qtl_t >= j0) & (qtl_t < j0 + m)
-        if bool(sel.any()):
-            cols = qtl_t[sel] - j0
-            xq = x[:n, cols].to(torch.float64)
-            gval += (xq - 2.0 * f[j0 + cols]) @ eff_t[sel]
-        del x, code, c4, byte
-    freq = (counts / (2.0 * n)).cpu().numpy()
-    gv = gval.cpu().numpy()
-    gv *= np.sqrt(h2 / gv.var())
-    y = 100.0 + gv + rs.standard_normal(n) * np.sqrt(1.0 - h2)
+    bed, freq, y, _ = synth.device_dataset(torch, n, p, seed, fmin=fmin, fmax=fmax, nqtl=nqtl, h2=h2)
     return bed, freq, y
 
 
-def traits_on_device(torch, n, p, seed, ntraits=16, nqtl=3000, shared=0.5, chunk=2048):
-    """SNP-major PLINK payload (p x ceil(n/4) bytes) generated in HBM, with
-    ``ntraits`` phenotypes: each trait has ``nqtl`` QTL with N(0, 1)
-    effects, a fraction ``shared`` of them common to all traits (genetic
-    correlation about ``shared``), and h2 drawn from U(0.1, 0.6)."""
-    import numpy as np
+def traits_on_device(torch, n, p, seed, ntraits=16, nqtl=3000, shared=0.5):
+    """PLINK payload in HBM with ntraits correlated phenotypes (synth.device_traits)."""
+    from sodabayes_amd import synth
 
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-    f = torch.rand(p, generator=g, device=dev, dtype=torch.float64) * 0.49 + 0.01
-    nb = (n + 3) // 4
-    npad = nb * 4
-    bed = torch.empty((p, nb), dtype=torch.uint8, device=dev)
-    counts = torch.empty(p, dtype=torch.float64, device=dev)
-    rs = np.random.default_rng(seed)
-    ns = int(nqtl * shared)
-    common = rs.choice(p, size=ns, replace=False)
-    eff = np.zeros((p, ntraits))
-    eff[common] = rs.standard_normal((ns, 1)) * np.ones((1, ntraits))   # shared QTL, same effect
-    for t in range(ntraits):
-        own = rs.choice(np.setdiff1d(np.arange(p), common), size=nqtl - ns, replace=False)
-        eff[own, t] = rs.standard_normal(nqtl - ns)
-    qtl = np.flatnonzero(np.abs(eff).sum(1) > 0)
-    qtl_t = torch.from_numpy(qtl).to(dev)
-    eff_t = torch.from_numpy(eff[qtl]).to(dev)
-    gval = torch.zeros((n, ntraits), dtype=torch.float64, device=dev)
-    for j0 in range(0, p, chunk):
-        m = min(chunk, p - j0)
-        fc = f[j0:j0 + m].float()
-        u = torch.rand((2, npad, m), generator=g, device=dev)
-        x = (u[0] < fc).to(torch.uint8) + (u[1] < fc).to(torch.uint8)
-        del u
-        if npad > n:
-            x[n:] = 0
-        counts[j0:j0 + m] = x.sum(0, dtype=torch.int64).to(torch.float64)
-        code = x + (x > 0).to(torch.uint8)
-        c4 = code.view(nb, 4, m)
-        bed[j0:j0 + m] = (c4[:, 0] | (c4[:, 1] << 2) | (c4[:, 2] << 4) | (c4[:, 3] << 6)).t()
-        sel = (qtl_t >= j0) & (qtl_t < j0 + m)
-        if bool(sel.any()):
-            cols = qtl_t[sel] - j0
-            gval += (x[:n, cols].to(torch.float64) - 2.0 * f[j0 + cols]) @ eff_t[sel]
-        del x, code, c4
-    freq = (counts / (2.0 * n)).cpu().numpy()
-    gv = gval.cpu().numpy()
-    h2 = rs.uniform(0.1, 0.6, size=ntraits)
-    gv *= np.sqrt(h2 / gv.var(0))
-    y = 100.0 + gv + rs.standard_normal((n, ntraits)) * np.sqrt(1.0 - h2)
-    return bed, freq, np.ascontiguousarray(y.T)
+    return synth.device_traits(torch, n, p, seed, ntraits=ntraits, nqtl=nqtl, shared=shared)
 
 
 def cpu_baseline(seed: int) -> dict | None:

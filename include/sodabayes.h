@@ -176,7 +176,8 @@ enum {
     SB_Q_SKIPPED,        /* SNPs skipped by the pivoted-Cholesky fallback, 1-based ids in update
                             order (i_skipped, soda.f90:505-507) (nskipped) */
     SB_Q_NAIND,          /* naind, nskipped (2) (soda.f90:440, 505, 514) */
-    SB_Q_WHY             /* the chain's phenotype (nind), held-out rows at their current draw */
+    SB_Q_WHY,            /* the chain's phenotype (nind), held-out rows at their current draw */
+    SB_Q_XA_BLOCK        /* augmenting matrix of block `chain` (n_b x n_b column-major, block positions) */
 };
 
 /* Kernel timing slots of SB_Q_KTIMES (diagnostics; no counterpart in the

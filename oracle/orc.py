@@ -81,6 +81,7 @@ def lib():
         L.orc_permutate.argtypes = [ip, C.c_int, C.c_int, rp]
         L.orc_run.argtypes = [C.POINTER(OrcOpts), u8p, dp, C.POINTER(C.c_int32), C.c_char_p]
         L.orc_run.restype = C.c_int
+        L.orc_gram_block.argtypes = [u8p, C.c_int, C.c_int, dp, dp]
         u32p = C.POINTER(C.c_uint32)
         L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.orc_chain_key.argtypes = [C.c_uint64, C.c_int]
@@ -215,6 +216,17 @@ def gram(bed: np.ndarray, nind: int, cols, f: np.ndarray) -> np.ndarray:
     for a in range(m):
         for b in range(a + 1):
             G[a, b] = G[b, a] = xxproduct(gb, cols[a], cols[b], nind, f)
+    return G
+
+
+def gram_block(bed_cols: np.ndarray, nind: int, f_cols: np.ndarray) -> np.ndarray:
+    """Standardized Gram of the SNPs whose .bed rows are bed_cols, every
+    entry by xxproduct (the same restatement as gram(), in one C call)."""
+    bed_cols = np.ascontiguousarray(bed_cols, dtype=np.uint8)
+    f_cols = np.ascontiguousarray(f_cols, dtype=np.float64)
+    m = bed_cols.shape[0]
+    G = np.zeros((m, m))
+    lib().orc_gram_block(_ptr(bed_cols, C.c_uint8), nind, m, _ptr(f_cols, C.c_double), _ptr(G, C.c_double))
     return G
 
 

@@ -291,7 +291,7 @@ void orc_predict(const uint8_t *bed, int nind, int nloci, const double *freq, co
 }
 
 /* Integer part of xxproduct, soda.f90:541-553: 3 nd + 2 ng + nc = sum x_a x_b */
-int64_t orc_gram_count(const int64_t *a, const int64_t *b, int64_t nbatch)
+__attribute__((target("popcnt"))) int64_t orc_gram_count(const int64_t *a, const int64_t *b, int64_t nbatch)
 {
     const uint64_t M5 = 0x5555555555555555ULL;
     int64_t nc = 0, nd = 0, ng = 0;
@@ -320,6 +320,24 @@ double orc_xxproduct(const int64_t *a, const int64_t *b, int64_t nbatch, int nin
     double res = (double)orc_gram_count(a, b, nbatch);
     res = (res - (double)nind * pj * pk) / sqrt(pj * pk * qj * qk);
     return res;
+}
+
+/* Standardised Gram of m SNP columns (get_aug_matrix's loop over pairs,
+ * soda.f90:472-483, each entry xxproduct soda.f90:520-556): bed holds the m
+ * columns' .bed rows (m x ceil(nind/4)), f their frequencies; G is m x m. */
+void orc_gram_block(const uint8_t *bed, int nind, int m, const double *f, double *G)
+{
+    const int64_t nbatch = orc_nbatch(nind);
+    int64_t *gb = malloc(sizeof(int64_t) * (size_t)nbatch * (size_t)m);
+    orc_pack_genbits(bed, nind, m, gb);
+    for (int a = 0; a < m; a++)
+        for (int b = 0; b <= a; b++) {
+            const double v = orc_xxproduct(gb + (int64_t)a * nbatch, gb + (int64_t)b * nbatch, nbatch, nind, f[a],
+                                           f[b]);
+            G[(int64_t)a * m + b] = v;
+            G[(int64_t)b * m + a] = v;
+        }
+    free(gb);
 }
 
 /* power_method, soda.f90:378-401.  A is n x n column-major.  The start

@@ -52,6 +52,7 @@ void     orc_standardize(const uint8_t *bed, int nind, int nloci, const double *
 double   orc_xxproduct(const int64_t *a, const int64_t *b, int64_t nbatch, int nind,
                        double fj, double fk);
 int64_t  orc_gram_count(const int64_t *a, const int64_t *b, int64_t nbatch);
+void     orc_gram_block(const uint8_t *bed, int nind, int m, const double *f, double *G);
 void     orc_predict(const uint8_t *bed, int nind, int nloci, const double *freq, const double *eff,
                      double mu, double *pred);
 int      orc_power_method(int n, const double *A, orc_rng *rng, double *lambda);

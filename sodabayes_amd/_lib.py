@@ -20,7 +20,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 
 Q = dict(G=1, CLASS=2, VARA_S=3, YADJ=4, YAADJ=5, P=6, SCALARS=7, GSTORE=8, INDISTSTORE=9,
          VARA_SSTORE=10, STORES=11, SNPSTORE=12, VARSTORE=13, PSTORE=14, XA=15, XXD=16, GRAM=17,
-         SNPINDIST=18, VARINDIST=19, KTIMES=20, SKIPPED=21, NAIND=22, WHY=23)
+         SNPINDIST=18, VARINDIST=19, KTIMES=20, SKIPPED=21, NAIND=22, WHY=23, XA_BLOCK=24)
 
 # kernel timing slots of the KTIMES query (order of SB_KT_* in the header)
 KT_SLOTS = ["rhs", "draw", "list", "update", "rquant", "xq", "ut0", "drawb", "dquant", "xupd", "ut1", "rared"]

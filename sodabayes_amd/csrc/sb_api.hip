@@ -502,8 +502,9 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
     if (!h || !out) return fail(SB_ERR_ARG, "sb_get: null argument");
     const Par &P = h->par;
     const int64_t p = P.p;
-    if (what != SB_Q_GRAM && (chain < 0 || chain >= P.nchains)) return fail(SB_ERR_ARG, "sb_get: bad chain");
-    const int64_t cp = what == SB_Q_GRAM ? 0 : (int64_t)chain * p;
+    const bool by_block = what == SB_Q_GRAM || what == SB_Q_XA_BLOCK;
+    if (!by_block && (chain < 0 || chain >= P.nchains)) return fail(SB_ERR_ARG, "sb_get: bad chain");
+    const int64_t cp = by_block ? 0 : (int64_t)chain * p;
     hipStream_t st = h->stream;
     auto need = [&](int64_t n) { return count >= n ? SB_OK : fail(SB_ERR_ARG, "sb_get: output too small"); };
     auto by_id = [&](const void *dsrc, bool is_int, int64_t ncol) -> int {
@@ -608,6 +609,24 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
                 for (int i = 0; i <= j; i++)
                     out[(int64_t)(p0 + j) * P.blocksize + i] = U[h->tri_off[b] + (int64_t)j * (j + 1) / 2 + i];
         }
+        return SB_OK;
+    }
+    case SB_Q_XA_BLOCK: {
+        // one block's augmenting matrix, n_b x n_b column-major (block positions)
+        if (chain < 0 || chain >= P.nblocks) return fail(SB_ERR_ARG, "block out of range");
+        const int b = chain, p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+        if (need((int64_t)nb * nb)) return SB_ERR_ARG;
+        std::memset(out, 0, sizeof(double) * (size_t)nb * nb);
+        if (b < (int)h->dense_off.size() && h->dense_off[b] >= 0)
+            return hip_check(hipMemcpy(out, h->Ud + h->dense_off[b], sizeof(double) * (size_t)nb * nb,
+                                       hipMemcpyDeviceToHost),
+                             "get dense XA block");
+        std::vector<double> U(h->tri_off[b + 1] - h->tri_off[b]);
+        if (hip_check(hipMemcpy(U.data(), h->U + h->tri_off[b], sizeof(double) * U.size(), hipMemcpyDeviceToHost),
+                      "get U block"))
+            return SB_ERR_HIP;
+        for (int j = 0; j < nb; j++)
+            for (int i = 0; i <= j; i++) out[(int64_t)j * nb + i] = U[(int64_t)j * (j + 1) / 2 + i];
         return SB_OK;
     }
     case SB_Q_GRAM: {

@@ -595,13 +595,42 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
 }
 
 // Centring constants K_c of the update: every chain's nkp workgroup parts
-// from k_dquant, summed in workgroup order into LDS (read after the first
-// barrier of the caller).
-__device__ __forceinline__ void load_kconst(const double *__restrict__ kpart, int nkp, int C, double *kc_s)
+// from k_dquant.  The update kernels load them into registers before their
+// main loop (KPre: up to 4 per thread, the first 4 * NT parts), so that no
+// wave waits for them, and sum them in workgroup order into LDS after it
+// (kconst_finish; the caller's next barrier publishes kc_s).
+struct KPre {
+    double v[4];
+};
+__device__ __forceinline__ KPre kconst_prefetch(const double *__restrict__ kpart, int nkp, int C)
 {
+    KPre k;
+    const int tot = nkp * C;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int i = min((int)threadIdx.x + u * NT, tot - 1);
+        k.v[u] = kpart[i];
+    }
+    return k;
+}
+// kp_s: scratch of at least min(nkp * C, 4 * NT) doubles (free LDS after the
+// main loop); kc_s: C doubles
+__device__ __forceinline__ void kconst_finish(const KPre &k, const double *__restrict__ kpart, int nkp, int C,
+                                              double *kp_s, double *kc_s)
+{
+    const int tot = nkp * C, held = min(tot, 4 * NT);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int i = (int)threadIdx.x + u * NT;
+        if (i < held) kp_s[i] = k.v[u];
+    }
+    __syncthreads();
     if (threadIdx.x < C) {
         double v = 0.0;
-        for (int w = 0; w < nkp; w++) v += kpart[(int64_t)threadIdx.x * nkp + w];
+        for (int w = 0; w < nkp; w++) {
+            const int i = (int)threadIdx.x * nkp + w;
+            v += i < held ? kp_s[i] : kpart[i];
+        }
         kc_s[threadIdx.x] = v;
     }
 }
@@ -620,7 +649,8 @@ __device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict_
     constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
     __shared__ int red[ROWS][COLS + 1];
     __shared__ double kc_s[64];
-    load_kconst(kpart, nkp, P.nchains, kc_s);
+    __shared__ double kp_s[4 * NT];
+    const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
     const int i0 = bx * ROWS;
@@ -651,7 +681,7 @@ __device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict_
                     acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[s][t]), acc[it][t], 0, 0, 0);
             }
     }
-    __syncthreads();
+    kconst_finish(kpre, kpart, nkp, P.nchains, kp_s, kc_s);   // includes a barrier
     // acc[it][t][e]: individual 16 it + 4 g + e, column 16 t + l16
 #pragma unroll
     for (int it = 0; it < IT; it++)
@@ -708,7 +738,7 @@ __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__
                                          double *__restrict__ rsum, uint64_t *__restrict__ rmax, int nwg, int bx)
 {
     __shared__ double kc_s[64];
-    load_kconst(kpart, nkp, P.nchains, kc_s);
+    const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
     constexpr int W = NT / WAVE;
     constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
     constexpr int SLAB = NTILE * 4, NB = SLAB / W, NBUF = 3;
@@ -757,7 +787,9 @@ __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__
             }
         }
     }
-    __syncthreads();   // the ring is reused for the row sums
+    __syncthreads();   // the ring is reused for the row sums (and first for the K parts)
+    kconst_finish(kpre, kpart, nkp, P.nchains, reinterpret_cast<double *>(lds), kc_s);
+    __syncthreads();
     int(*red)[COLS + 1] = reinterpret_cast<int(*)[COLS + 1]>(lds);
     // acc[it][t][e]: individual 16 (IT wave + it) + 4 g + e, column 16 t + l16
 #pragma unroll
