@@ -2,14 +2,15 @@
 //
 // C chains share one pass over a block's genotypes.  Each chain's residual
 // r_c is written as a fixed-point integer q = rint(r_c * 2^e_c) with
-// |q| < 2^(8L-2), split into L balanced radix-256 digits (int8), and
+// |q| < 2^(8L-1), split into L balanced radix-256 digits (int8), and
 //   sum_i x_ij r_ic  ~=  2^-e_c * sum_k 256^k * T_jck,   T_jck = sum_i x_ij d_ick
 // where every T is an exact int32 sum from int8 MFMA
 // (v_mfma_i32_16x16x64_i8: one 32-bit word of sixteen 2-bit genotypes is
 // one lane's 16-byte operand).  The only error is the rounding of r to
-// 8L-2 bits of its largest entry.  The residual update r_c += Z_s delta_c
-// is the same int8 product the other way round (individual-major genotypes
-// times the limbs of delta / s).  The augmented parts U_s' R_a and U_s delta
+// 8L-1 (or, at the top of a binade, 8L-2) bits of its largest entry.  The
+// residual update r_c += Z_s delta_c is the same int8 product the other way
+// round (individual-major genotypes times the limbs of delta / s), with each
+// SNP's rounding remainder carried into its next update.  The augmented parts U_s' R_a and U_s delta
 // stay FP64 (v_mfma_f64_16x16x4f64 over all chains).
 #include <hip/hip_runtime.h>
 
@@ -27,7 +28,7 @@ constexpr int NT = 256;
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr uint32_t M2 = 0x03030303u;
-constexpr int LIMBS = 3;  // int8 limbs per fixed-point value (22 bits)
+constexpr int LIMBS = 3;  // int8 limbs per fixed-point value (23 bits)
 
 // Exact value of a chain's limb sums t_0 .. t_{LIMBS-1} (limb k weighs 256^k)
 __device__ __forceinline__ long long limb_sum(const int *t)
@@ -110,15 +111,23 @@ __global__ __launch_bounds__(NT) void k_rstat(Par P, const double *r, double *rs
     }
 }
 
-// Fixed-point exponent e with max|v| * 2^e < 2^(8L-2) from a maximum given
-// as the bit pattern of a non-negative double.
+// Fixed-point exponent e from a maximum max|v| given as the bit pattern of
+// a non-negative double: the largest e with rint(max|v| 2^e) <= 127 S,
+// S = (256^L - 1) / 255.  L balanced digits in [-128, 127] represent
+// exactly the integers of [-128 S, 127 S], so every q = rint(v 2^e) has
+// its L digits; e is 8L - 1 - E or, when max|v| sits at the top of its
+// binade, 8L - 2 - E (max|v| = f 2^E, 0.5 <= f < 1): 23 or 22 bits of the
+// maximum for three limbs.
 __device__ __forceinline__ int fixed_exponent(unsigned long long m, int L)
 {
     double mv = __longlong_as_double((long long)m);
     if (mv == 0.0) return 0;
     int E;
     frexp(mv, &E);              // mv = f * 2^E, 0.5 <= f < 1
-    return 8 * L - 2 - E;
+    const double top = 127.0 * ((ldexp(1.0, 8 * L) - 1.0) / 255.0) - 1.0;
+    int e = 8 * L - 1 - E;
+    if (ldexp(mv, e) > top) e -= 1;
+    return e;
 }
 
 // One thread per (16-individual word, chain): balanced radix-256 digits of

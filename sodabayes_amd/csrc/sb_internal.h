@@ -247,7 +247,7 @@ struct sb_handle {
     int32_t *d_count = nullptr;
 
     // mixed-precision batched products (precision SB_PREC_MIXED)
-    int limbs = 3;                // int8 limbs per residual (8L-2 fixed-point bits)
+    int limbs = 3;                // int8 limbs per residual (8L-1 fixed-point bits)
     int nwg_r = 0;                // workgroups of the residual statistics (k_rstat)
     int rstat_nwg = 0;            // valid partials of (max |r|, sum r) in b_rsum / b_rmax (0: stale)
     int q_block = -1;             // block whose r limbs the last update already wrote to b_Q (-1: none)

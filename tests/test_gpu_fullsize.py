@@ -14,7 +14,11 @@ path under test:
   popcount restatement of xxproduct (soda.f90:520-556) on sampled blocks;
 * residual identity: the incrementally updated r against y - Z beta - mu
   (on all individuals at 20,000 x 50,000; on 4,096 sampled individuals at
-  the larger sizes, where Z has up to 6e10 entries);
+  the larger sizes, where Z has up to 6e10 entries).  The mixed path's r
+  differs from it by the remainders its fixed-point update carries to each
+  SNP's next update (at most half a quantum per SNP), a bounded difference
+  that does not grow with the iterations: below 5e-6 relative even when all
+  500,000 SNPs move every iteration (BayesA);
 * held-out rows are never read: folds whose held-out phenotypes are
   replaced by 1e6-scale noise give bit-identical chains;
 * traits are independent: changing every other trait's phenotype leaves a
@@ -183,7 +187,7 @@ def test_c3_products_and_identity(torch, c3_data, method):
     kw = dict(df_va=4.0, df_ve=4.0, scale_va=0.4 * vary / share * 1.5, scale_ve=0.6 * vary * 1.5)
     if method == "BayesB":
         kw["pi"] = 0.99
-    o = soda.SodaOptions(method=method, blocksize=4096, numit=20, burnin=0, seed=31, rng="counter",
+    o = soda.SodaOptions(method=method, blocksize=4096, numit=200, burnin=0, seed=31, rng="counter",
                          precision="mixed", chains=C, **kw)
     s = soda.SodaSampler(o, n, p, y, freq=freq, bed_device_ptr=bed.data_ptr())
     s.setup()
@@ -191,9 +195,16 @@ def test_c3_products_and_identity(torch, c3_data, method):
     s.run(20, trace=False)
     check_products(torch, s, bed, freq, sample_blocks(s.nblocks), list(range(C)), "mixed", f"c3 {method} x{C}")
     rows = np.random.default_rng(3).choice(n, size=4096, replace=False)
-    err = residual_identity(torch, s, bed, freq, list(range(C)), rows=rows)
-    print(f"c3 {method} x{C}: residual identity (4,096 sampled individuals, every chain): {err:.3e}")
-    assert err < 1e-7
+    err20 = residual_identity(torch, s, bed, freq, list(range(C)), rows=rows)
+    s.run(180, trace=False)
+    err200 = residual_identity(torch, s, bed, freq, list(range(C)), rows=rows)
+    print(f"c3 {method} x{C}: residual identity (4,096 sampled individuals, every chain) after 20 iterations "
+          f"{err20:.3e}, after 200 {err200:.3e}")
+    # r differs from y - Z beta - mu by the remainders the update's fixed
+    # point carries (at most half a quantum per SNP, each quantum 2^-23 of
+    # its block's largest |delta / s|): a sum over all p SNPs when every SNP
+    # moves (BayesA), bounded and not growing with the iterations
+    assert err200 < 5e-6 and err200 < 3.0 * max(err20, 1e-9)
     s.close()
 
 
@@ -231,7 +242,7 @@ def test_c4_folds_never_read_held_out_rows_and_keep_identities(torch, c4_data):
     rows = np.random.default_rng(5).choice(40000, size=4096, replace=False)
     err = residual_identity(torch, s, bed, freq, chains, rows=rows)
     print(f"c4 folds x64: residual identity with the drawn held-out phenotypes: {err:.3e}")
-    assert err < 1e-7
+    assert err < 5e-6
     s.close()
     junk = np.tile(y, (C, 1))
     junk[held] = np.random.default_rng(6).standard_normal(int(held.sum())) * 1e6
@@ -275,7 +286,7 @@ def test_c5_traits_products_identity_and_independence(torch, c5_data):
     rows = np.random.default_rng(7).choice(n, size=4096, replace=False)
     err = residual_identity(torch, s, bed, freq, chains, rows=rows)
     print(f"c5 traits x16: residual identity (4,096 sampled individuals): {err:.3e}")
-    assert err < 1e-7
+    assert err < 5e-6
     s.close()
     # trait 3's chain does not depend on the other traits' phenotypes
     Y2 = Y[::-1].copy()
