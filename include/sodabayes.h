@@ -224,6 +224,22 @@ int sb_get_info(sb_handle *h, double *out, int32_t n);
 int sb_predict(int32_t nind, int32_t nloci, const uint8_t *bed, int32_t bed_on_device, const double *freq,
                const double *effects, double mu, double *pred);
 
+/* Prediction that allocates nothing (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The same genomic values
+ * as sb_predict, from device pointers only: bed (nloci x ceil(nind/4) .bed
+ * payload), freq and effects (nloci doubles each) in, pred (nind doubles)
+ * out, on `stream` (a hipStream_t; NULL: the default stream), asynchronous
+ * like any kernel launch.  The per-chunk partial sums go to a fixed device
+ * array of the library of SB_PREDICT_NOALLOC_MAX_PARTIALS doubles (16 MiB;
+ * each device keeps its own copy until the process ends), which bounds
+ * nind to that many individuals (SB_ERR_ARG beyond).  No hipMalloc or
+ * hipFree is called.  Every call on a device uses that device's one array:
+ * calls that overlap on one device race. */
+#define SB_PREDICT_NOALLOC_MAX_PARTIALS 2097152
+int sb_predict_noalloc(int32_t nind, int32_t nloci, const uint8_t *bed, const double *freq,
+                       const double *effects, double mu, double *pred, void *stream);
+
 /* ---- device-state checkpoints (no counterpart in the reference; for long
  * chains that must survive a job's end) ---- */
 

@@ -28,7 +28,7 @@ KT_SLOTS = ["rhs", "draw", "list", "update", "rquant", "xq", "ut0", "drawb", "dq
 # every function include/sodabayes.h declares
 EXPORTS = ["sb_rng_seed", "sb_permutate", "sb_create", "sb_destroy", "sb_rng_set", "sb_rng_get",
            "sb_build_aug", "sb_test_shift_d", "sb_mcmc_init", "sb_mcmc_init_chains", "sb_mcmc_run", "sb_hyp_width", "sb_get",
-           "sb_block_rhs", "sb_get_info", "sb_predict", "sb_state_size", "sb_state_save", "sb_state_load",
+           "sb_block_rhs", "sb_get_info", "sb_predict", "sb_predict_noalloc", "sb_state_size", "sb_state_save", "sb_state_load",
            "sb_counter_uniforms", "sb_last_error"]
 
 
@@ -82,6 +82,7 @@ def load():
     L.sb_get_info.argtypes = [vp, dp, C.c_int32]
     L.sb_block_rhs.argtypes = [vp, C.c_int32, dp]
     L.sb_predict.argtypes = [C.c_int32, C.c_int32, vp, C.c_int32, dp, dp, C.c_double, dp]
+    L.sb_predict_noalloc.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, C.c_double, vp, vp]
     L.sb_state_size.argtypes = [vp, C.POINTER(C.c_int64)]
     L.sb_state_save.argtypes = [vp, vp, C.c_int64]
     L.sb_state_load.argtypes = [vp, vp, C.c_int64]

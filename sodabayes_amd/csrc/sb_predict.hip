@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sb_device.h"
@@ -90,7 +91,109 @@ __global__ void k_pred_sum(const double *__restrict__ partial, int nch, int64_t 
     pred[i] = base + s;
 }
 
+// ---- the variant that allocates nothing (sb_predict_noalloc) ----
+//
+// Its partial sums live in `scratch`, a fixed device array of the library
+// (each device holds its own copy until the process ends); the inputs stay
+// in the caller's .bed layout, read byte-wise, so no packed copy is needed.
+// Calls on one device share the array: calls that overlap on one device race.
+constexpr int64_t NOALLOC_MAX = SB_PREDICT_NOALLOC_MAX_PARTIALS;
+__device__ __align__(256) double scratch[NOALLOC_MAX];
+
+// partial[ch][i] = sum over the SNPs j of chunk ch of (x_ij - 2 f_j) c_j,
+// c_j = g_j / sqrt(2 f_j (1 - f_j)) (0 for monomorphic SNPs), for the 16
+// individuals 16 w .. 16 w + 15 of thread w (four .bed bytes)
+__global__ __launch_bounds__(PT) void k_pred_bytes(const uint8_t *__restrict__ bed, int64_t nbytes, int n, int p,
+                                                   int clen, const double *__restrict__ freq,
+                                                   const double *__restrict__ g, double *__restrict__ partial)
+{
+    const int w = blockIdx.x * PT + threadIdx.x;
+    const int ch = blockIdx.y;
+    const int j0 = ch * clen, j1 = min(p, j0 + clen);
+    if (16 * w >= n || j0 >= j1) return;
+    double acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = 0.0;
+    const int64_t b0 = 4 * (int64_t)w;
+    for (int j = j0; j < j1; j++) {
+        const double f = freq[j];
+        const bool mono = f < DEPS || (1.0 - f) < DEPS;
+        const double c = mono ? 0.0 : g[j] / sqrt(2.0 * f * (1.0 - f));
+        const double f2 = 2.0 * f;
+        const uint8_t *row = bed + (int64_t)j * nbytes;
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) v |= (uint32_t)(b0 + b < nbytes ? row[b0 + b] : 0) << (8 * b);
+        v = v - ((v >> 1) & 0x55555555u);   // codes -> counts, missing -> 1 (utilities.f90:446-451)
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[e] = fma((double)((v >> (2 * e)) & 3u) - f2, c, acc[e]);
+    }
+    double *out = partial + (int64_t)ch * n + 16 * (int64_t)w;
+#pragma unroll
+    for (int e = 0; e < 16; e++)
+        if (16 * w + e < n) out[e] = acc[e];
+}
+
+__global__ void k_pred_sum_n(const double *__restrict__ partial, int nch, int n, double mu,
+                             double *__restrict__ pred)
+{
+    const int i = blockIdx.x * PT + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int ch = 0; ch < nch; ch++) s += partial[(int64_t)ch * n + i];
+    pred[i] = mu + s;
+}
+
 }  // namespace
+
+// Device address of `scratch` on each device, looked up once per device
+// (hipGetSymbolAddress with that device current); host code uses only
+// these addresses, never the symbol itself.
+namespace {
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
+static int scratch_address(int dev, double **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(SB_ERR_ARG, "sb_predict_noalloc: device index out of range");
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const int rc = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return SB_OK;
+}
+
+int predict_noalloc(int32_t nind, int32_t nloci, const uint8_t *bed, const double *freq, const double *effects,
+                    double mu, double *pred, void *stream)
+{
+    if (nind < 1 || nloci < 1 || !bed || !freq || !effects || !pred)
+        return fail(SB_ERR_ARG, "sb_predict_noalloc: bad sizes or null argument");
+    if ((int64_t)nind > NOALLOC_MAX)
+        return fail(SB_ERR_ARG, "sb_predict_noalloc: nind " + std::to_string(nind) + " exceeds the scratch limit of " +
+                                    std::to_string(NOALLOC_MAX) + " individuals");
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    double *part = nullptr;
+    if (!rc) rc = scratch_address(dev, &part);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t nbytes = ((int64_t)nind + 3) / 4;
+    const int nw = (nind + 15) / 16, wgx = (nw + PT - 1) / PT;
+    // SNP chunks: enough workgroups to fill the device, within the scratch
+    const int64_t room = NOALLOC_MAX / nind;
+    const int want = (int)std::min<int64_t>(room, std::max(1, 2048 / wgx));
+    const int clen = std::max(1, (nloci + want - 1) / want);
+    const int nch = (nloci + clen - 1) / clen;   // nch <= want <= room: nch * nind fits
+    k_pred_bytes<<<dim3(wgx, nch), PT, 0, st>>>(bed, nbytes, nind, nloci, clen, freq, effects, part);
+    k_pred_sum_n<<<(nind + PT - 1) / PT, PT, 0, st>>>(part, nch, nind, mu, pred);
+    return hip_check(hipGetLastError(), "sb_predict_noalloc kernels");
+}
 
 int predict(int32_t nind, int32_t nloci, const uint8_t *bed, int32_t bed_on_device, const double *freq,
             const double *effects, double mu, double *pred)
@@ -162,4 +265,10 @@ extern "C" int sb_predict(int32_t nind, int32_t nloci, const uint8_t *bed, int32
                           const double *effects, double mu, double *pred)
 {
     return sb::predict(nind, nloci, bed, bed_on_device, freq, effects, mu, pred);
+}
+
+extern "C" int sb_predict_noalloc(int32_t nind, int32_t nloci, const uint8_t *bed, const double *freq,
+                                  const double *effects, double mu, double *pred, void *stream)
+{
+    return sb::predict_noalloc(nind, nloci, bed, freq, effects, mu, pred, stream);
 }
