@@ -67,6 +67,11 @@ def parse_args():
     ap.add_argument("--blocksize", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=20240607)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-only", action="store_true", help="only the reference CPU baseline (prints its JSON)")
+    ap.add_argument("--cpu-nloci", type=int, default=2048, help="SNPs of the CPU baseline's sample")
+    ap.add_argument("--cpu-blocksize", type=int, default=1024, help="SODA subset size of the CPU baseline's sample")
+    ap.add_argument("--cpu-iters", type=int, default=30, help="iterations of the CPU baseline's sample")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 accuracy report")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
@@ -100,19 +105,33 @@ def traits_on_device(torch, n, p, seed, ntraits=16, nqtl=3000, shared=0.5):
     return synth.device_traits(torch, n, p, seed, ntraits=ntraits, nqtl=nqtl, shared=shared)
 
 
-def cpu_baseline(seed: int) -> dict | None:
-    """The reference program (oracle/_ref/sodabayes_ref, built from the
-    reference sources) on a bounded sample of the workload: same n, BayesR
-    mixture, p = 2,048 SNPs in subsets of 1,024, 30 iterations, all host
-    cores.  SNP-updates/s = p * numit / the MCMC seconds of its .log."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "sodabayes_ref")
+def cpu_cores() -> int:
+    """Host threads the baseline may use: the CPU share the environment
+    grants this job (OMP_NUM_THREADS, e.g. 16 per GPU), else every core."""
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    n = os.cpu_count() or 1
+    return min(share, n) if share > 0 else n
+
+
+def cpu_baseline(seed: int, nloci: int = 2048, blocksize: int = 1024, numit: int = 30) -> dict | None:
+    """The reference program (oracle/_ref/o3/sodabayes_ref: the reference
+    sources built -O3 -fopenmp by oracle/Makefile) on a bounded sample of
+    the workload: the same n = 50,000 individuals and BayesR mixture, nloci
+    SNPs in subsets of blocksize, numit iterations, -nthreads = every core
+    the job may use.  SNP-updates/s = nloci * numit / the MCMC seconds of its
+    .log; the setup seconds (serial power method and Cholesky) are reported
+    apart."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "o3", "sodabayes_ref")
     if not os.path.exists(exe):
-        log("cpu baseline skipped: oracle/_ref/sodabayes_ref not built")
+        log("cpu baseline skipped: oracle/_ref/o3/sodabayes_ref not built")
         return None
     from sodabayes_amd import plink, synth
 
-    n, p, numit, bs = 50000, 2048, 30, 1024
-    cores = min(16, os.cpu_count() or 1)
+    n, p, numit, bs = 50000, nloci, numit, blocksize
+    cores = cpu_cores()
     with tempfile.TemporaryDirectory() as td:
         bed, y, _ = synth.dataset(n, p, seed=seed % 100000, nqtl=40, h2=0.4, fmin=0.01, fmax=0.5)
         plink.write_plink(os.path.join(td, "d"), bed, y)
@@ -120,20 +139,44 @@ def cpu_baseline(seed: int) -> dict | None:
                "-numit", str(numit), "-burnin", "0", "-blocksize", str(bs), "-nthreads", str(cores),
                "-seed", "7", "-out", "r"]
         env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-        log(f"cpu baseline: reference program, n={n} p={p} blocksize={bs} {numit} its, {cores} threads")
-        r = subprocess.run(cmd, cwd=td, env=env, capture_output=True, text=True, timeout=300)
-        txt = open(os.path.join(td, "r.log")).read() if os.path.exists(os.path.join(td, "r.log")) else ""
+        log(f"cpu baseline: reference program (-O3), n={n} p={p} blocksize={bs} {numit} its, {cores} threads")
+        r = subprocess.run(cmd, cwd=td, env=env, capture_output=True, text=True, timeout=3000)
+        path = os.path.join(td, "r.log")
+        txt = open(path, errors="replace").read() if os.path.exists(path) else ""
         m = re.search(r"Setup:\s*([\d.]+)\s+MCMC:\s*([\d.]+)", txt)
         if r.returncode != 0 or not m:
             log(f"cpu baseline failed: {r.stderr[-300:]}")
             return None
-        mcmc = float(m.group(2))
+        setup, mcmc = float(m.group(1)), float(m.group(2))
     return {"value": p * numit / mcmc, "unit": "SNP-effect updates/s", "cores": cores, "kind": "reference",
-            "sample": f"reference sodabayes (amdflang -O2 -fopenmp) BayesR n={n} p={p} blocksize={bs} "
-                      f"{numit} iterations, -nthreads {cores}; MCMC {mcmc:.2f} s, setup {float(m.group(1)):.2f} s"}
+            "setup_seconds": setup, "mcmc_seconds": mcmc,
+            "sample": f"reference sodabayes (amdflang -O3 -fopenmp) BayesR n={n} p={p} blocksize={bs} "
+                      f"{numit} iterations, -nthreads {cores}; MCMC {mcmc:.2f} s, setup {setup:.2f} s (not counted)"}
 
 
-def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int):
+def accuracy(torch, s, bed, freq, chains: int) -> dict:
+    """After the timed sweeps, outside the timed region: the worst norm-wise
+    error of the path's block products X_s'r + U_s'r_a against an FP64
+    evaluation of the same state over 8 blocks and every chain, and the
+    residual identity r = y - Z beta - mu over 4,096 sampled individuals and
+    every chain (sodabayes_amd.check)."""
+    import numpy as np
+
+    from sodabayes_amd import check
+
+    blocks = check.sample_blocks(s.nblocks, 8)
+    rows = np.random.default_rng(12345).choice(s.nind, size=min(4096, s.nind), replace=False)
+    t0 = time.time()
+    out = {"products_rel_err": check.products_error(torch, s, bed, freq, blocks, list(range(chains))),
+           "residual_identity_rel_err": check.residual_identity(torch, s, bed, freq, list(range(chains)), rows=rows),
+           "blocks": blocks, "chains": chains, "sampled_individuals": int(len(rows)),
+           "iterations_before": int(s.rep)}
+    log(f"accuracy ({time.time() - t0:.1f} s): products {out['products_rel_err']:.3e}, residual identity "
+        f"{out['residual_identity_rel_err']:.3e} after {s.rep} iterations")
+    return out
+
+
+def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int, bed=None):
     """Setup + timed sweeps of one product path; returns its metrics."""
     n, p, bs = a.nind, a.nloci, a.blocksize
     troof = 0 if a.no_roofline else max(3, a.steps // 3)
@@ -194,6 +237,8 @@ def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int):
         for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["share"]):
             log(f"[{precision} x{chains}]   {v['name']:<34s} {v['avg_us']:8.2f} us/launch  {v['gbs']:7.0f} GB/s"
                 f"  {v['share']:6.1%} of kernel time")
+    if bed is not None and not a.no_accuracy:
+        out["accuracy"] = accuracy(torch, s, bed, freq, chains)
     s.close()
     return out
 
@@ -349,6 +394,10 @@ def run_single_site(soda, torch, a) -> dict:
 
 def main():
     a = parse_args()
+    if a.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu_baseline(a.seed, a.cpu_nloci, a.cpu_blocksize, a.cpu_iters)}),
+              flush=True)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {a.gpus}); "
               "the sharding design is in DESIGN.md", file=sys.stderr)
@@ -369,8 +418,8 @@ def main():
     bed, freq, why = synth_on_device(torch, n, p, a.seed)
     torch.cuda.synchronize()
     log(f"genotypes generated in HBM: n={n} p={p} ({bed.numel() / 1e9:.2f} GB) in {time.time() - t0:.1f} s")
-    fp64 = None if a.chains_only else run_path(soda, torch, a, why, freq, bed.data_ptr(), "fp64", 1)
-    mixed = run_path(soda, torch, a, why, freq, bed.data_ptr(), "mixed", a.chains)
+    fp64 = None if a.chains_only else run_path(soda, torch, a, why, freq, bed.data_ptr(), "fp64", 1, bed=bed)
+    mixed = run_path(soda, torch, a, why, freq, bed.data_ptr(), "mixed", a.chains, bed=bed)
     c3 = None if a.no_c3 else run_c3(soda, torch, a, why, freq, bed.data_ptr())
     del bed
     torch.cuda.empty_cache()
@@ -381,7 +430,7 @@ def main():
     single = None if a.no_single_site else run_single_site(soda, torch, a)
     torch.cuda.empty_cache()
 
-    cpu = None if a.no_cpu_baseline else cpu_baseline(a.seed)
+    cpu = None if a.no_cpu_baseline else cpu_baseline(a.seed, a.cpu_nloci, a.cpu_blocksize, a.cpu_iters)
     out = {
         "metric": "SNP-effect updates/s (BayesR n=50k,p=500k): mixed+batched vs FP64 GPU, same acc.",
         "value": mixed["value"], "unit": "SNP-effect updates/s", "n_gpus": 1, "steps": a.steps,
@@ -393,12 +442,14 @@ def main():
                    "parallelism": "single GPU"},
         "fp64_gpu": None if fp64 is None else {
             "value": fp64["value"], "ms_per_step": fp64["ms"], "chains": 1,
-            "roofline": fp64.get("roofline"), "setup_seconds": round(fp64["setup_s"], 2)},
+            "roofline": fp64.get("roofline"), "setup_seconds": round(fp64["setup_s"], 2),
+            "accuracy": fp64.get("accuracy")},
         "speedup_vs_fp64_gpu": None if fp64 is None else mixed["value"] / fp64["value"],
         "setup_seconds": round(mixed["setup_s"], 2),
         "power_method_iterations": mixed["power_iters"],
         "iterations_per_s": a.steps / mixed["dt"],
         "tile_shape": mixed.get("tile_shape"),
+        "accuracy": mixed.get("accuracy"),
         "folds": folds,
         "traits": traits,
         "bayesa_bayesb": c3,

@@ -42,7 +42,7 @@ import numpy as np
 import pytest
 
 import orc
-from sodabayes_amd import soda, synth
+from sodabayes_amd import check, soda, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -58,69 +58,17 @@ def torch():
     return t
 
 
-def block_cols(s, b):
-    a, e = soda.block_bounds(s.nloci, s.blocksize, s.nblocks, s.blockt)[b]
-    return s.permvec[a:e]
-
-
-def fp64_products(torch, s, bed, freq, b, chains):
-    """X_s'r + U_s'r_a of block b for the given chains, FP64 in torch."""
-    cols = block_cols(s, b)
-    nb = len(cols)
-    Z = synth.device_standardized(torch, bed, s.nind, freq, cols)            # nb x n
-    R = torch.tensor(np.stack([s.get("YADJ", s.nind, c) for c in chains]), device=Z.device)
-    RA = torch.tensor(np.stack([s.get("YAADJ", s.blocksize, c)[:nb] for c in chains]), device=Z.device)
-    XA = torch.tensor(s.get("XA_BLOCK", nb * nb, b).reshape(nb, nb, order="F"), device=Z.device)
-    out = Z @ R.T + XA.T @ RA.T                                              # nb x C
-    return out.T.cpu().numpy()
-
-
 def check_products(torch, s, bed, freq, blocks, chains, precision, label):
-    worst = 0.0
-    for b in blocks:
-        want = fp64_products(torch, s, bed, freq, b, chains)
-        got = s.block_rhs(b)
-        for k, c in enumerate(chains):
-            worst = max(worst, np.linalg.norm(got[c] - want[k]) / np.linalg.norm(want[k]))
+    worst = check.products_error(torch, s, bed, freq, blocks, chains)
     print(f"{label}: worst norm-wise error of X'r + U'r_a over {len(blocks)} blocks x {len(chains)} chains "
           f"= {worst:.3e}")
     assert worst <= PROD_BAR[precision], label
     return worst
 
 
-def residual_identity(torch, s, bed, freq, chains, rows=None, snp_chunk=8192):
-    """max over chains of ||r - (y - Z beta - mu)|| / ||y - Z beta - mu||,
-    over all individuals or the given rows."""
-    n, p = s.nind, s.nloci
-    dev = bed.device
-    idx = np.arange(n) if rows is None else np.sort(np.asarray(rows))
-    it = torch.as_tensor(idx, device=dev)
-    byte, sh = it // 4, (2 * (it % 4)).to(torch.int32)
-    f = torch.as_tensor(np.asarray(freq, dtype=np.float64), device=dev)
-    eps = np.finfo(np.float64).eps
-    B = torch.tensor(np.stack([s.get("G", p, c) for c in chains]), device=dev)          # C x p
-    zb = torch.zeros((len(chains), len(idx)), dtype=torch.float64, device=dev)
-    for j0 in range(0, p, snp_chunk):
-        j1 = min(p, j0 + snp_chunk)
-        code = (bed[j0:j1][:, byte].to(torch.int32) >> sh) & 3                          # m x rows
-        x = (code - ((code >> 1) & 1)).to(torch.float64)
-        fc = f[j0:j1]
-        ok = (fc >= eps) & ((1.0 - fc) >= eps)
-        sd = torch.where(ok, torch.sqrt(2.0 * fc * (1.0 - fc)), torch.ones_like(fc))
-        z = torch.where(ok[:, None], (x - (2.0 * fc)[:, None]) / sd[:, None], torch.zeros_like(x))
-        zb += B[:, j0:j1] @ z
-    worst = 0.0
-    for k, c in enumerate(chains):
-        y = s.get("WHY", n, c)[idx]
-        mu = s.get("SCALARS", 4, c)[0]
-        r = s.get("YADJ", n, c)[idx]
-        want = y - zb[k].cpu().numpy() - mu
-        worst = max(worst, np.linalg.norm(r - want) / np.linalg.norm(want))
-    return worst
-
-
-def sample_blocks(nblocks, k=8):
-    return sorted(set(np.linspace(0, nblocks - 1, k).round().astype(int).tolist()))
+block_cols = check.block_cols
+residual_identity = check.residual_identity
+sample_blocks = check.sample_blocks
 
 
 # -------------------------------------------------------------------- c2 --
