@@ -75,6 +75,7 @@ def parse_args():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
+    ap.add_argument("--fp64-only", action="store_true", help="only the FP64 single-chain pass (no JSON line)")
     ap.add_argument("--no-folds", action="store_true", help="skip the cross-validation folds pass")
     ap.add_argument("--no-traits", action="store_true", help="skip the batched traits pass")
     ap.add_argument("--no-single-site", action="store_true", help="skip the single-site (-SODAOff) pass")
@@ -419,6 +420,8 @@ def main():
     torch.cuda.synchronize()
     log(f"genotypes generated in HBM: n={n} p={p} ({bed.numel() / 1e9:.2f} GB) in {time.time() - t0:.1f} s")
     fp64 = None if a.chains_only else run_path(soda, torch, a, why, freq, bed.data_ptr(), "fp64", 1, bed=bed)
+    if a.fp64_only:
+        return
     mixed = run_path(soda, torch, a, why, freq, bed.data_ptr(), "mixed", a.chains, bed=bed)
     c3 = None if a.no_c3 else run_c3(soda, torch, a, why, freq, bed.data_ptr())
     del bed

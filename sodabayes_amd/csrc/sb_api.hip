@@ -303,6 +303,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->u_kc, sizeof(double) * (C + 4));
     A((void **)&h->u_cnt, sizeof(int) * (C + 4));
     A((void **)&h->u_rapart, sizeof(double) * bs * C * ((bs + 255) / 256 + 1));
+    A((void **)&h->u_tstart, sizeof(int) * C * (bs / 64 + 2));
     A((void **)&h->d_count, sizeof(int32_t) * (C + 4));
     if (d.precision == SB_PREC_MIXED) {
         h->nwg_r = (int)std::min<int64_t>(64, (d.nind + 255) / 256);

@@ -837,27 +837,6 @@ __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__
     }
 }
 
-// Workgroup id -> (kind, local id) for two horizontally fused grids of n0
-// and n1 workgroups: interleaved in runs of 8 (one run per XCD, which take
-// workgroups round-robin), then the rest of the larger grid.
-__device__ __forceinline__ void split_ids(int id, int n0, int n1, int &kind, int &local)
-{
-    const int m = (min(n0, n1) / 8) * 8;
-    if (id < 2 * m) {
-        kind = (id >> 3) & 1;
-        local = ((id >> 4) << 3) + (id & 7);
-        return;
-    }
-    id -= 2 * m;
-    if (id < n0 - m) {
-        kind = 0;
-        local = m + id;
-    } else {
-        kind = 1;
-        local = m + id - (n0 - m);
-    }
-}
-
 // Row tiles per wave of the LDS-shared products: 2 or 4 for up to 3 limb
 // tiles (the limb slab of a super-step is then shared by 128 or 256 rows),
 // 2 for 4 to 6 tiles (register budget); the non-LDS path ignores it.

@@ -182,6 +182,27 @@ __device__ __forceinline__ int xcd_contiguous(int L, int n)
     return L < m ? (L % 8) * (m / 8) + L / 8 : L;
 }
 
+// Workgroup id -> (kind, local id) for two horizontally fused grids of n0
+// and n1 workgroups: interleaved in runs of 8 (one run per XCD, which take
+// workgroups round-robin), then the rest of the larger grid.
+__device__ __forceinline__ void split_ids(int id, int n0, int n1, int &kind, int &local)
+{
+    const int m = (min(n0, n1) / 8) * 8;
+    if (id < 2 * m) {
+        kind = (id >> 3) & 1;
+        local = ((id >> 4) << 3) + (id & 7);
+        return;
+    }
+    id -= 2 * m;
+    if (id < n0 - m) {
+        kind = 0;
+        local = m + id;
+    } else {
+        kind = 1;
+        local = m + id - (n0 - m);
+    }
+}
+
 // Fixed-order block reduction (deterministic for a fixed blockDim).
 template <int NT>
 __device__ double block_sum(double v, double *lds)

@@ -243,6 +243,7 @@ struct sb_handle {
     double *u_kc = nullptr;       // centring constant sum 2 f c per chain
     int *u_cnt = nullptr;         // list length per chain
     double *u_rapart = nullptr;   // r_a partial sums per list chunk (FP64 update)
+    int *u_tstart = nullptr;      // per chain, per 64-row tile of r_a: first list entry reaching it
     int32_t *resync_list = nullptr; // positions with beta != 0 in SNP-id order
     int32_t *d_count = nullptr;
 

@@ -91,8 +91,9 @@ __global__ void k_mu_post(Par P, double *r, const Scal *scal)
 
 // Workgroups [0, nxg): X part.  WG -> (chunk c, SNP group); each wave holds
 // r for 2048 individuals in registers (32 per lane) and streams `spw` SNP
-// columns (one 8-byte load per lane per SNP, 512 B per wave), reducing each
-// product across the wave.  Workgroups [nxg, ...): A part, one wave per
+// columns (a multiple of 8; one 8-byte load per lane per SNP, 512 B per
+// wave, eight SNPs' loads in flight), reducing eight products at a time
+// across the wave.  Workgroups [nxg, ...): A part, one wave per
 // column of the block's packed triangle (Ud == nullptr) or of its dense
 // n_b x n_b factor (blocks that took the pivoted-Cholesky fallback).
 __global__ __launch_bounds__(NT) void k_rhs(Par P, const uint32_t *__restrict__ geno,
@@ -104,6 +105,8 @@ __global__ __launch_bounds__(NT) void k_rhs(Par P, const uint32_t *__restrict__ 
                                             double *__restrict__ partA, double *__restrict__ partR)
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // genotype workgroups first, then the factor's (interleaving the two
+    // kinds in runs of 8 measured slower: 54 against 42 us per block)
     const int t = blockIdx.x;
     if (t < nxg) {
         const int c = t / ngs, sg = t % ngs;
@@ -126,30 +129,35 @@ __global__ __launch_bounds__(NT) void k_rhs(Par P, const uint32_t *__restrict__ 
         const int j0 = (sg * 4 + wave) * spw;
         const int j1 = min(nb, j0 + spw);
         const uint32_t *gbase = geno + (int64_t)p0 * P.ldw + w0;
-        int j = j0;
-        for (; j + 1 < j1; j += 2) {
-            uint64_t ga = *reinterpret_cast<const uint64_t *>(gbase + (int64_t)j * P.ldw);
-            uint64_t gb = *reinterpret_cast<const uint64_t *>(gbase + (int64_t)(j + 1) * P.ldw);
-            double a0 = 0.0, a1 = 0.0;
+        // eight SNPs per step: their genotype loads in flight together
+        // (clamped index), eight dot products, then one transposed
+        // butterfly that leaves SNP (lane >> 3) & 7's sum in lanes 8k .. 8k+7
+        for (int jb = j0; jb < j1; jb += 8) {
+            uint64_t gv[8];
 #pragma unroll
-            for (int e = 0; e < 32; e++) {
-                a0 = fma((double)(uint32_t)((ga >> (2 * e)) & 3u), rr[e], a0);
-                a1 = fma((double)(uint32_t)((gb >> (2 * e)) & 3u), rr[e], a1);
-            }
-            a0 = wave_sum(a0);
-            a1 = wave_sum(a1);
-            if (lane == 0) {
-                part[(int64_t)c * P.blocksize + j] = a0;
-                part[(int64_t)c * P.blocksize + j + 1] = a1;
-            }
-        }
-        if (j < j1) {
-            uint64_t ga = *reinterpret_cast<const uint64_t *>(gbase + (int64_t)j * P.ldw);
-            double a0 = 0.0;
+            for (int u = 0; u < 8; u++)
+                gv[u] = *reinterpret_cast<const uint64_t *>(gbase + (int64_t)min(jb + u, j1 - 1) * P.ldw);
+            double a[8];
 #pragma unroll
-            for (int e = 0; e < 32; e++) a0 = fma((double)(uint32_t)((ga >> (2 * e)) & 3u), rr[e], a0);
-            a0 = wave_sum(a0);
-            if (lane == 0) part[(int64_t)c * P.blocksize + j] = a0;
+            for (int u = 0; u < 8; u++) {
+                a[u] = 0.0;
+#pragma unroll
+                for (int e = 0; e < 32; e++) a[u] = fma((double)(uint32_t)((gv[u] >> (2 * e)) & 3u), rr[e], a[u]);
+            }
+            const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+            double b[4], d2[2];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                b[k] = (h5 ? a[k + 4] : a[k]) + __shfl_xor(h5 ? a[k] : a[k + 4], 32, WAVE);
+#pragma unroll
+            for (int k = 0; k < 2; k++)
+                d2[k] = (h4 ? b[k + 2] : b[k]) + __shfl_xor(h4 ? b[k] : b[k + 2], 16, WAVE);
+            double s = (h3 ? d2[1] : d2[0]) + __shfl_xor(h3 ? d2[0] : d2[1], 8, WAVE);
+            s += __shfl_xor(s, 4, WAVE);
+            s += __shfl_xor(s, 2, WAVE);
+            s += __shfl_xor(s, 1, WAVE);
+            const int u = (lane >> 3) & 7;
+            if ((lane & 7) == 0 && jb + u < j1) part[(int64_t)c * P.blocksize + jb + u] = s;
         }
     } else {
         const int jk = (t - nxg) * 4 + wave;
@@ -183,9 +191,22 @@ __device__ __forceinline__ double block_rhs(const Par &P, const double *part, co
                                             int p0, int jk)
 {
     double sx = 0.0, sr = 0.0;
-    for (int c = 0; c < P.nchunk; c++) {
-        sx += part[(int64_t)c * P.blocksize + jk];
-        sr += partR[c];
+    // eight chunks' loads in flight at a time (clamped, masked), summed in
+    // chunk order
+    for (int c0 = 0; c0 < P.nchunk; c0 += 8) {
+        double v[8], w[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = min(c0 + u, P.nchunk - 1);
+            v[u] = part[(int64_t)c * P.blocksize + jk];
+            w[u] = partR[c];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (c0 + u < P.nchunk) {
+                sx += v[u];
+                sr += w[u];
+            }
     }
     int pos = p0 + jk;
     double s = sdv[pos];
@@ -311,15 +332,22 @@ __global__ __launch_bounds__(NT) void k_draw_replay(Par P, Scal *sc, const doubl
 
 // Ordered list of the block's changed positions of chain blockIdx.z
 // (models.f90:416-422: the SNPs with gflag /= 1, in block order), written
-// once per block for the update kernel, with delta_j, c_j = delta_j / s_j
-// and the centring constant sum_j 2 f_j c_j.
+// once per block for the update kernel, with delta_j, c_j = delta_j / s_j,
+// the centring constant sum_j 2 f_j c_j and, per 64-row tile of r_a, the
+// first list entry whose column reaches it (U is upper triangular: row tile
+// rt meets only the columns jk >= 64 rt, a suffix of the ordered list; a
+// dense factor of the pivoted-Cholesky fallback meets them all).  One
+// workgroup: round k covers positions 256 k .. 256 k + 255, one per thread,
+// so that every load is coalesced; the loads of up to LR rounds are issued
+// before the first is used, then each round is compacted in order by wave
+// ballots and wave offsets.
+constexpr int LR = 16;   // rounds of loads in flight
 __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const double *delta,
                                              const double *f2, const double *sdv, int p0, int nb, int *lst,
                                              double *dl, double *cl, int *count, double *kconst,
-                                             double *ubytes)
+                                             int *tstart, int dense, double *ubytes)
 {
-    __shared__ int cnt[NT + 1];
-    __shared__ int wsum[NT / WAVE];
+    __shared__ int wsum[2][NT / WAVE];
     __shared__ double lds[NT / WAVE];
     const int64_t c = blockIdx.z;
     chg += c * P.p;
@@ -327,80 +355,57 @@ __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const do
     lst += c * P.blocksize;
     dl += c * P.blocksize;
     cl += c * P.blocksize;
-    const int R = (nb + NT - 1) / NT;
-    int a = threadIdx.x * R, e = min(nb, a + R), k = 0;
-    // up to LR positions per thread: flags and values in one batch of loads
-    // (clamped index, masked), so that no load waits on another
-    constexpr int LR = 16;
-    uint8_t fl[LR];
-    double dv[LR], sv[LR], fv[LR];
-    if (R <= LR) {
+    int *ts = tstart + c * (P.blocksize / WAVE + 2);
+    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;   // lanes < this one
+    int base = 0;
+    double kc = 0.0, ub = 0.0;
+    for (int c0 = 0; c0 < nb; c0 += NT * LR) {
+        bool fl[LR];
+        double dv[LR], sv[LR], fv[LR];
 #pragma unroll
         for (int u = 0; u < LR; u++) {
-            const int pos = p0 + min(a + u, nb - 1);
-            fl[u] = chg[pos];
+            const int j = c0 + u * NT + (int)threadIdx.x;
+            const int pos = p0 + min(j, nb - 1);
+            fl[u] = chg[pos] != 0 && j < nb;
             dv[u] = delta[pos];
             sv[u] = sdv[pos];
             fv[u] = f2[pos];
         }
 #pragma unroll
         for (int u = 0; u < LR; u++) {
-            if (a + u >= e) fl[u] = 0;
-            k += fl[u] ? 1 : 0;
-        }
-    } else {
-        for (int j = a; j < e; j++) k += chg[p0 + j] ? 1 : 0;
-    }
-    // exclusive scan of the per-thread counts: wave scan, then wave offsets
-    const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
-    int incl = k;
+            const int j = c0 + u * NT + (int)threadIdx.x;
+            if (c0 + u * NT >= nb) break;   // workgroup-uniform
+            const uint64_t mask = __ballot(fl[u]);
+            if (lane == 0) wsum[u & 1][wv] = __popcll(mask);
+            __syncthreads();
+            int woff = 0, tot = 0;
 #pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        int t = __shfl_up(incl, off, WAVE);
-        if (lane >= off) incl += t;
-    }
-    if (lane == WAVE - 1) wsum[wv] = incl;
-    __syncthreads();
-    int base = 0;
-    for (int w = 0; w < wv; w++) base += wsum[w];
-    cnt[threadIdx.x] = base + incl - k;
-    if (threadIdx.x == NT - 1) count[c] = base + incl;
-    __syncthreads();
-    int o = cnt[threadIdx.x];
-    double kc = 0.0;
-    if (R <= LR) {
-#pragma unroll
-        for (int u = 0; u < LR; u++)
+            for (int w = 0; w < NT / WAVE; w++) {
+                const int v = wsum[u & 1][w];
+                woff += w < wv ? v : 0;
+                tot += v;
+            }
+            const int o = base + woff + __popcll(mask & below);   // entries before position j
+            if (j < nb && (j & (WAVE - 1)) == 0) ts[j / WAVE] = dense ? 0 : o;
             if (fl[u]) {
                 const double cj = sv[u] > 0.0 ? dv[u] / sv[u] : 0.0;
-                lst[o] = a + u;
+                lst[o] = j;
                 dl[o] = dv[u];
                 cl[o] = cj;
                 kc = fma(fv[u], cj, kc);
-                o++;
+                // kernel timing: the update's algorithmic bytes, i.e. the
+                // SNP's two-bit genotype column and its column of the packed
+                // upper factor (block position j holds j + 1 doubles)
+                ub += (double)((P.n + 3) / 4) + 8.0 * (double)(j + 1);
             }
-    } else
-    for (int j = a; j < e; j++)
-        if (chg[p0 + j]) {
-            const int pos = p0 + j;
-            const double d = delta[pos], sd = sdv[pos];
-            const double cj = sd > 0.0 ? d / sd : 0.0;
-            lst[o] = j;
-            dl[o] = d;
-            cl[o] = cj;
-            kc = fma(f2[pos], cj, kc);
-            o++;
+            base += tot;
         }
+    }
+    if (threadIdx.x == 0) count[c] = base;
     kc = block_sum<NT>(kc, lds);
     if (threadIdx.x == 0) kconst[c] = kc;
     if (ubytes) {
-        // kernel timing: the update's algorithmic bytes, i.e. each changed
-        // SNP's two-bit genotype column and its column of the packed upper
-        // factor (block position j holds j + 1 doubles)
-        double ub = 0.0;
-        for (int q = cnt[threadIdx.x]; q < o; q++)
-            ub += (double)((P.n + 3) / 4) + 8.0 * (double)(lst[q] + 1);
-        __syncthreads();
         ub = block_sum<NT>(ub, lds);
         if (threadIdx.x == 0) atomicAdd(ubytes, ub);
     }
@@ -410,21 +415,28 @@ __global__ __launch_bounds__(NT) void k_list(Par P, const uint8_t *chg, const do
 // (models.f90:416-430), z = (x - 2f)/s.  The genotype part is evaluated in
 // centred form: sum_j x_ij c_j - sum_j 2 f_j c_j with c_j = delta_j / s_j.
 // Workgroups [0, nrw): 256 individuals each; lane l of every wave owns
-// individuals 4l..4l+3; the UW waves take interleaved runs of the list
-// (UNR loads in flight per wave) and their partial sums are added in wave
-// order through LDS.  Workgroups after that: (64 rows of r_a) x (RCH list
-// entries), so that the rows of the first tiles, which meet every changed
-// column of the triangle, spread over many workgroups; each writes its
-// partial sums (r_a is updated by k_ra_sum, in chunk order).
+// individuals 4l..4l+3 (one .bed-sized byte of every listed SNP); the UW
+// waves take interleaved runs of the list (UNR loads in flight per wave) and
+// their partial sums are added in wave order through LDS.  This half is
+// bound by FP64 VALU work (bit extract, convert, FMA per genotype), not by
+// its load instructions: a variant with one 32-bit word (16 individuals) per
+// lane and four SNPs per wave load measured slower.  Workgroups after that:
+// (64 rows of r_a) x (RCH list entries), each over the suffix of the list
+// that reaches its row tile (U is upper triangular; tstart from k_list), so
+// that the rows of the first tiles, which meet every changed column of the
+// triangle, spread over many workgroups; each writes its partial sums (r_a
+// is updated by k_ra_sum, in chunk order).  Their loads have clamped, valid
+// addresses and are masked afterwards (no branch per load).
 constexpr int UW = 16;   // waves per update workgroup
-constexpr int UNR = 8;   // list entries in flight per wave
+constexpr int UNR = 8;   // list entries in flight per lane
 constexpr int RCH = UW * UNR * 2;   // list entries per r_a workgroup
 __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__restrict__ geno,
                                                       const double *__restrict__ U, int64_t tri0,
                                                       const double *__restrict__ Ud, int p0,
                                                       int nb, int nrw, const int *lst, const double *dl,
                                                       const double *cl, const int *count,
-                                                      const double *kconst, double *r, double *rapart)
+                                                      const double *kconst, const int *tstart, double *r,
+                                                      double *rapart)
 {
     __shared__ double red[UW][WAVE][4];
     const int64_t c = blockIdx.z;
@@ -434,11 +446,14 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
     dl += c * P.blocksize;
     cl += c * P.blocksize;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // genotype workgroups first, then r_a's (interleaving the two kinds in
+    // runs of 8 measured slower: 70 against 44 us per block)
+    const int local = (int)blockIdx.x < nrw ? (int)blockIdx.x : (int)blockIdx.x - nrw;
     if ((int)blockIdx.x < nrw) {
         r += c * P.ldr;
         // neighbouring 256-individual ranges share 128-byte genotype lines:
         // keep them on one XCD (one L2)
-        const int bx = xcd_contiguous((int)blockIdx.x, nrw);
+        const int bx = xcd_contiguous(local, nrw);
         const int i0 = bx * 256 + 4 * lane;       // first of 4 individuals
         const int64_t byte = i0 >> 2;
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -483,34 +498,34 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
         }
     } else {
         const int nrt = (nb + WAVE - 1) / WAVE;
-        const int rt = (blockIdx.x - nrw) % nrt, cg = (blockIdx.x - nrw) / nrt;
-        const int qb = cg * RCH, qe = min(m, qb + RCH);
+        const int rt = local % nrt, cg = local / nrt;
+        // the list entries from the first column that reaches row tile rt
+        const int qb = tstart[c * (P.blocksize / WAVE + 2) + rt] + cg * RCH, qe = min(m, qb + RCH);
         if (qb >= m) return;   // chunk past the list (not summed)
         rapart += c * P.blocksize * ((P.blocksize + RCH - 1) / RCH) + (int64_t)cg * P.blocksize;
         const int i = rt * WAVE + lane;
+        const int ic = min(i, nb - 1);
         // column jk of the factor: packed triangle (rows <= jk) or dense n_b
-        // column; chosen by select, so that the loads below stay unbranched
+        // column; the row index is clamped into the column and the value
+        // masked, so that every load has a valid address and no branch
         const bool dense = Ud != nullptr;
         const double *Ub = dense ? Ud : U + tri0;
         double acc = 0.0;
-        if (i < nb)
-            for (int q0 = qb + wave * UNR; q0 < qe; q0 += UW * UNR) {
-                double uv[UNR], dv[UNR];
+        for (int q0 = qb + wave * UNR; q0 < qe; q0 += UW * UNR) {
+            double uv[UNR], dv[UNR];
 #pragma unroll
-                for (int u = 0; u < UNR; u++) {
-                    const int q = q0 + u;
-                    uv[u] = 0.0;
-                    dv[u] = 0.0;
-                    if (q < qe) {
-                        const int jk = lst[q];
-                        dv[u] = dl[q];
-                        const int64_t off = dense ? (int64_t)jk * nb : (int64_t)jk * (jk + 1) / 2;
-                        if (dense || i <= jk) uv[u] = Ub[off + i];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < UNR; u++) acc = fma(uv[u], dv[u], acc);
+            for (int u = 0; u < UNR; u++) {
+                const int q = q0 + u, qc = min(q, qe - 1);
+                const int jk = lst[qc];
+                const double d = dl[qc];
+                const int64_t off = dense ? (int64_t)jk * nb : (int64_t)jk * (jk + 1) / 2;
+                const double uval = Ub[off + (dense ? ic : min(ic, jk))];
+                uv[u] = (q < qe && (dense || i <= jk)) ? uval : 0.0;
+                dv[u] = q < qe ? d : 0.0;
             }
+#pragma unroll
+            for (int u = 0; u < UNR; u++) acc = fma(uv[u], dv[u], acc);
+        }
         red[wave][lane][0] = acc;
         __syncthreads();
         if (wave == 0 && i < nb) {
@@ -521,13 +536,15 @@ __global__ __launch_bounds__(UW * WAVE) void k_update(Par P, const uint32_t *__r
     }
 }
 
-// r_a += the list chunks' partial sums, in chunk order.
-__global__ __launch_bounds__(NT) void k_ra_sum(Par P, const double *rapart, const int *count, int nb, double *ra)
+// r_a += the list chunks' partial sums, in chunk order (chunks of the
+// suffix of the list that reaches the row's tile).
+__global__ __launch_bounds__(NT) void k_ra_sum(Par P, const double *rapart, const int *count, const int *tstart,
+                                               int nb, double *ra)
 {
     const int i = blockIdx.x * NT + threadIdx.x;
     const int64_t c = blockIdx.z;
     if (i >= nb) return;
-    const int ncg = (count[c] + RCH - 1) / RCH;
+    const int ncg = (count[c] - tstart[c * (P.blocksize / WAVE + 2) + i / WAVE] + RCH - 1) / RCH;
     rapart += c * P.blocksize * ((P.blocksize + RCH - 1) / RCH) + i;
     double s = 0.0;
     for (int g = 0; g < ncg; g++) s += rapart[(int64_t)g * P.blocksize];
@@ -1163,7 +1180,7 @@ int block_rhs(sb_handle *h, int b, double *out_host)
         rc = batch_block(h, b, 0.0, 0);
         if (!rc) rc = batch_draw(h, b, 0.0, 0, h->sc.delta, reinterpret_cast<uint8_t *>(h->sc.flag), dout);
     } else {
-        int spw = (int)std::max<int64_t>(2, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 2048));
+        int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
         int ngs = (nb + 4 * spw - 1) / (4 * spw);
         int nxg = ngs * P.nchunk;
         k_rhs<<<nxg + (nb + 3) / 4, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, h->tri_off[b], dense_block(h, b), p0,
@@ -1224,7 +1241,7 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
             const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
             const int64_t tri0 = h->tri_off[b];
             // X part: ~2048 waves, each reusing its residual registers over spw SNPs
-            int spw = (int)std::max<int64_t>(2, std::min<int64_t>(32, (int64_t)nb * P.nchunk / 2048));
+            int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
             int ngs = (nb + 4 * spw - 1) / (4 * spw);
             int nxg = ngs * P.nchunk;
             int nag = (nb + 3) / 4;
@@ -1266,14 +1283,15 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
                 kc.begin(st);
                 kc.last(KS_LIST, 25.0 * nb);
                 launch(kc, k_list, dim3(1, 1, C), dim3(NT), 0, st, P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst,
-                       h->u_dl, h->u_cl, h->u_cnt, h->u_kc, kc.extra(KS_UPDATE));
+                       h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, dense_block(h, b) != nullptr ? 1 : 0,
+                       kc.extra(KS_UPDATE));
                 kc.begin(st);
                 launch(kc, k_update, dim3(nrw + nra, 1, C), dim3(UW * WAVE), 0, st, P, h->geno, h->U, tri0,
-                       dense_block(h, b), p0, nb, nrw, h->u_lst, h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->r,
-                       h->u_rapart);
+                       dense_block(h, b), p0, nb, nrw, h->u_lst, h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->u_tstart,
+                       h->r, h->u_rapart);
                 kc.last(KS_UPDATE, 16.0 * P.n + 16.0 * nb);
-                launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt, nb,
-                       h->ra);
+                launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt,
+                       h->u_tstart, nb, h->ra);
             }
         }
         if (rc) break;
