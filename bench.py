@@ -238,6 +238,15 @@ def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int, be
         for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["share"]):
             log(f"[{precision} x{chains}]   {v['name']:<34s} {v['avg_us']:8.2f} us/launch  {v['gbs']:7.0f} GB/s"
                 f"  {v['share']:6.1%} of kernel time")
+    if troof:
+        # the operand layouts alone: one read-only pass per block over every
+        # block (sb_layout_stream), the bandwidth the product kernels' loads
+        # could reach with nothing else to do
+        ls = s.layout_stream()
+        out["layout_stream"] = {k: {"bytes_per_block": int(v[0]), "us_per_block": round(v[1], 2),
+                                    "gbs": round(v[2], 1)} for k, v in ls.items()}
+        log(f"[{precision} x{chains}] read-only layout passes: "
+            + ", ".join(f"{k} {v[1]:.1f} us ({v[2]:.0f} GB/s)" for k, v in ls.items()))
     if bed is not None and not a.no_accuracy:
         out["accuracy"] = accuracy(torch, s, bed, freq, chains)
     s.close()
@@ -445,7 +454,8 @@ def main():
                    "parallelism": "single GPU"},
         "fp64_gpu": None if fp64 is None else {
             "value": fp64["value"], "ms_per_step": fp64["ms"], "chains": 1,
-            "roofline": fp64.get("roofline"), "setup_seconds": round(fp64["setup_s"], 2),
+            "roofline": fp64.get("roofline"), "layout_stream": fp64.get("layout_stream"),
+            "setup_seconds": round(fp64["setup_s"], 2),
             "accuracy": fp64.get("accuracy")},
         "speedup_vs_fp64_gpu": None if fp64 is None else mixed["value"] / fp64["value"],
         "setup_seconds": round(mixed["setup_s"], 2),
@@ -453,6 +463,7 @@ def main():
         "iterations_per_s": a.steps / mixed["dt"],
         "tile_shape": mixed.get("tile_shape"),
         "accuracy": mixed.get("accuracy"),
+        "layout_stream": mixed.get("layout_stream"),
         "folds": folds,
         "traits": traits,
         "bayesa_bayesb": c3,

@@ -257,6 +257,15 @@ int sb_state_save(sb_handle *h, void *buf, int64_t size);
  * gives the chain the uninterrupted run would have given. */
 int sb_state_load(sb_handle *h, const void *buf, int64_t size);
 
+/* Diagnostics (no counterpart in the reference): the read bandwidth of this
+ * handle's operand layouts alone.  For each layout, one read-only pass per
+ * block (1 KB per wave load, one launch per block as the sweep launches its
+ * kernels) over every block; out[3k .. 3k+2] = bytes per block,
+ * microseconds per block, GB/s, for k = 0..3: the batched path's SNP-major
+ * and individual-major genotype tiles and its two FP32 U tile layouts, or
+ * the FP64 path's genotype words and packed U (k = 2, 3 zero).  12 doubles. */
+int sb_layout_stream(sb_handle *h, double *out);
+
 /* Diagnostics of the COUNTER generator (no counterpart in the reference):
  * for each of n draw sites k, the first `per` uniforms in [0, 1) of the
  * stream the sampler opens for chain `chain` of a handle seeded `seed`, at

@@ -29,7 +29,7 @@ KT_SLOTS = ["rhs", "draw", "list", "update", "rquant", "xq", "ut0", "drawb", "dq
 EXPORTS = ["sb_rng_seed", "sb_permutate", "sb_create", "sb_destroy", "sb_rng_set", "sb_rng_get",
            "sb_build_aug", "sb_test_shift_d", "sb_mcmc_init", "sb_mcmc_init_chains", "sb_mcmc_run", "sb_hyp_width", "sb_get",
            "sb_block_rhs", "sb_get_info", "sb_predict", "sb_predict_noalloc", "sb_state_size", "sb_state_save", "sb_state_load",
-           "sb_counter_uniforms", "sb_last_error"]
+           "sb_counter_uniforms", "sb_layout_stream", "sb_last_error"]
 
 
 class SbDims(C.Structure):
@@ -86,6 +86,7 @@ def load():
     L.sb_state_size.argtypes = [vp, C.POINTER(C.c_int64)]
     L.sb_state_save.argtypes = [vp, vp, C.c_int64]
     L.sb_state_load.argtypes = [vp, vp, C.c_int64]
+    L.sb_layout_stream.argtypes = [vp, dp]
     L.sb_counter_uniforms.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_uint32), C.c_int32, C.c_int32, dp]
     L.sb_last_error.restype = C.c_char_p
     for f in EXPORTS:

@@ -465,6 +465,13 @@ int sb_counter_uniforms(int64_t seed, int32_t chain, const uint32_t *ctr, int32_
     return counter_uniforms(seed, chain, ctr, n, per, out);
 }
 
+int sb_layout_stream(sb_handle *h, double *out)
+{
+    if (!h || !out) return fail(SB_ERR_ARG, "sb_layout_stream: null argument");
+    if (!h->aug_built) return fail(SB_ERR_STATE, "sb_layout_stream before sb_build_aug");
+    return layout_stream(h, out);
+}
+
 int sb_test_shift_d(sb_handle *h, int32_t block, double amount)
 {
     if (!h) return fail(SB_ERR_ARG, "sb_test_shift_d: null handle");

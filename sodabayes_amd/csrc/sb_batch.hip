@@ -14,6 +14,7 @@
 // stay FP64 (v_mfma_f64_16x16x4f64 over all chains).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -1205,6 +1206,102 @@ int batch_autotune(sb_handle *h)
     h->rstat_nwg = 0;   // the trial updates overwrote the residual statistics
     h->q_block = -1;
     return hip_check(hipGetLastError(), "tile-shape tuning");
+}
+
+// Read-only pass over a layout: every wave reads contiguous 1 KB (one uint4
+// per lane) per load, four loads in flight per lane, grid-stride, as the
+// product kernels read their operand tiles.  The XOR of everything read is
+// stored only if it equals a value it cannot take when `never` is 0, which
+// keeps the loads alive.
+__global__ __launch_bounds__(NT) void k_stream(const uint4 *__restrict__ p, int64_t n, uint4 *sink, int never)
+{
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    const int64_t stride = (int64_t)gridDim.x * NT * 4;
+    for (int64_t base = (int64_t)blockIdx.x * NT * 4 + threadIdx.x; base < n; base += stride) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t i = min(base + (int64_t)u * NT, n - 1);
+            v[u] = p[i];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            acc.x ^= v[u].x;
+            acc.y ^= v[u].y;
+            acc.z ^= v[u].z;
+            acc.w ^= v[u].w;
+        }
+    }
+    if (never && acc.x == 0x9E3779B9u && acc.y == 0x7F4A7C15u) sink[threadIdx.x] = acc;
+}
+
+// For each operand layout of the handle's path, the time of one read-only
+// pass per block, one launch per block as the sweep launches its kernels,
+// over every block (a working set far beyond the caches): out[3 k ..] =
+// (bytes per block, microseconds per block, GB/s) for layout k: the batched
+// path's geno_sw, geno_t, Ut0, Ut1; the FP64 path's genotype words and
+// packed U (then slots 2, 3 are zero).
+int layout_stream(sb_handle *h, double *out)
+{
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    for (int k = 0; k < 12; k++) out[k] = 0.0;
+    if (P.nblocks <= 0) return fail(SB_ERR_STATE, "sb_layout_stream: no SODA blocks");
+    uint4 *sink = nullptr;
+    TmpDev tmp;
+    int rc = tmp.alloc(&sink, sizeof(uint4) * NT, "stream sink");
+    if (rc) return rc;
+    struct Lay {
+        const uint4 *base;
+        std::vector<int64_t> off;   // uint4 offset of each block, nblocks + 1
+    };
+    std::vector<Lay> lays;
+    const int nb = P.nblocks;
+    auto mk = [&](const void *b, auto &&f) {
+        Lay l{static_cast<const uint4 *>(b), std::vector<int64_t>(nb + 1)};
+        for (int i = 0; i <= nb; i++) l.off[i] = f(i);
+        lays.push_back(l);
+    };
+    if (P.precision == SB_PREC_MIXED) {
+        mk(h->geno_sw, [&](int i) { return h->gs_off[i]; });
+        mk(h->geno_t, [&](int i) { return h->gt_off[i]; });
+        mk(h->Ut0, [&](int i) { return h->tt_off[i] * 64; });
+        mk(h->Ut1, [&](int i) { return h->tt_off[i] * 64; });
+    } else {
+        mk(h->geno, [&](int i) { return (int64_t)h->ls[i] * P.ldw / 4; });
+        // packed triangles of doubles: block starts rounded down to whole
+        // 16-byte elements (a block's last double may be left out)
+        mk(h->U, [&](int i) { return h->tri_off[i] / 2; });
+    }
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    for (size_t k = 0; k < lays.size() && !rc; k++) {
+        const Lay &l = lays[k];
+        auto pass = [&]() {
+            for (int b = 0; b < nb; b++) {
+                const int64_t n = l.off[b + 1] - l.off[b];
+                if (n <= 0) continue;
+                const int grid = (int)std::min<int64_t>((n + 4 * NT - 1) / (4 * NT), 8192);
+                k_stream<<<grid, NT, 0, st>>>(l.base + l.off[b], n, sink, 0);
+            }
+        };
+        pass();   // warm-up (page tables, code)
+        hipEventRecord(e0, st);
+        pass();
+        hipEventRecord(e1, st);
+        rc = hip_check(hipEventSynchronize(e1), "layout stream");
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        const double bytes = 16.0 * (double)(l.off[nb] - l.off[0]) / nb, us = 1e3 * ms / nb;
+        out[3 * k] = bytes;
+        out[3 * k + 1] = us;
+        out[3 * k + 2] = us > 0 ? bytes / (us * 1e-6) / 1e9 : 0.0;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!rc) rc = hip_check(hipGetLastError(), "layout stream");
+    return rc;
 }
 
 // Every chain's carried quantisation remainders to zero: when r is exact

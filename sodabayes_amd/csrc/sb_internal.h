@@ -338,6 +338,7 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
                double *rhs_out = nullptr);
 int batch_update(sb_handle *h, int b);
 int batch_clear_carry(sb_handle *h);
+int layout_stream(sb_handle *h, double *out);
 int batch_transpose(sb_handle *h);
 int batch_words(int nb);
 int batch_kss(int ntile);

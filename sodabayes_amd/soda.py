@@ -349,6 +349,14 @@ class SodaSampler:
         _lib.check(_lib.load().sb_block_rhs(self.handle, block, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def layout_stream(self) -> dict:
+        """Read-only bandwidth of this handle's operand layouts
+        (sb_layout_stream): {layout: (bytes per block, us per block, GB/s)}."""
+        v = np.zeros(12)
+        _lib.check(_lib.load().sb_layout_stream(self.handle, v.ctypes.data_as(C.POINTER(C.c_double))))
+        names = (["geno_sw", "geno_t", "Ut0", "Ut1"] if self.opts.precision == "mixed" else ["geno", "U"])
+        return {k: (v[3 * i], v[3 * i + 1], v[3 * i + 2]) for i, k in enumerate(names)}
+
     def info(self) -> dict:
         v = self.info_raw()
         return dict(setup_seconds=v[0], iterations=v[1], sweep_ms=v[2], device_bytes=v[3],
