@@ -301,6 +301,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->u_dl, sizeof(double) * bs * C);
     A((void **)&h->u_cl, sizeof(double) * bs * C);
     A((void **)&h->u_kc, sizeof(double) * (C + 4));
+    A((void **)&h->d_it, sizeof(int) * 4);
     A((void **)&h->u_cnt, sizeof(int) * (C + 4));
     A((void **)&h->u_rapart, sizeof(double) * bs * C * ((bs + 255) / 256 + 1));
     A((void **)&h->u_tstart, sizeof(int) * C * (bs / 64 + 2));
@@ -430,6 +431,8 @@ int sb_destroy(sb_handle *h)
 {
     if (!h) return SB_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    if (h->hyp) hipFree(h->hyp);
     for (void *p : h->allocs) hipFree(p);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
@@ -759,6 +762,7 @@ int sb_state_load(sb_handle *h, const void *buf, int64_t size)
     if (rc) return rc;
     if (hip_check(hipStreamSynchronize(h->stream), "state load")) return SB_ERR_HIP;
     h->initialised = true;
+    h->graph_ok = false;   // a captured sweep may name another state (held-out rows)
     return SB_OK;
 }
 

@@ -913,7 +913,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
                                                const double *rapart, int spant, const double *f2,
                                                const double *sdv, const int32_t *grp,
                                                const int32_t *snp_of_pos, int p0, int nb, double zz,
-                                               int rep, double *beta, int32_t *cls, double *vara_s,
+                                               const int *it, double *beta, int32_t *cls, double *vara_s,
                                                double *delta, uint8_t *chg, double *rhs_out, uint64_t *dmax,
                                                const double *carry, double *cj_out, float *dl_sw)
 {
@@ -968,7 +968,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
             rhs_out[(int64_t)c * nb + jk] = rhs;
         } else {
             const Scal *scc = sc + c;
-            PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)sid, TAG_SNP);
+            PhiloxSrc src(chain_key(P, c), (uint32_t)it[0], (uint32_t)sid, TAG_SNP);
             const SnpDraw d = mixm ? draw_mix_fast<NM>(P, scc, src, mc, rhs, zz, gp, gk, cold)
                                    : draw_core<PhiloxSrc, NM>(P, scc, src, rhs, zz, zz / scc->vare, gp, gk, cold, vs);
             beta[o + pos] = d.beta;
@@ -1161,6 +1161,7 @@ int batch_min_kss(int ntile)
 int batch_autotune(sb_handle *h)
 {
     h->tuned = true;
+    h->graph_ok = false;   // the tile shape changes the captured launches
     const Par &P = h->par;
     hipStream_t st = h->stream;
     if (!h->tune_r && dev_alloc(h, (void **)&h->tune_r, sizeof(double) * (size_t)P.ldr * P.nchains))
@@ -1355,7 +1356,7 @@ int batch_tiles(sb_handle *h)
     return rc;
 }
 
-int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *chg, double *rhs_out)
+int batch_draw(sb_handle *h, int b, double zz, double *delta, uint8_t *chg, double *rhs_out)
 {
     const Par &P = h->par;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
@@ -1369,8 +1370,8 @@ int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *
     with_nmix(P.nmix, [&](auto nm) {
         launch(h->kc, k_draw_b<decltype(nm)::value>, dim3((nb + NT - 1) / NT, P.nchains), dim3(NT), 0, h->stream,
                P, h->scal, reinterpret_cast<const long long *>(h->b_T), tstride, nch, h->b_expo, h->b_rtot,
-               h->b_rapart, h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, rep, h->beta, h->cls, h->vara_s, delta, chg,
-               rhs_out, h->b_dmax, h->b_carry, h->b_cj, h->dl_sw);
+               h->b_rapart, h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, h->d_it, h->beta, h->cls, h->vara_s,
+               delta, chg, rhs_out, h->b_dmax, h->b_carry, h->b_cj, h->dl_sw);
     });
     return hip_check(hipGetLastError(), "batch draw");
 }

@@ -234,8 +234,13 @@ struct sb_handle {
     double *vara_s = nullptr;     // nloci per chain
     double *gstore = nullptr, *vara_sstore = nullptr, *indiststore = nullptr;
     sb::Scal *scal = nullptr;     // per chain
-    double *hyp = nullptr;        // device hyp ring
+    double *hyp = nullptr;        // device .hyp records of one sb_mcmc_run call (hyp_cap records)
     int hyp_cap = 0;
+    int *d_it = nullptr;          // device iteration state: [iteration, next .hyp record]
+    // one sweep iteration captured as a HIP graph (kept across sb_mcmc_run
+    // calls while its launch parameters stay valid; freed by sb_destroy)
+    hipGraphExec_t gexec = nullptr;
+    bool graph_ok = false;        // gexec matches the handle's current state
     sb::Scratch sc{};
     int *u_lst = nullptr;         // changed positions of the current block, per chain (blocksize each)
     double *u_dl = nullptr;       // their beta_old - beta_new
@@ -334,8 +339,7 @@ int counter_uniforms(int64_t seed, int chain, const uint32_t *ctr, int n, int pe
 
 // chain-batched mixed-precision block products and draws (sb_batch.hip)
 int batch_block(sb_handle *h, int b, double zz, int rep);
-int batch_draw(sb_handle *h, int b, double zz, int rep, double *delta, uint8_t *chg,
-               double *rhs_out = nullptr);
+int batch_draw(sb_handle *h, int b, double zz, double *delta, uint8_t *chg, double *rhs_out = nullptr);
 int batch_update(sb_handle *h, int b);
 int batch_clear_carry(sb_handle *h);
 int layout_stream(sb_handle *h, double *out);

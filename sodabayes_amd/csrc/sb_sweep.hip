@@ -50,8 +50,9 @@ __global__ __launch_bounds__(NT) void k_mu_pre(Par P, double *r, const Scal *sca
 
 // mu ~ N(sum(r)/n, vare/n) (models.f90:131) and the per-sweep constants
 // log_p, vare_gp (models.f90:330-338)
-__global__ void k_mu_draw(Par P, Scal *scal, const double *red, int nred, int rep)
+__global__ void k_mu_draw(Par P, Scal *scal, const double *red, int nred, const int *it)
 {
+    const int rep = it[0];
     const int c = threadIdx.x;
     if (c >= P.nchains) return;
     scal += c;
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
                                                      const double *partA, const double *partR,
                                                      const double *f2, const double *sdv,
                                                      const int32_t *grp, const int32_t *snp_of_pos,
-                                                     int p0, int nb, double zz, int rep, double *beta,
+                                                     int p0, int nb, double zz, const int *it, double *beta,
                                                      int32_t *cls, double *vara_s, double *delta,
                                                      uint8_t *chg, const uint8_t *skip)
 {
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
         return;
     }
     double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
-    PhiloxSrc src(chain_key(P, 0), (uint32_t)rep, (uint32_t)snp_of_pos[pos], TAG_SNP);
+    PhiloxSrc src(chain_key(P, 0), (uint32_t)it[0], (uint32_t)snp_of_pos[pos], TAG_SNP);
     if (mixm) {
         const SnpDraw d = draw_mix_fast<NM>(P, sc, src, mc, rhs, zz, grp[pos], beta[pos], cls[pos]);
         beta[pos] = d.beta;
@@ -599,10 +600,15 @@ __global__ __launch_bounds__(NT) void k_bins(Par P, const double *beta, const in
 
 // Hyperparameter draws, stores and the trace record (models.f90:496-569).
 __global__ void k_hyper(Par P, Scal *sc, const double *red_rtr, int nrtr, const double *cnt,
-                        const double *var, int nwg, int rep, int store, double *hyp, int hyp_idx)
+                        const double *var, int nwg, int *it, double *hyp)
 {
     const int ch = threadIdx.x;
     if (ch >= P.nchains) return;
+    // it[0]: this iteration; it[1]: its record index in hyp (models.f90:526, 549)
+    const int rep = it[0];
+    const bool store = rep > P.burnin && rep % P.thin == 0;
+    const bool record = hyp != nullptr && rep % P.thin == 0;
+    const int hyp_idx = record ? it[1] : -1;
     const int nb = P.nbins;
     sc += ch;
     red_rtr += (int64_t)ch * nrtr;
@@ -690,6 +696,7 @@ __global__ void k_hyper(Par P, Scal *sc, const double *red_rtr, int nrtr, const 
             h[5 + nb + b] = sc->varindist[b];
         }
     }
+    if (ch == 0 && record) it[1] = hyp_idx + 1;
 }
 
 // fresh r_a ~ N(0, vare) (models.f90:523-525; init 118-120)
@@ -704,8 +711,9 @@ __global__ void k_ra_replay(Par P, Scal *sc, double *ra)
 }
 
 // (ra_sw, when given, receives the same values as a swizzled panel)
-__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag, float *ra_sw)
+__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, const int *it, int rep, int tag, float *ra_sw)
 {
+    if (it) rep = it[0];   // the sweep's iteration (rep: the initial draws' fixed counter)
     int i = blockIdx.x * NT + threadIdx.x;
     const int c = blockIdx.z;
     if (i >= P.naind) return;
@@ -716,9 +724,11 @@ __global__ void k_ra_counter(Par P, const Scal *sc, double *ra, int rep, int tag
 }
 
 // posterior sums per SNP (models.f90:528-544)
-__global__ void k_accum(Par P, const double *beta, const int32_t *cls, const double *vara_s,
+__global__ void k_accum(Par P, const int *it, const double *beta, const int32_t *cls, const double *vara_s,
                         double *gstore, double *indiststore, double *vara_sstore)
 {
+    const int rep = it[0];
+    if (!(rep > P.burnin && rep % P.thin == 0)) return;   // models.f90:526
     int pos = blockIdx.x * NT + threadIdx.x;
     if (pos >= P.p) return;
     const int64_t c = blockIdx.z;
@@ -840,8 +850,9 @@ __global__ void k_init_r(Par P, const double *why, const Scal *sc, double *r)
 // so after vare's update every held-out y_i is redrawn, which makes its
 // residual a fresh N(0, vare) draw.  The phenotype keeps the drawn value
 // (y = y - r + e), so that the resync recomputes the same residual.
-__global__ void k_impute(Par P, const Scal *sc, const uint8_t *heldout, double *why, double *r, int rep)
+__global__ void k_impute(Par P, const Scal *sc, const uint8_t *heldout, double *why, double *r, const int *it)
 {
+    const int rep = it[0];
     const int i = blockIdx.x * NT + threadIdx.x;
     const int64_t c = blockIdx.z;
     if (i >= P.n) return;
@@ -963,6 +974,17 @@ __global__ void k_rhs_out(Par P, const double *part, const double *partA, const 
 // ------------------------------------------------------------- host side --
 
 // dense factor of block b (pivoted-Cholesky fallback), or nullptr
+// The sweep's device iteration state: it[0] = the iteration being run,
+// it[1] = the index of its .hyp record.  Set at the start of each
+// sb_mcmc_run call; every iteration begins by advancing it[0], so that one
+// captured iteration replays for any iteration number.
+__global__ void k_set_it(int *it, int rep, int rec)
+{
+    it[0] = rep;
+    it[1] = rec;
+}
+__global__ void k_next_it(int *it) { it[0] += 1; }
+
 // Diagnostics of the counter-based generator: for site k, `per` uniforms of
 // the stream the sampler's kernels open as PhiloxSrc(chain_key(seed, chain),
 // ctr[3k], ctr[3k+1], ctr[3k+2]) (iteration, site, tag), in draw order.
@@ -1147,12 +1169,13 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
         if (P.rng_mode == SB_RNG_REPLAY)
             k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
         else
-            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->scal, h->ra,
+            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->scal, h->ra, nullptr,
                                                                                    0x7FFFFFFF, TAG_INIT, h->ra_sw);
     }
     if (hip_check(hipGetLastError(), "init kernels")) return SB_ERR_HIP;
     if (hip_check(hipStreamSynchronize(h->stream), "init")) return SB_ERR_HIP;
     h->initialised = true;
+    h->graph_ok = false;   // a captured sweep may name another state (held-out rows)
     h->iterations = 0;
     return SB_OK;
 }
@@ -1178,7 +1201,7 @@ int block_rhs(sb_handle *h, int b, double *out_host)
     if (P.precision == SB_PREC_MIXED) {
         h->rstat_nwg = 0;
         rc = batch_block(h, b, 0.0, 0);
-        if (!rc) rc = batch_draw(h, b, 0.0, 0, h->sc.delta, reinterpret_cast<uint8_t *>(h->sc.flag), dout);
+        if (!rc) rc = batch_draw(h, b, 0.0, h->sc.delta, reinterpret_cast<uint8_t *>(h->sc.flag), dout);
     } else {
         int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
         int ngs = (nb + 4 * spw - 1) / (4 * spw);
@@ -1196,19 +1219,18 @@ int block_rhs(sb_handle *h, int b, double *out_host)
     return rc;
 }
 
-int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
+// One sweep iteration (models.f90:129-570, without the resync): mu, the
+// block loop, the skipped SNPs or the single-site sweep, the
+// hyperparameters, fresh r_a, held-out imputation and the posterior sums.
+// Iteration-dependent values come from the device state h->d_it, which the
+// first kernel advances, so the same launches serve any iteration (and can
+// be captured as a graph).  `rep` is used only by the paths that are never
+// captured (the single-site sweep and the skipped SNPs).
+static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd)
 {
-    if (!h->initialised) return fail(SB_ERR_STATE, "sb_mcmc_run before sb_mcmc_init");
-    if (count < 0 || first < 1) return fail(SB_ERR_ARG, "bad iteration range");
     const Par &P = h->par;
     hipStream_t st = h->stream;
     const int nred = std::min(1024, (P.n + NT - 1) / NT);
-    const int hw = 5 + 2 * P.nbins;
-    int nrec = 0;
-    for (int rep = first; rep < first + count; rep++) nrec += (rep % P.thin) == 0;
-    double *dhyp = nullptr;
-    if (hyp_host && nrec > 0)
-        if (hip_check(hipMalloc(&dhyp, sizeof(double) * (size_t)nrec * hw), "hipMalloc(hyp)")) return SB_ERR_NOMEM;
     const int C = P.nchains;
     const bool mixed = P.precision == SB_PREC_MIXED;
     const int bin_chunk = 8192;
@@ -1216,125 +1238,186 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     double *bcnt = h->sc.red + (int64_t)1024 * C, *bvar = bcnt + (int64_t)C * P.nbins * nwg_bins;
     uint8_t *chg = reinterpret_cast<uint8_t *>(h->sc.flag);
     double *delta = h->sc.delta;
+    KClock &kc = h->kc;
+    int rc = SB_OK;
+    k_next_it<<<1, 1, 0, st>>>(h->d_it);
+    k_mu_pre<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
+    k_mu_draw<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, h->d_it);
+    k_mu_post<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal);
+    h->rstat_nwg = 0;   // r changed: the batched path recomputes its statistics
+    for (int b = 0; b < P.nblocks && !rc; b++) {
+        const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+        const int64_t tri0 = h->tri_off[b];
+        // X part: ~2048 waves, each reusing its residual registers over spw SNPs
+        int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
+        int ngs = (nb + 4 * spw - 1) / (4 * spw);
+        int nxg = ngs * P.nchunk;
+        int nag = (nb + 3) / 4;
+        const double gbytes = (double)nb * (((double)P.n + 3.0) / 4.0);   // 2-bit genotypes of the block
+        const double ubytes = 8.0 * (double)nb * (nb + 1) / 2.0;           // packed upper factor
+        if (mixed) {
+            rc = batch_block(h, b, xxd[b], rep);
+            if (rc) break;
+            rc = batch_draw(h, b, xxd[b], delta, chg);
+            if (rc) break;
+            rc = batch_update(h, b);
+            if (rc) break;
+            continue;
+        }
+        kc.begin(st);
+        kc.last(KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
+        launch(kc, k_rhs, dim3(nxg + nag), dim3(NT), 0, st, P, h->geno, h->r, h->ra, h->U, tri0, dense_block(h, b),
+               p0, nb, spw, ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
+        kc.begin(st);
+        kc.last(KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
+        if (P.rng_mode == SB_RNG_REPLAY)
+            launch(kc, k_draw_replay, dim3(1), dim3(NT), 0, st, P, h->scal, h->sc.part, h->sc.partA, h->sc.partR,
+                   h->f2, h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp, h->sc.uu, h->sc.chg, h->beta,
+                   h->cls, h->vara_s, delta, chg, h->skip);
+        else
+            with_nmix(P.nmix, [&](auto nm) {
+                launch(kc, k_draw_counter<decltype(nm)::value>, dim3((nb + NT - 1) / NT), dim3(NT), 0, st, P,
+                       h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p,
+                       p0, nb, xxd[b], h->d_it, h->beta, h->cls, h->vara_s, delta, chg, h->skip);
+            });
+        const int nrw = (P.n + 255) / 256;
+        const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
+        kc.begin(st);
+        kc.last(KS_LIST, 25.0 * nb);
+        launch(kc, k_list, dim3(1, 1, C), dim3(NT), 0, st, P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
+               h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, dense_block(h, b) != nullptr ? 1 : 0, kc.extra(KS_UPDATE));
+        kc.begin(st);
+        launch(kc, k_update, dim3(nrw + nra, 1, C), dim3(UW * WAVE), 0, st, P, h->geno, h->U, tri0, dense_block(h, b),
+               p0, nb, nrw, h->u_lst, h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, h->r, h->u_rapart);
+        kc.last(KS_UPDATE, 16.0 * P.n + 16.0 * nb);
+        launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt, h->u_tstart,
+               nb, h->ra);
+    }
+    if (rc) return rc;
+    if (P.regular) {
+        // -SODAOff: the single-site sweep over every SNP
+        rc = single_sweep(h, rep);
+        if (rc) return rc;
+    } else if (h->nskipped > 0) {
+        // the FP64 path only (setup refuses the fallback for the mixed path)
+        with_nmix(P.nmix, [&](auto nm) {
+            k_skipped<decltype(nm)::value><<<C, SKT, 0, st>>>(P, h->scal, h->geno, h->zlut, h->d_skip_list,
+                                                              h->nskipped, h->xpx, h->grp, h->d_pos_by_id + P.p,
+                                                              rep, h->r, h->beta, h->cls, h->vara_s, delta, chg);
+        });
+    }
+    k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);
+    k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+    k_hyper<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, h->d_it, h->hyp);
+    if (P.naind > 0) {
+        if (P.rng_mode == SB_RNG_REPLAY)
+            k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
+        else
+            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, h->d_it, 0, TAG_RA,
+                                                                             h->ra_sw);
+    }
+    if (h->use_heldout) {
+        k_impute<<<dim3((P.n + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->heldout, h->why, h->r, h->d_it);
+        h->rstat_nwg = 0;
+    }
+    k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->d_it, h->beta, h->cls, h->vara_s, h->gstore,
+                                                           h->indiststore, h->vara_sstore);
+    return hip_check(hipGetLastError(), "sweep launch");
+}
+
+// compute_residuals every P.resync iterations (models.f90:571-573)
+static int sweep_resync(sb_handle *h)
+{
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    const int C = P.nchains;
+    k_nonzero<<<dim3(1, 1, C), NT, 0, st>>>(P, h->d_pos_by_id, h->beta, h->resync_list, h->d_count);
+    k_resync<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->geno, h->zlut, h->beta, h->resync_list,
+                                                              h->d_count, h->why, h->scal, h->r);
+    h->rstat_nwg = 0;
+    int rc = batch_clear_carry(h);   // r is exact again
+    if (!rc) rc = hip_check(hipGetLastError(), "resync launch");
+    return rc;
+}
+
+// HIP graphs for the sweep, opt-in with SODABAYES_GRAPH=1: replaying one
+// captured iteration measured 11.18 against 11.20 ms per iteration (mixed,
+// 16 chains) and 13.00 against 13.08 ms (FP64) at 50,000 x 500,000 with
+// subsets of 4,096 -- the host already enqueues far ahead of the device, and
+// a graph's kernel nodes are still dispatched one by one -- so eager
+// launches stay the default.  Never for the per-kernel timing, the
+// single-site sampler or the skipped-SNP updates (their launches depend on
+// the iteration on the host).
+static bool graph_wanted(const sb_handle *h)
+{
+    const char *e = std::getenv("SODABAYES_GRAPH");
+    if (!e || e[0] != '1') return false;
+    return !h->kc.on && !h->par.regular && h->nskipped == 0 && h->par.nblocks > 0;
+}
+
+int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
+{
+    if (!h->initialised) return fail(SB_ERR_STATE, "sb_mcmc_run before sb_mcmc_init");
+    if (count < 0 || first < 1) return fail(SB_ERR_ARG, "bad iteration range");
+    const Par &P = h->par;
+    hipStream_t st = h->stream;
+    const int hw = 5 + 2 * P.nbins;
+    int nrec = 0;
+    for (int rep = first; rep < first + count; rep++) nrec += (rep % P.thin) == 0;
+    const bool mixed = P.precision == SB_PREC_MIXED;
     std::vector<double> xxd(P.nblocks);
     if (P.nblocks > 0) hipMemcpy(xxd.data(), h->xxd, sizeof(double) * P.nblocks, hipMemcpyDeviceToHost);
     int rc = SB_OK;
     if (mixed && !h->tuned) rc = batch_autotune(h);
-    if (rc) {
-        if (dhyp) hipFree(dhyp);
-        return rc;
+    if (rc) return rc;
+    // the call's .hyp records go to a buffer of the handle (a larger call
+    // replaces it, and with it any captured graph that names it)
+    if (nrec > h->hyp_cap) {
+        if (h->gexec) {
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+        }
+        h->graph_ok = false;
+        if (h->hyp) hipFree(h->hyp);
+        h->hyp = nullptr;
+        h->hyp_cap = 0;
+        const int cap = std::max(nrec, 1024);
+        if (hip_check(hipMalloc(&h->hyp, sizeof(double) * (size_t)cap * hw), "hipMalloc(hyp records)"))
+            return SB_ERR_NOMEM;
+        h->hyp_cap = cap;
     }
     // optional per-kernel timing on this stream
     const char *kt = std::getenv("SODABAYES_KERNEL_TIMING");
     h->kc.on = kt && kt[0] == '1';
     KClock &kc = h->kc;
+    const bool use_graph = graph_wanted(h);
     auto t0 = std::chrono::steady_clock::now();
-    int rec = 0;
-    // every failure below leaves the loops with rc set and goes through the
-    // one exit at the end (stream drained, clock reset, hyp buffer freed)
+    k_set_it<<<1, 1, 0, st>>>(h->d_it, first - 1, 0);
+    // every failure below leaves the loop with rc set and goes through the
+    // one exit at the end (stream drained, clock reset)
     for (int rep = first; rep < first + count && !rc; rep++) {
-        k_mu_pre<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
-        k_mu_draw<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, rep);
-        k_mu_post<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal);
-        h->rstat_nwg = 0;   // r changed: the batched path recomputes its statistics
-        for (int b = 0; b < P.nblocks && !rc; b++) {
-            const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
-            const int64_t tri0 = h->tri_off[b];
-            // X part: ~2048 waves, each reusing its residual registers over spw SNPs
-            int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
-            int ngs = (nb + 4 * spw - 1) / (4 * spw);
-            int nxg = ngs * P.nchunk;
-            int nag = (nb + 3) / 4;
-            const double gbytes = (double)nb * (((double)P.n + 3.0) / 4.0);   // 2-bit genotypes of the block
-            const double ubytes = 8.0 * (double)nb * (nb + 1) / 2.0;           // packed upper factor
-            if (mixed) {
-                rc = batch_block(h, b, xxd[b], rep);
-                if (rc) break;
-            } else {
-                kc.begin(st);
-                kc.last(KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
-                launch(kc, k_rhs, dim3(nxg + nag), dim3(NT), 0, st, P, h->geno, h->r, h->ra, h->U, tri0,
-                       dense_block(h, b), p0, nb, spw, ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
+        if (use_graph && !h->graph_ok) {
+            // capture one iteration; its launches run when the graph does
+            if (h->gexec) {
+                hipGraphExecDestroy(h->gexec);
+                h->gexec = nullptr;
             }
-            if (mixed) {
-                rc = batch_draw(h, b, xxd[b], rep, delta, chg);
-                if (rc) break;
-            } else {
-                kc.begin(st);
-                kc.last(KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
-                if (P.rng_mode == SB_RNG_REPLAY)
-                    launch(kc, k_draw_replay, dim3(1), dim3(NT), 0, st, P, h->scal, h->sc.part, h->sc.partA,
-                           h->sc.partR, h->f2, h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp, h->sc.uu,
-                           h->sc.chg, h->beta, h->cls, h->vara_s, delta, chg, h->skip);
-                else
-                    with_nmix(P.nmix, [&](auto nm) {
-                        launch(kc, k_draw_counter<decltype(nm)::value>, dim3((nb + NT - 1) / NT), dim3(NT), 0, st,
-                               P, h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
-                               h->d_pos_by_id + P.p, p0, nb, xxd[b], rep, h->beta, h->cls, h->vara_s, delta, chg,
-                               h->skip);
-                    });
-            }
-            if (mixed) {
-                rc = batch_update(h, b);
-                if (rc) break;
-            } else {
-                const int nrw = (P.n + 255) / 256;
-                const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
-                kc.begin(st);
-                kc.last(KS_LIST, 25.0 * nb);
-                launch(kc, k_list, dim3(1, 1, C), dim3(NT), 0, st, P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst,
-                       h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, dense_block(h, b) != nullptr ? 1 : 0,
-                       kc.extra(KS_UPDATE));
-                kc.begin(st);
-                launch(kc, k_update, dim3(nrw + nra, 1, C), dim3(UW * WAVE), 0, st, P, h->geno, h->U, tri0,
-                       dense_block(h, b), p0, nb, nrw, h->u_lst, h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->u_tstart,
-                       h->r, h->u_rapart);
-                kc.last(KS_UPDATE, 16.0 * P.n + 16.0 * nb);
-                launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt,
-                       h->u_tstart, nb, h->ra);
-            }
-        }
-        if (rc) break;
-        if (P.regular) {
-            // -SODAOff: the single-site sweep over every SNP
-            rc = single_sweep(h, rep);
+            hipGraph_t g = nullptr;
+            rc = hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture");
             if (rc) break;
-        } else if (h->nskipped > 0) {
-            // the FP64 path only (setup refuses the fallback for the mixed path)
-            with_nmix(P.nmix, [&](auto nm) {
-                k_skipped<decltype(nm)::value><<<C, SKT, 0, st>>>(P, h->scal, h->geno, h->zlut, h->d_skip_list,
-                                                                  h->nskipped, h->xpx, h->grp, h->d_pos_by_id + P.p,
-                                                                  rep, h->r, h->beta, h->cls, h->vara_s, delta, chg);
-            });
+            const int rb = sweep_iteration(h, rep, xxd);
+            const int re = hip_check(hipStreamEndCapture(st, &g), "end capture");
+            rc = rb ? rb : re;
+            if (!rc) rc = hip_check(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0), "graph instantiate");
+            if (g) hipGraphDestroy(g);
+            if (rc) break;
+            h->graph_ok = true;
         }
-        k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);
-        k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
-        const int store = (rep > P.burnin) && (rep % P.thin == 0);
-        const int hidx = (dhyp && rep % P.thin == 0) ? rec++ : -1;
-        k_hyper<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, rep, store, dhyp, hidx);
-        if (P.naind > 0) {
-            if (P.rng_mode == SB_RNG_REPLAY)
-                k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
-            else
-                k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, rep, TAG_RA,
-                                                                                 h->ra_sw);
-        }
-        if (h->use_heldout) {
-            k_impute<<<dim3((P.n + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->heldout, h->why, h->r, rep);
-            h->rstat_nwg = 0;
-        }
-        if (store)
-            k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->vara_s, h->gstore,
-                                                                   h->indiststore, h->vara_sstore);
-        if (rep % P.resync == 0) {
-            k_nonzero<<<dim3(1, 1, C), NT, 0, st>>>(P, h->d_pos_by_id, h->beta, h->resync_list, h->d_count);
-            k_resync<<<dim3((P.ldw + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->geno, h->zlut, h->beta,
-                                                                      h->resync_list, h->d_count, h->why,
-                                                                      h->scal, h->r);
-            h->rstat_nwg = 0;
-            rc = batch_clear_carry(h);   // r is exact again
-        }
-        if (!rc) rc = hip_check(hipGetLastError(), "sweep launch");
+        if (use_graph)
+            rc = hip_check(hipGraphLaunch(h->gexec, st), "graph launch");
+        else
+            rc = sweep_iteration(h, rep, xxd);
+        if (!rc && rep % P.resync == 0) rc = sweep_resync(h);
     }
     const int rs = hip_check(hipStreamSynchronize(st), "sweep");
     if (!rc) rc = rs;
@@ -1343,11 +1426,9 @@ int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)
     kc.on = false;
     if (count > 0) h->last_sweep_ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / count;
     h->iterations += count;
-    if (!rc && dhyp) {
-        rc = hip_check(hipMemcpy(hyp_host, dhyp, sizeof(double) * (size_t)nrec * hw, hipMemcpyDeviceToHost),
+    if (!rc && hyp_host && nrec > 0)
+        rc = hip_check(hipMemcpy(hyp_host, h->hyp, sizeof(double) * (size_t)nrec * hw, hipMemcpyDeviceToHost),
                        "download hyp");
-    }
-    if (dhyp) hipFree(dhyp);
     return rc;
 }
 
