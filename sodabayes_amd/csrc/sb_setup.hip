@@ -1073,18 +1073,10 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
     if (fail_block) *fail_block = 0;
     if (fail_col) *fail_col = 0;
     if (!rc && !failed.empty()) {
-        if (P.precision == SB_PREC_MIXED) {
-            // the batched kernels keep triangular factor tiles and no skip lists
-            const int b = failed[0];
-            if (fail_block) *fail_block = b + 1;
-            if (fail_col) *fail_col = hfail[b];
-            rc = fail(SB_ERR_NOT_PD, "D - X'X is not positive definite in block " + std::to_string(b + 1) +
-                                         " at column " + std::to_string(hfail[b]) +
-                                         "; the pivoted-Cholesky fallback runs on the FP64 path only");
-        } else {
-            progress("pivoted-Cholesky fallback", 0, (int)failed.size());
-            rc = psd_fallback(h, gb, failed, hxxd);
-        }
+        // both paths: the mixed path sweeps these blocks with the FP64 block
+        // kernels (their factors are dense, with skipped SNPs)
+        progress("pivoted-Cholesky fallback", 0, (int)failed.size());
+        rc = psd_fallback(h, gb, failed, hxxd);
     }
     if (!rc && h->Ut0) rc = batch_tiles(h);
     const int rs = hip_check(hipStreamSynchronize(h->stream), "build_aug");

@@ -215,13 +215,14 @@ __device__ __forceinline__ double block_rhs(const Par &P, const double *part, co
     return rx + partA[jk];
 }
 
-// COUNTER mode: one thread per SNP of the block.
+// COUNTER mode: one thread per SNP of the block, for chain `chain` (sc,
+// beta, cls, vara_s, delta and chg point at that chain's state).
 template <int NM>
 __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, const double *part,
                                                      const double *partA, const double *partR,
                                                      const double *f2, const double *sdv,
                                                      const int32_t *grp, const int32_t *snp_of_pos,
-                                                     int p0, int nb, double zz, const int *it, double *beta,
+                                                     int p0, int nb, double zz, const int *it, int chain, double *beta,
                                                      int32_t *cls, double *vara_s, double *delta,
                                                      uint8_t *chg, const uint8_t *skip)
 {
@@ -236,7 +237,7 @@ __global__ __launch_bounds__(NT) void k_draw_counter(Par P, const Scal *sc, cons
         return;
     }
     double rhs = block_rhs(P, part, partA, partR, f2, sdv, p0, jk);
-    PhiloxSrc src(chain_key(P, 0), (uint32_t)it[0], (uint32_t)snp_of_pos[pos], TAG_SNP);
+    PhiloxSrc src(chain_key(P, chain), (uint32_t)it[0], (uint32_t)snp_of_pos[pos], TAG_SNP);
     if (mixm) {
         const SnpDraw d = draw_mix_fast<NM>(P, sc, src, mc, rhs, zz, grp[pos], beta[pos], cls[pos]);
         beta[pos] = d.beta;
@@ -1188,6 +1189,8 @@ int init_xpx(sb_handle *h)
     return hip_check(hipGetLastError(), "xpx");
 }
 
+static void fp64_products(sb_handle *h, int b, int c);
+
 int block_rhs(sb_handle *h, int b, double *out_host)
 {
     if (!h->initialised) return fail(SB_ERR_STATE, "sb_block_rhs before sb_mcmc_init");
@@ -1198,18 +1201,18 @@ int block_rhs(sb_handle *h, int b, double *out_host)
     double *dout = nullptr;
     if (hip_check(hipMalloc(&dout, sizeof(double) * (size_t)nb * P.nchains), "hipMalloc(rhs)")) return SB_ERR_NOMEM;
     int rc = SB_OK;
-    if (P.precision == SB_PREC_MIXED) {
+    if (P.precision == SB_PREC_MIXED && !dense_block(h, b)) {
         h->rstat_nwg = 0;
         rc = batch_block(h, b, 0.0, 0);
         if (!rc) rc = batch_draw(h, b, 0.0, h->sc.delta, reinterpret_cast<uint8_t *>(h->sc.flag), dout);
     } else {
-        int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
-        int ngs = (nb + 4 * spw - 1) / (4 * spw);
-        int nxg = ngs * P.nchunk;
-        k_rhs<<<nxg + (nb + 3) / 4, NT, 0, st>>>(P, h->geno, h->r, h->ra, h->U, h->tri_off[b], dense_block(h, b), p0,
-                                                 nb, spw, ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
-        k_rhs_out<<<(nb + NT - 1) / NT, NT, 0, st>>>(P, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, p0,
-                                                     nb, dout);
+        // FP64 products, chain by chain (the FP64 path, and the mixed path's
+        // pivoted-Cholesky blocks)
+        for (int c = 0; c < P.nchains; c++) {
+            fp64_products(h, b, c);
+            k_rhs_out<<<(nb + NT - 1) / NT, NT, 0, st>>>(P, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv,
+                                                         p0, nb, dout + (int64_t)c * nb);
+        }
     }
     if (!rc) rc = hip_check(hipMemcpyAsync(out_host, dout, sizeof(double) * (size_t)nb * P.nchains,
                                            hipMemcpyDeviceToHost, st),
@@ -1217,6 +1220,89 @@ int block_rhs(sb_handle *h, int b, double *out_host)
     if (!rc) rc = hip_check(hipStreamSynchronize(st), "block rhs");
     hipFree(dout);
     return rc;
+}
+
+// r_a of every chain into the FP32 swizzled panel of the batched products
+// (after an FP64 block update changed r_a outside the batched kernels)
+__global__ void k_ra_panel(Par P, const double *ra, float *ra_sw)
+{
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= P.blocksize) return;
+    ra_sw[rsw_index(i, c, (P.nchains + 15) / 16)] = (float)ra[(int64_t)c * P.blocksize + i];
+}
+
+// The FP64 block products X_s'r + U_s'r_a of chain c into the per-block
+// scratch (part, partA, partR).
+static void fp64_products(sb_handle *h, int b, int c)
+{
+    const Par &P = h->par;
+    KClock &kc = h->kc;
+    hipStream_t st = h->stream;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
+    int ngs = (nb + 4 * spw - 1) / (4 * spw);
+    int nxg = ngs * P.nchunk;
+    int nag = (nb + 3) / 4;
+    const double gbytes = (double)nb * (((double)P.n + 3.0) / 4.0);   // 2-bit genotypes of the block
+    const double ubytes = 8.0 * (double)nb * (nb + 1) / 2.0;           // packed upper factor
+    kc.begin(st);
+    kc.last(KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
+    launch(kc, k_rhs, dim3(nxg + nag), dim3(NT), 0, st, P, h->geno, h->r + (int64_t)c * P.ldr,
+           h->ra + (int64_t)c * P.blocksize, h->U, h->tri_off[b], dense_block(h, b), p0, nb, spw, ngs, nxg,
+           h->sc.part, h->sc.partA, h->sc.partR);
+}
+
+// One block on the FP64 kernels for every chain: products and draws chain by
+// chain, then the ordered changed lists, r and r_a updates of all chains.
+// The FP64 path's only sweep; in a mixed run, the blocks that took the
+// pivoted-Cholesky fallback (dense factor, skipped SNPs), after which the
+// FP32 r_a panel is refreshed and the next block recomputes its r limbs.
+static int fp64_block(sb_handle *h, int b, const std::vector<double> &xxd)
+{
+    const Par &P = h->par;
+    KClock &kc = h->kc;
+    hipStream_t st = h->stream;
+    const int C = P.nchains;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    const int64_t tri0 = h->tri_off[b];
+    uint8_t *chg = reinterpret_cast<uint8_t *>(h->sc.flag);
+    double *delta = h->sc.delta;
+    for (int c = 0; c < C; c++) {
+        fp64_products(h, b, c);
+        const int64_t o = (int64_t)c * P.p;
+        kc.begin(st);
+        kc.last(KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
+        if (P.rng_mode == SB_RNG_REPLAY)
+            launch(kc, k_draw_replay, dim3(1), dim3(NT), 0, st, P, h->scal, h->sc.part, h->sc.partA, h->sc.partR,
+                   h->f2, h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp, h->sc.uu, h->sc.chg, h->beta,
+                   h->cls, h->vara_s, delta, chg, h->skip);
+        else
+            with_nmix(P.nmix, [&](auto nm) {
+                launch(kc, k_draw_counter<decltype(nm)::value>, dim3((nb + NT - 1) / NT), dim3(NT), 0, st, P,
+                       h->scal + c, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp,
+                       h->d_pos_by_id + P.p, p0, nb, xxd[b], h->d_it, c, h->beta + o, h->cls + o, h->vara_s + o,
+                       delta + o, chg + o, h->skip);
+            });
+    }
+    const int nrw = (P.n + 255) / 256;
+    const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
+    kc.begin(st);
+    kc.last(KS_LIST, 25.0 * nb);
+    launch(kc, k_list, dim3(1, 1, C), dim3(NT), 0, st, P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
+           h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, dense_block(h, b) != nullptr ? 1 : 0, kc.extra(KS_UPDATE));
+    kc.begin(st);
+    launch(kc, k_update, dim3(nrw + nra, 1, C), dim3(UW * WAVE), 0, st, P, h->geno, h->U, tri0, dense_block(h, b),
+           p0, nb, nrw, h->u_lst, h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, h->r, h->u_rapart);
+    kc.last(KS_UPDATE, 16.0 * P.n + 16.0 * nb);
+    launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt, h->u_tstart,
+           nb, h->ra);
+    if (P.precision == SB_PREC_MIXED) {
+        k_ra_panel<<<dim3((P.blocksize + NT - 1) / NT, C), NT, 0, st>>>(P, h->ra, h->ra_sw);
+        h->rstat_nwg = 0;   // r moved outside the batched update: fresh statistics and limbs
+        h->q_block = -1;
+    }
+    return hip_check(hipGetLastError(), "FP64 block");
 }
 
 // One sweep iteration (models.f90:129-570, without the resync): mu, the
@@ -1246,52 +1332,13 @@ static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd
     k_mu_post<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal);
     h->rstat_nwg = 0;   // r changed: the batched path recomputes its statistics
     for (int b = 0; b < P.nblocks && !rc; b++) {
-        const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
-        const int64_t tri0 = h->tri_off[b];
-        // X part: ~2048 waves, each reusing its residual registers over spw SNPs
-        int spw = 8 * (int)std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)nb * P.nchunk / 16384));
-        int ngs = (nb + 4 * spw - 1) / (4 * spw);
-        int nxg = ngs * P.nchunk;
-        int nag = (nb + 3) / 4;
-        const double gbytes = (double)nb * (((double)P.n + 3.0) / 4.0);   // 2-bit genotypes of the block
-        const double ubytes = 8.0 * (double)nb * (nb + 1) / 2.0;           // packed upper factor
-        if (mixed) {
+        if (mixed && !dense_block(h, b)) {
             rc = batch_block(h, b, xxd[b], rep);
-            if (rc) break;
-            rc = batch_draw(h, b, xxd[b], delta, chg);
-            if (rc) break;
-            rc = batch_update(h, b);
-            if (rc) break;
-            continue;
+            if (!rc) rc = batch_draw(h, b, xxd[b], delta, chg);
+            if (!rc) rc = batch_update(h, b);
+        } else {
+            rc = fp64_block(h, b, xxd);
         }
-        kc.begin(st);
-        kc.last(KS_RHS, gbytes + ubytes + 8.0 * ((double)P.n + 2.0 * nb));
-        launch(kc, k_rhs, dim3(nxg + nag), dim3(NT), 0, st, P, h->geno, h->r, h->ra, h->U, tri0, dense_block(h, b),
-               p0, nb, spw, ngs, nxg, h->sc.part, h->sc.partA, h->sc.partR);
-        kc.begin(st);
-        kc.last(KS_DRAW, 8.0 * nb * (P.nchunk + 8.0));
-        if (P.rng_mode == SB_RNG_REPLAY)
-            launch(kc, k_draw_replay, dim3(1), dim3(NT), 0, st, P, h->scal, h->sc.part, h->sc.partA, h->sc.partR,
-                   h->f2, h->sdv, h->grp, p0, nb, xxd[b], h->sc.rhs, h->sc.stemp, h->sc.uu, h->sc.chg, h->beta,
-                   h->cls, h->vara_s, delta, chg, h->skip);
-        else
-            with_nmix(P.nmix, [&](auto nm) {
-                launch(kc, k_draw_counter<decltype(nm)::value>, dim3((nb + NT - 1) / NT), dim3(NT), 0, st, P,
-                       h->scal, h->sc.part, h->sc.partA, h->sc.partR, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p,
-                       p0, nb, xxd[b], h->d_it, h->beta, h->cls, h->vara_s, delta, chg, h->skip);
-            });
-        const int nrw = (P.n + 255) / 256;
-        const int nra = ((nb + WAVE - 1) / WAVE) * ((nb + RCH - 1) / RCH);
-        kc.begin(st);
-        kc.last(KS_LIST, 25.0 * nb);
-        launch(kc, k_list, dim3(1, 1, C), dim3(NT), 0, st, P, chg, delta, h->f2, h->sdv, p0, nb, h->u_lst, h->u_dl,
-               h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, dense_block(h, b) != nullptr ? 1 : 0, kc.extra(KS_UPDATE));
-        kc.begin(st);
-        launch(kc, k_update, dim3(nrw + nra, 1, C), dim3(UW * WAVE), 0, st, P, h->geno, h->U, tri0, dense_block(h, b),
-               p0, nb, nrw, h->u_lst, h->u_dl, h->u_cl, h->u_cnt, h->u_kc, h->u_tstart, h->r, h->u_rapart);
-        kc.last(KS_UPDATE, 16.0 * P.n + 16.0 * nb);
-        launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt, h->u_tstart,
-               nb, h->ra);
     }
     if (rc) return rc;
     if (P.regular) {
@@ -1299,7 +1346,8 @@ static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd
         rc = single_sweep(h, rep);
         if (rc) return rc;
     } else if (h->nskipped > 0) {
-        // the FP64 path only (setup refuses the fallback for the mixed path)
+        // the SNPs the pivoted-Cholesky fallback skipped, one by one, every
+        // chain (models.f90:433-495)
         with_nmix(P.nmix, [&](auto nm) {
             k_skipped<decltype(nm)::value><<<C, SKT, 0, st>>>(P, h->scal, h->geno, h->zlut, h->d_skip_list,
                                                               h->nskipped, h->xpx, h->grp, h->d_pos_by_id + P.p,

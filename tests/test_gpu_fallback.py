@@ -6,7 +6,8 @@ The reference fixture pins BayesA (setup and chain: test_gpu_parity); here
 the other methods are checked against the oracle in REPLAY mode (classes
 equal, continuous state within 1e-9 relative), COUNTER mode through the
 residual identity r = y - Z g - mu (1e-11) and the block products against
-numpy FP64 (1e-12), and the mixed path must refuse the fallback.
+numpy FP64 (1e-12); the mixed batched path sweeps the pivoted-Cholesky
+block with the FP64 block kernels (identity, products, posterior).
 """
 import numpy as np
 import pytest
@@ -98,11 +99,76 @@ def test_counter_fallback_keeps_residuals(dshift, method):
     s.close()
 
 
-def test_mixed_path_refuses_fallback(dshift):
-    c, o = case_opts("BayesR")
-    s = make(c, o, dshift, rng="counter", precision="mixed", chains=2)
-    with pytest.raises(_lib.SbError) as ei:
-        s.setup()
-    assert ei.value.code == _lib.SB_ERR_NOT_PD
-    assert "FP64 path only" in str(ei.value)
+@pytest.mark.parametrize("method", ["BayesA", "BayesR"])
+def test_mixed_path_sweeps_fallback_blocks(dshift, method):
+    """The mixed batched path takes the same fallback as the FP64 path (same
+    skipped SNPs and naind), sweeps the pivoted-Cholesky block with the FP64
+    block kernels chain by chain and the skipped SNPs one by one, and keeps
+    r = y - Z g - mu for every chain (to the fixed-point remainders of its
+    batched blocks); the products of every block, the dense one included,
+    match numpy FP64 of the same state."""
+    C = 4
+    c, o = case_opts(method, numit=30)
+    n, p = int(c["nind"]), int(c["nloci"])
+    ref = make(c, o, dshift, rng="counter")
+    ref.setup()
+    s = make(c, o, dshift, rng="counter", precision="mixed", chains=C)
+    s.setup()
+    assert s.nskipped == ref.nskipped > 0 and s.naind == ref.naind
+    assert np.array_equal(s.skipped(), ref.skipped())
+    ref.close()
+    s.init()
+    s.run(30, trace=False)
+    f = orc.freq(c["bed"], n)
+    Z = orc.standardize(c["bed"], n, f)
+    XA = s.get("XA", s.blocksize * p).reshape(s.blocksize, p, order="F")
+    bounds = soda.block_bounds(p, s.blocksize, s.nblocks, s.blockt)
+    worst_r, worst_p = 0.0, 0.0
+    for ch in range(C):
+        g, mu, r = s.get("G", p, ch), s.get("SCALARS", 4, ch)[0], s.get("YADJ", n, ch)
+        want = c["why"] - Z @ g - mu
+        worst_r = max(worst_r, np.linalg.norm(r - want) / np.linalg.norm(want))
+        ra = s.get("YAADJ", s.blocksize, ch)
+        for b, (a, e) in enumerate(bounds):
+            nb = e - a
+            want_b = Z[:, s.permvec[a:e]].T @ r + XA[:nb, a:e].T @ ra[:nb]
+            got_b = s.block_rhs(b)[ch]
+            worst_p = max(worst_p, np.linalg.norm(got_b - want_b) / np.linalg.norm(want_b))
+    print(f"mixed x{C} {method} with a pivoted-Cholesky block: residual identity {worst_r:.2e}, "
+          f"products {worst_p:.2e}")
+    # r carries the fixed-point remainders of the batched blocks' updates (at
+    # most half a quantum per SNP): the bar of the full-size mixed runs
+    assert worst_r < 5e-6 and worst_p < 1e-6
     s.close()
+
+
+def test_mixed_fallback_posterior_matches_fp64(dshift):
+    """Posterior means of mu, sigma2_e, sigma2_g and model size from 16
+    mixed chains with the fallback block against 16 FP64 counter chains of
+    the same job: within 4 combined standard errors of the chain means."""
+    c, o = case_opts("BayesR", numit=1500, burnin=300)
+    n, p = int(c["nind"]), int(c["nloci"])
+    keys = ("mu_store", "vare_store", "vara_store", "included_store")
+
+    def means(rs):
+        return np.array([[float(np.ravel(r[k])[0]) for k in keys] for r in rs])
+
+    s = make(c, o, dshift, rng="counter", precision="mixed", chains=16, seed=101)
+    s.setup()
+    s.init()
+    s.run(trace=False)
+    mixed = means([s.results(k) for k in range(16)])
+    s.close()
+    fp = []
+    for k in range(16):
+        t = make(c, o, dshift, rng="counter", seed=201 + k)
+        t.setup()
+        t.init()
+        t.run(trace=False)
+        fp.append(t.results())
+        t.close()
+    fp64 = means(fp)
+    z = (mixed.mean(0) - fp64.mean(0)) / np.sqrt(mixed.var(0, ddof=1) / 16 + fp64.var(0, ddof=1) / 16)
+    for k, zk, a, b in zip(keys, z, mixed.mean(0), fp64.mean(0)):
+        print(f"fallback {k}: mixed x16 {a:.6g} vs FP64 x16 {b:.6g}  z={zk:.2f}")
+    assert np.all(np.abs(z) < 4.0)
