@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNEL_NAMES = {
     "rhs": "k_rhs (X_s'r + U_s'r_a, FP64)", "draw": "k_draw_counter", "list": "k_list",
     "update": "k_update (r += Z delta, r_a += U delta)", "rquant": "k_rstat + k_quant (r limbs, first block)",
-    "xq": "k_prod (X_s'Q int8 MFMA | U_s'R_a FP32 U, FP64 MFMA)", "ut0": "k_utile<0> (U_s'R_a, FP64 MFMA)",
+    "xq": "k_prod (X_s'Q, int8 MFMA)", "ut0": "k_ut<0> (U_s'R_a, FP32 U on FP64 MFMA)",
     "drawb": "k_draw_b (limb sums + draw)", "dquant": "k_dquant (delta limbs)",
-    "xupd": "k_upd (R += X_s C int8 MFMA | U_s D FP32 U, FP64 MFMA)", "ut1": "k_utile<1> (U_s D, FP64 MFMA)",
+    "xupd": "k_upd (R += X_s C, int8 MFMA)", "ut1": "k_ut<1> (U_s D, FP32 U on FP64 MFMA)",
     "rared": "k_rared_quant (r_a reduction + next block's r limbs)",
 }
 

@@ -54,6 +54,55 @@ __device__ __forceinline__ uint32_t word_of(const uint4 &u, int s)
 }
 __device__ __forceinline__ int slot_of(int t) { return 4 * (t & 3) + (t >> 2); }
 
+// 64-bit values moved across lanes by DPP (two 32-bit halves): CTRL 0xB1 =
+// quad_perm [1,0,3,2] (lane ^ 1), 0x4E = quad_perm [2,3,0,1] (lane ^ 2),
+// 0x124 / 0x128 = row_ror 4 / 8 (rotate right within the 16-lane row)
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v)
+{
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)v, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(v >> 32), (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf,
+                                               false);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int l)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Sum (double) and maximum (unsigned 64-bit) over the 64 lanes, as VALU
+// work: quad and row steps by DPP (every lane of a 16-lane row then holds
+// the row's total), the four row totals read from lanes 0, 16, 32, 48 and
+// combined in that order.  The result is wave-uniform and deterministic.
+__device__ __forceinline__ void wave_sum_max(double &sm, uint64_t &mx)
+{
+#define SB_STEP(CTRL)                                                              \
+    {                                                                              \
+        sm += __longlong_as_double((long long)dpp_u64<CTRL>((uint64_t)__double_as_longlong(sm))); \
+        const uint64_t o = dpp_u64<CTRL>(mx);                                      \
+        mx = o > mx ? o : mx;                                                      \
+    }
+    SB_STEP(0xB1)
+    SB_STEP(0x4E)
+    SB_STEP(0x124)
+    SB_STEP(0x128)
+#undef SB_STEP
+    const uint64_t s0 = readlane_u64((uint64_t)__double_as_longlong(sm), 0);
+    const uint64_t s1 = readlane_u64((uint64_t)__double_as_longlong(sm), 16);
+    const uint64_t s2 = readlane_u64((uint64_t)__double_as_longlong(sm), 32);
+    const uint64_t s3 = readlane_u64((uint64_t)__double_as_longlong(sm), 48);
+    sm = ((__longlong_as_double((long long)s0) + __longlong_as_double((long long)s1)) +
+          __longlong_as_double((long long)s2)) + __longlong_as_double((long long)s3);
+    uint64_t m = readlane_u64(mx, 0);
+    const uint64_t m1 = readlane_u64(mx, 16), m2 = readlane_u64(mx, 32), m3 = readlane_u64(mx, 48);
+    m = m1 > m ? m1 : m;
+    m = m2 > m ? m2 : m;
+    m = m3 > m ? m3 : m;
+    mx = m;
+}
+
 // rows of 16 per wave for a given number of 16-column tiles (register budget)
 constexpr int rows_tiles(int ntile) { return ntile <= 4 ? 4 : (ntile <= 8 ? 2 : 1); }
 
@@ -287,11 +336,102 @@ __device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ 
     }
 }
 
+// The int8 main loop of the LDS-shared products and updates:
+//   acc[it][t] += sum over super-steps S in [S0, S1) of A(it, S) x B(S, t)
+// A(it, S): the wave's genotype tile it of super-step S, one 1 KB load per
+// wave at abase + (toff[it] + S) * 64 (4 words of 16 2-bit genotypes per
+// lane); B(S, .): the super-step's limb slab of NTILE x 4 blocks of 1 KB at
+// B + S * SLAB * 64, shared by the four waves through LDS.
+// Pipelined so that the loads of a super-step are in flight while the
+// previous one computes: the genotype tiles of step S + 1 are loaded at the
+// start of step S, and each wave stages its quarter of slab S + 2 in
+// registers after writing slab S + 1 (loaded one step earlier) to the other
+// half of a two-slab LDS ring; one barrier per super-step publishes it (a
+// barrier with an LDS-only release, so it waits for no global load).  Ends
+// with a barrier: the ring is free.
+template <int NTILE, int IT>
+__device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, const int64_t (&toff)[IT],
+                                              const uint4 *__restrict__ B, int S0, int S1, uint4 *lds,
+                                              v4i (&out)[IT][NTILE])
+{
+    v4i acc[IT][NTILE];
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
+    constexpr int W = NT / WAVE;
+    constexpr int SLAB = NTILE * 4, NB = SLAB / W;
+    static_assert(SLAB % W == 0, "limb blocks per super-step split evenly over the waves");
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (S0 >= S1) {
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; it++)
+#pragma unroll
+            for (int t = 0; t < NTILE; t++) out[it][t] = acc[it][t];
+        return;
+    }
+    const int Sl = S1 - 1;
+    // operands as native 4 x 32-bit vectors (register arrays of HIP's uint4
+    // struct that a loop carries are not always kept in registers)
+    const v4i *Bw = reinterpret_cast<const v4i *>(B) + (int64_t)wave * 64 + lane;   // blocks wave + W j
+    const v4i *Av = reinterpret_cast<const v4i *>(abase);
+    v4i a[IT], an[IT], bs[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)S0 * SLAB + W * j) * 64];
+#pragma unroll
+    for (int it = 0; it < IT; it++) a[it] = Av[(toff[it] + S0) * 64];
+    v4i *Lw = reinterpret_cast<v4i *>(lds) + wave * 64 + lane;
+#pragma unroll
+    for (int j = 0; j < NB; j++) Lw[W * j * 64] = bs[j];
+    {
+        const int Sb = min(S0 + 1, Sl);
+#pragma unroll
+        for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)Sb * SLAB + W * j) * 64];
+    }
+    __syncthreads();
+    // step S: A(S) in a, A(S + 1) loading into an; B(S) in LDS half
+    // (S - S0) & 1, B(S + 1) loading into bs.  Every load is issued (past the
+    // range it re-reads step Sl, an L2 hit), so that the compiler counts the
+    // loads in flight exactly and waits for no more than the operand at hand.
+    for (int S = S0; S < S1; S++) {
+        const int buf = (S - S0) & 1;
+        const int Sa = min(S + 1, Sl), Sb = min(S + 2, Sl);
+#pragma unroll
+        for (int it = 0; it < IT; it++) an[it] = Av[(toff[it] + Sa) * 64];
+        const v4i *Bl = reinterpret_cast<const v4i *>(lds) + buf * SLAB * 64 + lane;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            v4i b[NTILE];
+#pragma unroll
+            for (int t = 0; t < NTILE; t++) b[t] = Bl[(t * 4 + s) * 64];
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const v4i av = unpack_word((uint32_t)a[it][s]);
+#pragma unroll
+                for (int t = 0; t < NTILE; t++)
+                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, b[t], acc[it][t], 0, 0, 0);
+            }
+        }
+        v4i *Ln = Lw + (buf ^ 1) * SLAB * 64;
+#pragma unroll
+        for (int j = 0; j < NB; j++) Ln[W * j * 64] = bs[j];
+#pragma unroll
+        for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)Sb * SLAB + W * j) * 64];
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; it++) a[it] = an[it];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+#pragma unroll
+        for (int t = 0; t < NTILE; t++) out[it][t] = acc[it][t];
+}
+
 // X_s'Q with the limb operand shared through LDS: the four waves split the
 // rows (IT row tiles each; one workgroup = 64 IT SNPs) and walk the chunk's
-// super-steps together; each super-step's limb blocks are fetched once per
-// workgroup by direct global-to-LDS loads, one super-step ahead, into a ring
-// of three buffers.  Chunk partials as in xq_tile.
+// super-steps together (limb_mainloop), each super-step's limb blocks read
+// once per workgroup.  Chunk partials as in xq_tile.
 constexpr int prod_rows(int ntile) { return ntile <= 6 ? 2 : 1; }   // row tiles per wave (LDS path)
 constexpr bool prod_lds(int ntile) { return ntile <= 6; }
 template <int NTILE, int IT>
@@ -301,59 +441,26 @@ __device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ g
 {
     constexpr int W = NT / WAVE;
     constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
-    constexpr int SLAB = NTILE * 4, NB = SLAB / W, NBUF = 3;
+    constexpr int SLAB = NTILE * 4, NBUF = 2;
     constexpr int RED4 = (ROWS * (COLS + 1) + 3) / 4;
     constexpr int LDS4 = NBUF * SLAB * 64 > RED4 ? NBUF * SLAB * 64 : RED4;
-    static_assert(SLAB % W == 0, "limb blocks per super-step split evenly over the waves");
     __shared__ uint4 lds[LDS4];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
     const int r0 = bx * ROWS;
     const int nS = P.ldw / 16;
     const int S0 = by * kss, S1 = min(nS, S0 + kss);
     const int tile0 = (bx * W + wave) * IT;
-    bool valid[IT];
+    // row tiles past the block re-read its last tile: their rows are never written
+    int64_t toff[IT];
 #pragma unroll
-    for (int it = 0; it < IT; it++) valid[it] = tile0 + it < nbt;
-    const uint4 *abase = gsw + (int64_t)tile0 * nS * 64 + lane;
-    auto slab = [&](int S) {
-        uint4 *dst = lds + (S % NBUF) * SLAB * 64;
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-            const int q = wave + W * j;
-            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(Q + ((int64_t)S * SLAB + q) * 64 + lane),
-                                             (__attribute__((address_space(3))) void *)(dst + q * 64), 16, 0, 0);
-        }
-    };
+    for (int it = 0; it < IT; it++) toff[it] = (int64_t)min(tile0 + it, nbt - 1) * nS;
     v4i acc[IT][NTILE];
 #pragma unroll
     for (int it = 0; it < IT; it++)
 #pragma unroll
         for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
-    slab(S0);
-    for (int S = S0; S < S1; S++) {
-        if (S + 1 < S1) slab(S + 1);
-        uint4 a[IT];
-#pragma unroll
-        for (int it = 0; it < IT; it++)
-            a[it] = valid[it] ? abase[((int64_t)it * nS + S) * 64] : make_uint4(0, 0, 0, 0);
-        __syncthreads();   // waits for this wave's loads; then every wave's part of slabs S, S+1 is in LDS
-        const uint4 *B = lds + (S % NBUF) * SLAB * 64 + lane;
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            uint4 b[NTILE];
-#pragma unroll
-            for (int t = 0; t < NTILE; t++) b[t] = B[(t * 4 + s) * 64];
-#pragma unroll
-            for (int it = 0; it < IT; it++) {
-                const v4i av = unpack_word(word_of(a[it], s));
-#pragma unroll
-                for (int t = 0; t < NTILE; t++)
-                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[t]), acc[it][t], 0, 0, 0);
-            }
-        }
-    }
-    __syncthreads();   // the ring is reused for the transpose of the partials
+    limb_mainloop<NTILE, IT>(gsw + lane, toff, Q, S0, S1, lds, acc);
     int(*red)[COLS + 1] = reinterpret_cast<int(*)[COLS + 1]>(lds);
     // acc[it][t][e]: SNP row 16 (IT wave + it) + 4 g + e, column 16 t + l16
 #pragma unroll
@@ -379,15 +486,22 @@ __device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ g
 // R (r_a for mode 0, delta for mode 1) is an FP32 swizzled panel
 // (rsw_index).  A wave owns one 16-row/column output tile; a workgroup four
 // of them over one group of `spant` tiles; waves with no tile of the group on
-// their side of the diagonal write nothing.
-constexpr int UKU = 4;  // U tiles in flight per wave
+// their side of the diagonal write nothing.  The tiles are walked UKU at a
+// time with the next UKU already loading (two register sets): every load
+// address is clamped into the wave's range and issued unconditionally, and
+// the tiles past the range are skipped at the MFMAs (a uniform branch), so
+// that no load is ever waited for before its use.  Its own launch, apart
+// from the int8 products, whose LDS would otherwise cap the occupancy.
+constexpr int UKU = 4;  // U tiles per register set
 template <int MODE, int NTC>
 __device__ __forceinline__ void ut_tile(const Par &P, const float4 *__restrict__ Ut, int nbt, int nb,
                                         const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
                                         int bx, int by)
 {
     const int C = P.nchains;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // the wave index as a scalar: the tile range and every branch below are
+    // then uniform (scalar branches, no exec masking around the loads)
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int l16 = lane & 15, lg = lane >> 4;
     const int O = bx * 4 + wave;
     if (O >= nbt) return;
@@ -402,26 +516,37 @@ __device__ __forceinline__ void ut_tile(const Par &P, const float4 *__restrict__
     v4d acc[NTC];
 #pragma unroll
     for (int t = 0; t < NTC; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int K = K0; K < K1; K += UKU) {
-        float4 a[UKU], r[UKU][NTC];
+    float4 a[2][UKU], r[2][UKU][NTC];
+    auto load = [&](float4 (&av)[UKU], float4 (&rv)[UKU][NTC], int K) {
 #pragma unroll
         for (int u = 0; u < UKU; u++) {
             const int KK = min(K + u, K1 - 1);
             const int I = MODE == 0 ? KK : O, J = MODE == 0 ? O : KK;
-            a[u] = Ut[((int64_t)J * (J + 1) / 2 + I) * 64 + lane];
-            if (K + u >= K1) a[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            av[u] = Ut[((int64_t)J * (J + 1) / 2 + I) * 64 + lane];
 #pragma unroll
-            for (int t = 0; t < NTC; t++) r[u][t] = Rsw[((int64_t)KK * NTC + t) * 64 + lane];
+            for (int t = 0; t < NTC; t++) rv[u][t] = Rsw[((int64_t)KK * NTC + t) * 64 + lane];
         }
+    };
+    auto mma = [&](const float4 (&av)[UKU], const float4 (&rv)[UKU][NTC], int K) {
 #pragma unroll
-        for (int u = 0; u < UKU; u++)
+        for (int u = 0; u < UKU; u++) {
+            if (K + u >= K1) break;
 #pragma unroll
             for (int t = 0; t < NTC; t++) {
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].x, (double)r[u][t].x, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].y, (double)r[u][t].y, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].z, (double)r[u][t].z, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u].w, (double)r[u][t].w, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].x, (double)rv[u][t].x, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].y, (double)rv[u][t].y, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].z, (double)rv[u][t].z, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].w, (double)rv[u][t].w, acc[t], 0, 0, 0);
             }
+        }
+    };
+    load(a[0], r[0], K0);
+    for (int K = K0; K < K1; K += 2 * UKU) {
+        load(a[1], r[1], K + UKU);
+        mma(a[0], r[0], K);
+        if (K + UKU >= K1) break;
+        load(a[0], r[0], K + 2 * UKU);
+        mma(a[1], r[1], K + UKU);
     }
     // D[m][n] of the f64 16x16x4 MFMA: m = lg + 4*v (rows interleaved across
     // the 4 lane groups), n = l16 (chain within tile t)
@@ -432,6 +557,22 @@ __device__ __forceinline__ void ut_tile(const Par &P, const float4 *__restrict__
             const int o = 16 * O + lg + 4 * v, ch = 16 * t + l16;
             if (o < nb && ch < C) part[((int64_t)by * C + ch) * P.blocksize + o] = acc[t][v];
         }
+}
+
+// U products of one block over the (output tile quad, group) pairs
+// id = bx + g1x by, id < npair: workgroup k takes pair k and its mirror
+// npair - 1 - k.  A wave's output tile O = 4 bx + w meets min(16, O + 1 -
+// 16 by) tiles of group by (mode 0); in the mirror, O' = nbt - 1 - O meets
+// the complement, so every wave walks 16 or 17 tiles (when nbt is a multiple
+// of the group) instead of 0 to 16, and the work per CU evens out.
+template <int MODE, int NTC>
+__global__ __launch_bounds__(NT) void k_ut(Par P, const float4 *__restrict__ Ut, int nbt, int nb,
+                                           const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
+                                           int g1x, int npair)
+{
+    const int k = blockIdx.x, m = npair - 1 - k;
+    ut_tile<MODE, NTC>(P, Ut, nbt, nb, Rsw, spant, part, k % g1x, k / g1x);
+    if (m != k) ut_tile<MODE, NTC>(P, Ut, nbt, nb, Rsw, spant, part, m % g1x, m / g1x);
 }
 
 // FP32 U tiles of every block from the packed FP64 triangles: tile (I, J),
@@ -733,14 +874,12 @@ __device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict_
 
 // The same update with the limb operand shared through LDS: the four waves
 // split the rows (IT row tiles each, one workgroup = 64 IT individuals) and
-// walk every super-step together.  Each super-step's limb blocks (NTILE * 4
-// KB) are fetched once per workgroup by direct global-to-LDS loads, one
-// super-step ahead, into a ring of three buffers (a buffer is refilled two
-// steps after it was read, past one barrier); the genotype tiles come
-// straight to registers.  The limb bytes read from L2 drop by the number of
-// waves.  Statistics of the new r go out per 64 rows.
+// walk every super-step together (limb_mainloop): each super-step's limb
+// blocks (NTILE * 4 KB) are read once per workgroup, so the limb bytes read
+// from L2 drop by the number of waves.  Statistics of the new r go out per
+// 64 rows.
 constexpr int upd_rows(int ntile) { return ntile <= 6 ? 2 : 1; }   // row tiles per wave (LDS path)
-constexpr bool upd_lds(int ntile) { return ntile <= 6; }            // ring of 3 slabs fits in LDS
+constexpr bool upd_lds(int ntile) { return ntile <= 6; }            // two slabs fit in LDS
 template <int NTILE, int IT>
 __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__ gt, int nss,
                                          const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
@@ -751,53 +890,22 @@ __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__
     const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
     constexpr int W = NT / WAVE;
     constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
-    constexpr int SLAB = NTILE * 4, NB = SLAB / W, NBUF = 3;
+    constexpr int SLAB = NTILE * 4, NBUF = 2;
     constexpr int RED4 = (ROWS * (COLS + 1) + 3) / 4;
     constexpr int LDS4 = NBUF * SLAB * 64 > RED4 ? NBUF * SLAB * 64 : RED4;
-    static_assert(SLAB % W == 0, "limb blocks per super-step split evenly over the waves");
     __shared__ uint4 lds[LDS4];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
     const int i0 = bx * ROWS;
-    const uint4 *abase = gt + (int64_t)(bx * W + wave) * IT * nss * 64 + lane;
-    auto slab = [&](int ss) {
-        uint4 *dst = lds + (ss % NBUF) * SLAB * 64;
+    int64_t toff[IT];
 #pragma unroll
-        for (int j = 0; j < NB; j++) {
-            const int q = wave + W * j;
-            __builtin_amdgcn_global_load_lds(reinterpret_cast<const void *>(Dq + ((int64_t)ss * SLAB + q) * 64 + lane),
-                                             (__attribute__((address_space(3))) void *)(dst + q * 64),
-                                             16, 0, 0);
-        }
-    };
+    for (int it = 0; it < IT; it++) toff[it] = ((int64_t)(bx * W + wave) * IT + it) * nss;
     v4i acc[IT][NTILE];
 #pragma unroll
     for (int it = 0; it < IT; it++)
 #pragma unroll
         for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
-    slab(0);
-    for (int ss = 0; ss < nss; ss++) {
-        if (ss + 1 < nss) slab(ss + 1);
-        uint4 a[IT];
-#pragma unroll
-        for (int it = 0; it < IT; it++) a[it] = abase[((int64_t)it * nss + ss) * 64];
-        __syncthreads();   // waits for this wave's loads; then every wave's part of slabs ss, ss+1 is in LDS
-        const uint4 *B = lds + (ss % NBUF) * SLAB * 64 + lane;
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            uint4 b[NTILE];
-#pragma unroll
-            for (int t = 0; t < NTILE; t++) b[t] = B[(t * 4 + s) * 64];
-#pragma unroll
-            for (int it = 0; it < IT; it++) {
-                const v4i av = unpack_word(word_of(a[it], s));
-#pragma unroll
-                for (int t = 0; t < NTILE; t++)
-                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[t]), acc[it][t], 0, 0, 0);
-            }
-        }
-    }
-    __syncthreads();   // the ring is reused for the row sums (and first for the K parts)
+    limb_mainloop<NTILE, IT>(gt + lane, toff, Dq, 0, nss, lds, acc);
     kconst_finish(kpre, kpart, nkp, P.nchains, reinterpret_cast<double *>(lds), kc_s);
     __syncthreads();
     int(*red)[COLS + 1] = reinterpret_cast<int(*)[COLS + 1]>(lds);
@@ -809,43 +917,64 @@ __device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__
 #pragma unroll
             for (int e = 0; e < 4; e++) red[16 * (IT * wave + it) + 4 * g + e][16 * t + l16] = acc[it][t][e];
     __syncthreads();
-    // r update, then per (64 rows, chain) max |r| and sum r of the new values
-    // (one wave = one segment of one chain) for the next block's quantisation
+    // r update: wave w takes chains w, w + 4, ... over the workgroup's ROWS
+    // rows (RPL per lane), CB chains at a time with every r load of the batch
+    // issued before the first is used; then per (64 rows, chain) max |r| and
+    // sum r of the new values for the next block's quantisation
+    (void)L;
     const int C = P.nchains;
-    for (int pr = threadIdx.x; pr < ROWS * C; pr += NT) {
-        const int c = pr / ROWS, li = pr - c * ROWS;
-        const int i = i0 + li;
-        double nv = 0.0;
-        if (i < P.n) {
-            long long S = 0;
-            for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
-            double *rc = r + (int64_t)c * P.ldr;
-            nv = rc[i] + (ldexp((double)S, -expo[c]) - kc_s[c]);
-            rc[i] = nv;
-        }
-        double sm = nv;
-        unsigned long long mx = (unsigned long long)__double_as_longlong(fabs(nv));
+    constexpr int RPL = ROWS / WAVE, CB = 8;
+    for (int c0 = wave; c0 < C; c0 += 4 * CB) {
+        double rv[CB][RPL];
 #pragma unroll
-        for (int off = WAVE / 2; off > 0; off >>= 1) {
-            sm += __shfl_xor(sm, off, WAVE);
-            const unsigned long long o = __shfl_xor(mx, off, WAVE);
-            mx = o > mx ? o : mx;
+        for (int cb = 0; cb < CB; cb++) {
+            const int c = c0 + 4 * cb;
+#pragma unroll
+            for (int q = 0; q < RPL; q++) {
+                const int i = i0 + q * WAVE + lane;
+                rv[cb][q] = c < C && i < P.n ? r[(int64_t)c * P.ldr + i] : 0.0;
+            }
         }
-        if (lane == 0) {
-            rsum[(int64_t)c * nwg + i / WAVE] = sm;
-            rmax[(int64_t)c * nwg + i / WAVE] = mx;
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            const int c = c0 + 4 * cb;
+            if (c >= C) break;
+            const double sc = ldexp(1.0, -expo[c]), kc = kc_s[c];   // S * sc == ldexp(S, -e) exactly
+#pragma unroll
+            for (int q = 0; q < RPL; q++) {
+                const int li = q * WAVE + lane, i = i0 + li;
+                const int *rw = red[li] + c * LIMBS;
+                long long S = 0;
+#pragma unroll
+                for (int k = LIMBS - 1; k >= 0; k--) S = S * 256 + (long long)rw[k];
+                double sm = 0.0;
+                uint64_t mx = 0;
+                if (i < P.n) {
+                    const double nv = rv[cb][q] + ((double)S * sc - kc);
+                    r[(int64_t)c * P.ldr + i] = nv;
+                    sm = nv;
+                    mx = (uint64_t)__double_as_longlong(fabs(nv));
+                }
+                // per 64 rows, whatever the rows per workgroup (the tile
+                // shape must not change the statistics)
+                wave_sum_max(sm, mx);
+                if (lane == 0) {
+                    rsum[(int64_t)c * nwg + (i0 >> 6) + q] = sm;
+                    rmax[(int64_t)c * nwg + (i0 >> 6) + q] = mx;
+                }
+            }
         }
     }
 }
 
-// Row tiles per wave of the LDS-shared products: 2 or 4 for up to 3 limb
-// tiles (the limb slab of a super-step is then shared by 128 or 256 rows),
-// 2 for 4 to 6 tiles (register budget); the non-LDS path ignores it.
+// Row tiles per wave of the LDS-shared products: 2 or 4 (the limb slab of a
+// super-step is then shared by 128 or 256 rows; 4 halves the LDS reads per
+// MFMA); the non-LDS path ignores it.
 template <int NTL, class F>
 void with_rows(int rw, F &&f)
 {
     using std::integral_constant;
-    if constexpr (NTL <= 3) {
+    if constexpr (NTL <= 6) {
         if (rw == 4) {
             f(integral_constant<int, 4>{});
             return;
@@ -854,48 +983,31 @@ void with_rows(int rw, F &&f)
     f(integral_constant<int, 2>{});
 }
 
-// Block products of one block in one launch: the X_s'Q workgroups (n0, grid
-// g0x x .) and the U_s'R_a workgroups (n1, grid g1x x .) run side by side.
-template <int NTILE, int IT, int NTC, int RW>
+// X_s'Q of one block (grid g0x row groups x chunks of kss super-steps)
+template <int NTILE, int IT, int RW>
 __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
                                              const uint4 *__restrict__ Q, int kss, long long *__restrict__ Tp,
-                                             int64_t tstride, int g0x, int n0, const float4 *__restrict__ Ut,
-                                             const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
-                                             int g1x, int n1)
+                                             int64_t tstride, int g0x)
 {
-    int kind, local;
-    split_ids(blockIdx.x, n0, n1, kind, local);
-    if (kind == 0) {
-        if constexpr (prod_lds(NTILE))
-            xq_lds<NTILE, RW>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
-        else
-            xq_tile<NTILE, IT>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
-    } else {
-        ut_tile<0, NTC>(P, Ut, nbt, nb, Rsw, spant, part, local % g1x, local / g1x);
-    }
+    const int local = blockIdx.x;
+    if constexpr (prod_lds(NTILE))
+        xq_lds<NTILE, RW>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
+    else
+        xq_tile<NTILE, IT>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
 }
 
-// Updates of one block in one launch: r += X_s C - K (n0 workgroups) beside
-// U_s D (n1 workgroups, grid g1x x .).
-template <int NTILE, int IT, int NTC, int RW>
+// r += X_s C - K of one block (n0 workgroups of rows)
+template <int NTILE, int IT, int RW>
 __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss,
                                             const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
                                             const double *__restrict__ kpart, int nkp, double *__restrict__ r,
-                                            double *__restrict__ rsum, uint64_t *__restrict__ rmax, int n0,
-                                            const float4 *__restrict__ Ut, int nbt, int nb,
-                                            const float4 *__restrict__ Dsw, int spant, double *__restrict__ part,
-                                            int g1x, int n1)
+                                            double *__restrict__ rsum, uint64_t *__restrict__ rmax, int n0)
 {
-    int kind, local;
-    split_ids(blockIdx.x, n0, n1, kind, local);
-    if (kind == 0) {
-        if constexpr (upd_lds(NTILE))
-            xupd_lds<NTILE, RW>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, (int)(P.ldr / WAVE), local);
-        else
-            xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, n0, local);
-    } else {
-        ut_tile<1, NTC>(P, Ut, nbt, nb, Dsw, spant, part, local % g1x, local / g1x);
-    }
+    const int local = blockIdx.x;
+    if constexpr (upd_lds(NTILE))
+        xupd_lds<NTILE, RW>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, (int)(P.ldr / WAVE), local);
+    else
+        xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, n0, local);
 }
 
 // One thread per (SNP, chain): exact limb combination of the chunk
@@ -1006,8 +1118,8 @@ int batch_words(int nb) { return ((nb + 255) / 256) * 16; }
 // workgroups cover 4x the rows (measured best at 4,096 SNPs x 50,000)
 int batch_kss(int ntile) { return prod_lds(ntile) ? 16 : 32; }
 
-// X_s'Q beside U_s'R_a of block b with row tiles per wave rw and chunks of
-// kss super-steps (one launch, through the handle's kernel clock)
+// X_s'Q of block b with row tiles per wave rw and chunks of kss super-steps
+// (one launch, through the handle's kernel clock)
 int prod_launch(sb_handle *h, int b, int rw_sel, int kss)
 {
     const Par &P = h->par;
@@ -1017,48 +1129,72 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int kss)
     const uint4 *Q = reinterpret_cast<const uint4 *>(h->b_Q);
     const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
     const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
-    const int spant = h->rows_per_group / 16, nrg = (nbt + spant - 1) / spant;
-    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut0) + h->tt_off[b] * 64;
     KClock &kc = h->kc;
-    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
-        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
+    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
+        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
         with_rows<NTL>(rw_sel, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
             const int rt = prod_lds(NTL) ? 4 * RW : IT;   // row tiles per workgroup
             const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
-            const int g1x = (nbt + 3) / 4, n1 = g1x * nrg;
-            launch(kc, k_prod<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss,
-                   reinterpret_cast<long long *>(h->b_T),
-                   tstride, g0x, n0, ut, reinterpret_cast<const float4 *>(h->ra_sw), spant, h->b_rapart, g1x, n1);
+            launch(kc, k_prod<NTL, IT, RW>, dim3(n0), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss,
+                   reinterpret_cast<long long *>(h->b_T), tstride, g0x);
         });
     });
 }
 
-// r += X_s C - K beside U_s D of block b with row tiles per wave rw, on the
-// residuals r (the handle's, or a scratch copy while tuning)
+// r += X_s C - K of block b with row tiles per wave rw, on the residuals r
+// (the handle's, or a scratch copy while tuning)
 int upd_launch(sb_handle *h, int b, int rw_sel, double *r)
 {
     const Par &P = h->par;
     hipStream_t st = h->stream;
-    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
     const int C = P.nchains, L = h->limbs;
     const int nsw = batch_words(nb);
     const uint4 *Dq = reinterpret_cast<const uint4 *>(h->b_Dq);
     const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
-    const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
-    const float4 *ut = reinterpret_cast<const float4 *>(h->Ut1) + h->tt_off[b] * 64;
     KClock &kc = h->kc;
-    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto ntc) {
-        constexpr int NTL = decltype(ntl)::value, NTC = decltype(ntc)::value, IT = rows_tiles(NTL);
+    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
+        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
         with_rows<NTL>(rw_sel, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
             const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * RW : 16 * IT));
-            const int g1x = (nbt + 3) / 4, n1 = g1x * ncg;
-            launch(kc, k_upd<NTL, IT, NTC, RW>, dim3(n0 + n1), dim3(NT), 0, st, P, gt, nsw / 16, Dq, L, h->b_expo2,
-                   h->b_kpart, (nb + NT - 1) / NT, r, h->b_rsum, h->b_rmax, n0, ut, nbt, nb, reinterpret_cast<const float4 *>(h->dl_sw),
-                   spant, h->b_uapart, g1x, n1);
+            launch(kc, k_upd<NTL, IT, RW>, dim3(n0), dim3(NT), 0, st, P, gt, nsw / 16, Dq, L, h->b_expo2,
+                   h->b_kpart, (nb + NT - 1) / NT, r, h->b_rsum, h->b_rmax, n0);
         });
     });
+}
+
+// U products of block b: mode 0 U_s'R_a (r_a panel) into the r_a partials of
+// the draw, mode 1 U_s D (delta panel) into the partials of the r_a update
+int ut_launch(sb_handle *h, int b, int mode)
+{
+    const Par &P = h->par;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
+    const int C = P.nchains;
+    const int spant = h->rows_per_group / 16, ngrp = (nbt + spant - 1) / spant;
+    const int g1x = (nbt + 3) / 4;
+    const float4 *ut = reinterpret_cast<const float4 *>(mode == 0 ? h->Ut0 : h->Ut1) + h->tt_off[b] * 64;
+    const float4 *rsw = reinterpret_cast<const float4 *>(mode == 0 ? h->ra_sw : h->dl_sw);
+    double *part = mode == 0 ? h->b_rapart : h->b_uapart;
+    KClock &kc = h->kc;
+    const int ntc = (C + 15) / 16;
+    auto go = [&](auto m, auto t) {
+        const int npair = g1x * ngrp;
+        launch(kc, k_ut<decltype(m)::value, decltype(t)::value>, dim3((npair + 1) / 2), dim3(NT), 0, h->stream, P,
+               ut, nbt, nb, rsw, spant, part, g1x, npair);
+    };
+    using std::integral_constant;
+    auto by_ntc = [&](auto m) {
+        switch (ntc) {
+        case 1: go(m, integral_constant<int, 1>{}); return 0;
+        case 2: go(m, integral_constant<int, 2>{}); return 0;
+        case 3: go(m, integral_constant<int, 3>{}); return 0;
+        case 4: go(m, integral_constant<int, 4>{}); return 0;
+        default: return (int)SB_ERR_ARG;
+        }
+    };
+    return mode == 0 ? by_ntc(integral_constant<int, 0>{}) : by_ntc(integral_constant<int, 1>{});
 }
 
 int batch_block(sb_handle *h, int b, double zz, int rep)
@@ -1087,8 +1223,11 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     }
     h->q_block = -1;
     kc.begin(st);
-    kc.last(KS_XQ, gbytes + cl * n + 8.0 * C * nb + tbytes + 12.0 * C * nb);
+    kc.last(KS_XQ, gbytes + cl * n + 8.0 * C * nb);
     int rc = prod_launch(h, b, h->b_rw, h->kss);
+    kc.begin(st);
+    kc.last(KS_UT0, tbytes + 12.0 * C * nb);
+    if (!rc) rc = ut_launch(h, b, 0);
     if (rc) return fail(rc, "unsupported chain count");
     return hip_check(hipGetLastError(), "batch products");
 }
@@ -1110,8 +1249,11 @@ int batch_update(sb_handle *h, int b)
            h->b_dmax, nwg_d, Dq, h->b_expo2, h->b_carry, h->b_kpart);
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
     kc.begin(st);
-    kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
+    kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n);
     int rc = upd_launch(h, b, h->b_rw, h->r);
+    kc.begin(st);
+    kc.last(KS_UT1, tbytes + 12.0 * C * nb);
+    if (!rc) rc = ut_launch(h, b, 1);
     if (rc) return fail(rc, "unsupported chain count");
     // partials of the new r for the next block: per 64 rows (LDS path) or per workgroup
     h->rstat_nwg = (int)(P.ldr / (upd_lds(h->b_ntile) ? WAVE : 16 * rows_tiles(h->b_ntile)));
@@ -1144,7 +1286,7 @@ int batch_update(sb_handle *h, int b)
 void tune_candidates(int ntile, std::vector<int> &rws, std::vector<int> &ksss)
 {
     rws.assign({2});
-    if (ntile <= 3) rws.push_back(4);
+    if (ntile <= 6) rws.push_back(4);
     if (prod_lds(ntile))
         ksss.assign({8, 12, 16, 25});
     else
