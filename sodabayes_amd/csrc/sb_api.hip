@@ -308,41 +308,42 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     A((void **)&h->d_count, sizeof(int32_t) * (C + 4));
     if (d.precision == SB_PREC_MIXED) {
         h->nwg_r = (int)std::min<int64_t>(64, (d.nind + 255) / 256);
-        int need = (int)((C * h->limbs + 15) / 16);
-        static const int tiles[] = {1, 2, 3, 4, 6, 8, 12};
-        h->b_ntile = 12;
-        for (int t : tiles)
-            if (t >= need) {
-                h->b_ntile = t;
-                break;
-            }
-        h->b_ldq = ldr;
-        h->b_ldt = 16 * h->b_ntile;
-        const int nS = P.ldw / 16;
-        h->kss = sb::batch_kss(h->b_ntile);
-        // the chunk partials are sized for the shortest chunk the tuner tries
-        int kss_min = sb::batch_min_kss(h->b_ntile);
-        if (const char *e = std::getenv("SODABAYES_KSS")) {
-            h->kss = std::max(1, std::atoi(e));
-            kss_min = std::min(kss_min, h->kss);
-            h->tuned = true;
-        }
-        if (const char *e = std::getenv("SODABAYES_ROWS_PER_WAVE")) {
-            h->b_rw = std::atoi(e) == 4 ? 4 : 2;
-            h->tuned = true;
-        }
-        const int nch = (nS + kss_min - 1) / kss_min;
-        const int nwg_d = (int)((bs + 255) / 256);
         const int ntc = (int)((C + 15) / 16);
+        h->b_ntile = h->limbs * ntc;   // limb-major rows: limb k of chain c at row k * 16 ntc + c
+        {
+            int dev = 0, ncu = 0;
+            hipGetDevice(&dev);
+            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+                ncu = 1;
+            h->ncu = ncu;
+        }
+        h->b_ldq = ldr;
+        const int nS = P.ldw / 16;
+        h->b_rw = h->b_ntile <= 6 ? 4 : 2;
+        h->b_nch = sb::prod_chunks(h, h->b_rw, 1);
+        if (const char *e = std::getenv("SODABAYES_ROWS_PER_WAVE")) {
+            h->b_rw = (std::atoi(e) == 4 && h->b_ntile <= 6) ? 4 : 2;
+            h->b_nch = sb::prod_chunks(h, h->b_rw, 1);
+            h->tuned = true;
+        }
+        if (const char *e = std::getenv("SODABAYES_KSS")) {
+            const int k = std::max(1, std::atoi(e));
+            h->b_nch = std::max(1, (nS + k - 1) / k);
+            h->tuned = true;
+        }
+        h->kss = (nS + h->b_nch - 1) / h->b_nch;
+        // the chunk partials are sized for the most chunks the tuner tries
+        const int nch = sb::batch_max_chunks(h);
+        const int nwg_d = (int)((bs + 255) / 256);
         const int64_t rows16 = 16 * ((bs + 15) / 16);
         int nrg = (int)((bs + h->rows_per_group - 1) / h->rows_per_group);
         const int64_t nwg_stat = std::max<int64_t>(h->nwg_r, ldr / 16);  // k_rstat or update workgroups
-        A((void **)&h->b_rsum, sizeof(double) * C * nwg_stat);
+        h->b_nq = (int)((P.ldw + 255) / 256);
         A((void **)&h->b_rmax, sizeof(uint64_t) * C * nwg_stat);
-        A((void **)&h->b_rtot, sizeof(double) * C);
+        A((void **)&h->b_qsum, sizeof(long long) * C * h->b_nq);
         A((void **)&h->b_Q, (size_t)16 * h->b_ntile * h->b_ldq);
         A((void **)&h->b_expo, sizeof(int) * C);
-        A((void **)&h->b_T, sizeof(int) * (size_t)nch * bs * h->b_ldt);
+        A((void **)&h->b_T, sizeof(long long) * (size_t)nch * bs * C);
         A((void **)&h->b_rapart, sizeof(double) * (size_t)nrg * bs * C);
         A((void **)&h->b_uapart, sizeof(double) * (size_t)nrg * bs * C);
         h->b_ldd = (int64_t)16 * sb::batch_words((int)bs);

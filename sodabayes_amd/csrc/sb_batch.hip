@@ -103,50 +103,49 @@ __device__ __forceinline__ void wave_sum_max(double &sm, uint64_t &mx)
     mx = m;
 }
 
-// rows of 16 per wave for a given number of 16-column tiles (register budget)
-constexpr int rows_tiles(int ntile) { return ntile <= 4 ? 4 : (ntile <= 8 ? 2 : 1); }
+// Limb-major operand rows: limb k of chain c is row k * CP + c, CP = 16 *
+// ntc (chains padded to whole 16-column tiles), so NTILE = LIMBS * ntc
+// column tiles and the LIMBS limbs of a chain sit in the same lane of
+// column tiles tc, ntc + tc, 2 ntc + tc.
+constexpr int limb_row(int k, int c, int ntc) { return k * 16 * ntc + c; }
+static_assert(LIMBS == 3, "the products combine three limbs per chain");
 
-// (16-column limb tiles, 16-chain tiles) pairs that 3 limbs per chain give
+// Row tiles per wave of the int8 products: four while the accumulators of
+// NTILE column tiles fit in 96 registers, else two.
+constexpr int prod_rw(int ntile) { return ntile <= 6 ? 4 : 2; }
+
+// (16-column limb tiles, 16-chain tiles) of LIMBS limbs per chain
 template <class F>
 int with_tiles(int ntile, int ntc, F &&f)
 {
     using std::integral_constant;
-#define SB_TILE_CASE(a, b)                                              \
-    if (ntile == a && ntc == b) {                                       \
-        f(integral_constant<int, a>{}, integral_constant<int, b>{});    \
-        return 0;                                                       \
+#define SB_TILE_CASE(b)                                                                   \
+    if (ntc == b && ntile == LIMBS * b) {                                                 \
+        f(integral_constant<int, LIMBS * b>{}, integral_constant<int, b>{});              \
+        return 0;                                                                         \
     }
-    SB_TILE_CASE(1, 1)
-    SB_TILE_CASE(2, 1)
-    SB_TILE_CASE(3, 1)
-    SB_TILE_CASE(4, 2)
-    SB_TILE_CASE(6, 2)
-    SB_TILE_CASE(8, 3)
-    SB_TILE_CASE(12, 3)
-    SB_TILE_CASE(12, 4)
+    SB_TILE_CASE(1)
+    SB_TILE_CASE(2)
+    SB_TILE_CASE(3)
+    SB_TILE_CASE(4)
 #undef SB_TILE_CASE
     return SB_ERR_ARG;
 }
 
 }  // namespace
 
-// per chain, per workgroup: max |r| (as the bit pattern of a non-negative
-// double, which orders like the value) and sum r over [0, n)
-__global__ __launch_bounds__(NT) void k_rstat(Par P, const double *r, double *rsum, uint64_t *rmax)
+// per chain, per workgroup: max |r| over [0, n), as the bit pattern of a
+// non-negative double (which orders like the value)
+__global__ __launch_bounds__(NT) void k_rstat(Par P, const double *r, uint64_t *rmax)
 {
-    __shared__ double lds[NT / WAVE];
     __shared__ unsigned long long mx[NT / WAVE];
     const int c = blockIdx.z;
     r += (int64_t)c * P.ldr;
-    double s = 0.0;
     unsigned long long m = 0;
     for (int i = blockIdx.x * NT + threadIdx.x; i < P.n; i += gridDim.x * NT) {
-        double v = r[i];
-        s += v;
-        unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
+        unsigned long long b = (unsigned long long)__double_as_longlong(fabs(r[i]));
         m = b > m ? b : m;
     }
-    s = block_sum<NT>(s, lds);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(m, off, WAVE);
@@ -156,7 +155,6 @@ __global__ __launch_bounds__(NT) void k_rstat(Par P, const double *r, double *rs
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < NT / WAVE; w++) m = mx[w] > m ? mx[w] : m;
-        rsum[(int64_t)c * gridDim.x + blockIdx.x] = s;
         rmax[(int64_t)c * gridDim.x + blockIdx.x] = m;
     }
 }
@@ -181,159 +179,92 @@ __device__ __forceinline__ int fixed_exponent(unsigned long long m, int L)
 }
 
 // One thread per (16-individual word, chain): balanced radix-256 digits of
-// q = rint(r * 2^e) for limb rows c*L + k, written as MFMA B-operand
-// blocks: Q[((S * NTILE + t) * 4 + s) * 64 + lane] (16 bytes each) holds,
-// for lane = l16 + 16 g, the 16 slots (order of unpack_word) of row
-// 16 t + l16 at word 16 S + 4 g + s.  Every workgroup reduces the nwg
-// partials (max |r|, sum r) of its chain; workgroup 0 stores e, sum r.
-__device__ __forceinline__ void quant_wg(const Par &P, int L, int ntile, const double *r, const double *rsum,
-                                         const uint64_t *rmax, int nwg, uint4 *Q, int *expo, double *rtot,
-                                         int bx, int c)
+// q = rint(r * 2^e) in the limb-major rows limb_row(k, c), written as MFMA
+// B-operand blocks: Q[((S * NTILE + t) * 4 + s) * 64 + lane] (16 bytes
+// each) holds, for lane = l16 + 16 g, the 16 slots (order of unpack_word) of
+// row 16 t + l16 at word 16 S + 4 g + s.  Every workgroup reduces the nwg
+// partial maxima of |r| of its chain to e; workgroup 0 stores e.  Each
+// workgroup also leaves the exact integer sum of its q (qsum), so that the
+// centring term uses sum_i q_i 2^-e, the sum of the very values the
+// products see (qsum[c * nq + bx], nq workgroups per chain).
+__device__ __forceinline__ void quant_wg(const Par &P, int ntile, const double *r, const uint64_t *rmax, int nwg,
+                                         uint4 *Q, int *expo, long long *qsum, int nq, int bx, int c)
 {
     __shared__ int e_s;
-    __shared__ double sl[NT / WAVE];
     __shared__ unsigned long long ml[NT / WAVE];
+    __shared__ long long ql[NT / WAVE];
     unsigned long long m = 0;
-    double sr = 0.0;
     for (int w0 = 0; w0 < nwg; w0 += 4 * NT) {
         unsigned long long v[4];
-        double sv[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int w = w0 + u * NT + threadIdx.x;
             v[u] = w < nwg ? rmax[(int64_t)c * nwg + w] : 0ull;
-            sv[u] = w < nwg ? rsum[(int64_t)c * nwg + w] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            m = v[u] > m ? v[u] : m;
-            sr += sv[u];
-        }
+        for (int u = 0; u < 4; u++) m = v[u] > m ? v[u] : m;
     }
-    sr = wave_sum(sr);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         unsigned long long o = __shfl_xor(m, off, WAVE);
         m = o > m ? o : m;
     }
-    if ((threadIdx.x & 63) == 0) {
-        sl[threadIdx.x >> 6] = sr;
-        ml[threadIdx.x >> 6] = m;
-    }
+    if ((threadIdx.x & 63) == 0) ml[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < NT / WAVE; w++) {
-            sr += sl[w];
-            m = ml[w] > m ? ml[w] : m;
-        }
-        e_s = fixed_exponent(m, L);
-        if (bx == 0) {
-            expo[c] = e_s;
-            rtot[c] = sr;
-        }
+        for (int w = 1; w < NT / WAVE; w++) m = ml[w] > m ? ml[w] : m;
+        e_s = fixed_exponent(m, LIMBS);
+        if (bx == 0) expo[c] = e_s;
     }
     __syncthreads();
     const int e = e_s;
     const int w = bx * NT + threadIdx.x;
-    if (w >= P.ldw) return;
-    const double *rc = r + (int64_t)c * P.ldr + (int64_t)w * 16;
-    int32_t q[16];
+    long long qs = 0;
+    if (w < P.ldw) {
+        const double *rc = r + (int64_t)c * P.ldr + (int64_t)w * 16;
+        int32_t q[16];
 #pragma unroll
-    for (int t = 0; t < 16; t++) q[t] = (int32_t)rint(ldexp(rc[t], e));
-    const int S = w >> 4, g = (w >> 2) & 3, s = w & 3;
-    for (int k = 0; k < L; k++) {
-        uint32_t out[4];
-#pragma unroll
-        for (int m = 0; m < 4; m++) {
-            uint32_t v = 0;
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                int t = 4 * b + m;                     // individual within the word
-                int8_t d = (int8_t)(q[t] & 0xFF);      // balanced digit in [-128, 127]
-                v |= ((uint32_t)(uint8_t)d) << (8 * b);
-            }
-            out[m] = v;
+        for (int t = 0; t < 16; t++) {
+            q[t] = (int32_t)rint(ldexp(rc[t], e));
+            qs += q[t];
         }
+        const int S = w >> 4, g = (w >> 2) & 3, s = w & 3;
+        const int ntc = ntile / LIMBS;
+        for (int k = 0; k < LIMBS; k++) {
+            uint32_t out[4];
 #pragma unroll
-        for (int t = 0; t < 16; t++) q[t] = (q[t] - (int32_t)(int8_t)(q[t] & 0xFF)) >> 8;
-        const int row = c * L + k;
-        Q[(((int64_t)S * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
-            make_uint4(out[0], out[1], out[2], out[3]);
-    }
-}
-
-__global__ __launch_bounds__(NT) void k_quant(Par P, int L, int ntile, const double *r, const double *rsum,
-                                              const uint64_t *rmax, int nwg, uint4 *Q, int *expo, double *rtot)
-{
-    quant_wg(P, L, ntile, r, rsum, rmax, nwg, Q, expo, rtot, blockIdx.x, blockIdx.z);
-}
-
-// Tp[chunk][c][jk] = sum over the chunk's individuals and the limbs k of
-// 256^k x_(p0+jk, i) * Q[c L + k][i]: one exact int64 per (chunk, chain, SNP),
-// the limb columns combined after the exact int32 sums (columns of length
-// blocksize).
-// Workgroup: 16*IT SNPs x all 16*NTILE columns over one chunk of `kss`
-// super-steps (16 words = 256 individuals).  The four waves split the
-// chunk's super-steps; every operand load is one contiguous 1 KB per wave
-// (A tiles of geno_sw, B blocks of Q); the waves are summed exactly in LDS.
-template <int NTILE, int IT>
-__device__ __forceinline__ void xq_tile(const Par &P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                        const uint4 *__restrict__ Q, int kss, long long *__restrict__ Tp,
-                                        int64_t tstride, int bx, int by)
-{
-    constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
-    __shared__ int red[ROWS][COLS + 1];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = lane >> 4, l16 = lane & 15;
-    const int r0 = bx * ROWS;
-    for (int k = threadIdx.x; k < ROWS * (COLS + 1); k += NT) (&red[0][0])[k] = 0;
-    const int nS = P.ldw / 16;
-    const int S0 = by * kss, nss = min(nS, S0 + kss) - S0;
-    const int sb = S0 + wave * nss / 4, se = S0 + (wave + 1) * nss / 4;
-    bool valid[IT];
+            for (int m4 = 0; m4 < 4; m4++) {
+                uint32_t v = 0;
 #pragma unroll
-    for (int it = 0; it < IT; it++) valid[it] = bx * IT + it < nbt;
-    const uint4 *abase = gsw + (int64_t)bx * IT * nS * 64 + lane;
-    v4i acc[IT][NTILE];
-#pragma unroll
-    for (int it = 0; it < IT; it++)
-#pragma unroll
-        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
-    for (int S = sb; S < se; S++) {
-        uint4 a[IT];
-#pragma unroll
-        for (int it = 0; it < IT; it++)
-            a[it] = valid[it] ? abase[((int64_t)it * nS + S) * 64] : make_uint4(0, 0, 0, 0);
-        uint4 b[4][NTILE];
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-            for (int t = 0; t < NTILE; t++) b[s][t] = Q[(((int64_t)S * NTILE + t) * 4 + s) * 64 + lane];
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-            for (int it = 0; it < IT; it++) {
-                const v4i av = unpack_word(word_of(a[it], s));
-#pragma unroll
-                for (int t = 0; t < NTILE; t++)
-                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[s][t]), acc[it][t], 0, 0, 0);
+                for (int b = 0; b < 4; b++) {
+                    int t = 4 * b + m4;                    // individual within the word
+                    int8_t d = (int8_t)(q[t] & 0xFF);      // balanced digit in [-128, 127]
+                    v |= ((uint32_t)(uint8_t)d) << (8 * b);
+                }
+                out[m4] = v;
             }
+#pragma unroll
+            for (int t = 0; t < 16; t++) q[t] = (q[t] - (int32_t)(int8_t)(q[t] & 0xFF)) >> 8;
+            const int row = limb_row(k, c, ntc);
+            Q[(((int64_t)S * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
+                make_uint4(out[0], out[1], out[2], out[3]);
+        }
     }
+    // exact: |q| < 2^23 and at most ldw * 16 terms
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) qs += __shfl_xor(qs, off, WAVE);
+    if ((threadIdx.x & 63) == 0) ql[threadIdx.x >> 6] = qs;
     __syncthreads();
-    // acc[it][t][e]: SNP row 16 it + 4 g + e, column 16 t + l16
-#pragma unroll
-    for (int it = 0; it < IT; it++)
-#pragma unroll
-        for (int t = 0; t < NTILE; t++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                if (acc[it][t][e] != 0) atomicAdd(&red[16 * it + 4 * g + e][16 * t + l16], acc[it][t][e]);
-    __syncthreads();
-    long long *out = Tp + (int64_t)by * tstride;
-    for (int k = threadIdx.x; k < ROWS * P.nchains; k += NT) {
-        const int c = k / ROWS, row = k - c * ROWS;
-        if (r0 + row < nb) out[(int64_t)c * P.blocksize + r0 + row] = limb_sum(red[row] + LIMBS * c);
+    if (threadIdx.x == 0) {
+        for (int wv = 1; wv < NT / WAVE; wv++) qs += ql[wv];
+        qsum[(int64_t)c * nq + bx] = qs;
     }
+}
+
+__global__ __launch_bounds__(NT) void k_quant(Par P, int ntile, const double *r, const uint64_t *rmax, int nwg,
+                                              uint4 *Q, int *expo, long long *qsum)
+{
+    quant_wg(P, ntile, r, rmax, nwg, Q, expo, qsum, gridDim.x, blockIdx.x, blockIdx.z);
 }
 
 // The int8 main loop of the LDS-shared products and updates:
@@ -428,52 +359,51 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
         for (int t = 0; t < NTILE; t++) out[it][t] = acc[it][t];
 }
 
-// X_s'Q with the limb operand shared through LDS: the four waves split the
-// rows (IT row tiles each; one workgroup = 64 IT SNPs) and walk the chunk's
-// super-steps together (limb_mainloop), each super-step's limb blocks read
-// once per workgroup.  Chunk partials as in xq_tile.
-constexpr int prod_rows(int ntile) { return ntile <= 6 ? 2 : 1; }   // row tiles per wave (LDS path)
-constexpr bool prod_lds(int ntile) { return ntile <= 6; }
-template <int NTILE, int IT>
-__device__ __forceinline__ void xq_lds(const Par &P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                       const uint4 *__restrict__ Q, int kss, long long *__restrict__ Tp, int64_t tstride,
-                                       int bx, int by)
+// X_s'Q of one block, by (row group, chunk) workgroups, id = rg + nrg * ch:
+//   Tp[ch][c][jk] = sum over the chunk's individuals i and the limbs k of
+//                   256^k x_(p0+jk, i) q_k(i, c),
+// one exact int64 per (chunk, chain, SNP) (columns of length blocksize).
+// The four waves take RW consecutive SNP tiles each (a workgroup covers
+// 64 RW SNPs) and walk the chunk's super-steps together (limb_mainloop), so
+// each super-step's limb slab is read once per workgroup.  Chunk ch covers
+// super-steps [ch nS / nch, (ch + 1) nS / nch) of 256 individuals.  A lane
+// holds the three limbs of its chains in the same accumulator position of
+// column tiles tc, ntc + tc, 2 ntc + tc and forms the int64 value itself.
+// Tiles past the block re-read its last tile and store nothing.
+template <int NTILE, int RW>
+__global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
+                                             const uint4 *__restrict__ Q, int nch, long long *__restrict__ Tp,
+                                             int64_t tstride, int nrg)
 {
-    constexpr int W = NT / WAVE;
-    constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
-    constexpr int SLAB = NTILE * 4, NBUF = 2;
-    constexpr int RED4 = (ROWS * (COLS + 1) + 3) / 4;
-    constexpr int LDS4 = NBUF * SLAB * 64 > RED4 ? NBUF * SLAB * 64 : RED4;
-    __shared__ uint4 lds[LDS4];
+    constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
+    __shared__ uint4 lds[2 * SLAB * 64];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int r0 = bx * ROWS;
+    const int rg = blockIdx.x % nrg, ch = blockIdx.x / nrg;
     const int nS = P.ldw / 16;
-    const int S0 = by * kss, S1 = min(nS, S0 + kss);
-    const int tile0 = (bx * W + wave) * IT;
-    // row tiles past the block re-read its last tile: their rows are never written
-    int64_t toff[IT];
+    const int S0 = (int)((int64_t)ch * nS / nch), S1 = (int)((int64_t)(ch + 1) * nS / nch);
+    const int tile0 = (rg * W + wave) * RW;
+    int64_t toff[RW];
 #pragma unroll
-    for (int it = 0; it < IT; it++) toff[it] = (int64_t)min(tile0 + it, nbt - 1) * nS;
-    v4i acc[IT][NTILE];
+    for (int it = 0; it < RW; it++) toff[it] = (int64_t)min(tile0 + it, nbt - 1) * nS;
+    v4i acc[RW][NTILE];
+    limb_mainloop<NTILE, RW>(gsw + lane, toff, Q, S0, S1, lds, acc);
+    const int C = P.nchains;
+    long long *out = Tp + (int64_t)ch * tstride;
 #pragma unroll
-    for (int it = 0; it < IT; it++)
+    for (int it = 0; it < RW; it++) {
+        const int jk0 = 16 * (tile0 + it) + 4 * g;
 #pragma unroll
-        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
-    limb_mainloop<NTILE, IT>(gsw + lane, toff, Q, S0, S1, lds, acc);
-    int(*red)[COLS + 1] = reinterpret_cast<int(*)[COLS + 1]>(lds);
-    // acc[it][t][e]: SNP row 16 (IT wave + it) + 4 g + e, column 16 t + l16
+        for (int tc = 0; tc < NTC; tc++) {
+            const int c = 16 * tc + l16;
+            if (c >= C) continue;
+            long long *o = out + (int64_t)c * P.blocksize + jk0;
 #pragma unroll
-    for (int it = 0; it < IT; it++)
-#pragma unroll
-        for (int t = 0; t < NTILE; t++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) red[16 * (IT * wave + it) + 4 * g + e][16 * t + l16] = acc[it][t][e];
-    __syncthreads();
-    long long *out = Tp + (int64_t)by * tstride;
-    for (int k = threadIdx.x; k < ROWS * P.nchains; k += NT) {
-        const int c = k / ROWS, row = k - c * ROWS;
-        if (r0 + row < nb) out[(int64_t)c * P.blocksize + r0 + row] = limb_sum(red[row] + LIMBS * c);
+            for (int e = 0; e < 4; e++)
+                if (jk0 + e < nb)
+                    o[e] = (long long)acc[it][tc][e] + 256LL * acc[it][NTC + tc][e] +
+                           65536LL * acc[it][2 * NTC + tc][e];
+        }
     }
 }
 
@@ -633,13 +563,13 @@ __global__ __launch_bounds__(NT) void k_ra_reduce(Par P, const double *part, int
 // need only the update just done): workgroups [0, nq * C) quantise r,
 // the rest reduce r_a.
 __global__ __launch_bounds__(NT) void k_rared_quant(Par P, const double *part, int spant, int ngroups, int nb,
-                                                    double *ra, float *ra_sw, int nra, int L, int ntile,
-                                                    const double *r, const double *rsum, const uint64_t *rmax,
-                                                    int nwg, uint4 *Q, int *expo, double *rtot, int nq)
+                                                    double *ra, float *ra_sw, int nra, int ntile, const double *r,
+                                                    const uint64_t *rmax, int nwg, uint4 *Q, int *expo,
+                                                    long long *qsum, int nq)
 {
     const int id = blockIdx.x, C = P.nchains;
     if (id < nq * C)
-        quant_wg(P, L, ntile, r, rsum, rmax, nwg, Q, expo, rtot, id % nq, id / nq);
+        quant_wg(P, ntile, r, rmax, nwg, Q, expo, qsum, nq, id % nq, id / nq);
     else
         ra_reduce_wg(P, part, spant, ngroups, nb, ra, ra_sw, (id - nq * C) % nra, (id - nq * C) / nra);
 }
@@ -701,7 +631,7 @@ __global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, 
 // back to carry_j for the SNP's next update, and each workgroup leaves its
 // part of the centring constant K = sum_j 2 f_j c~_j (kpart), formed from
 // the same c~_j, so that r moves by exactly sum_j (x_ij - 2 f_j) c~_j.
-__global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const double *cjv, const double *f2,
+__global__ __launch_bounds__(NT) void k_dquant(Par P, int ntile, const double *cjv, const double *f2,
                                                int p0, int nb, const uint64_t *dmax, int nwg_d, uint4 *Dq,
                                                int *expo, double *carry, double *kpart)
 {
@@ -716,7 +646,7 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
 #pragma unroll
         for (int u = 0; u < 8; u++) m = v[u] > m ? v[u] : m;
     }
-    const int e = fixed_exponent(m, L);
+    const int e = fixed_exponent(m, LIMBS);
     if (jk == 0) expo[c] = e;
     const double cj = jk < nb ? cjv[(int64_t)c * P.blocksize + jk] : 0.0;
     int32_t q = (int32_t)rint(ldexp(cj, e));
@@ -729,14 +659,14 @@ __global__ __launch_bounds__(NT) void k_dquant(Par P, int L, int ntile, const do
     const int t = jk & 15, sw = jk >> 4;
     const int ss = sw >> 4, g = (sw >> 2) & 3, s = sw & 3;
     const int src = (threadIdx.x & (WAVE - 1) & ~15) + (t & 3);
-    for (int k = 0; k < L; k++) {
+    for (int k = 0; k < LIMBS; k++) {
         const int32_t d = (int32_t)(int8_t)(q & 0xFF);   // balanced digit in [-128, 127]
         q = (q - d) >> 8;
         const uint32_t byte = (uint32_t)(uint8_t)d;
         const uint32_t v = __shfl(byte, src, WAVE) | (__shfl(byte, src + 4, WAVE) << 8) |
                            (__shfl(byte, src + 8, WAVE) << 16) | (__shfl(byte, src + 12, WAVE) << 24);
         if (t < 4) {
-            const int row = c * L + k;
+            const int row = limb_row(k, c, ntile / LIMBS);
             reinterpret_cast<uint32_t *>(Dq + (((int64_t)ss * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) +
                                          16 * g)[t] = v;
         }
@@ -786,228 +716,98 @@ __device__ __forceinline__ void kconst_finish(const KPre &k, const double *__res
     }
 }
 
-// r_c += X_s c - K_c for every chain, as the int8 MFMA product of the
+// r_c += X_s c - K_c for every chain: the int8 MFMA product of the
 // individual-major genotype tiles with the limbs of c (exact int32 sums),
-// then the FP64 limb combination.  Workgroup = 16*IT individuals; the four
-// waves split the block's super-steps (16 SNP words) and are summed exactly
-// in LDS; every operand load is one contiguous 1 KB per wave.
-template <int NTILE, int IT>
-__device__ __forceinline__ void xupd_tile(const Par &P, const uint4 *__restrict__ gt, int nss,
-                                          const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
-                                          const double *__restrict__ kpart, int nkp, double *__restrict__ r,
-                                          double *__restrict__ rsum, uint64_t *__restrict__ rmax, int nwg, int bx)
+// then r_c(i) += (sum_k 256^k T_k(i, c)) 2^-e_c - K_c in FP64.  Workgroup wg
+// of ng takes the 16-individual tiles [wg ntr / ng, (wg + 1) ntr / ng) (ntr
+// = ceil(n / 16)); its four waves split them into runs of at most RW and walk
+// every super-step of the block together (limb_mainloop), so each limb slab
+// is read once per workgroup.  ng is a multiple of the CU count, so every CU
+// carries the same share of the rows.  Each lane holds the three limbs of
+// its chains in one accumulator position and updates four consecutive
+// residuals per (tile, chain tile) directly from registers.  Leaves per
+// (chain, workgroup) the maximum |r| of the new values (rmax[c * ng + wg])
+// for the next block's quantisation.
+template <int NTILE, int RW>
+__global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss, int ntr,
+                                            const uint4 *__restrict__ Dq, const int *__restrict__ expo,
+                                            const double *__restrict__ kpart, int nkp, double *__restrict__ r,
+                                            uint64_t *__restrict__ rmax, int ng)
 {
-    constexpr int ROWS = 16 * IT, COLS = 16 * NTILE;
-    __shared__ int red[ROWS][COLS + 1];
+    constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
+    __shared__ uint4 lds[2 * SLAB * 64];   // limb slab ring, then the K-constant parts
     __shared__ double kc_s[64];
-    __shared__ double kp_s[4 * NT];
+    __shared__ unsigned long long mx_s[W][16 * NTC];
     const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = lane >> 4, l16 = lane & 15;
-    const int i0 = bx * ROWS;
-    for (int k = threadIdx.x; k < ROWS * (COLS + 1); k += NT) (&red[0][0])[k] = 0;
-    const int sb = wave * nss / 4, se = (wave + 1) * nss / 4;
-    v4i acc[IT][NTILE];
-#pragma unroll
-    for (int it = 0; it < IT; it++)
-#pragma unroll
-        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
-    const uint4 *abase = gt + (int64_t)bx * IT * nss * 64 + lane;
-    for (int ss = sb; ss < se; ss++) {
-        uint4 a[IT];
-#pragma unroll
-        for (int it = 0; it < IT; it++) a[it] = abase[((int64_t)it * nss + ss) * 64];
-        uint4 b[4][NTILE];
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-            for (int t = 0; t < NTILE; t++) b[s][t] = Dq[(((int64_t)ss * NTILE + t) * 4 + s) * 64 + lane];
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-#pragma unroll
-            for (int it = 0; it < IT; it++) {
-                const v4i av = unpack_word(word_of(a[it], s));
-#pragma unroll
-                for (int t = 0; t < NTILE; t++)
-                    acc[it][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, as_v4i(b[s][t]), acc[it][t], 0, 0, 0);
-            }
-    }
-    kconst_finish(kpre, kpart, nkp, P.nchains, kp_s, kc_s);   // includes a barrier
-    // acc[it][t][e]: individual 16 it + 4 g + e, column 16 t + l16
-#pragma unroll
-    for (int it = 0; it < IT; it++)
-#pragma unroll
-        for (int t = 0; t < NTILE; t++)
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                if (acc[it][t][e] != 0) atomicAdd(&red[16 * it + 4 * g + e][16 * t + l16], acc[it][t][e]);
-    __syncthreads();
-    // r update, then per (workgroup, chain) max |r| and sum r of the new
-    // values over segments of ROWS lanes (one chain per segment) for the
-    // quantisation of the next block (nwg = number of xupd workgroups)
-    const int C = P.nchains;
-    for (int pr = threadIdx.x; pr < ROWS * C; pr += NT) {
-        const int c = pr / ROWS, li = pr - c * ROWS;
-        const int i = i0 + li;
-        double nv = 0.0;
-        if (i < P.n) {
-            long long S = 0;
-            for (int k = L - 1; k >= 0; k--) S = S * 256 + (long long)red[li][c * L + k];
-            double *rc = r + (int64_t)c * P.ldr;
-            nv = rc[i] + (ldexp((double)S, -expo[c]) - kc_s[c]);
-            rc[i] = nv;
-        }
-        double sm = nv;
-        unsigned long long mx = (unsigned long long)__double_as_longlong(fabs(nv));
-#pragma unroll
-        for (int off = ROWS / 2; off > 0; off >>= 1) {
-            sm += __shfl_xor(sm, off, WAVE);
-            const unsigned long long o = __shfl_xor(mx, off, WAVE);
-            mx = o > mx ? o : mx;
-        }
-        if (li == 0) {
-            rsum[(int64_t)c * nwg + bx] = sm;
-            rmax[(int64_t)c * nwg + bx] = mx;
-        }
-    }
-}
-
-// The same update with the limb operand shared through LDS: the four waves
-// split the rows (IT row tiles each, one workgroup = 64 IT individuals) and
-// walk every super-step together (limb_mainloop): each super-step's limb
-// blocks (NTILE * 4 KB) are read once per workgroup, so the limb bytes read
-// from L2 drop by the number of waves.  Statistics of the new r go out per
-// 64 rows.
-constexpr int upd_rows(int ntile) { return ntile <= 6 ? 2 : 1; }   // row tiles per wave (LDS path)
-constexpr bool upd_lds(int ntile) { return ntile <= 6; }            // two slabs fit in LDS
-template <int NTILE, int IT>
-__device__ __forceinline__ void xupd_lds(const Par &P, const uint4 *__restrict__ gt, int nss,
-                                         const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
-                                         const double *__restrict__ kpart, int nkp, double *__restrict__ r,
-                                         double *__restrict__ rsum, uint64_t *__restrict__ rmax, int nwg, int bx)
-{
-    __shared__ double kc_s[64];
-    const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
-    constexpr int W = NT / WAVE;
-    constexpr int ROWS = 16 * IT * W, COLS = 16 * NTILE;
-    constexpr int SLAB = NTILE * 4, NBUF = 2;
-    constexpr int RED4 = (ROWS * (COLS + 1) + 3) / 4;
-    constexpr int LDS4 = NBUF * SLAB * 64 > RED4 ? NBUF * SLAB * 64 : RED4;
-    __shared__ uint4 lds[LDS4];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int i0 = bx * ROWS;
-    int64_t toff[IT];
+    const int wg = blockIdx.x;
+    const int t0 = (int)((int64_t)wg * ntr / ng), T = (int)((int64_t)(wg + 1) * ntr / ng) - t0;
+    const int w0 = t0 + wave * T / W, nt = t0 + (wave + 1) * T / W - w0;   // nt <= RW
+    int64_t toff[RW];
 #pragma unroll
-    for (int it = 0; it < IT; it++) toff[it] = ((int64_t)(bx * W + wave) * IT + it) * nss;
-    v4i acc[IT][NTILE];
-#pragma unroll
-    for (int it = 0; it < IT; it++)
-#pragma unroll
-        for (int t = 0; t < NTILE; t++) acc[it][t] = v4i{0, 0, 0, 0};
-    limb_mainloop<NTILE, IT>(gt + lane, toff, Dq, 0, nss, lds, acc);
+    for (int it = 0; it < RW; it++) toff[it] = (int64_t)min(w0 + it, ntr - 1) * nss;
+    v4i acc[RW][NTILE];
+    limb_mainloop<NTILE, RW>(gt + lane, toff, Dq, 0, nss, lds, acc);
     kconst_finish(kpre, kpart, nkp, P.nchains, reinterpret_cast<double *>(lds), kc_s);
     __syncthreads();
-    int(*red)[COLS + 1] = reinterpret_cast<int(*)[COLS + 1]>(lds);
-    // acc[it][t][e]: individual 16 (IT wave + it) + 4 g + e, column 16 t + l16
-#pragma unroll
-    for (int it = 0; it < IT; it++)
-#pragma unroll
-        for (int t = 0; t < NTILE; t++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) red[16 * (IT * wave + it) + 4 * g + e][16 * t + l16] = acc[it][t][e];
-    __syncthreads();
-    // r update: wave w takes chains w, w + 4, ... over the workgroup's ROWS
-    // rows (RPL per lane), CB chains at a time with every r load of the batch
-    // issued before the first is used; then per (64 rows, chain) max |r| and
-    // sum r of the new values for the next block's quantisation
-    (void)L;
     const int C = P.nchains;
-    constexpr int RPL = ROWS / WAVE, CB = 8;
-    for (int c0 = wave; c0 < C; c0 += 4 * CB) {
-        double rv[CB][RPL];
+    double sc[NTC], kc[NTC];
+    unsigned long long mx[NTC];
 #pragma unroll
-        for (int cb = 0; cb < CB; cb++) {
-            const int c = c0 + 4 * cb;
+    for (int tc = 0; tc < NTC; tc++) {
+        const int c = min(16 * tc + l16, C - 1);
+        sc[tc] = ldexp(1.0, -expo[c]);   // S * sc == ldexp(S, -e) exactly
+        kc[tc] = kc_s[c];
+        mx[tc] = 0;
+    }
 #pragma unroll
-            for (int q = 0; q < RPL; q++) {
-                const int i = i0 + q * WAVE + lane;
-                rv[cb][q] = c < C && i < P.n ? r[(int64_t)c * P.ldr + i] : 0.0;
-            }
+    for (int it = 0; it < RW; it++) {
+        if (it >= nt) break;
+        const int i0 = 16 * (w0 + it) + 4 * g;
+        double2 rv[NTC][2];
+#pragma unroll
+        for (int tc = 0; tc < NTC; tc++) {
+            const double2 *rp = reinterpret_cast<const double2 *>(
+                r + (int64_t)min(16 * tc + l16, C - 1) * P.ldr + i0);
+            rv[tc][0] = rp[0];
+            rv[tc][1] = rp[1];
         }
 #pragma unroll
-        for (int cb = 0; cb < CB; cb++) {
-            const int c = c0 + 4 * cb;
-            if (c >= C) break;
-            const double sc = ldexp(1.0, -expo[c]), kc = kc_s[c];   // S * sc == ldexp(S, -e) exactly
+        for (int tc = 0; tc < NTC; tc++) {
+            const int c = 16 * tc + l16;
+            if (c >= C) continue;
+            double *rp = r + (int64_t)c * P.ldr + i0;
 #pragma unroll
-            for (int q = 0; q < RPL; q++) {
-                const int li = q * WAVE + lane, i = i0 + li;
-                const int *rw = red[li] + c * LIMBS;
-                long long S = 0;
-#pragma unroll
-                for (int k = LIMBS - 1; k >= 0; k--) S = S * 256 + (long long)rw[k];
-                double sm = 0.0;
-                uint64_t mx = 0;
-                if (i < P.n) {
-                    const double nv = rv[cb][q] + ((double)S * sc - kc);
-                    r[(int64_t)c * P.ldr + i] = nv;
-                    sm = nv;
-                    mx = (uint64_t)__double_as_longlong(fabs(nv));
-                }
-                // per 64 rows, whatever the rows per workgroup (the tile
-                // shape must not change the statistics)
-                wave_sum_max(sm, mx);
-                if (lane == 0) {
-                    rsum[(int64_t)c * nwg + (i0 >> 6) + q] = sm;
-                    rmax[(int64_t)c * nwg + (i0 >> 6) + q] = mx;
-                }
+            for (int e = 0; e < 4; e++) {
+                if (i0 + e >= P.n) break;
+                const long long S = (long long)acc[it][tc][e] + 256LL * acc[it][NTC + tc][e] +
+                                    65536LL * acc[it][2 * NTC + tc][e];
+                const double old = e == 0 ? rv[tc][0].x : e == 1 ? rv[tc][0].y : e == 2 ? rv[tc][1].x : rv[tc][1].y;
+                const double nv = old + ((double)S * sc[tc] - kc[tc]);
+                rp[e] = nv;
+                const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(nv));
+                mx[tc] = b > mx[tc] ? b : mx[tc];
             }
         }
     }
-}
-
-// Row tiles per wave of the LDS-shared products: 2 or 4 (the limb slab of a
-// super-step is then shared by 128 or 256 rows; 4 halves the LDS reads per
-// MFMA); the non-LDS path ignores it.
-template <int NTL, class F>
-void with_rows(int rw, F &&f)
-{
-    using std::integral_constant;
-    if constexpr (NTL <= 6) {
-        if (rw == 4) {
-            f(integral_constant<int, 4>{});
-            return;
+    // the four lanes of a chain (g = 0..3), then the four waves
+#pragma unroll
+    for (int tc = 0; tc < NTC; tc++) {
+#pragma unroll
+        for (int off = 16; off < WAVE; off <<= 1) {
+            const unsigned long long o = __shfl_xor(mx[tc], off, WAVE);
+            mx[tc] = o > mx[tc] ? o : mx[tc];
         }
+        if (g == 0) mx_s[wave][16 * tc + l16] = mx[tc];
     }
-    f(integral_constant<int, 2>{});
-}
-
-// X_s'Q of one block (grid g0x row groups x chunks of kss super-steps)
-template <int NTILE, int IT, int RW>
-__global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                             const uint4 *__restrict__ Q, int kss, long long *__restrict__ Tp,
-                                             int64_t tstride, int g0x)
-{
-    const int local = blockIdx.x;
-    if constexpr (prod_lds(NTILE))
-        xq_lds<NTILE, RW>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
-    else
-        xq_tile<NTILE, IT>(P, gsw, nbt, nb, Q, kss, Tp, tstride, local % g0x, local / g0x);
-}
-
-// r += X_s C - K of one block (n0 workgroups of rows)
-template <int NTILE, int IT, int RW>
-__global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss,
-                                            const uint4 *__restrict__ Dq, int L, const int *__restrict__ expo,
-                                            const double *__restrict__ kpart, int nkp, double *__restrict__ r,
-                                            double *__restrict__ rsum, uint64_t *__restrict__ rmax, int n0)
-{
-    const int local = blockIdx.x;
-    if constexpr (upd_lds(NTILE))
-        xupd_lds<NTILE, RW>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, (int)(P.ldr / WAVE), local);
-    else
-        xupd_tile<NTILE, IT>(P, gt, nss, Dq, L, expo, kpart, nkp, r, rsum, rmax, n0, local);
+    __syncthreads();
+    if ((int)threadIdx.x < C) {
+        unsigned long long m = mx_s[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < W; w++) m = mx_s[w][threadIdx.x] > m ? mx_s[w][threadIdx.x] : m;
+        rmax[(int64_t)threadIdx.x * ng + wg] = m;
+    }
 }
 
 // One thread per (SNP, chain): exact limb combination of the chunk
@@ -1021,7 +821,7 @@ constexpr int DCH = 16;  // chunk partials unrolled in k_draw_b
 constexpr int DRG = 16;  // r_a row-group partials unrolled in k_draw_b
 template <int NM>
 __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long long *Tp,
-                                               int64_t tstride, int nch, const int *expo, const double *rtot,
+                                               int64_t tstride, int nch, const int *expo, const long long *qsum, int nq,
                                                const double *rapart, int spant, const double *f2,
                                                const double *sdv, const int32_t *grp,
                                                const int32_t *snp_of_pos, int p0, int nb, double zz,
@@ -1065,14 +865,21 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
         double ra[DRG];
 #pragma unroll
         for (int u = 0; u < DRG; u++) ra[u] = rp[(int64_t)min(u, ng - 1) * gstride];
+        // sum r of the quantised residuals: the exact integer sum of the
+        // quantisation workgroups' sums of q, formed by every wave (one part
+        // per lane, then a butterfly; integer, so in any order)
+        long long Sq = 0;
+        for (int u = threadIdx.x & (WAVE - 1); u < nq; u += WAVE) Sq += qsum[(int64_t)c * nq + u];
         if (mixm && !rhs_out) mix_const(P, sc + c, zz, mc);   // workgroup-uniform
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) Sq += __shfl_xor(Sq, off, WAVE);   // every lane, before any exits
         if (jk >= nb) goto reduce;
         long long S = 0;
 #pragma unroll
         for (int u = 0; u < DCH; u++) S += u < nch ? v[u] : 0;
         for (int ch = DCH; ch < nch; ch++) S += tcol[(int64_t)ch * tstride];   // more chunks than unrolled
-        const double sx = ldexp((double)S, -expo[c]);
-        double rhs = s > 0.0 ? (sx - f2p * rtot[c]) / s : 0.0;
+        const double sx = ldexp((double)S, -expo[c]), rt = ldexp((double)Sq, -expo[c]);
+        double rhs = s > 0.0 ? (sx - f2p * rt) / s : 0.0;
 #pragma unroll
         for (int u = 0; u < DRG; u++) rhs += u < ng ? ra[u] : 0.0;
         for (int g = DRG; g < ng; g++) rhs += rp[(int64_t)g * gstride];
@@ -1114,53 +921,90 @@ reduce:
 // SNP words of a block rounded up to whole super-steps (16 words)
 int batch_words(int nb) { return ((nb + 255) / 256) * 16; }
 
-// super-steps per X_s'Q chunk: shorter chunks for the LDS path, whose
-// workgroups cover 4x the rows (measured best at 4,096 SNPs x 50,000)
-int batch_kss(int ntile) { return prod_lds(ntile) ? 16 : 32; }
+// Row tiles per wave of the int8 products: rw_sel where the accumulators
+// allow (2 or 4), else 2.
+template <int NTL, class F>
+void with_rw(int rw_sel, F &&f)
+{
+    using std::integral_constant;
+    if constexpr (prod_rw(NTL) == 4) {
+        if (rw_sel == 4) {
+            f(integral_constant<int, 4>{});
+            return;
+        }
+    }
+    f(integral_constant<int, 2>{});
+}
 
-// X_s'Q of block b with row tiles per wave rw and chunks of kss super-steps
-// (one launch, through the handle's kernel clock)
-int prod_launch(sb_handle *h, int b, int rw_sel, int kss)
+// Row groups (of 64 rw SNPs) of the handle's full-size block
+int prod_groups(const sb_handle *h, int rw)
+{
+    const int nbt = (h->par.blocksize + 15) / 16;
+    return std::max(1, (nbt + 4 * rw - 1) / (4 * rw));
+}
+
+// X_s'Q chunks for about m workgroups per CU (row groups x chunks), each
+// chunk at least two super-steps when the individuals allow
+int prod_chunks(const sb_handle *h, int rw, int m)
+{
+    const int nS = h->par.ldw / 16, nrg = prod_groups(h, rw);
+    const int want = std::max(1, (m * h->ncu + nrg / 2) / nrg);
+    return std::max(1, std::min(want, std::max(1, nS / 2)));
+}
+
+// Update workgroups for row tiles per wave rw: a multiple of the CU count,
+// at most 4 rw tiles of 16 individuals each, at most one per tile
+int upd_groups(const sb_handle *h, int rw)
+{
+    const int ntr = (h->par.n + 15) / 16;
+    const int per = h->ncu * 4 * rw;
+    return std::max(1, std::min(ntr, h->ncu * ((ntr + per - 1) / per)));
+}
+
+// X_s'Q of block b with row tiles per wave rw and nch chunks (one launch,
+// through the handle's kernel clock)
+int prod_launch(sb_handle *h, int b, int rw_sel, int nch)
 {
     const Par &P = h->par;
     hipStream_t st = h->stream;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
-    const int C = P.nchains, nS = P.ldw / 16;
+    const int C = P.nchains;
     const uint4 *Q = reinterpret_cast<const uint4 *>(h->b_Q);
     const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
     const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
     KClock &kc = h->kc;
     return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
-        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
-        with_rows<NTL>(rw_sel, [&](auto rw) {
+        constexpr int NTL = decltype(ntl)::value;
+        with_rw<NTL>(rw_sel, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
-            const int rt = prod_lds(NTL) ? 4 * RW : IT;   // row tiles per workgroup
-            const int g0x = (nbt + rt - 1) / rt, n0 = g0x * ((nS + kss - 1) / kss);
-            launch(kc, k_prod<NTL, IT, RW>, dim3(n0), dim3(NT), 0, st, P, gsw, nbt, nb, Q, kss,
-                   reinterpret_cast<long long *>(h->b_T), tstride, g0x);
+            const int nrg = (nbt + 4 * RW - 1) / (4 * RW);
+            launch(kc, k_prod<NTL, RW>, dim3(nrg * nch), dim3(NT), 0, st, P, gsw, nbt, nb, Q, nch,
+                   reinterpret_cast<long long *>(h->b_T), tstride, nrg);
         });
     });
 }
 
 // r += X_s C - K of block b with row tiles per wave rw, on the residuals r
-// (the handle's, or a scratch copy while tuning)
-int upd_launch(sb_handle *h, int b, int rw_sel, double *r)
+// (the handle's, or a scratch copy while tuning); returns the number of
+// max |r| partials it leaves per chain through *ng_out
+int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out)
 {
     const Par &P = h->par;
     hipStream_t st = h->stream;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
-    const int C = P.nchains, L = h->limbs;
-    const int nsw = batch_words(nb);
+    const int C = P.nchains;
+    const int nsw = batch_words(nb), ntr = (P.n + 15) / 16;
     const uint4 *Dq = reinterpret_cast<const uint4 *>(h->b_Dq);
     const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
     KClock &kc = h->kc;
     return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
-        constexpr int NTL = decltype(ntl)::value, IT = rows_tiles(NTL);
-        with_rows<NTL>(rw_sel, [&](auto rw) {
+        constexpr int NTL = decltype(ntl)::value;
+        with_rw<NTL>(rw_sel, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
-            const int n0 = (int)(P.ldr / (upd_lds(NTL) ? 64 * RW : 16 * IT));
-            launch(kc, k_upd<NTL, IT, RW>, dim3(n0), dim3(NT), 0, st, P, gt, nsw / 16, Dq, L, h->b_expo2,
-                   h->b_kpart, (nb + NT - 1) / NT, r, h->b_rsum, h->b_rmax, n0);
+            const int ng = upd_groups(h, RW);
+            *ng_out = ng;
+            launch(kc, k_upd<NTL, RW>, dim3(ng), dim3(NT), 0, st, P, gt, nsw / 16, ntr, Dq, h->b_expo2,
+                   h->b_kpart, (nb + NT - 1) / NT, r, h->b_rmax, ng);
         });
     });
 }
@@ -1214,17 +1058,17 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
         // r changed outside the batched update (first block of a sweep)
         kc.begin(st);
         if (h->rstat_nwg == 0) {
-            launch(kc, k_rstat, dim3(nwg_r, 1, C), dim3(NT), 0, st, P, h->r, h->b_rsum, h->b_rmax);
+            launch(kc, k_rstat, dim3(nwg_r, 1, C), dim3(NT), 0, st, P, h->r, h->b_rmax);
             h->rstat_nwg = nwg_r;
         }
         kc.last(KS_RQUANT, 16.0 * C * n + cl * n);
-        launch(kc, k_quant, dim3((P.ldw + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, L, h->b_ntile, h->r, h->b_rsum,
-               h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_rtot);
+        launch(kc, k_quant, dim3(h->b_nq, 1, C), dim3(NT), 0, st, P, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q,
+               h->b_expo, h->b_qsum);
     }
     h->q_block = -1;
     kc.begin(st);
-    kc.last(KS_XQ, gbytes + cl * n + 8.0 * C * nb);
-    int rc = prod_launch(h, b, h->b_rw, h->kss);
+    kc.last(KS_XQ, gbytes + cl * n + 8.0 * C * nb * h->b_nch);
+    int rc = prod_launch(h, b, h->b_rw, h->b_nch);
     kc.begin(st);
     kc.last(KS_UT0, tbytes + 12.0 * C * nb);
     if (!rc) rc = ut_launch(h, b, 0);
@@ -1245,28 +1089,27 @@ int batch_update(sb_handle *h, int b)
     const double tbytes = 4.0 * 256.0 * nbt * (nbt + 1) / 2.0;
     kc.begin(st);
     kc.last(KS_DQUANT, 16.0 * C * nb + cl * nb);
-    launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, L, h->b_ntile, h->b_cj, h->f2, p0, nb,
+    launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, h->b_ntile, h->b_cj, h->f2, p0, nb,
            h->b_dmax, nwg_d, Dq, h->b_expo2, h->b_carry, h->b_kpart);
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
     kc.begin(st);
     kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n);
-    int rc = upd_launch(h, b, h->b_rw, h->r);
+    int ng = 0;
+    int rc = upd_launch(h, b, h->b_rw, h->r, &ng);
     kc.begin(st);
     kc.last(KS_UT1, tbytes + 12.0 * C * nb);
     if (!rc) rc = ut_launch(h, b, 1);
     if (rc) return fail(rc, "unsupported chain count");
-    // partials of the new r for the next block: per 64 rows (LDS path) or per workgroup
-    h->rstat_nwg = (int)(P.ldr / (upd_lds(h->b_ntile) ? WAVE : 16 * rows_tiles(h->b_ntile)));
+    h->rstat_nwg = ng;   // max |r| partials of the new r, per update workgroup
     const int nra = (nb + NT - 1) / NT;
     kc.begin(st);
     if (b + 1 < P.nblocks) {
         // the next block's r limbs in the same launch
-        const int nq = (P.ldw + NT - 1) / NT;
+        const int nq = h->b_nq;
         uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
-        kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 16.0 * C * n + cl * n);
+        kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 8.0 * C * n + cl * n);
         launch(kc, k_rared_quant, dim3((nq + nra) * C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra,
-               h->ra_sw, nra, L, h->b_ntile, h->r, h->b_rsum, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_rtot,
-               nq);
+               h->ra_sw, nra, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_qsum, nq);
         h->q_block = b + 1;
     } else {
         kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0));
@@ -1275,29 +1118,35 @@ int batch_update(sb_handle *h, int b)
     return hip_check(hipGetLastError(), "batch update");
 }
 
-// Tile shape of the batched products: the chunk length of X_s'Q (kss
-// super-steps of 256 individuals) and the row tiles per wave of the
-// LDS-shared X products (2 or 4).  Every shape gives bit-identical results
-// (the limb products are exact int32 sums and the chunk partials are added
-// as 64-bit integers), so the choice affects speed only.  Picked once per
-// handle, before its first sweep, by timing block 0's products (k_prod) and
-// update (k_upd, on a scratch copy of the residuals) for every candidate;
-// SODABAYES_KSS / SODABAYES_ROWS_PER_WAVE fix the shape instead.
-void tune_candidates(int ntile, std::vector<int> &rws, std::vector<int> &ksss)
+// Tile shape of the batched products: the X_s'Q chunk count (about 1, 2 or
+// 3 workgroups per CU) and the row tiles per wave of the int8 products (2
+// or 4).  Every shape gives bit-identical results (the limb products are
+// exact int32 sums and the chunk partials are added as 64-bit integers), so
+// the choice affects speed only.  Picked once per handle, before its first
+// sweep, by timing block 0's products (k_prod) and update (k_upd, on a
+// scratch copy of the residuals) for every candidate; SODABAYES_KSS (super-
+// steps per chunk) / SODABAYES_ROWS_PER_WAVE fix the shape instead.
+void tune_rws(const sb_handle *h, std::vector<int> &rws)
 {
     rws.assign({2});
-    if (ntile <= 6) rws.push_back(4);
-    if (prod_lds(ntile))
-        ksss.assign({8, 12, 16, 25});
-    else
-        ksss.assign({16, 24, 32, 48});
+    if (prod_rw(h->b_ntile) == 4) rws.push_back(4);
+}
+void tune_chunks(const sb_handle *h, int rw, std::vector<int> &nchs)
+{
+    nchs.clear();
+    for (int m = 1; m <= 3; m++) {
+        const int c = prod_chunks(h, rw, m);
+        if (std::find(nchs.begin(), nchs.end(), c) == nchs.end()) nchs.push_back(c);
+    }
 }
 
-int batch_min_kss(int ntile)
+// chunks the partial buffer must hold: the most any candidate uses
+int batch_max_chunks(const sb_handle *h)
 {
-    std::vector<int> rws, ksss;
-    tune_candidates(ntile, rws, ksss);
-    return ksss[0];
+    int mx = 1;
+    for (int rw : {2, 4})
+        for (int m = 1; m <= 3; m++) mx = std::max(mx, prod_chunks(h, rw, m));
+    return std::max(mx, h->b_nch);
 }
 
 int batch_autotune(sb_handle *h)
@@ -1308,8 +1157,6 @@ int batch_autotune(sb_handle *h)
     hipStream_t st = h->stream;
     if (!h->tune_r && dev_alloc(h, (void **)&h->tune_r, sizeof(double) * (size_t)P.ldr * P.nchains))
         return SB_ERR_NOMEM;
-    std::vector<int> rws, ksss;
-    tune_candidates(h->b_ntile, rws, ksss);
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
@@ -1327,15 +1174,18 @@ int batch_autotune(sb_handle *h)
         return ms / REPS;
     };
     double best = 1e30;
-    int brw = h->b_rw, bkss = h->kss, rc = 0;
+    int brw = h->b_rw, bnch = h->b_nch, rc = 0, ng = 0;
+    std::vector<int> rws, nchs;
+    tune_rws(h, rws);
     for (int rw : rws) {
-        const float tu = time_it([&] { rc |= upd_launch(h, 0, rw, h->tune_r); });
-        for (int kss : ksss) {
-            const float tp = time_it([&] { rc |= prod_launch(h, 0, rw, kss); });
+        tune_chunks(h, rw, nchs);
+        const float tu = time_it([&] { rc |= upd_launch(h, 0, rw, h->tune_r, &ng); });
+        for (int nch : nchs) {
+            const float tp = time_it([&] { rc |= prod_launch(h, 0, rw, nch); });
             if (tp + tu < best) {
                 best = tp + tu;
                 brw = rw;
-                bkss = kss;
+                bnch = nch;
             }
         }
     }
@@ -1344,7 +1194,8 @@ int batch_autotune(sb_handle *h)
     hipEventDestroy(e1);
     if (rc) return fail(rc, "unsupported chain count");
     h->b_rw = brw;
-    h->kss = bkss;
+    h->b_nch = bnch;
+    h->kss = (P.ldw / 16 + bnch - 1) / bnch;
     h->tune_us = best * 1e3;
     h->rstat_nwg = 0;   // the trial updates overwrote the residual statistics
     h->q_block = -1;
@@ -1503,15 +1354,14 @@ int batch_draw(sb_handle *h, int b, double zz, double *delta, uint8_t *chg, doub
     const Par &P = h->par;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
     const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
-    const int nS = P.ldw / 16;
-    const int nch = (nS + h->kss - 1) / h->kss;
+    const int nch = h->b_nch;
     const double C = P.nchains;
     h->kc.begin(h->stream);
     // chunk partials (one int64 each), r_a partials, effect state read and written
     h->kc.last(KS_DRAWB, 8.0 * nch * C * nb + 8.0 * C * nb * ((nb + 255) / 256 / 2.0 + 1.0) + 60.0 * C * nb);
     with_nmix(P.nmix, [&](auto nm) {
         launch(h->kc, k_draw_b<decltype(nm)::value>, dim3((nb + NT - 1) / NT, P.nchains), dim3(NT), 0, h->stream,
-               P, h->scal, reinterpret_cast<const long long *>(h->b_T), tstride, nch, h->b_expo, h->b_rtot,
+               P, h->scal, reinterpret_cast<const long long *>(h->b_T), tstride, nch, h->b_expo, h->b_qsum, h->b_nq,
                h->b_rapart, h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, h->d_it, h->beta, h->cls, h->vara_s,
                delta, chg, rhs_out, h->b_dmax, h->b_carry, h->b_cj, h->dl_sw);
     });

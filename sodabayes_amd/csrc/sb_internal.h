@@ -255,22 +255,23 @@ struct sb_handle {
     // mixed-precision batched products (precision SB_PREC_MIXED)
     int limbs = 3;                // int8 limbs per residual (8L-1 fixed-point bits)
     int nwg_r = 0;                // workgroups of the residual statistics (k_rstat)
-    int rstat_nwg = 0;            // valid partials of (max |r|, sum r) in b_rsum / b_rmax (0: stale)
+    int ncu = 1;                  // compute units of the device (grid sizes of the int8 products)
+    int rstat_nwg = 0;            // valid partials of max |r| in b_rmax (0: stale)
     int q_block = -1;             // block whose r limbs the last update already wrote to b_Q (-1: none)
-    double *b_rsum = nullptr;     // nchains * nwg_r partial sums of r
-    uint64_t *b_rmax = nullptr;   // nchains * nwg_r partial max |r| (bit patterns)
-    int8_t *b_Q = nullptr;        // (16 * b_ntile) x ldq int8 limbs, limb-major
+    uint64_t *b_rmax = nullptr;   // nchains x partials of max |r| (bit patterns)
+    int8_t *b_Q = nullptr;        // (16 * b_ntile) x ldq int8 limbs, limb-major rows k * 16 ntc + c
     int64_t b_ldq = 0;
     int *b_expo = nullptr;        // fixed-point exponent per chain
-    double *b_rtot = nullptr;     // sum of r per chain
-    int kss = 32;                 // super-steps (16 genotype words) per MFMA workgroup chunk (batch_kss)
-    int b_rw = 2;                 // row tiles per wave of the LDS-shared products (2 or 4)
+    long long *b_qsum = nullptr;  // nchains x b_nq exact sums of the quantised residuals
+    int b_nq = 1;                 // quantisation workgroups per chain
+    int kss = 16;                 // super-steps (16 genotype words) per X_s'Q chunk, about nS / b_nch
+    int b_nch = 1;                // X_s'Q chunks (of the individuals) per block
+    int b_rw = 4;                 // row tiles per wave of the int8 products (2 or 4)
     bool tuned = false;           // tile shape chosen (batch_autotune, or fixed by the environment)
     double tune_us = 0.0;         // block-0 products + update time of the chosen shape
     double *tune_r = nullptr;     // scratch residuals of the tuning updates
-    int b_ntile = 1;              // 16-column MFMA tiles covering nchains * limbs
-    int *b_T = nullptr;           // chunks x blocksize x b_ldt int32 limb products per individual chunk
-    int b_ldt = 16;
+    int b_ntile = 3;              // 16-column MFMA tiles: limbs x ceil(nchains / 16)
+    int *b_T = nullptr;           // chunks x nchains x blocksize int64 partials of X_s'Q
     int rows_per_group = 256;     // row split of the augmented product
     double *b_rapart = nullptr;   // (blocksize / rows_per_group) x blocksize x nchains
     double *b_uapart = nullptr;   // same shape: partial U_s delta of the update
@@ -345,8 +346,8 @@ int batch_clear_carry(sb_handle *h);
 int layout_stream(sb_handle *h, double *out);
 int batch_transpose(sb_handle *h);
 int batch_words(int nb);
-int batch_kss(int ntile);
-int batch_min_kss(int ntile);
+int prod_chunks(const sb_handle *h, int rw, int m);
+int batch_max_chunks(const sb_handle *h);
 int batch_autotune(sb_handle *h);
 int init_xpx(sb_handle *h);
 // single-site sampler (sb_single.hip)
