@@ -112,7 +112,6 @@ std::vector<Section> state_sections(sb_handle *h, bool heldout)
         {h->indiststore, sizeof(double) * p * (size_t)P.nmix * C},
     };
     if (heldout) v.push_back({h->heldout, ldr * C});
-    if (h->ra_sw) v.push_back({h->ra_sw, sizeof(float) * 16 * ((bs + 15) / 16) * 16 * ((C + 15) / 16)});
     // the mixed path's carried quantisation remainders: r is exact only with them
     if (h->b_carry) v.push_back({h->b_carry, sizeof(double) * p * C});
     return v;
@@ -335,7 +334,6 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         // the chunk partials are sized for the most chunks the tuner tries
         const int nch = sb::batch_max_chunks(h);
         const int nwg_d = (int)((bs + 255) / 256);
-        const int64_t rows16 = 16 * ((bs + 15) / 16);
         int nrg = (int)((bs + h->rows_per_group - 1) / h->rows_per_group);
         const int64_t nwg_stat = std::max<int64_t>(h->nwg_r, ldr / 16);  // k_rstat or update workgroups
         h->b_nq = (int)((P.ldw + 255) / 256);
@@ -353,21 +351,25 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         A((void **)&h->b_kpart, sizeof(double) * C * nwg_d);
         A((void **)&h->b_cj, sizeof(double) * C * bs);
         A((void **)&h->b_carry, sizeof(double) * C * (size_t)d.nloci);
-        A((void **)&h->ra_sw, sizeof(float) * rows16 * 16 * ntc);
-        A((void **)&h->dl_sw, sizeof(float) * rows16 * 16 * ntc);
+        A((void **)&h->b_Raq, (size_t)16 * h->b_ntile * 64 * ((bs + 63) / 64));
+        A((void **)&h->b_expoa, sizeof(int) * C);
         h->gt_off.assign(d.nblocks + 1, 0);
         h->gs_off.assign(d.nblocks + 1, 0);
-        h->tt_off.assign(d.nblocks + 1, 0);
+        h->uq0_off.assign(d.nblocks + 1, 0);
+        h->uq1_off.assign(d.nblocks + 1, 0);
         for (int b = 0; b < d.nblocks; b++) {
             const int64_t nbb = ls[b + 1] - ls[b], nbt = (nbb + 15) / 16;
             h->gt_off[b + 1] = h->gt_off[b] + (ldr / 16) * (sb::batch_words((int)nbb) / 16) * 64;
             h->gs_off[b + 1] = h->gs_off[b] + nbt * nS * 64;
-            h->tt_off[b + 1] = h->tt_off[b] + nbt * (nbt + 1) / 2;
+            h->uq0_off[b + 1] = h->uq0_off[b] + sb::batch_uq_elems((int)nbb, 0);
+            h->uq1_off[b + 1] = h->uq1_off[b] + sb::batch_uq_elems((int)nbb, 1);
         }
         A((void **)&h->geno_t, (size_t)16 * h->gt_off[d.nblocks]);
         A((void **)&h->geno_sw, (size_t)16 * h->gs_off[d.nblocks]);
-        A((void **)&h->Ut0, sizeof(float) * 256 * (size_t)h->tt_off[d.nblocks]);
-        A((void **)&h->Ut1, sizeof(float) * 256 * (size_t)h->tt_off[d.nblocks]);
+        A((void **)&h->Uq0, (size_t)16 * h->uq0_off[d.nblocks]);
+        A((void **)&h->Uq1, (size_t)16 * h->uq1_off[d.nblocks]);
+        A((void **)&h->ue0, sizeof(int) * (size_t)d.nblocks * bs);
+        A((void **)&h->ue1, sizeof(int) * (size_t)d.nblocks * bs);
     }
     if (rc) {
         sb_destroy(h);
@@ -412,9 +414,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     if (h->b_Q) {
         hipMemsetAsync(h->b_Q, 0, (size_t)16 * h->b_ntile * h->b_ldq, st);
         hipMemsetAsync(h->b_Dq, 0, (size_t)16 * h->b_ntile * h->b_ldd, st);
-        const int64_t rows16 = 16 * ((bs + 15) / 16), ntc = (C + 15) / 16;
-        hipMemsetAsync(h->ra_sw, 0, sizeof(float) * rows16 * 16 * ntc, st);
-        hipMemsetAsync(h->dl_sw, 0, sizeof(float) * rows16 * 16 * ntc, st);
+        hipMemsetAsync(h->b_Raq, 0, (size_t)16 * h->b_ntile * 64 * ((bs + 63) / 64), st);
     }
     hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
     hipMemsetAsync(h->skip, 0, (size_t)p + 64, st);

@@ -359,6 +359,72 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
         for (int t = 0; t < NTILE; t++) out[it][t] = acc[it][t];
 }
 
+// One workgroup per chain: r_a(i, c), i < nb, quantised to LIMBS limbs at the
+// chain's exponent (from max |r_a| over the block's rows), as the mode-0 B
+// operand blocks of the U products; rows past nb up to whole 64-row groups
+// are zero.  Coalesced reads (thread t takes rows t, t + 256, ...); the limb
+// bytes of 4,096 rows at a time are assembled in LDS (bytes: 12 KB).
+__device__ __forceinline__ void raq_wg(const Par &P, int ntile, int nb, const double *__restrict__ ra,
+                                       uint4 *__restrict__ Raq, int *__restrict__ expoa, int c, uint8_t *bytes)
+{
+    __shared__ unsigned long long ml[NT / WAVE];
+    __shared__ int e_s;
+    constexpr int CH = 4096;   // rows per LDS pass
+    const int ntc = ntile / LIMBS;
+    const int n64 = 64 * ((nb + 63) / 64);
+    const double *rc = ra + (int64_t)c * P.blocksize;
+    unsigned long long m = 0;
+    for (int i0 = 0; i0 < nb; i0 += 4 * NT) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * NT + (int)threadIdx.x;
+            v[u] = rc[min(i, nb - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v[u]));
+            m = b > m ? b : m;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(m, off, WAVE);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) ml[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / WAVE; w++) m = ml[w] > m ? ml[w] : m;
+        e_s = fixed_exponent(m, LIMBS);
+        expoa[c] = e_s;
+    }
+    __syncthreads();
+    const int e = e_s;
+    for (int base = 0; base < n64; base += CH) {
+        const int len = min(CH, n64 - base);
+        for (int r = threadIdx.x; r < len; r += NT) {
+            const int i = base + r;
+            int32_t q = (int32_t)rint(ldexp(i < nb ? rc[i] : 0.0, e));
+#pragma unroll
+            for (int k = 0; k < LIMBS; k++) {
+                const int32_t d = (int32_t)(int8_t)(q & 0xFF);
+                bytes[k * CH + r] = (uint8_t)d;
+                q = (q - d) >> 8;
+            }
+        }
+        __syncthreads();
+        // blocks (I4, limb k): lane l16 + 16 g holds rows 64 I4 + 16 g .. + 15
+        for (int u = threadIdx.x; u < (len / 64) * 4 * LIMBS; u += NT) {
+            const int k = u % LIMBS, rest = u / LIMBS, g = rest & 3, I4l = rest >> 2;
+            const uint4 w = *reinterpret_cast<const uint4 *>(bytes + k * CH + 64 * I4l + 16 * g);
+            const int row = limb_row(k, c, ntc), I4 = base / 64 + I4l;
+            Raq[((int64_t)I4 * ntile + (row >> 4)) * 64 + (row & 15) + 16 * g] = w;
+        }
+        __syncthreads();
+    }
+}
+
 // X_s'Q of one block, by (row group, chunk) workgroups, id = rg + nrg * ch:
 //   Tp[ch][c][jk] = sum over the chunk's individuals i and the limbs k of
 //                   256^k x_(p0+jk, i) q_k(i, c),
@@ -369,17 +435,26 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
 // super-steps [ch nS / nch, (ch + 1) nS / nch) of 256 individuals.  A lane
 // holds the three limbs of its chains in the same accumulator position of
 // column tiles tc, ntc + tc, 2 ntc + tc and forms the int64 value itself.
-// Tiles past the block re-read its last tile and store nothing.
+// Tiles past the block re-read its last tile and store nothing.  The
+// first nchains workgroups instead quantise r_a for the block's U_s'r_a
+// (raq_wg), which this launch precedes.
 template <int NTILE, int RW>
 __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
                                              const uint4 *__restrict__ Q, int nch, long long *__restrict__ Tp,
-                                             int64_t tstride, int nrg)
+                                             int64_t tstride, int nrg, const double *__restrict__ ra,
+                                             uint4 *__restrict__ Raq, int *__restrict__ expoa)
 {
     constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
+    static_assert(2 * SLAB * 64 * 16 >= LIMBS * 4096, "the r_a limb bytes fit the slab ring");
     __shared__ uint4 lds[2 * SLAB * 64];
+    if ((int)blockIdx.x < P.nchains) {   // the first nchains workgroups: r_a limbs for U_s'r_a
+        raq_wg(P, NTILE, nb, ra, Raq, expoa, blockIdx.x, reinterpret_cast<uint8_t *>(lds));
+        return;
+    }
+    const int id = blockIdx.x - P.nchains;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int rg = blockIdx.x % nrg, ch = blockIdx.x / nrg;
+    const int rg = id % nrg, ch = id / nrg;
     const int nS = P.ldw / 16;
     const int S0 = (int)((int64_t)ch * nS / nch), S1 = (int)((int64_t)(ch + 1) * nS / nch);
     const int tile0 = (rg * W + wave) * RW;
@@ -407,133 +482,253 @@ __global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gs
     }
 }
 
-// Products with a block's upper factor U on FP64 MFMA (v_mfma_f64_16x16x4f64),
-// U held in FP32 as 16 x 16 tiles (I <= J at tile J(J+1)/2 + I) in the operand
-// order of the mode, so that a wave reads a tile as one contiguous 1 KB load
-// (the operands are widened to FP64 in registers; sums are FP64):
-//   mode 0 (rhs):    part[g][c][jk] = sum_{row tiles I of group g, I <= J} U(i, jk) R(i, c)
-//   mode 1 (update): part[g][c][i]  = sum_{col tiles J of group g, J >= I} U(i, j) R(j, c)
-// R (r_a for mode 0, delta for mode 1) is an FP32 swizzled panel
-// (rsw_index).  A wave owns one 16-row/column output tile; a workgroup four
-// of them over one group of `spant` tiles; waves with no tile of the group on
-// their side of the diagonal write nothing.  The tiles are walked UKU at a
-// time with the next UKU already loading (two register sets): every load
-// address is clamped into the wave's range and issued unconditionally, and
-// the tiles past the range are skipped at the MFMAs (a uniform branch), so
-// that no load is ever waited for before its use.  Its own launch, apart
-// from the int8 products, whose LDS would otherwise cap the occupancy.
-constexpr int UKU = 4;  // U tiles per register set
-template <int MODE, int NTC>
-__device__ __forceinline__ void ut_tile(const Par &P, const float4 *__restrict__ Ut, int nbt, int nb,
-                                        const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
-                                        int bx, int by)
+// Products with a block's upper factor U on int8 MFMA, both operands as
+// balanced radix-256 limbs with exact int32 sums:
+//   mode 0 (rhs):    part[g][c][j] = sum over rows i <= j of group g of U(i, j) r_a(i, c)
+//   mode 1 (update): part[g][c][i] = sum over columns j >= i of group g of U(i, j) s_j c~(j, c)
+// U is held as ULIMBS limbs of rint(U 2^e) with one exponent per column j
+// (mode 0) or per row i of U diag(s) (mode 1), set from that column's or
+// row's largest entry (31 bits of it), so every entry keeps its absolute
+// precision 2^-31 max.  r_a has three limbs per chain at one exponent per
+// chain (quantised by the X'Q launch before this one, raq_wg); mode 1
+// multiplies with the limbs of c~ = delta / s + carry that the genotype
+// update applies (k_dquant), which is why its U columns carry s_j.  A group
+// is 256 rows (mode 0) or one super-step of 256 columns (mode 1), as the
+// partial sums of the draw and of the r_a reduction expect.
+//
+// K steps of one MFMA (64 values of the summed index):
+//   mode 0: row group I4 (rows 64 I4 + 16 g + b, byte b of lane group g);
+//   mode 1: (super-step ss, sub-step s): columns 16 (16 ss + 4 g + s) + m + 4 b
+//           at byte 4 m + b (the slot order of the genotype words and Dq).
+// A operand tiles (1 KB, lane l16 + 16 g, 16 bytes), per block:
+//   mode 0: Uq0[((off0(J) + I4) * ULIMBS + ku) * 64 + lane], column j = 16 J + l16;
+//   mode 1: Uq1[(((off1(I) + ss - I / 16) * 4 + s) * ULIMBS + ku) * 64 + lane], row i = 16 I + l16.
+// B operand blocks (limb-major rows, as the genotype products):
+//   mode 0: Raq[(I4 * NTILE + t) * 64 + lane];  mode 1: Dq[((ss * NTILE + t) * 4 + s) * 64 + lane].
+constexpr int ULIMBS = 4;   // limbs of U (31 bits)
+__host__ __device__ __forceinline__ int64_t uq_off0(int J)
 {
-    const int C = P.nchains;
-    // the wave index as a scalar: the tile range and every branch below are
-    // then uniform (scalar branches, no exec masking around the loads)
+    const int64_t a = J >> 2, r = J & 3;   // sum over J' < J of (J' / 4 + 1)
+    return (a + 1) * (2 * a + r);
+}
+__host__ __device__ __forceinline__ int64_t uq_off1(int I, int nss)
+{
+    const int64_t q = I >> 4, r = I & 15;  // sum over I' < I of (nss - I' / 16)
+    return (int64_t)I * nss - (8 * q * (q - 1) + r * q);
+}
+
+// U products of one block over (quad, group) pairs, workgroup id = pair:
+//   mode 0: quad bx = output columns J = 4 bx + w (wave w), group by of
+//           rows: K steps I4 in [4 by, min(4 by + 4, bx + 1));
+//   mode 1: quad bx = output rows I = 4 bx + w, group ss (one super-step):
+//           K steps (ss, s), s = 0..3, ss >= bx / 4.
+// A pair has at most four K steps: every wave loads its ULIMBS A tiles of
+// all of them at once, the four waves load the steps' B blocks into LDS
+// together, and one barrier publishes them.  Waves past the block still
+// load and meet the barrier, but do no MFMA and store nothing.
+template <int MODE, int NTILE>
+__global__ __launch_bounds__(NT) void k_uq(Par P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
+                                           const uint4 *__restrict__ Bq, const int *__restrict__ ue,
+                                           const int *__restrict__ ce, double *__restrict__ part)
+{
+    constexpr int W = NT / WAVE, NTC = NTILE / LIMBS;
+    __shared__ uint4 lds[4 * NTILE * 64];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int l16 = lane & 15, lg = lane >> 4;
-    const int O = bx * 4 + wave;
-    if (O >= nbt) return;
-    int K0 = by * spant, K1 = min(nbt, K0 + spant);
+    const int g = lane >> 4, l16 = lane & 15;
+    const int k = blockIdx.x;
+    int bx, gi, k0, k1;
     if (MODE == 0) {
-        if (K0 > O) return;
-        K1 = min(K1, O + 1);
+        // pairs of quad bx: bx / 4 + 1 groups; uq_off0(bx) pairs before it
+        bx = (int)sqrtf(2.0f * (float)k);
+        while (bx > 0 && uq_off0(bx) > k) bx--;
+        while (uq_off0(bx + 1) <= k) bx++;
+        gi = (int)(k - uq_off0(bx));
+        k0 = 4 * gi;
+        k1 = min(4 * gi + 4, bx + 1);
     } else {
-        if (K1 <= O) return;
-        K0 = max(K0, O);
+        // pairs of quad bx: nss - bx / 4 super-steps
+        auto offq = [&](int x) { return (int64_t)x * (nss + 1) - uq_off0(x); };
+        bx = 0;
+        while (offq(bx + 1) <= k) bx++;
+        gi = bx / 4 + (int)(k - offq(bx));   // the super-step
+        k0 = 4 * gi;
+        k1 = 4 * gi + 4;
     }
-    v4d acc[NTC];
-#pragma unroll
-    for (int t = 0; t < NTC; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    float4 a[2][UKU], r[2][UKU][NTC];
-    auto load = [&](float4 (&av)[UKU], float4 (&rv)[UKU][NTC], int K) {
-#pragma unroll
-        for (int u = 0; u < UKU; u++) {
-            const int KK = min(K + u, K1 - 1);
-            const int I = MODE == 0 ? KK : O, J = MODE == 0 ? O : KK;
-            av[u] = Ut[((int64_t)J * (J + 1) / 2 + I) * 64 + lane];
-#pragma unroll
-            for (int t = 0; t < NTC; t++) rv[u][t] = Rsw[((int64_t)KK * NTC + t) * 64 + lane];
-        }
+    const int O = 4 * bx + wave;      // the wave's output tile (J or I)
+    const bool live = O < nbt;
+    const int Oc = min(O, nbt - 1);
+    // A tile of K step kk, limb ku
+    auto aidx = [&](int kk, int ku) -> int64_t {
+        if (MODE == 0) return ((uq_off0(Oc) + kk) * ULIMBS + ku) * 64 + lane;
+        return (((uq_off1(Oc, nss) + (kk >> 2) - Oc / 16) * 4 + (kk & 3)) * ULIMBS + ku) * 64 + lane;
     };
-    auto mma = [&](const float4 (&av)[UKU], const float4 (&rv)[UKU][NTC], int K) {
-#pragma unroll
-        for (int u = 0; u < UKU; u++) {
-            if (K + u >= K1) break;
-#pragma unroll
-            for (int t = 0; t < NTC; t++) {
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].x, (double)rv[u][t].x, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].y, (double)rv[u][t].y, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].z, (double)rv[u][t].z, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[u].w, (double)rv[u][t].w, acc[t], 0, 0, 0);
-            }
-        }
+    // B block t of K step kk
+    auto bidx = [&](int kk, int t) -> int64_t {
+        if (MODE == 0) return ((int64_t)kk * NTILE + t) * 64 + lane;
+        return (((int64_t)(kk >> 2) * NTILE + t) * 4 + (kk & 3)) * 64 + lane;
     };
-    load(a[0], r[0], K0);
-    for (int K = K0; K < K1; K += 2 * UKU) {
-        load(a[1], r[1], K + UKU);
-        mma(a[0], r[0], K);
-        if (K + UKU >= K1) break;
-        load(a[0], r[0], K + 2 * UKU);
-        mma(a[1], r[1], K + UKU);
+    // every load of the pair in flight at once: the wave's A tiles of all
+    // (at most four) K steps in registers, the B blocks of all steps in LDS
+    // (each wave loads every fourth block), one barrier
+    constexpr int KS = 4;
+    const v4i *Av = reinterpret_cast<const v4i *>(Ua);
+    const v4i *Bv = reinterpret_cast<const v4i *>(Bq);
+    v4i *L = reinterpret_cast<v4i *>(lds);
+    v4i a[KS][ULIMBS];
+#pragma unroll
+    for (int s = 0; s < KS; s++)
+#pragma unroll
+        for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(min(k0 + s, k1 - 1), u)];
+    constexpr int NBQ = (KS * NTILE + W - 1) / W;   // B blocks per wave, all loads before any store
+    v4i bq[NBQ];
+#pragma unroll
+    for (int j = 0; j < NBQ; j++) {
+        const int q = wave + W * j, s = min(q / NTILE, k1 - k0 - 1), t = q % NTILE;
+        bq[j] = Bv[bidx(k0 + s, t)];
     }
-    // D[m][n] of the f64 16x16x4 MFMA: m = lg + 4*v (rows interleaved across
-    // the 4 lane groups), n = l16 (chain within tile t)
 #pragma unroll
-    for (int t = 0; t < NTC; t++)
+    for (int j = 0; j < NBQ; j++) {
+        const int q = wave + W * j;
+        if (q < (k1 - k0) * NTILE) L[q * 64 + lane] = bq[j];
+    }
+    // one accumulator per chain tile and weight 256^(ku + kr): the limb
+    // products of equal weight are summed exactly in int32 (at most three
+    // products of 64 terms per K step, four K steps)
+    constexpr int NWT = ULIMBS + LIMBS - 1;
+    v4i acc[NWT][NTC];
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int o = 16 * O + lg + 4 * v, ch = 16 * t + l16;
-            if (o < nb && ch < C) part[((int64_t)by * C + ch) * P.blocksize + o] = acc[t][v];
+    for (int w = 0; w < NWT; w++)
+#pragma unroll
+        for (int tc = 0; tc < NTC; tc++) acc[w][tc] = v4i{0, 0, 0, 0};
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            if (k0 + s >= k1) break;
+            const v4i *Lb = L + s * NTILE * 64 + lane;
+#pragma unroll
+            for (int kr = 0; kr < LIMBS; kr++)
+#pragma unroll
+                for (int tc = 0; tc < NTC; tc++) {
+                    const v4i bt = Lb[(kr * NTC + tc) * 64];
+#pragma unroll
+                    for (int u = 0; u < ULIMBS; u++)
+                        acc[u + kr][tc] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[s][u], bt, acc[u + kr][tc], 0, 0, 0);
+                }
         }
+    }
+    if (!live) return;
+    // D[m][n] of the 16x16 MFMA: m = 4 g + e (column j in mode 0, row i in
+    // mode 1), n = l16 (chain 16 tc + l16); the weights combined in FP64
+    // (Horner from the top weight), then scaled by 2^-(e_U + e_chain).
+    const int C = P.nchains;
+#pragma unroll
+    for (int tc = 0; tc < NTC; tc++) {
+        const int c = 16 * tc + l16;
+        if (c >= C) continue;
+        const int ec = ce[c];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int o = 16 * O + 4 * g + e;
+            if (o >= nb) continue;
+            double S = (double)acc[NWT - 1][tc][e];
+#pragma unroll
+            for (int w = NWT - 2; w >= 0; w--) S = S * 256.0 + (double)acc[w][tc][e];
+            part[((int64_t)gi * C + c) * P.blocksize + o] = ldexp(S, -(ue[o] + ec));
+        }
+    }
 }
 
-// U products of one block over the (output tile quad, group) pairs
-// id = bx + g1x by, id < npair: workgroup k takes pair k and its mirror
-// npair - 1 - k.  A wave's output tile O = 4 bx + w meets min(16, O + 1 -
-// 16 by) tiles of group by (mode 0); in the mirror, O' = nbt - 1 - O meets
-// the complement, so every wave walks 16 or 17 tiles (when nbt is a multiple
-// of the group) instead of 0 to 16, and the work per CU evens out.
-template <int MODE, int NTC>
-__global__ __launch_bounds__(NT) void k_ut(Par P, const float4 *__restrict__ Ut, int nbt, int nb,
-                                           const float4 *__restrict__ Rsw, int spant, double *__restrict__ part,
-                                           int g1x, int npair)
+// Limb exponents of U, per block: ue0[j] from max over i <= j of |U(i, j)|,
+// ue1[i] from max over j >= i of |U(i, j) s_j| (ULIMBS limbs, 31 bits).  One
+// thread per (index k, block): column k and row k of the packed triangle.
+__global__ __launch_bounds__(NT) void k_uexp(const double *U, const int64_t *tri_off, const int32_t *ls,
+                                             const double *sdv, int bs, int *ue0, int *ue1)
 {
-    const int k = blockIdx.x, m = npair - 1 - k;
-    ut_tile<MODE, NTC>(P, Ut, nbt, nb, Rsw, spant, part, k % g1x, k / g1x);
-    if (m != k) ut_tile<MODE, NTC>(P, Ut, nbt, nb, Rsw, spant, part, m % g1x, m / g1x);
-}
-
-// FP32 U tiles of every block from the packed FP64 triangles: tile (I, J),
-// element th = lane * 4 + s (lane = l16 + 16 lg) holds
-//   Ut0: U(16 I + 4 lg + s, 16 J + l16)     Ut1: U(16 I + l16, 16 J + 4 lg + s)
-// (zero below the diagonal and beyond the block).
-__global__ __launch_bounds__(256) void k_pack_tiles(const double *U, const int64_t *tri_off, const int32_t *ls,
-                                                    const int64_t *tt_off, float *Ut0, float *Ut1)
-{
-    const int b = blockIdx.y, nb = ls[b + 1] - ls[b], nbt = (nb + 15) / 16;
-    const int64_t T = blockIdx.x;
-    if (T >= (int64_t)nbt * (nbt + 1) / 2) return;
-    int J = (int)((sqrt(8.0 * (double)T + 1.0) - 1.0) * 0.5);
-    while ((int64_t)(J + 1) * (J + 2) / 2 <= T) J++;
-    while ((int64_t)J * (J + 1) / 2 > T) J--;
-    const int I = (int)(T - (int64_t)J * (J + 1) / 2);
-    const int th = threadIdx.x, s = th & 3, lane = th >> 2;
-    const int l16 = lane & 15, lg = lane >> 4;
+    const int b = blockIdx.y, p0 = ls[b], nb = ls[b + 1] - p0;
+    const int k = blockIdx.x * NT + threadIdx.x;
+    if (k >= nb) return;
     const double *Ub = U + tri_off[b];
+    unsigned long long mc = 0, mr = 0;
+    const double *col = Ub + (int64_t)k * (k + 1) / 2;
+    for (int i = 0; i <= k; i++) {
+        const unsigned long long u = (unsigned long long)__double_as_longlong(fabs(col[i]));
+        mc = u > mc ? u : mc;
+    }
+    for (int j = k; j < nb; j++) {
+        const double v = Ub[(int64_t)j * (j + 1) / 2 + k] * sdv[p0 + j];
+        const unsigned long long u = (unsigned long long)__double_as_longlong(fabs(v));
+        mr = u > mr ? u : mr;
+    }
+    ue0[(int64_t)b * bs + k] = fixed_exponent(mc, ULIMBS);
+    ue1[(int64_t)b * bs + k] = fixed_exponent(mr, ULIMBS);
+}
+
+// The ULIMBS limb tiles of one block's U: one thread per (tile entry, lane).
+// Mode 0 entries (J, I4), mode 1 entries (I, ss, s); see k_uq for the byte
+// order.  Entries past the triangle or the block are zero.
+__global__ __launch_bounds__(NT) void k_pack_uq(const double *U, int64_t tri0, int nb, int nbt, int nss, int p0,
+                                                const double *sdv, const int *ue0, const int *ue1, int mode,
+                                                uint4 *__restrict__ Uq)
+{
+    const int64_t tid = (int64_t)blockIdx.x * NT + threadIdx.x;
+    const int64_t nent = mode == 0 ? uq_off0(nbt) : uq_off1(nbt, nss) * 4;
+    if (tid >= nent * 64) return;
+    const int64_t ent = tid >> 6;
+    const int lane = (int)(tid & 63), g = lane >> 4, l16 = lane & 15;
     auto u = [&](int i, int j) -> double {
-        return (i <= j && j < nb) ? Ub[(int64_t)j * (j + 1) / 2 + i] : 0.0;
+        return (i <= j && j < nb) ? U[tri0 + (int64_t)j * (j + 1) / 2 + i] : 0.0;
     };
-    const int64_t o = (tt_off[b] + T) * 256 + th;
-    Ut0[o] = (float)u(16 * I + 4 * lg + s, 16 * J + l16);
-    Ut1[o] = (float)u(16 * I + l16, 16 * J + 4 * lg + s);
+    int32_t q[16];
+    if (mode == 0) {
+        int J = (int)sqrtf(2.0f * (float)ent);
+        while (J > 0 && uq_off0(J) > ent) J--;
+        while (uq_off0(J + 1) <= ent) J++;
+        const int I4 = (int)(ent - uq_off0(J));
+        const int j = 16 * J + l16, e = j < nb ? ue0[j] : 0;
+#pragma unroll
+        for (int b = 0; b < 16; b++) q[b] = (int32_t)rint(ldexp(u(64 * I4 + 16 * g + b, j), e));
+    } else {
+        const int64_t e2 = ent >> 2;   // (I, ss) entry
+        const int s = (int)(ent & 3);
+        int I = 0;
+        {
+            int lo = 0, hi = nbt;   // largest I with uq_off1(I) <= e2
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) / 2;
+                if (uq_off1(mid, nss) <= e2) lo = mid; else hi = mid;
+            }
+            I = lo;
+        }
+        const int ss = I / 16 + (int)(e2 - uq_off1(I, nss));
+        const int i = 16 * I + l16, e = i < nb ? ue1[i] : 0;
+        const int sw = 16 * ss + 4 * g + s;
+#pragma unroll
+        for (int bb = 0; bb < 16; bb++) {
+            const int m = bb >> 2, b = bb & 3;   // byte 4 m + b holds column 16 sw + m + 4 b
+            const int j = 16 * sw + m + 4 * b;
+            q[bb] = (int32_t)rint(ldexp(j < nb ? u(i, j) * sdv[p0 + j] : 0.0, e));
+        }
+    }
+    for (int ku = 0; ku < ULIMBS; ku++) {
+        uint32_t w4[4];
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) v |= ((uint32_t)(uint8_t)(int8_t)(q[4 * d + b] & 0xFF)) << (8 * b);
+            w4[d] = v;
+        }
+#pragma unroll
+        for (int b = 0; b < 16; b++) q[b] = (q[b] - (int32_t)(int8_t)(q[b] & 0xFF)) >> 8;
+        Uq[(ent * ULIMBS + ku) * 64 + lane] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    }
 }
 
 // r_a(i, c) += sum of the mode-1 partials of the column groups that reach row
-// i's tile (groups left of the diagonal wrote nothing); plain and swizzled.
+// i's tile (groups left of the diagonal wrote nothing).
 __device__ __forceinline__ void ra_reduce_wg(const Par &P, const double *part, int spant, int ngroups, int nb,
-                                             double *ra, float *ra_sw, int bx, int c)
+                                             double *ra, int bx, int c)
 {
     const int i = bx * NT + threadIdx.x;
     if (i >= nb) return;
@@ -550,20 +745,19 @@ __device__ __forceinline__ void ra_reduce_wg(const Par &P, const double *part, i
     }
     const double v = ra[(int64_t)c * P.blocksize + i] + s;
     ra[(int64_t)c * P.blocksize + i] = v;
-    ra_sw[rsw_index(i, c, (C + 15) / 16)] = (float)v;
 }
 
 __global__ __launch_bounds__(NT) void k_ra_reduce(Par P, const double *part, int spant, int ngroups, int nb,
-                                                  double *ra, float *ra_sw)
+                                                  double *ra)
 {
-    ra_reduce_wg(P, part, spant, ngroups, nb, ra, ra_sw, blockIdx.x, blockIdx.y);
+    ra_reduce_wg(P, part, spant, ngroups, nb, ra, blockIdx.x, blockIdx.y);
 }
 
 // The r_a reduction of this block beside the r limbs of the next block (both
 // need only the update just done): workgroups [0, nq * C) quantise r,
 // the rest reduce r_a.
 __global__ __launch_bounds__(NT) void k_rared_quant(Par P, const double *part, int spant, int ngroups, int nb,
-                                                    double *ra, float *ra_sw, int nra, int ntile, const double *r,
+                                                    double *ra, int nra, int ntile, const double *r,
                                                     const uint64_t *rmax, int nwg, uint4 *Q, int *expo,
                                                     long long *qsum, int nq)
 {
@@ -571,7 +765,7 @@ __global__ __launch_bounds__(NT) void k_rared_quant(Par P, const double *part, i
     if (id < nq * C)
         quant_wg(P, ntile, r, rmax, nwg, Q, expo, qsum, nq, id % nq, id / nq);
     else
-        ra_reduce_wg(P, part, spant, ngroups, nb, ra, ra_sw, (id - nq * C) % nra, (id - nq * C) / nra);
+        ra_reduce_wg(P, part, spant, ngroups, nb, ra, (id - nq * C) % nra, (id - nq * C) / nra);
 }
 
 // SNP-major genotype tiles of one block for k_xq: element ((tile * nS + S) *
@@ -827,7 +1021,7 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
                                                const int32_t *snp_of_pos, int p0, int nb, double zz,
                                                const int *it, double *beta, int32_t *cls, double *vara_s,
                                                double *delta, uint8_t *chg, double *rhs_out, uint64_t *dmax,
-                                               const double *carry, double *cj_out, float *dl_sw)
+                                               const double *carry, double *cj_out)
 {
     __shared__ unsigned long long mx[NT / WAVE];
     __shared__ MixConst mc;
@@ -895,7 +1089,6 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
             if (ab) vara_s[o + pos] = d.vs;
             delta[o + pos] = d.delta;
             chg[o + pos] = d.chg;
-            dl_sw[rsw_index(jk, c, (C + 15) / 16)] = (float)d.delta;
             cj = (s > 0.0 ? d.delta / s : 0.0) + cy;
             cj_out[(int64_t)c * P.blocksize + jk] = cj;
         }
@@ -961,8 +1154,9 @@ int upd_groups(const sb_handle *h, int rw)
     return std::max(1, std::min(ntr, h->ncu * ((ntr + per - 1) / per)));
 }
 
-// X_s'Q of block b with row tiles per wave rw and nch chunks (one launch,
-// through the handle's kernel clock)
+// X_s'Q of block b with row tiles per wave rw and nch chunks, and the r_a
+// limbs of the block's U_s'r_a (one launch, through the handle's kernel
+// clock)
 int prod_launch(sb_handle *h, int b, int rw_sel, int nch)
 {
     const Par &P = h->par;
@@ -978,8 +1172,9 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int nch)
         with_rw<NTL>(rw_sel, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
             const int nrg = (nbt + 4 * RW - 1) / (4 * RW);
-            launch(kc, k_prod<NTL, RW>, dim3(nrg * nch), dim3(NT), 0, st, P, gsw, nbt, nb, Q, nch,
-                   reinterpret_cast<long long *>(h->b_T), tstride, nrg);
+            launch(kc, k_prod<NTL, RW>, dim3(C + nrg * nch), dim3(NT), 0, st, P, gsw, nbt, nb, Q, nch,
+                   reinterpret_cast<long long *>(h->b_T), tstride, nrg, h->ra, reinterpret_cast<uint4 *>(h->b_Raq),
+                   h->b_expoa);
         });
     });
 }
@@ -1009,36 +1204,32 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out)
     });
 }
 
-// U products of block b: mode 0 U_s'R_a (r_a panel) into the r_a partials of
-// the draw, mode 1 U_s D (delta panel) into the partials of the r_a update
+// U products of block b on int8 limbs: mode 0 U_s'r_a (r_a limbs) into the
+// r_a partials of the draw, mode 1 U_s diag(s) c~ (the update's limbs Dq)
+// into the partials of the r_a update
 int ut_launch(sb_handle *h, int b, int mode)
 {
     const Par &P = h->par;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
-    const int C = P.nchains;
-    const int spant = h->rows_per_group / 16, ngrp = (nbt + spant - 1) / spant;
-    const int g1x = (nbt + 3) / 4;
-    const float4 *ut = reinterpret_cast<const float4 *>(mode == 0 ? h->Ut0 : h->Ut1) + h->tt_off[b] * 64;
-    const float4 *rsw = reinterpret_cast<const float4 *>(mode == 0 ? h->ra_sw : h->dl_sw);
+    const int C = P.nchains, nss = batch_words(nb) / 16;
+    const int nq4 = (nbt + 3) / 4;
+    const int64_t npair = mode == 0 ? uq_off0(nq4) : (int64_t)nq4 * (nss + 1) - uq_off0(nq4);
+    const uint4 *ua = reinterpret_cast<const uint4 *>(mode == 0 ? h->Uq0 : h->Uq1) +
+                      (mode == 0 ? h->uq0_off[b] : h->uq1_off[b]);
+    const uint4 *bq = reinterpret_cast<const uint4 *>(mode == 0 ? h->b_Raq : h->b_Dq);
+    const int *ue = (mode == 0 ? h->ue0 : h->ue1) + (int64_t)b * P.blocksize;
+    const int *ce = mode == 0 ? h->b_expoa : h->b_expo2;
     double *part = mode == 0 ? h->b_rapart : h->b_uapart;
     KClock &kc = h->kc;
-    const int ntc = (C + 15) / 16;
-    auto go = [&](auto m, auto t) {
-        const int npair = g1x * ngrp;
-        launch(kc, k_ut<decltype(m)::value, decltype(t)::value>, dim3((npair + 1) / 2), dim3(NT), 0, h->stream, P,
-               ut, nbt, nb, rsw, spant, part, g1x, npair);
-    };
-    using std::integral_constant;
-    auto by_ntc = [&](auto m) {
-        switch (ntc) {
-        case 1: go(m, integral_constant<int, 1>{}); return 0;
-        case 2: go(m, integral_constant<int, 2>{}); return 0;
-        case 3: go(m, integral_constant<int, 3>{}); return 0;
-        case 4: go(m, integral_constant<int, 4>{}); return 0;
-        default: return (int)SB_ERR_ARG;
-        }
-    };
-    return mode == 0 ? by_ntc(integral_constant<int, 0>{}) : by_ntc(integral_constant<int, 1>{});
+    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
+        constexpr int NTL = decltype(ntl)::value;
+        if (mode == 0)
+            launch(kc, k_uq<0, NTL>, dim3((unsigned)npair), dim3(NT), 0, h->stream, P, ua, nbt, nb, nss, bq, ue, ce,
+                   part);
+        else
+            launch(kc, k_uq<1, NTL>, dim3((unsigned)npair), dim3(NT), 0, h->stream, P, ua, nbt, nb, nss, bq, ue, ce,
+                   part);
+    });
 }
 
 int batch_block(sb_handle *h, int b, double zz, int rep)
@@ -1053,7 +1244,7 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
     uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
     KClock &kc = h->kc;
     const double n = P.n, cl = (double)C * L, gbytes = (double)nb * ((n + 3.0) / 4.0);
-    const double tbytes = 4.0 * 256.0 * nbt * (nbt + 1) / 2.0;   // FP32 U tiles of the block
+    const double tbytes = 1024.0 * ULIMBS * (double)uq_off0(nbt);   // U limb tiles of the block (mode 0)
     if (h->q_block != b || h->rstat_nwg == 0) {
         // r changed outside the batched update (first block of a sweep)
         kc.begin(st);
@@ -1086,7 +1277,7 @@ int batch_update(sb_handle *h, int b)
     uint4 *Dq = reinterpret_cast<uint4 *>(h->b_Dq);
     KClock &kc = h->kc;
     const double n = P.n, cl = (double)C * L, gbytes = (double)nb * ((n + 3.0) / 4.0);
-    const double tbytes = 4.0 * 256.0 * nbt * (nbt + 1) / 2.0;
+    const double tbytes = 4096.0 * ULIMBS * (double)uq_off1(nbt, nsw / 16);   // U limb tiles (mode 1)
     kc.begin(st);
     kc.last(KS_DQUANT, 16.0 * C * nb + cl * nb);
     launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, h->b_ntile, h->b_cj, h->f2, p0, nb,
@@ -1109,11 +1300,11 @@ int batch_update(sb_handle *h, int b)
         uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
         kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 8.0 * C * n + cl * n);
         launch(kc, k_rared_quant, dim3((nq + nra) * C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra,
-               h->ra_sw, nra, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_qsum, nq);
+               nra, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_qsum, nq);
         h->q_block = b + 1;
     } else {
         kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0));
-        launch(kc, k_ra_reduce, dim3(nra, C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra, h->ra_sw);
+        launch(kc, k_ra_reduce, dim3(nra, C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra);
     }
     return hip_check(hipGetLastError(), "batch update");
 }
@@ -1233,7 +1424,7 @@ __global__ __launch_bounds__(NT) void k_stream(const uint4 *__restrict__ p, int6
 // pass per block, one launch per block as the sweep launches its kernels,
 // over every block (a working set far beyond the caches): out[3 k ..] =
 // (bytes per block, microseconds per block, GB/s) for layout k: the batched
-// path's geno_sw, geno_t, Ut0, Ut1; the FP64 path's genotype words and
+// path's geno_sw, geno_t, Uq0, Uq1; the FP64 path's genotype words and
 // packed U (then slots 2, 3 are zero).
 int layout_stream(sb_handle *h, double *out)
 {
@@ -1259,8 +1450,8 @@ int layout_stream(sb_handle *h, double *out)
     if (P.precision == SB_PREC_MIXED) {
         mk(h->geno_sw, [&](int i) { return h->gs_off[i]; });
         mk(h->geno_t, [&](int i) { return h->gt_off[i]; });
-        mk(h->Ut0, [&](int i) { return h->tt_off[i] * 64; });
-        mk(h->Ut1, [&](int i) { return h->tt_off[i] * 64; });
+        mk(h->Uq0, [&](int i) { return h->uq0_off[i]; });
+        mk(h->Uq1, [&](int i) { return h->uq1_off[i]; });
     } else {
         mk(h->geno, [&](int i) { return (int64_t)h->ls[i] * P.ldw / 4; });
         // packed triangles of doubles: block starts rounded down to whole
@@ -1333,20 +1524,35 @@ int batch_transpose(sb_handle *h)
     return rc;
 }
 
-// U tiles of every block (after the factorisation has filled h->U).
+// U limb tiles of every block (after the factorisation has filled h->U):
+// the exponents of every column and row, then the two tile sets.
 int batch_tiles(sb_handle *h)
 {
     const Par &P = h->par;
-    const int bt = (P.blocksize + 15) / 16;
-    int64_t *dtt = nullptr;
-    if (hip_check(hipMalloc(&dtt, sizeof(int64_t) * (P.nblocks + 1)), "hipMalloc(tile offsets)")) return SB_ERR_NOMEM;
-    hipMemcpyAsync(dtt, h->tt_off.data(), sizeof(int64_t) * (P.nblocks + 1), hipMemcpyHostToDevice, h->stream);
-    k_pack_tiles<<<dim3(bt * (bt + 1) / 2, P.nblocks), 256, 0, h->stream>>>(h->U, h->d_tri_off, h->d_ls, dtt,
-                                                                          h->Ut0, h->Ut1);
-    int rc = hip_check(hipGetLastError(), "U tiles");
-    if (!rc) rc = hip_check(hipStreamSynchronize(h->stream), "U tiles");
-    hipFree(dtt);
+    hipStream_t st = h->stream;
+    k_uexp<<<dim3((P.blocksize + NT - 1) / NT, P.nblocks), NT, 0, st>>>(h->U, h->d_tri_off, h->d_ls, h->sdv,
+                                                                        P.blocksize, h->ue0, h->ue1);
+    for (int b = 0; b < P.nblocks; b++) {
+        const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16, nss = batch_words(nb) / 16;
+        const int *ue0 = h->ue0 + (int64_t)b * P.blocksize, *ue1 = h->ue1 + (int64_t)b * P.blocksize;
+        for (int mode = 0; mode < 2; mode++) {
+            const int64_t nent = mode == 0 ? uq_off0(nbt) : uq_off1(nbt, nss) * 4;
+            uint4 *dst = reinterpret_cast<uint4 *>(mode == 0 ? h->Uq0 : h->Uq1) +
+                         (mode == 0 ? h->uq0_off[b] : h->uq1_off[b]);
+            k_pack_uq<<<(unsigned)((nent * 64 + NT - 1) / NT), NT, 0, st>>>(h->U, h->tri_off[b], nb, nbt, nss, p0,
+                                                                            h->sdv, ue0, ue1, mode, dst);
+        }
+    }
+    int rc = hip_check(hipGetLastError(), "U limb tiles");
+    if (!rc) rc = hip_check(hipStreamSynchronize(st), "U limb tiles");
     return rc;
+}
+
+// uint4 elements of one block's U limb tiles (mode 0 or 1)
+int64_t batch_uq_elems(int nb, int mode)
+{
+    const int nbt = (nb + 15) / 16, nss = batch_words(nb) / 16;
+    return (mode == 0 ? uq_off0(nbt) : uq_off1(nbt, nss) * 4) * ULIMBS * 64;
 }
 
 int batch_draw(sb_handle *h, int b, double zz, double *delta, uint8_t *chg, double *rhs_out)
@@ -1363,7 +1569,7 @@ int batch_draw(sb_handle *h, int b, double zz, double *delta, uint8_t *chg, doub
         launch(h->kc, k_draw_b<decltype(nm)::value>, dim3((nb + NT - 1) / NT, P.nchains), dim3(NT), 0, h->stream,
                P, h->scal, reinterpret_cast<const long long *>(h->b_T), tstride, nch, h->b_expo, h->b_qsum, h->b_nq,
                h->b_rapart, h->rows_per_group / 16, h->f2, h->sdv, h->grp, h->d_pos_by_id + P.p, p0, nb, zz, h->d_it, h->beta, h->cls, h->vara_s,
-               delta, chg, rhs_out, h->b_dmax, h->b_carry, h->b_cj, h->dl_sw);
+               delta, chg, rhs_out, h->b_dmax, h->b_carry, h->b_cj);
     });
     return hip_check(hipGetLastError(), "batch draw");
 }

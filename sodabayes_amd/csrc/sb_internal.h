@@ -286,11 +286,14 @@ struct sb_handle {
     std::vector<int64_t> gt_off;  // 16-byte element offset of each block in geno_t
     uint32_t *geno_sw = nullptr;  // SNP-major genotypes per block, MFMA operand tiles (product GEMM)
     std::vector<int64_t> gs_off;  // 16-byte element offset of each block in geno_sw
-    float *Ut0 = nullptr;         // U (FP32) as 16x16 tiles in the operand order of U' R (mode 0)
-    float *Ut1 = nullptr;         // U (FP32) as 16x16 tiles in the operand order of U D (mode 1)
-    std::vector<int64_t> tt_off;  // tile offset of each block in Ut0 / Ut1 (256 values per tile)
-    float *ra_sw = nullptr;       // r_a of every chain as an FP32 swizzled panel (rsw_index)
-    float *dl_sw = nullptr;       // delta of the current block, FP32 swizzled panel
+    int8_t *Uq0 = nullptr;        // U as 4 int8 limbs per entry, A-operand tiles of U' r_a (mode 0)
+    int8_t *Uq1 = nullptr;        // U diag(s) as 4 int8 limbs per entry, A-operand tiles of U c~ (mode 1)
+    std::vector<int64_t> uq0_off; // uint4 offset of each block in Uq0
+    std::vector<int64_t> uq1_off; // uint4 offset of each block in Uq1
+    int *ue0 = nullptr;           // nblocks x blocksize limb exponents of U's columns (mode 0)
+    int *ue1 = nullptr;           // nblocks x blocksize limb exponents of U diag(s)'s rows (mode 1)
+    int8_t *b_Raq = nullptr;      // r_a limbs of every chain, B-operand blocks of U' r_a
+    int *b_expoa = nullptr;       // fixed-point exponent of r_a per chain
 
     std::vector<void *> allocs;
 };
@@ -346,6 +349,7 @@ int batch_clear_carry(sb_handle *h);
 int layout_stream(sb_handle *h, double *out);
 int batch_transpose(sb_handle *h);
 int batch_words(int nb);
+int64_t batch_uq_elems(int nb, int mode);
 int prod_chunks(const sb_handle *h, int rw, int m);
 int batch_max_chunks(const sb_handle *h);
 int batch_autotune(sb_handle *h);

@@ -1078,7 +1078,7 @@ int setup_build_aug(sb_handle *h, double *xxd_out, int32_t *fail_block, int32_t 
         progress("pivoted-Cholesky fallback", 0, (int)failed.size());
         rc = psd_fallback(h, gb, failed, hxxd);
     }
-    if (!rc && h->Ut0) rc = batch_tiles(h);
+    if (!rc && h->Uq0) rc = batch_tiles(h);
     const int rs = hip_check(hipStreamSynchronize(h->stream), "build_aug");
     if (!rc) rc = rs;
     gram_free(gb);

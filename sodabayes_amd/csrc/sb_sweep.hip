@@ -711,8 +711,7 @@ __global__ void k_ra_replay(Par P, Scal *sc, double *ra)
     sc->rng[1] = src.st.s2;
 }
 
-// (ra_sw, when given, receives the same values as a swizzled panel)
-__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, const int *it, int rep, int tag, float *ra_sw)
+__global__ void k_ra_counter(Par P, const Scal *sc, double *ra, const int *it, int rep, int tag)
 {
     if (it) rep = it[0];   // the sweep's iteration (rep: the initial draws' fixed counter)
     int i = blockIdx.x * NT + threadIdx.x;
@@ -721,7 +720,6 @@ __global__ void k_ra_counter(Par P, const Scal *sc, double *ra, const int *it, i
     PhiloxSrc src(chain_key(P, c), (uint32_t)rep, (uint32_t)i, tag);
     const double v = draw_normal(src, 0.0, sqrt(sc[c].vare));
     ra[(int64_t)c * P.blocksize + i] = v;
-    if (ra_sw) ra_sw[rsw_index(i, c, (P.nchains + 15) / 16)] = (float)v;
 }
 
 // posterior sums per SNP (models.f90:528-544)
@@ -1171,7 +1169,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
             k_ra_replay<<<1, 1, 0, h->stream>>>(P, h->scal, h->ra);
         else
             k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, h->stream>>>(P, h->scal, h->ra, nullptr,
-                                                                                   0x7FFFFFFF, TAG_INIT, h->ra_sw);
+                                                                                   0x7FFFFFFF, TAG_INIT);
     }
     if (hip_check(hipGetLastError(), "init kernels")) return SB_ERR_HIP;
     if (hip_check(hipStreamSynchronize(h->stream), "init")) return SB_ERR_HIP;
@@ -1222,16 +1220,6 @@ int block_rhs(sb_handle *h, int b, double *out_host)
     return rc;
 }
 
-// r_a of every chain into the FP32 swizzled panel of the batched products
-// (after an FP64 block update changed r_a outside the batched kernels)
-__global__ void k_ra_panel(Par P, const double *ra, float *ra_sw)
-{
-    const int i = blockIdx.x * NT + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= P.blocksize) return;
-    ra_sw[rsw_index(i, c, (P.nchains + 15) / 16)] = (float)ra[(int64_t)c * P.blocksize + i];
-}
-
 // The FP64 block products X_s'r + U_s'r_a of chain c into the per-block
 // scratch (part, partA, partR).
 static void fp64_products(sb_handle *h, int b, int c)
@@ -1257,7 +1245,7 @@ static void fp64_products(sb_handle *h, int b, int c)
 // chain, then the ordered changed lists, r and r_a updates of all chains.
 // The FP64 path's only sweep; in a mixed run, the blocks that took the
 // pivoted-Cholesky fallback (dense factor, skipped SNPs), after which the
-// FP32 r_a panel is refreshed and the next block recomputes its r limbs.
+// next block recomputes its r limbs (its X_s'Q launch also requantises r_a).
 static int fp64_block(sb_handle *h, int b, const std::vector<double> &xxd)
 {
     const Par &P = h->par;
@@ -1298,7 +1286,6 @@ static int fp64_block(sb_handle *h, int b, const std::vector<double> &xxd)
     launch(kc, k_ra_sum, dim3((nb + NT - 1) / NT, 1, C), dim3(NT), 0, st, P, h->u_rapart, h->u_cnt, h->u_tstart,
            nb, h->ra);
     if (P.precision == SB_PREC_MIXED) {
-        k_ra_panel<<<dim3((P.blocksize + NT - 1) / NT, C), NT, 0, st>>>(P, h->ra, h->ra_sw);
         h->rstat_nwg = 0;   // r moved outside the batched update: fresh statistics and limbs
         h->q_block = -1;
     }
@@ -1361,8 +1348,7 @@ static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd
         if (P.rng_mode == SB_RNG_REPLAY)
             k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
         else
-            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, h->d_it, 0, TAG_RA,
-                                                                             h->ra_sw);
+            k_ra_counter<<<dim3((P.naind + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->ra, h->d_it, 0, TAG_RA);
     }
     if (h->use_heldout) {
         k_impute<<<dim3((P.n + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->heldout, h->why, h->r, h->d_it);
