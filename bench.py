@@ -36,24 +36,24 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNEL_NAMES = {
     "rhs": "k_rhs (X_s'r + U_s'r_a, FP64)", "draw": "k_draw_counter", "list": "k_list",
     "update": "k_update (r += Z delta, r_a += U delta)", "rquant": "k_rstat + k_quant (r limbs, first block)",
-    "xq": "k_prod (X_s'Q, int8 MFMA)", "ut0": "k_ut<0> (U_s'R_a, FP32 U on FP64 MFMA)",
+    "xq": "k_prod (X_s'Q + r_a limbs, int8 MFMA)", "ut0": "k_uq<0> (U_s'r_a, int8 limbs of U and r_a)",
     "drawb": "k_draw_b (limb sums + draw)", "dquant": "k_dquant (delta limbs)",
-    "xupd": "k_upd (R += X_s C, int8 MFMA)", "ut1": "k_ut<1> (U_s D, FP32 U on FP64 MFMA)",
+    "xupd": "k_upd (R += X_s C, int8 MFMA)", "ut1": "k_uq<1> (U_s diag(s) c, int8 limbs)",
     "rared": "k_rared_quant (r_a reduction + next block's r limbs)",
 }
 
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2, the factor a 1 GiB
 # device copy shows on this GPU) + WRITE_SIZE (x1), per (path, kernel slot),
-# measured on 4,096-SNP blocks at n = 50,000 (16 chains for the batched path),
+# measured on 4,096-SNP blocks at n = 50,000 (32 chains for the batched path),
 # the per-launch shape of this workload
-# (profiles/soda_bayesr_n50k_bs4096_x16_pmc_traffic.md); None where not measured
-TRAFFIC_CHAINS = 16
+# (profiles/soda_bayesr_n50k_bs4096_x32_pmc_traffic.md); None where not measured
+TRAFFIC_CHAINS = 32
 TRAFFIC = {
-    ("mixed", "xq"): int((52670 * 2 + 11008) * 1024),     # k_prod<3,4,1,2>
-    ("mixed", "xupd"): int((46982 * 2 + 11002) * 1024),   # k_upd<3,4,1,2>
-    ("mixed", "drawb"): int((7100 * 2 + 1616) * 1024),    # k_draw_b<4>
-    ("fp64", "rhs"): int((60273 * 2 + 891) * 1024),       # k_rhs
-    ("fp64", "update"): int((50600 * 2 + 807) * 1024),    # k_update (k_ra_sum not counted)
+    ("mixed", "xupd"): int((32878 * 2 + 13012) * 1024),   # k_upd<6, 2>
+    ("mixed", "xq"): int((45411 * 2 + 8961) * 1024),      # k_prod<6, 2>
+    ("mixed", "drawb"): int((9968 * 2 + 3728) * 1024),    # k_draw_b<4>
+    ("fp64", "rhs"): int((60337 * 2 + 918) * 1024),       # k_rhs
+    ("fp64", "update"): int((50645 * 2 + 604) * 1024),    # k_update (k_ra_sum not counted)
 }
 
 
@@ -68,9 +68,9 @@ def parse_args():
     ap.add_argument("--seed", type=int, default=20240607)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-only", action="store_true", help="only the reference CPU baseline (prints its JSON)")
-    ap.add_argument("--cpu-nloci", type=int, default=2048, help="SNPs of the CPU baseline's sample")
+    ap.add_argument("--cpu-nloci", type=int, default=8192, help="SNPs of the CPU baseline's sample")
     ap.add_argument("--cpu-blocksize", type=int, default=1024, help="SODA subset size of the CPU baseline's sample")
-    ap.add_argument("--cpu-iters", type=int, default=30, help="iterations of the CPU baseline's sample")
+    ap.add_argument("--cpu-iters", type=int, default=100, help="iterations of the CPU baseline's sample")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 accuracy report")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--chains", type=int, default=32,
@@ -118,7 +118,7 @@ def cpu_cores() -> int:
     return min(share, n) if share > 0 else n
 
 
-def cpu_baseline(seed: int, nloci: int = 2048, blocksize: int = 1024, numit: int = 30) -> dict | None:
+def cpu_baseline(seed: int, nloci: int = 8192, blocksize: int = 1024, numit: int = 100) -> dict | None:
     """The reference program (oracle/_ref/o3/sodabayes_ref: the reference
     sources built -O3 -fopenmp by oracle/Makefile) on a bounded sample of
     the workload: the same n = 50,000 individuals and BayesR mixture, nloci
