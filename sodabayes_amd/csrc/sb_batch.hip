@@ -527,6 +527,23 @@ __host__ __device__ __forceinline__ int64_t uq_off0(int J)
     const int64_t a = J >> 2, r = J & 3;   // sum over J' < J of (J' / 4 + 1)
     return (a + 1) * (2 * a + r);
 }
+// pairs before quad bx: mode 0, bx / UQ_GROUP + 1 row groups per quad
+__host__ __device__ __forceinline__ int64_t uq_pairs0(int bx)
+{
+    const int64_t a = bx / UQ_GROUP, r = bx % UQ_GROUP;   // sum over x < bx of (x / G + 1)
+    return (a + 1) * ((int64_t)UQ_GROUP * a / 2 + r);
+}
+// mode 1: ng - bx / UQ_GROUP column groups per quad (ng groups in all)
+__host__ __device__ __forceinline__ int64_t uq_pairs1(int bx, int ng)
+{
+    return (int64_t)bx * (ng + 1) - uq_pairs0(bx);
+}
+// (quad, group) pairs of a block of nbt tiles, nss super-steps
+__host__ __device__ __forceinline__ int64_t uq_pairs(int nbt, int nss, int mode)
+{
+    const int nq4 = (nbt + 3) / 4;
+    return mode == 0 ? uq_pairs0(nq4) : uq_pairs1(nq4, (nss * 4 + UQ_GROUP - 1) / UQ_GROUP);
+}
 __host__ __device__ __forceinline__ int64_t uq_off1(int I, int nss)
 {
     const int64_t q = I >> 4, r = I & 15;  // sum over I' < I of (nss - I' / 16)
@@ -542,33 +559,36 @@ __host__ __device__ __forceinline__ int64_t uq_off1(int I, int nss)
 // all of them at once, the four waves load the steps' B blocks into LDS
 // together, and one barrier publishes them.  Waves past the block still
 // load and meet the barrier, but do no MFMA and store nothing.
+constexpr int uq_lds(int ntile) { return UQ_GROUP * ntile * 64; }   // uint4 of LDS per pair
+
 template <int MODE, int NTILE>
-__global__ __launch_bounds__(NT) void k_uq(Par P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
-                                           const uint4 *__restrict__ Bq, const int *__restrict__ ue,
-                                           const int *__restrict__ ce, double *__restrict__ part)
+__device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
+                                        const uint4 *__restrict__ Bq, const int *__restrict__ ue,
+                                        const int *__restrict__ ce, double *__restrict__ part, int k,
+                                        uint4 *lds)
 {
     constexpr int W = NT / WAVE, NTC = NTILE / LIMBS;
-    __shared__ uint4 lds[4 * NTILE * 64];
+    constexpr int KR = 4, NR = UQ_GROUP / KR;   // K steps per round, rounds per pair
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int k = blockIdx.x;
     int bx, gi, k0, k1;
     if (MODE == 0) {
-        // pairs of quad bx: bx / 4 + 1 groups; uq_off0(bx) pairs before it
-        bx = (int)sqrtf(2.0f * (float)k);
-        while (bx > 0 && uq_off0(bx) > k) bx--;
-        while (uq_off0(bx + 1) <= k) bx++;
-        gi = (int)(k - uq_off0(bx));
-        k0 = 4 * gi;
-        k1 = min(4 * gi + 4, bx + 1);
+        // pairs of quad bx: bx / UQ_GROUP + 1 groups; uq_pairs0(bx) pairs before it
+        bx = (int)sqrtf(2.0f * UQ_GROUP * (float)k);
+        while (bx > 0 && uq_pairs0(bx) > k) bx--;
+        while (uq_pairs0(bx + 1) <= k) bx++;
+        gi = (int)(k - uq_pairs0(bx));
+        k0 = UQ_GROUP * gi;
+        k1 = min(UQ_GROUP * gi + UQ_GROUP, bx + 1);
     } else {
-        // pairs of quad bx: nss - bx / 4 super-steps
-        auto offq = [&](int x) { return (int64_t)x * (nss + 1) - uq_off0(x); };
+        // pairs of quad bx: the column groups (UQ_GROUP / 4 super-steps each)
+        // from the one holding super-step bx / 4
+        const int ng = (nss * 4 + UQ_GROUP - 1) / UQ_GROUP;
         bx = 0;
-        while (offq(bx + 1) <= k) bx++;
-        gi = bx / 4 + (int)(k - offq(bx));   // the super-step
-        k0 = 4 * gi;
-        k1 = 4 * gi + 4;
+        while (uq_pairs1(bx + 1, ng) <= k) bx++;
+        gi = bx / UQ_GROUP + (int)(k - uq_pairs1(bx, ng));
+        k0 = max(UQ_GROUP * gi, 4 * (bx / 4));
+        k1 = min(UQ_GROUP * gi + UQ_GROUP, 4 * nss);
     }
     const int O = 4 * bx + wave;      // the wave's output tile (J or I)
     const bool live = O < nbt;
@@ -583,33 +603,37 @@ __global__ __launch_bounds__(NT) void k_uq(Par P, const uint4 *__restrict__ Ua, 
         if (MODE == 0) return ((int64_t)kk * NTILE + t) * 64 + lane;
         return (((int64_t)(kk >> 2) * NTILE + t) * 4 + (kk & 3)) * 64 + lane;
     };
-    // every load of the pair in flight at once: the wave's A tiles of all
-    // (at most four) K steps in registers, the B blocks of all steps in LDS
-    // (each wave loads every fourth block), one barrier
-    constexpr int KS = 4;
+    // The pair's K steps in rounds of four: each wave holds the A tiles of one
+    // round in registers (loaded together); the B blocks of every K step of
+    // the pair go to LDS before the first barrier (each wave loads every
+    // fourth block, all loads of a round before its stores).
     const v4i *Av = reinterpret_cast<const v4i *>(Ua);
     const v4i *Bv = reinterpret_cast<const v4i *>(Bq);
     v4i *L = reinterpret_cast<v4i *>(lds);
-    v4i a[KS][ULIMBS];
+    const int nk = k1 - k0;
+    v4i a[KR][ULIMBS];
 #pragma unroll
-    for (int s = 0; s < KS; s++)
+    for (int s = 0; s < KR; s++)
 #pragma unroll
-        for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(min(k0 + s, k1 - 1), u)];
-    constexpr int NBQ = (KS * NTILE + W - 1) / W;   // B blocks per wave, all loads before any store
-    v4i bq[NBQ];
+        for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(k0 + min(s, nk - 1), u)];
+    constexpr int NBQ = (KR * NTILE + W - 1) / W;   // B blocks per wave and round
 #pragma unroll
-    for (int j = 0; j < NBQ; j++) {
-        const int q = wave + W * j, s = min(q / NTILE, k1 - k0 - 1), t = q % NTILE;
-        bq[j] = Bv[bidx(k0 + s, t)];
-    }
+    for (int r = 0; r < NR; r++) {
+        v4i bq[NBQ];
 #pragma unroll
-    for (int j = 0; j < NBQ; j++) {
-        const int q = wave + W * j;
-        if (q < (k1 - k0) * NTILE) L[q * 64 + lane] = bq[j];
+        for (int j = 0; j < NBQ; j++) {
+            const int q = KR * NTILE * r + wave + W * j, s = min(q / NTILE, nk - 1), t = q % NTILE;
+            bq[j] = Bv[bidx(k0 + s, t)];
+        }
+#pragma unroll
+        for (int j = 0; j < NBQ; j++) {
+            const int q = KR * NTILE * r + wave + W * j;
+            if (wave + W * j < KR * NTILE && q < nk * NTILE) L[q * 64 + lane] = bq[j];
+        }
     }
     // one accumulator per chain tile and weight 256^(ku + kr): the limb
     // products of equal weight are summed exactly in int32 (at most three
-    // products of 64 terms per K step, four K steps)
+    // products of 64 terms per K step, UQ_GROUP K steps)
     constexpr int NWT = ULIMBS + LIMBS - 1;
     v4i acc[NWT][NTC];
 #pragma unroll
@@ -617,20 +641,31 @@ __global__ __launch_bounds__(NT) void k_uq(Par P, const uint4 *__restrict__ Ua, 
 #pragma unroll
         for (int tc = 0; tc < NTC; tc++) acc[w][tc] = v4i{0, 0, 0, 0};
     __syncthreads();
-    if (live) {
 #pragma unroll
-        for (int s = 0; s < KS; s++) {
-            if (k0 + s >= k1) break;
-            const v4i *Lb = L + s * NTILE * 64 + lane;
+    for (int r = 0; r < NR; r++) {
+        if (r > 0) {
+            if (KR * r >= nk) break;   // workgroup-uniform
 #pragma unroll
-            for (int kr = 0; kr < LIMBS; kr++)
+            for (int s = 0; s < KR; s++)
 #pragma unroll
-                for (int tc = 0; tc < NTC; tc++) {
-                    const v4i bt = Lb[(kr * NTC + tc) * 64];
+                for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(k0 + min(KR * r + s, nk - 1), u)];
+        }
+        if (live) {
 #pragma unroll
-                    for (int u = 0; u < ULIMBS; u++)
-                        acc[u + kr][tc] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[s][u], bt, acc[u + kr][tc], 0, 0, 0);
-                }
+            for (int s = 0; s < KR; s++) {
+                if (KR * r + s >= nk) break;
+                const v4i *Lb = L + (KR * r + s) * NTILE * 64 + lane;
+#pragma unroll
+                for (int kr = 0; kr < LIMBS; kr++)
+#pragma unroll
+                    for (int tc = 0; tc < NTC; tc++) {
+                        const v4i bt = Lb[(kr * NTC + tc) * 64];
+#pragma unroll
+                        for (int u = 0; u < ULIMBS; u++)
+                            acc[u + kr][tc] =
+                                __builtin_amdgcn_mfma_i32_16x16x64_i8(a[s][u], bt, acc[u + kr][tc], 0, 0, 0);
+                    }
+            }
         }
     }
     if (!live) return;
@@ -653,6 +688,15 @@ __global__ __launch_bounds__(NT) void k_uq(Par P, const uint4 *__restrict__ Ua, 
             part[((int64_t)gi * C + c) * P.blocksize + o] = ldexp(S, -(ue[o] + ec));
         }
     }
+}
+
+template <int MODE, int NTILE>
+__global__ __launch_bounds__(NT) void k_uq(Par P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
+                                           const uint4 *__restrict__ Bq, const int *__restrict__ ue,
+                                           const int *__restrict__ ce, double *__restrict__ part)
+{
+    __shared__ uint4 lds[uq_lds(NTILE)];
+    uq_pair<MODE, NTILE>(P, Ua, nbt, nb, nss, Bq, ue, ce, part, blockIdx.x, lds);
 }
 
 // Limb exponents of U, per block: ue0[j] from max over i <= j of |U(i, j)|,
@@ -942,12 +986,20 @@ template <int NTILE, int RW>
 __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss, int ntr,
                                             const uint4 *__restrict__ Dq, const int *__restrict__ expo,
                                             const double *__restrict__ kpart, int nkp, double *__restrict__ r,
-                                            uint64_t *__restrict__ rmax, int ng)
+                                            uint64_t *__restrict__ rmax, int ng, const uint4 *__restrict__ Ua,
+                                            int nbt, int nb, const int *__restrict__ ue, double *__restrict__ upart)
 {
     constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
+    static_assert(2 * SLAB * 64 >= uq_lds(NTILE), "LDS of the U pairs");
     __shared__ uint4 lds[2 * SLAB * 64];   // limb slab ring, then the K-constant parts
     __shared__ double kc_s[64];
     __shared__ unsigned long long mx_s[W][16 * NTC];
+    if ((int)blockIdx.x >= ng) {
+        // the workgroups after the update's: pairs of U_s diag(s) c~ (same
+        // limbs Dq and exponents), filling the CUs the update leaves idle
+        uq_pair<1, NTILE>(P, Ua, nbt, nb, nss, Dq, ue, expo, upart, (int)blockIdx.x - ng, lds);
+        return;
+    }
     const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
@@ -1198,11 +1250,12 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int nch)
 // r += X_s C - K of block b with row tiles per wave rw, on the residuals r
 // (the handle's, or a scratch copy while tuning); returns the number of
 // max |r| partials it leaves per chain through *ng_out
-int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out)
+int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out, bool with_u)
 {
     const Par &P = h->par;
     hipStream_t st = h->stream;
-    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0;
+    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
+    const int64_t nup = with_u ? uq_pairs(nbt, batch_words(nb) / 16, 1) : 0;   // U_s diag(s) c~ pairs
     const int C = P.nchains;
     const int nsw = batch_words(nb), ntr = (P.n + 15) / 16;
     const uint4 *Dq = reinterpret_cast<const uint4 *>(h->b_Dq);
@@ -1214,8 +1267,10 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out)
             constexpr int RW = decltype(rw)::value;
             const int ng = upd_groups(h, RW);
             *ng_out = ng;
-            launch(kc, k_upd<NTL, RW>, dim3(ng), dim3(NT), 0, st, P, gt, nsw / 16, ntr, Dq, h->b_expo2,
-                   h->b_kpart, (nb + NT - 1) / NT, r, h->b_rmax, ng);
+            launch(kc, k_upd<NTL, RW>, dim3(ng + (unsigned)nup), dim3(NT), 0, st, P, gt, nsw / 16, ntr, Dq,
+                   h->b_expo2, h->b_kpart, (nb + NT - 1) / NT, r, h->b_rmax, ng,
+                   reinterpret_cast<const uint4 *>(h->Uq1) + h->uq1_off[b], nbt, nb, h->ue1 + (int64_t)b * P.blocksize,
+                   h->b_uapart);
         });
     });
 }
@@ -1228,8 +1283,7 @@ int ut_launch(sb_handle *h, int b, int mode)
     const Par &P = h->par;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
     const int C = P.nchains, nss = batch_words(nb) / 16;
-    const int nq4 = (nbt + 3) / 4;
-    const int64_t npair = mode == 0 ? uq_off0(nq4) : (int64_t)nq4 * (nss + 1) - uq_off0(nq4);
+    const int64_t npair = uq_pairs(nbt, nss, mode);
     const uint4 *ua = reinterpret_cast<const uint4 *>(mode == 0 ? h->Uq0 : h->Uq1) +
                       (mode == 0 ? h->uq0_off[b] : h->uq1_off[b]);
     const uint4 *bq = reinterpret_cast<const uint4 *>(mode == 0 ? h->b_Raq : h->b_Dq);
@@ -1300,12 +1354,9 @@ int batch_update(sb_handle *h, int b)
            h->b_dmax, nwg_d, Dq, h->b_expo2, h->b_carry, h->b_kpart);
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
     kc.begin(st);
-    kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n);
+    kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
     int ng = 0;
-    int rc = upd_launch(h, b, h->b_rw, h->r, &ng);
-    kc.begin(st);
-    kc.last(KS_UT1, tbytes + 12.0 * C * nb);
-    if (!rc) rc = ut_launch(h, b, 1);
+    const int rc = upd_launch(h, b, h->b_rw, h->r, &ng, true);   // with U_s diag(s) c~
     if (rc) return fail(rc, "unsupported chain count");
     h->rstat_nwg = ng;   // max |r| partials of the new r, per update workgroup
     const int nra = (nb + NT - 1) / NT;
@@ -1386,7 +1437,7 @@ int batch_autotune(sb_handle *h)
     tune_rws(h, rws);
     for (int rw : rws) {
         tune_chunks(h, rw, nchs);
-        const float tu = time_it([&] { rc |= upd_launch(h, 0, rw, h->tune_r, &ng); });
+        const float tu = time_it([&] { rc |= upd_launch(h, 0, rw, h->tune_r, &ng, true); });
         for (int nch : nchs) {
             const float tp = time_it([&] { rc |= prod_launch(h, 0, rw, nch); });
             if (tp + tu < best) {
