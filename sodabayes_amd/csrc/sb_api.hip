@@ -351,8 +351,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         A((void **)&h->b_kpart, sizeof(double) * C * nwg_d);
         A((void **)&h->b_cj, sizeof(double) * C * bs);
         A((void **)&h->b_carry, sizeof(double) * C * (size_t)d.nloci);
-        A((void **)&h->b_Raq, (size_t)16 * h->b_ntile * 64 * ((bs + 63) / 64));
-        A((void **)&h->b_expoa, sizeof(int) * C);
+        A((void **)&h->b_ramax, sizeof(unsigned long long) * 2 * C);
         h->gt_off.assign(d.nblocks + 1, 0);
         h->gs_off.assign(d.nblocks + 1, 0);
         h->uq0_off.assign(d.nblocks + 1, 0);
@@ -414,7 +413,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
     if (h->b_Q) {
         hipMemsetAsync(h->b_Q, 0, (size_t)16 * h->b_ntile * h->b_ldq, st);
         hipMemsetAsync(h->b_Dq, 0, (size_t)16 * h->b_ntile * h->b_ldd, st);
-        hipMemsetAsync(h->b_Raq, 0, (size_t)16 * h->b_ntile * 64 * ((bs + 63) / 64), st);
+        hipMemsetAsync(h->b_ramax, 0, sizeof(unsigned long long) * 2 * C, st);
     }
     hipMemsetAsync(h->U, 0, sizeof(double) * (size_t)h->tri_off[d.nblocks], st);
     hipMemsetAsync(h->skip, 0, (size_t)p + 64, st);
@@ -759,6 +758,7 @@ int sb_state_load(sb_handle *h, const void *buf, int64_t size)
     h->iterations = hd.iterations;
     h->rstat_nwg = 0;
     h->q_block = -1;
+    h->ra_block = -1;
     int rc = sb::init_xpx(h);
     if (rc) return rc;
     if (hip_check(hipStreamSynchronize(h->stream), "state load")) return SB_ERR_HIP;

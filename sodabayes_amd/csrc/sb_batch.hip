@@ -375,129 +375,6 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
         for (int t = 0; t < NTILE; t++) out[it][t] = acc[it][t];
 }
 
-// One workgroup per chain: r_a(i, c), i < nb, quantised to LIMBS limbs at the
-// chain's exponent (from max |r_a| over the block's rows), as the mode-0 B
-// operand blocks of the U products; rows past nb up to whole 64-row groups
-// are zero.  Coalesced reads (thread t takes rows t, t + 256, ...); the limb
-// bytes of 4,096 rows at a time are assembled in LDS (bytes: 12 KB).
-__device__ __forceinline__ void raq_wg(const Par &P, int ntile, int nb, const double *__restrict__ ra,
-                                       uint4 *__restrict__ Raq, int *__restrict__ expoa, int c, uint8_t *bytes)
-{
-    __shared__ unsigned long long ml[NT / WAVE];
-    __shared__ int e_s;
-    constexpr int CH = 4096;   // rows per LDS pass
-    const int ntc = ntile / LIMBS;
-    const int n64 = 64 * ((nb + 63) / 64);
-    const double *rc = ra + (int64_t)c * P.blocksize;
-    unsigned long long m = 0;
-    for (int i0 = 0; i0 < nb; i0 += 4 * NT) {
-        double v[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int i = i0 + u * NT + (int)threadIdx.x;
-            v[u] = rc[min(i, nb - 1)];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v[u]));
-            m = b > m ? b : m;
-        }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(m, off, WAVE);
-        m = o > m ? o : m;
-    }
-    if ((threadIdx.x & 63) == 0) ml[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < NT / WAVE; w++) m = ml[w] > m ? ml[w] : m;
-        e_s = fixed_exponent(m, LIMBS);
-        expoa[c] = e_s;
-    }
-    __syncthreads();
-    const int e = e_s;
-    for (int base = 0; base < n64; base += CH) {
-        const int len = min(CH, n64 - base);
-        for (int r = threadIdx.x; r < len; r += NT) {
-            const int i = base + r;
-            int32_t q = (int32_t)rint(ldexp(i < nb ? rc[i] : 0.0, e));
-#pragma unroll
-            for (int k = 0; k < LIMBS; k++) {
-                const int32_t d = (int32_t)(int8_t)(q & 0xFF);
-                bytes[k * CH + r] = (uint8_t)d;
-                q = (q - d) >> 8;
-            }
-        }
-        __syncthreads();
-        // blocks (I4, limb k): lane l16 + 16 g holds rows 64 I4 + 16 g .. + 15
-        for (int u = threadIdx.x; u < (len / 64) * 4 * LIMBS; u += NT) {
-            const int k = u % LIMBS, rest = u / LIMBS, g = rest & 3, I4l = rest >> 2;
-            const uint4 w = *reinterpret_cast<const uint4 *>(bytes + k * CH + 64 * I4l + 16 * g);
-            const int row = limb_row(k, c, ntc), I4 = base / 64 + I4l;
-            Raq[((int64_t)I4 * ntile + (row >> 4)) * 64 + (row & 15) + 16 * g] = w;
-        }
-        __syncthreads();
-    }
-}
-
-// X_s'Q of one block, by (row group, chunk) workgroups, id = rg + nrg * ch:
-//   Tp[ch][c][jk] = sum over the chunk's individuals i and the limbs k of
-//                   256^k x_(p0+jk, i) q_k(i, c),
-// one exact int64 per (chunk, chain, SNP) (columns of length blocksize).
-// The four waves take RW consecutive SNP tiles each (a workgroup covers
-// 64 RW SNPs) and walk the chunk's super-steps together (limb_mainloop), so
-// each super-step's limb slab is read once per workgroup.  Chunk ch covers
-// super-steps [ch nS / nch, (ch + 1) nS / nch) of 256 individuals.  A lane
-// holds the three limbs of its chains in the same accumulator position of
-// column tiles tc, ntc + tc, 2 ntc + tc and forms the int64 value itself.
-// Tiles past the block re-read its last tile and store nothing.  The
-// first nchains workgroups instead quantise r_a for the block's U_s'r_a
-// (raq_wg), which this launch precedes.
-template <int NTILE, int RW>
-__global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
-                                             const uint4 *__restrict__ Q, int nch, long long *__restrict__ Tp,
-                                             int64_t tstride, int nrg, const double *__restrict__ ra,
-                                             uint4 *__restrict__ Raq, int *__restrict__ expoa)
-{
-    constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
-    static_assert(2 * SLAB * 64 * 16 >= LIMBS * 4096, "the r_a limb bytes fit the slab ring");
-    __shared__ uint4 lds[2 * SLAB * 64];
-    if ((int)blockIdx.x < P.nchains) {   // the first nchains workgroups: r_a limbs for U_s'r_a
-        raq_wg(P, NTILE, nb, ra, Raq, expoa, blockIdx.x, reinterpret_cast<uint8_t *>(lds));
-        return;
-    }
-    const int id = blockIdx.x - P.nchains;
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int g = lane >> 4, l16 = lane & 15;
-    const int rg = id % nrg, ch = id / nrg;
-    const int nS = P.ldw / 16;
-    const int S0 = (int)((int64_t)ch * nS / nch), S1 = (int)((int64_t)(ch + 1) * nS / nch);
-    const int tile0 = (rg * W + wave) * RW;
-    int64_t toff[RW];
-#pragma unroll
-    for (int it = 0; it < RW; it++) toff[it] = (int64_t)min(tile0 + it, nbt - 1) * nS;
-    v4i acc[RW][NTILE];
-    limb_mainloop<NTILE, RW>(gsw + lane, toff, Q, S0, S1, lds, acc);
-    const int C = P.nchains;
-    long long *out = Tp + (int64_t)ch * tstride;
-#pragma unroll
-    for (int it = 0; it < RW; it++) {
-        const int jk0 = 16 * (tile0 + it) + 4 * g;
-#pragma unroll
-        for (int tc = 0; tc < NTC; tc++) {
-            const int c = 16 * tc + l16;
-            if (c >= C) continue;
-            long long *o = out + (int64_t)c * P.blocksize + jk0;
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                if (jk0 + e < nb)
-                    o[e] = (long long)acc[it][tc][e] + 256LL * acc[it][NTC + tc][e] +
-                           65536LL * acc[it][2 * NTC + tc][e];
-        }
-    }
-}
-
 // Products with a block's upper factor U on int8 MFMA, both operands as
 // balanced radix-256 limbs with exact int32 sums:
 //   mode 0 (rhs):    part[g][c][j] = sum over rows i <= j of group g of U(i, j) r_a(i, c)
@@ -563,7 +440,8 @@ constexpr int uq_lds(int ntile) { return UQ_GROUP * ntile * 64; }   // uint4 of 
 
 template <int MODE, int NTILE>
 __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
-                                        const uint4 *__restrict__ Bq, const int *__restrict__ ue,
+                                        const uint4 *__restrict__ Bq, const double *__restrict__ ra,
+                                        const unsigned long long *__restrict__ amax, const int *__restrict__ ue,
                                         const int *__restrict__ ce, double *__restrict__ part, int k,
                                         uint4 *lds)
 {
@@ -616,19 +494,66 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
     for (int s = 0; s < KR; s++)
 #pragma unroll
         for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(k0 + min(s, nk - 1), u)];
-    constexpr int NBQ = (KR * NTILE + W - 1) / W;   // B blocks per wave and round
+    const int C = P.nchains;
+    int ea[NTC];   // mode 0: the exponent of chain 16 tc + l16's r_a limbs
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-        v4i bq[NBQ];
+    for (int tc = 0; tc < NTC; tc++) ea[tc] = MODE == 0 ? fixed_exponent(amax[min(16 * tc + l16, C - 1)], LIMBS) : 0;
+    if (MODE == 0) {
+        // B from r_a itself, as LIMBS balanced limbs of rint(r_a 2^e) with e
+        // from the chain's max |r_a| over the block's rows (zero past the
+        // block's rows and the chains).  Unit (s, g4, tc): lane l16 + 16 q
+        // takes rows 64 (k0 + s) + 16 g4 + 4 q .. + 3 of chain 16 tc + l16
+        // (32 contiguous bytes) and writes dword q of its limbs' B entries
+        // (lane l16 + 16 g4 of blocks (s, kk NTC + tc)).
+        uint32_t *Lw = reinterpret_cast<uint32_t *>(lds);
+        const int q = lane >> 4;
+        for (int u = wave; u < nk * 4 * NTC; u += W) {   // wave-uniform
+            const int tc = u % NTC, sg = u / NTC, s = sg >> 2, g4 = sg & 3;
+            const int c = 16 * tc + l16, i0 = 64 * (k0 + s) + 16 * g4 + 4 * q;
+            const double *rp = ra + (int64_t)min(c, C - 1) * P.blocksize + i0;
+            double v[4];
+            if (c < C && i0 + 4 <= nb && (P.blocksize & 1) == 0) {
+                const double2 x = reinterpret_cast<const double2 *>(rp)[0], y = reinterpret_cast<const double2 *>(rp)[1];
+                v[0] = x.x;
+                v[1] = x.y;
+                v[2] = y.x;
+                v[3] = y.y;
+            } else {
 #pragma unroll
-        for (int j = 0; j < NBQ; j++) {
-            const int q = KR * NTILE * r + wave + W * j, s = min(q / NTILE, nk - 1), t = q % NTILE;
-            bq[j] = Bv[bidx(k0 + s, t)];
+                for (int b = 0; b < 4; b++) v[b] = (c < C && i0 + b < nb) ? rp[b] : 0.0;
+            }
+            int e = ea[0];
+#pragma unroll
+            for (int t = 1; t < NTC; t++) e = tc == t ? ea[t] : e;
+            uint32_t w[LIMBS] = {};
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                int32_t qv = (int32_t)rint(ldexp(v[b], e));
+#pragma unroll
+                for (int kk = 0; kk < LIMBS; kk++) {
+                    const int32_t d = (int32_t)(int8_t)(qv & 0xFF);
+                    w[kk] |= (uint32_t)(uint8_t)d << (8 * b);
+                    qv = (qv - d) >> 8;
+                }
+            }
+#pragma unroll
+            for (int kk = 0; kk < LIMBS; kk++) Lw[((s * NTILE + kk * NTC + tc) * 64 + l16 + 16 * g4) * 4 + q] = w[kk];
         }
+    } else {
+        constexpr int NBQ = (KR * NTILE + W - 1) / W;   // B blocks per wave and round
 #pragma unroll
-        for (int j = 0; j < NBQ; j++) {
-            const int q = KR * NTILE * r + wave + W * j;
-            if (wave + W * j < KR * NTILE && q < nk * NTILE) L[q * 64 + lane] = bq[j];
+        for (int r = 0; r < NR; r++) {
+            v4i bq[NBQ];
+#pragma unroll
+            for (int j = 0; j < NBQ; j++) {
+                const int q = KR * NTILE * r + wave + W * j, s = min(q / NTILE, nk - 1), t = q % NTILE;
+                bq[j] = Bv[bidx(k0 + s, t)];
+            }
+#pragma unroll
+            for (int j = 0; j < NBQ; j++) {
+                const int q = KR * NTILE * r + wave + W * j;
+                if (wave + W * j < KR * NTILE && q < nk * NTILE) L[q * 64 + lane] = bq[j];
+            }
         }
     }
     // one accumulator per chain tile and weight 256^(ku + kr): the limb
@@ -672,12 +597,11 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
     // D[m][n] of the 16x16 MFMA: m = 4 g + e (column j in mode 0, row i in
     // mode 1), n = l16 (chain 16 tc + l16); the weights combined in FP64
     // (Horner from the top weight), then scaled by 2^-(e_U + e_chain).
-    const int C = P.nchains;
 #pragma unroll
     for (int tc = 0; tc < NTC; tc++) {
         const int c = 16 * tc + l16;
         if (c >= C) continue;
-        const int ec = ce[c];
+        const int ec = MODE == 0 ? ea[tc] : ce[c];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const int o = 16 * O + 4 * g + e;
@@ -690,13 +614,70 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
     }
 }
 
-template <int MODE, int NTILE>
-__global__ __launch_bounds__(NT) void k_uq(Par P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
-                                           const uint4 *__restrict__ Bq, const int *__restrict__ ue,
-                                           const int *__restrict__ ce, double *__restrict__ part)
+
+// X_s'Q of one block, by (row group, chunk) workgroups, id = rg + nrg * ch:
+//   Tp[ch][c][jk] = sum over the chunk's individuals i and the limbs k of
+//                   256^k x_(p0+jk, i) q_k(i, c),
+// one exact int64 per (chunk, chain, SNP) (columns of length blocksize).
+// The four waves take RW consecutive SNP tiles each (a workgroup covers
+// 64 RW SNPs) and walk the chunk's super-steps together (limb_mainloop), so
+// each super-step's limb slab is read once per workgroup.  Chunk ch covers
+// super-steps [ch nS / nch, (ch + 1) nS / nch) of 256 individuals.  A lane
+// holds the three limbs of its chains in the same accumulator position of
+// column tiles tc, ntc + tc, 2 ntc + tc and forms the int64 value itself.
+// Tiles past the block re-read its last tile and store nothing.  The
+// workgroups after the nrg nch of X_s'Q are the pairs of U_s'r_a (uq_pair,
+// mode 0), which quantise r_a themselves with the chain exponents from
+// amax (max |r_a| bits per chain over the block's rows); workgroup 0 also
+// clears anext, where the r_a reduction after this block's update collects
+// the next block's maxima.
+template <int NTILE, int RW>
+__global__ __launch_bounds__(NT) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
+                                             const uint4 *__restrict__ Q, int nch, long long *__restrict__ Tp,
+                                             int64_t tstride, int nrg, const uint4 *__restrict__ Ua,
+                                             const double *__restrict__ ra,
+                                             const unsigned long long *__restrict__ amax,
+                                             unsigned long long *__restrict__ anext, const int *__restrict__ ue,
+                                             double *__restrict__ upart)
 {
-    __shared__ uint4 lds[uq_lds(NTILE)];
-    uq_pair<MODE, NTILE>(P, Ua, nbt, nb, nss, Bq, ue, ce, part, blockIdx.x, lds);
+    constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
+    static_assert(2 * SLAB * 64 >= uq_lds(NTILE), "LDS of the U pairs");
+    __shared__ uint4 lds[2 * SLAB * 64];
+    const int id = blockIdx.x;
+    if (id == 0 && (int)threadIdx.x < P.nchains) anext[threadIdx.x] = 0;
+    if (id >= nrg * nch) {
+        uq_pair<0, NTILE>(P, Ua, nbt, nb, (nb + 255) / 256, nullptr, ra, amax, ue, nullptr, upart, id - nrg * nch,
+                          lds);
+        return;
+    }
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int rg = id % nrg, ch = id / nrg;
+    const int nS = P.ldw / 16;
+    const int S0 = (int)((int64_t)ch * nS / nch), S1 = (int)((int64_t)(ch + 1) * nS / nch);
+    const int tile0 = (rg * W + wave) * RW;
+    int64_t toff[RW];
+#pragma unroll
+    for (int it = 0; it < RW; it++) toff[it] = (int64_t)min(tile0 + it, nbt - 1) * nS;
+    v4i acc[RW][NTILE];
+    limb_mainloop<NTILE, RW>(gsw + lane, toff, Q, S0, S1, lds, acc);
+    const int C = P.nchains;
+    long long *out = Tp + (int64_t)ch * tstride;
+#pragma unroll
+    for (int it = 0; it < RW; it++) {
+        const int jk0 = 16 * (tile0 + it) + 4 * g;
+#pragma unroll
+        for (int tc = 0; tc < NTC; tc++) {
+            const int c = 16 * tc + l16;
+            if (c >= C) continue;
+            long long *o = out + (int64_t)c * P.blocksize + jk0;
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (jk0 + e < nb)
+                    o[e] = (long long)acc[it][tc][e] + 256LL * acc[it][NTC + tc][e] +
+                           65536LL * acc[it][2 * NTC + tc][e];
+        }
+    }
 }
 
 // Limb exponents of U, per block: ue0[j] from max over i <= j of |U(i, j)|,
@@ -785,47 +766,91 @@ __global__ __launch_bounds__(NT) void k_pack_uq(const double *U, int64_t tri0, i
     }
 }
 
+// max over the workgroup of m (any value from threads past the data is 0)
+__device__ __forceinline__ unsigned long long wg_max(unsigned long long m)
+{
+    __shared__ unsigned long long ml[NT / WAVE];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(m, off, WAVE);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) ml[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = ml[0];
+#pragma unroll
+    for (int w = 1; w < NT / WAVE; w++) m = ml[w] > m ? ml[w] : m;
+    return m;
+}
+
 // r_a(i, c) += sum of the mode-1 partials of the column groups that reach row
-// i's tile (groups left of the diagonal wrote nothing).
+// i's tile (groups left of the diagonal wrote nothing), rows i < nb.  With
+// amax, also the max |r_a(i, c)| bits over the next block's rows i < nbn
+// (atomic max into amax[c], cleared by the launch of the block's products).
 __device__ __forceinline__ void ra_reduce_wg(const Par &P, const double *part, int spant, int ngroups, int nb,
-                                             double *ra, int bx, int c)
+                                             double *ra, int bx, int c, int nbn, unsigned long long *amax)
 {
     const int i = bx * NT + threadIdx.x;
-    if (i >= nb) return;
     const int C = P.nchains;
     const int64_t gstride = (int64_t)C * P.blocksize;
-    const double *pp = part + (int64_t)c * P.blocksize + i;
-    double s = 0.0;
-    for (int g0 = (i >> 4) / spant; g0 < ngroups; g0 += 8) {
-        double v[8];
+    double *rc = ra + (int64_t)c * P.blocksize;
+    double v = 0.0;
+    if (i < nb) {
+        const double *pp = part + (int64_t)c * P.blocksize + i;
+        double s = 0.0;
+        for (int g0 = (i >> 4) / spant; g0 < ngroups; g0 += 8) {
+            double t[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = g0 + u < ngroups ? pp[(g0 + u) * gstride] : 0.0;
+            for (int u = 0; u < 8; u++) t[u] = g0 + u < ngroups ? pp[(g0 + u) * gstride] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 8; u++) s += v[u];
+            for (int u = 0; u < 8; u++) s += t[u];
+        }
+        v = rc[i] + s;
+        rc[i] = v;
+    } else if (i < nbn) {
+        v = rc[i];
     }
-    const double v = ra[(int64_t)c * P.blocksize + i] + s;
-    ra[(int64_t)c * P.blocksize + i] = v;
+    if (amax) {   // workgroup-uniform
+        const unsigned long long m = wg_max(i < nbn ? (unsigned long long)__double_as_longlong(fabs(v)) : 0ull);
+        if (threadIdx.x == 0) atomicMax(amax + c, m);
+    }
 }
 
 __global__ __launch_bounds__(NT) void k_ra_reduce(Par P, const double *part, int spant, int ngroups, int nb,
                                                   double *ra)
 {
-    ra_reduce_wg(P, part, spant, ngroups, nb, ra, blockIdx.x, blockIdx.y);
+    ra_reduce_wg(P, part, spant, ngroups, nb, ra, blockIdx.x, blockIdx.y, 0, nullptr);
+}
+
+// amax[c] = max |r_a(i, c)| bits over rows i < nb, one workgroup per chain
+// (r_a set outside the batched update: the first block of a sweep)
+__global__ __launch_bounds__(NT) void k_ramax(Par P, const double *ra, int nb, unsigned long long *amax)
+{
+    const int c = blockIdx.x;
+    const double *rc = ra + (int64_t)c * P.blocksize;
+    unsigned long long m = 0;
+    for (int i = threadIdx.x; i < nb; i += NT) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(rc[i]));
+        m = b > m ? b : m;
+    }
+    m = wg_max(m);
+    if (threadIdx.x == 0) amax[c] = m;
 }
 
 // The r_a reduction of this block beside the r limbs of the next block (both
 // need only the update just done): workgroups [0, nq * C) quantise r,
-// the rest reduce r_a.
+// the rest (nra per chain, covering max(nb, nbn) rows) reduce r_a and
+// collect the next block's max |r_a| (nbn rows) into amax.
 __global__ __launch_bounds__(NT) void k_rared_quant(Par P, const double *part, int spant, int ngroups, int nb,
                                                     double *ra, int nra, int ntile, const double *r,
                                                     const uint64_t *rmax, int nwg, uint4 *Q, int *expo,
-                                                    long long *qsum, int nq)
+                                                    long long *qsum, int nq, int nbn, unsigned long long *amax)
 {
     const int id = blockIdx.x, C = P.nchains;
     if (id < nq * C)
         quant_wg(P, ntile, r, rmax, nwg, Q, expo, qsum, nq, id % nq, id / nq);
     else
-        ra_reduce_wg(P, part, spant, ngroups, nb, ra, (id - nq * C) % nra, (id - nq * C) / nra);
+        ra_reduce_wg(P, part, spant, ngroups, nb, ra, (id - nq * C) % nra, (id - nq * C) / nra, nbn, amax);
 }
 
 // SNP-major genotype tiles of one block for k_xq: element ((tile * nS + S) *
@@ -997,7 +1022,7 @@ __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt,
     if ((int)blockIdx.x >= ng) {
         // the workgroups after the update's: pairs of U_s diag(s) c~ (same
         // limbs Dq and exponents), filling the CUs the update leaves idle
-        uq_pair<1, NTILE>(P, Ua, nbt, nb, nss, Dq, ue, expo, upart, (int)blockIdx.x - ng, lds);
+        uq_pair<1, NTILE>(P, Ua, nbt, nb, nss, Dq, nullptr, nullptr, ue, expo, upart, (int)blockIdx.x - ng, lds);
         return;
     }
     const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
@@ -1225,12 +1250,14 @@ int upd_groups(const sb_handle *h, int rw)
 // X_s'Q of block b with row tiles per wave rw and nch chunks, and the r_a
 // limbs of the block's U_s'r_a (one launch, through the handle's kernel
 // clock)
-int prod_launch(sb_handle *h, int b, int rw_sel, int nch)
+int prod_launch(sb_handle *h, int b, int rw_sel, int nch, bool with_u)
 {
     const Par &P = h->par;
     hipStream_t st = h->stream;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
     const int C = P.nchains;
+    const int64_t nup = with_u ? uq_pairs(nbt, batch_words(nb) / 16, 0) : 0;   // U_s'r_a pairs
+    unsigned long long *amax = h->b_ramax + (int64_t)(b & 1) * C, *anext = h->b_ramax + (int64_t)((b + 1) & 1) * C;
     const uint4 *Q = reinterpret_cast<const uint4 *>(h->b_Q);
     const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
     const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
@@ -1240,9 +1267,10 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int nch)
         with_rw<NTL>(rw_sel, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
             const int nrg = (nbt + 4 * RW - 1) / (4 * RW);
-            launch(kc, k_prod<NTL, RW>, dim3(C + nrg * nch), dim3(NT), 0, st, P, gsw, nbt, nb, Q, nch,
-                   reinterpret_cast<long long *>(h->b_T), tstride, nrg, h->ra, reinterpret_cast<uint4 *>(h->b_Raq),
-                   h->b_expoa);
+            launch(kc, k_prod<NTL, RW>, dim3(nrg * nch + (unsigned)nup), dim3(NT), 0, st, P, gsw, nbt, nb, Q, nch,
+                   reinterpret_cast<long long *>(h->b_T), tstride, nrg,
+                   reinterpret_cast<const uint4 *>(h->Uq0) + h->uq0_off[b], h->ra, amax, anext,
+                   h->ue0 + (int64_t)b * P.blocksize, h->b_rapart);
         });
     });
 }
@@ -1275,33 +1303,6 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out, bool wit
     });
 }
 
-// U products of block b on int8 limbs: mode 0 U_s'r_a (r_a limbs) into the
-// r_a partials of the draw, mode 1 U_s diag(s) c~ (the update's limbs Dq)
-// into the partials of the r_a update
-int ut_launch(sb_handle *h, int b, int mode)
-{
-    const Par &P = h->par;
-    const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
-    const int C = P.nchains, nss = batch_words(nb) / 16;
-    const int64_t npair = uq_pairs(nbt, nss, mode);
-    const uint4 *ua = reinterpret_cast<const uint4 *>(mode == 0 ? h->Uq0 : h->Uq1) +
-                      (mode == 0 ? h->uq0_off[b] : h->uq1_off[b]);
-    const uint4 *bq = reinterpret_cast<const uint4 *>(mode == 0 ? h->b_Raq : h->b_Dq);
-    const int *ue = (mode == 0 ? h->ue0 : h->ue1) + (int64_t)b * P.blocksize;
-    const int *ce = mode == 0 ? h->b_expoa : h->b_expo2;
-    double *part = mode == 0 ? h->b_rapart : h->b_uapart;
-    KClock &kc = h->kc;
-    return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
-        constexpr int NTL = decltype(ntl)::value;
-        if (mode == 0)
-            launch(kc, k_uq<0, NTL>, dim3((unsigned)npair), dim3(NT), 0, h->stream, P, ua, nbt, nb, nss, bq, ue, ce,
-                   part);
-        else
-            launch(kc, k_uq<1, NTL>, dim3((unsigned)npair), dim3(NT), 0, h->stream, P, ua, nbt, nb, nss, bq, ue, ce,
-                   part);
-    });
-}
-
 int batch_block(sb_handle *h, int b, double zz, int rep)
 {
     (void)zz;
@@ -1327,12 +1328,15 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
                h->b_expo, h->b_qsum);
     }
     h->q_block = -1;
+    if (h->ra_block != b) {
+        // r_a set outside the batched update (drawn for the sweep): the
+        // chain maxima of its limbs (once per sweep, not timed)
+        launch(kc, k_ramax, dim3(C), dim3(NT), 0, st, P, h->ra, nb, h->b_ramax + (int64_t)(b & 1) * C);
+    }
+    h->ra_block = -1;
     kc.begin(st);
-    kc.last(KS_XQ, gbytes + cl * n + 8.0 * C * nb * h->b_nch);
-    int rc = prod_launch(h, b, h->b_rw, h->b_nch);
-    kc.begin(st);
-    kc.last(KS_UT0, tbytes + 12.0 * C * nb);
-    if (!rc) rc = ut_launch(h, b, 0);
+    kc.last(KS_XQ, gbytes + cl * n + 8.0 * C * nb * h->b_nch + tbytes + 16.0 * C * nb);
+    const int rc = prod_launch(h, b, h->b_rw, h->b_nch, true);   // with U_s'r_a
     if (rc) return fail(rc, "unsupported chain count");
     return hip_check(hipGetLastError(), "batch products");
 }
@@ -1359,17 +1363,20 @@ int batch_update(sb_handle *h, int b)
     const int rc = upd_launch(h, b, h->b_rw, h->r, &ng, true);   // with U_s diag(s) c~
     if (rc) return fail(rc, "unsupported chain count");
     h->rstat_nwg = ng;   // max |r| partials of the new r, per update workgroup
-    const int nra = (nb + NT - 1) / NT;
     kc.begin(st);
     if (b + 1 < P.nblocks) {
-        // the next block's r limbs in the same launch
-        const int nq = h->b_nq;
+        // the next block's r limbs and r_a maxima in the same launch
+        const int nq = h->b_nq, nbn = h->ls[b + 2] - h->ls[b + 1];
+        const int nra = (std::max(nb, nbn) + NT - 1) / NT;
         uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
         kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 8.0 * C * n + cl * n);
         launch(kc, k_rared_quant, dim3((nq + nra) * C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra,
-               nra, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_qsum, nq);
+               nra, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_qsum, nq, nbn,
+               h->b_ramax + (int64_t)((b + 1) & 1) * C);
         h->q_block = b + 1;
+        h->ra_block = b + 1;
     } else {
+        const int nra = (nb + NT - 1) / NT;
         kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0));
         launch(kc, k_ra_reduce, dim3(nra, C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra);
     }
@@ -1439,7 +1446,7 @@ int batch_autotune(sb_handle *h)
         tune_chunks(h, rw, nchs);
         const float tu = time_it([&] { rc |= upd_launch(h, 0, rw, h->tune_r, &ng, true); });
         for (int nch : nchs) {
-            const float tp = time_it([&] { rc |= prod_launch(h, 0, rw, nch); });
+            const float tp = time_it([&] { rc |= prod_launch(h, 0, rw, nch, true); });
             if (tp + tu < best) {
                 best = tp + tu;
                 brw = rw;
@@ -1457,6 +1464,7 @@ int batch_autotune(sb_handle *h)
     h->tune_us = best * 1e3;
     h->rstat_nwg = 0;   // the trial updates overwrote the residual statistics
     h->q_block = -1;
+    h->ra_block = -1;
     return hip_check(hipGetLastError(), "tile-shape tuning");
 }
 

@@ -295,8 +295,8 @@ struct sb_handle {
     std::vector<int64_t> uq1_off; // uint4 offset of each block in Uq1
     int *ue0 = nullptr;           // nblocks x blocksize limb exponents of U's columns (mode 0)
     int *ue1 = nullptr;           // nblocks x blocksize limb exponents of U diag(s)'s rows (mode 1)
-    int8_t *b_Raq = nullptr;      // r_a limbs of every chain, B-operand blocks of U' r_a
-    int *b_expoa = nullptr;       // fixed-point exponent of r_a per chain
+    unsigned long long *b_ramax = nullptr;   // 2 x nchains: max |r_a| bits per chain, block parity
+    int ra_block = -1;            // block whose r_a maxima the last update already collected (-1: none)
 
     std::vector<void *> allocs;
 };
