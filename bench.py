@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 KERNEL_NAMES = {
     "rhs": "k_rhs (X_s'r + U_s'r_a, FP64)", "draw": "k_draw_counter", "list": "k_list",
     "update": "k_update (r += Z delta, r_a += U delta)", "rquant": "k_rstat + k_quant (r limbs, first block)",
-    "xq": "k_prod (X_s'Q + r_a limbs, int8 MFMA)", "ut0": "k_uq<0> (U_s'r_a, int8 limbs of U and r_a)",
+    "xq": "k_prod (X_s'Q + U_s'r_a, int8 MFMA)", "ut0": "U_s'r_a (inside k_prod)",
     "drawb": "k_draw_b (limb sums + draw)", "dquant": "k_dquant (delta limbs)",
-    "xupd": "k_upd (R += X_s C, int8 MFMA)", "ut1": "k_uq<1> (U_s diag(s) c, int8 limbs)",
+    "xupd": "k_upd (R += X_s C + U_s diag(s) c, int8 MFMA)", "ut1": "U_s diag(s) c (inside k_upd)",
     "rared": "k_rared_quant (r_a reduction + next block's r limbs)",
 }
 

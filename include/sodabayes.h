@@ -182,9 +182,10 @@ enum {
 
 /* Kernel timing slots of SB_Q_KTIMES (diagnostics; no counterpart in the
  * reference).  FP64 path: block products, draw, changed-SNP list, update.
- * Batched path: residual statistics + limb quantisation, X_s'Q, U_s'R_a,
- * limb combination + draw, delta quantisation, X_s C update, U_s D, r_a
- * reduction. */
+ * Batched path: residual statistics + limb quantisation, X_s'Q with U_s'R_a,
+ * limb combination + draw, delta quantisation, X_s C update with U_s D, r_a
+ * reduction.  The U products run inside the X_s'Q and update launches, so
+ * SB_KT_UT0 and SB_KT_UT1 stay empty (kept for the slot order). */
 enum {
     SB_KT_RHS = 0, SB_KT_DRAW, SB_KT_LIST, SB_KT_UPDATE,
     SB_KT_RQUANT, SB_KT_XQ, SB_KT_UT0, SB_KT_DRAWB, SB_KT_DQUANT, SB_KT_XUPD, SB_KT_UT1, SB_KT_RARED,

@@ -383,11 +383,14 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
 // (mode 0) or per row i of U diag(s) (mode 1), set from that column's or
 // row's largest entry (31 bits of it), so every entry keeps its absolute
 // precision 2^-31 max.  r_a has three limbs per chain at one exponent per
-// chain (quantised by the X'Q launch before this one, raq_wg); mode 1
-// multiplies with the limbs of c~ = delta / s + carry that the genotype
-// update applies (k_dquant), which is why its U columns carry s_j.  A group
-// is 256 rows (mode 0) or one super-step of 256 columns (mode 1), as the
-// partial sums of the draw and of the r_a reduction expect.
+// chain, from the chain's max |r_a| over the block's rows (each pair
+// quantises the rows it needs); mode 1 multiplies with the limbs of
+// c~ = delta / s + carry that the genotype update applies (k_dquant), which
+// is why its U columns carry s_j.  A group is 64 UQ_GROUP rows (mode 0) or
+// columns (mode 1), the row split of the partial sums that the draw and the
+// r_a reduction add up.  The pairs run as the trailing workgroups of the
+// genotype launches (mode 0 after X_s'Q, mode 1 after the residual update),
+// where they fill the CUs that the genotype workgroups leave idle.
 //
 // K steps of one MFMA (64 values of the summed index):
 //   mode 0: row group I4 (rows 64 I4 + 16 g + b, byte b of lane group g);
@@ -397,7 +400,8 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
 //   mode 0: Uq0[((off0(J) + I4) * ULIMBS + ku) * 64 + lane], column j = 16 J + l16;
 //   mode 1: Uq1[(((off1(I) + ss - I / 16) * 4 + s) * ULIMBS + ku) * 64 + lane], row i = 16 I + l16.
 // B operand blocks (limb-major rows, as the genotype products):
-//   mode 0: Raq[(I4 * NTILE + t) * 64 + lane];  mode 1: Dq[((ss * NTILE + t) * 4 + s) * 64 + lane].
+//   mode 0: built in LDS from r_a (block (I4, t), lane l16 + 16 g: rows 64 I4 + 16 g + b);
+//   mode 1: Dq[((ss * NTILE + t) * 4 + s) * 64 + lane].
 constexpr int ULIMBS = 4;   // limbs of U (31 bits)
 __host__ __device__ __forceinline__ int64_t uq_off0(int J)
 {
@@ -427,14 +431,14 @@ __host__ __device__ __forceinline__ int64_t uq_off1(int I, int nss)
     return (int64_t)I * nss - (8 * q * (q - 1) + r * q);
 }
 
-// U products of one block over (quad, group) pairs, workgroup id = pair:
-//   mode 0: quad bx = output columns J = 4 bx + w (wave w), group by of
-//           rows: K steps I4 in [4 by, min(4 by + 4, bx + 1));
-//   mode 1: quad bx = output rows I = 4 bx + w, group ss (one super-step):
-//           K steps (ss, s), s = 0..3, ss >= bx / 4.
-// A pair has at most four K steps: every wave loads its ULIMBS A tiles of
-// all of them at once, the four waves load the steps' B blocks into LDS
-// together, and one barrier publishes them.  Waves past the block still
+// U products of one block over (quad, group) pairs, pair index k:
+//   mode 0: quad bx = output columns J = 4 bx + w (wave w), row group gi:
+//           K steps I4 in [G gi, min(G gi + G, bx + 1));
+//   mode 1: quad bx = output rows I = 4 bx + w, column group gi: K steps
+//           kk = 4 ss + s in [max(G gi, 4 (bx / 4)), min(G gi + G, 4 nss))
+// (G = UQ_GROUP).  The B blocks of all the pair's K steps go to LDS before
+// one barrier; each wave then takes the K steps in rounds of four, its
+// ULIMBS A tiles of a round loaded together.  Waves past the block still
 // load and meet the barrier, but do no MFMA and store nothing.
 constexpr int uq_lds(int ntile) { return UQ_GROUP * ntile * 64; }   // uint4 of LDS per pair
 
@@ -706,7 +710,7 @@ __global__ __launch_bounds__(NT) void k_uexp(const double *U, const int64_t *tri
 }
 
 // The ULIMBS limb tiles of one block's U: one thread per (tile entry, lane).
-// Mode 0 entries (J, I4), mode 1 entries (I, ss, s); see k_uq for the byte
+// Mode 0 entries (J, I4), mode 1 entries (I, ss, s); see uq_pair for the byte
 // order.  Entries past the triangle or the block are zero.
 __global__ __launch_bounds__(NT) void k_pack_uq(const double *U, int64_t tri0, int nb, int nbt, int nss, int p0,
                                                 const double *sdv, const int *ue0, const int *ue1, int mode,
