@@ -1005,12 +1005,14 @@ __device__ __forceinline__ void kconst_finish(const KPre &k, const double *__res
 // of ng takes the 16-individual tiles [wg ntr / ng, (wg + 1) ntr / ng) (ntr
 // = ceil(n / 16)); its four waves split them into runs of at most RW and walk
 // every super-step of the block together (limb_mainloop), so each limb slab
-// is read once per workgroup.  ng is a multiple of the CU count, so every CU
-// carries the same share of the rows.  Each lane holds the three limbs of
-// its chains in one accumulator position and updates four consecutive
-// residuals per (tile, chain tile) directly from registers.  Leaves per
-// (chain, workgroup) the maximum |r| of the new values (rmax[c * ng + wg])
-// for the next block's quantisation.
+// is read once per workgroup.  ng = ceil(ntr / (4 RW)), so nearly every wave
+// has RW tiles (padding tiles would cost MFMA time on their SIMD); the CUs
+// that hold fewer update workgroups take the U pairs that follow them in the
+// launch (workgroups ng and up, uq_pair mode 1).  Each lane holds the three
+// limbs of its chains in one accumulator position and updates four
+// consecutive residuals per (tile, chain tile) directly from registers.
+// Leaves per (chain, workgroup) the maximum |r| of the new values
+// (rmax[c * ng + wg]) for the next block's quantisation.
 template <int NTILE, int RW>
 __global__ __launch_bounds__(NT) void k_upd(Par P, const uint4 *__restrict__ gt, int nss, int ntr,
                                             const uint4 *__restrict__ Dq, const int *__restrict__ expo,
@@ -1247,12 +1249,11 @@ int prod_chunks(const sb_handle *h, int rw, int m)
 int upd_groups(const sb_handle *h, int rw)
 {
     const int ntr = (h->par.n + 15) / 16;
-    const int per = h->ncu * 4 * rw;
-    return std::max(1, std::min(ntr, h->ncu * ((ntr + per - 1) / per)));
+    return std::max(1, (ntr + 4 * rw - 1) / (4 * rw));
 }
 
-// X_s'Q of block b with row tiles per wave rw and nch chunks, and the r_a
-// limbs of the block's U_s'r_a (one launch, through the handle's kernel
+// X_s'Q of block b with row tiles per wave rw and nch chunks, and (with_u)
+// the pairs of the block's U_s'r_a (one launch, through the handle's kernel
 // clock)
 int prod_launch(sb_handle *h, int b, int rw_sel, int nch, bool with_u)
 {
