@@ -635,11 +635,11 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
 // amax (max |r_a| bits per chain over the block's rows); workgroup 0 also
 // clears anext, where the r_a reduction after this block's update collects
 // the next block's maxima.
-// Register budget for three waves per SIMD up to six chain-limb tiles (two
-// beyond): the compiler then keeps the accumulators in VGPRs, without
-// spills, and three workgroups share a CU's LDS.
+// Register budget for three waves per SIMD up to 12 accumulator tiles per
+// wave (RW x NTILE), two beyond: the compiler then keeps the accumulators in
+// VGPRs, without spills, and three workgroups share a CU's LDS.
 template <int NTILE, int RW>
-__global__ __launch_bounds__(NT, NTILE <= 6 ? 3 : 2) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
+__global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
                                              const uint4 *__restrict__ Q, int nch, long long *__restrict__ Tp,
                                              int64_t tstride, int nrg, const uint4 *__restrict__ Ua,
                                              const double *__restrict__ ra,
@@ -1017,7 +1017,7 @@ __device__ __forceinline__ void kconst_finish(const KPre &k, const double *__res
 // Leaves per (chain, workgroup) the maximum |r| of the new values
 // (rmax[c * ng + wg]) for the next block's quantisation.
 template <int NTILE, int RW>
-__global__ __launch_bounds__(NT, NTILE <= 6 ? 3 : 2) void k_upd(Par P, const uint4 *__restrict__ gt, int nss, int ntr,
+__global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, const uint4 *__restrict__ gt, int nss, int ntr,
                                             const uint4 *__restrict__ Dq, const int *__restrict__ expo,
                                             const double *__restrict__ kpart, int nkp, double *__restrict__ r,
                                             uint64_t *__restrict__ rmax, int ng, const uint4 *__restrict__ Ua,
