@@ -1247,8 +1247,8 @@ int prod_chunks(const sb_handle *h, int rw, int m)
     return std::max(1, std::min(want, std::max(1, nS / 2)));
 }
 
-// Update workgroups for row tiles per wave rw: a multiple of the CU count,
-// at most 4 rw tiles of 16 individuals each, at most one per tile
+// Update workgroups for row tiles per wave rw: ceil(tiles / (4 rw)), so each
+// takes at most 4 rw tiles of 16 individuals
 int upd_groups(const sb_handle *h, int rw)
 {
     const int ntr = (h->par.n + 15) / 16;
