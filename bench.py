@@ -49,11 +49,11 @@ KERNEL_NAMES = {
 # (profiles/soda_bayesr_n50k_bs4096_x32_pmc_traffic.md); None where not measured
 TRAFFIC_CHAINS = 32
 TRAFFIC = {
-    ("mixed", "xupd"): int((32878 * 2 + 13012) * 1024),   # k_upd<6, 2>
-    ("mixed", "xq"): int((45411 * 2 + 8961) * 1024),      # k_prod<6, 2>
-    ("mixed", "drawb"): int((9968 * 2 + 3728) * 1024),    # k_draw_b<4>
-    ("fp64", "rhs"): int((60337 * 2 + 918) * 1024),       # k_rhs
-    ("fp64", "update"): int((50645 * 2 + 604) * 1024),    # k_update (k_ra_sum not counted)
+    ("mixed", "xupd"): int((50580 * 2 + 17515) * 1024),  # k_upd<6, 2> with the U_s diag(s) c pairs
+    ("mixed", "xq"): int((65067 * 2 + 12814) * 1024),    # k_prod<6, 2> with the U_s'r_a pairs
+    ("mixed", "drawb"): int((7918 * 2 + 3728) * 1024),   # k_draw_b<4>
+    ("fp64", "rhs"): int((60338 * 2 + 919) * 1024),      # k_rhs
+    ("fp64", "update"): int((50644 * 2 + 604) * 1024),   # k_update (k_ra_sum not counted)
 }
 
 
