@@ -1113,8 +1113,8 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
 // quantisation of SNP j's update left unapplied (error feedback, so the
 // rounding of the fixed-point update never accumulates in r); c_j goes to
 // cj_out for k_dquant, and per workgroup max |c_j| sets its exponent.
-constexpr int DCH = 16;  // chunk partials unrolled in k_draw_b
-constexpr int DRG = 16;  // r_a row-group partials unrolled in k_draw_b
+constexpr int DCH = 8;   // chunk partials unrolled in k_draw_b
+constexpr int DRG = 8;   // r_a row-group partials unrolled in k_draw_b (4,096 rows)
 template <int NM>
 __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long long *Tp,
                                                int64_t tstride, int nch, const int *expo, const long long *qsum, int nq,
