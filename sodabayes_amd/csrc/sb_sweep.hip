@@ -599,22 +599,85 @@ __global__ __launch_bounds__(NT) void k_bins(Par P, const double *beta, const in
     }
 }
 
+// The same, every bin in one pass over the chunk (nbins <= NBM): each thread
+// adds to its bins' sums in the order of the per-bin pass, so the sums are
+// bit-identical, and beta / cls / grp are read once instead of once per bin.
+template <int NBM>
+__global__ __launch_bounds__(NT) void k_bins_all(Par P, const double *beta, const int32_t *cls,
+                                                 const int32_t *grp, int chunk, double *cnt_out, double *var_out)
+{
+    __shared__ double lds[NT / WAVE];
+    const int wg = blockIdx.x, ch = blockIdx.z;
+    beta += (int64_t)ch * P.p;
+    cls += (int64_t)ch * P.p;
+    cnt_out += (int64_t)ch * P.nbins * gridDim.x;
+    var_out += (int64_t)ch * P.nbins * gridDim.x;
+    const bool all = P.method == SB_BAYES_A;
+    const int a = wg * chunk, e = min(P.p, a + chunk);
+    double c[NBM], v[NBM];
+#pragma unroll
+    for (int b = 0; b < NBM; b++) c[b] = v[b] = 0.0;
+    for (int pos = a + threadIdx.x; pos < e; pos += NT) {
+        const int gq = grp[pos], kq = cls[pos] - 1;   // bin (class kq, group gq), as k_bins matches it
+        const int bp = (gq >= 0 && gq < P.ngroup && kq >= 0 && kq < P.nmix) ? kq * P.ngroup + gq : -1;
+        const double x = beta[pos] * beta[pos];
+#pragma unroll
+        for (int b = 0; b < NBM; b++) {
+            const bool in = all || b == bp;
+            c[b] += in ? 1.0 : 0.0;
+            v[b] += in ? x : 0.0;
+        }
+    }
+    for (int b = 0; b < P.nbins; b++) {   // workgroup-uniform
+        double cb = 0.0, vb = 0.0;
+#pragma unroll
+        for (int u = 0; u < NBM; u++) {
+            cb = u == b ? c[u] : cb;
+            vb = u == b ? v[u] : vb;
+        }
+        cb = block_sum<NT>(cb, lds);
+        vb = block_sum<NT>(vb, lds);
+        if (threadIdx.x == 0) {
+            cnt_out[(int64_t)b * gridDim.x + wg] = cb;
+            var_out[(int64_t)b * gridDim.x + wg] = vb;
+        }
+    }
+}
+
 // Hyperparameter draws, stores and the trace record (models.f90:496-569).
+// One workgroup per chain: its threads stage the chain's partial sums (bins,
+// r'r) in LDS with coalesced loads, then thread 0 adds them in the order of
+// a serial loop and draws.
+constexpr int HYP_LDS = 4096;   // doubles of staged partials
 __global__ void k_hyper(Par P, Scal *sc, const double *red_rtr, int nrtr, const double *cnt,
                         const double *var, int nwg, int *it, double *hyp)
 {
-    const int ch = threadIdx.x;
-    if (ch >= P.nchains) return;
-    // it[0]: this iteration; it[1]: its record index in hyp (models.f90:526, 549)
+    __shared__ double stg[HYP_LDS];
+    const int ch = blockIdx.x;
+    const int nb = P.nbins;
+    const int64_t nbw = (int64_t)nb * nwg;
+    red_rtr += (int64_t)ch * nrtr;
+    cnt += (int64_t)ch * nbw;
+    var += (int64_t)ch * nbw;
+    if (2 * nbw + nrtr <= HYP_LDS) {   // workgroup-uniform
+        for (int i = threadIdx.x; i < nbw; i += blockDim.x) {
+            stg[i] = cnt[i];
+            stg[nbw + i] = var[i];
+        }
+        for (int i = threadIdx.x; i < nrtr; i += blockDim.x) stg[2 * nbw + i] = red_rtr[i];
+        __syncthreads();
+        cnt = stg;
+        var = stg + nbw;
+        red_rtr = stg + 2 * nbw;
+    }
+    if (threadIdx.x != 0) return;
+    // it[0]: this iteration; it[1]: its record index in hyp, read and advanced
+    // by chain 0's workgroup only (models.f90:526, 549)
     const int rep = it[0];
     const bool store = rep > P.burnin && rep % P.thin == 0;
     const bool record = hyp != nullptr && rep % P.thin == 0;
-    const int hyp_idx = record ? it[1] : -1;
-    const int nb = P.nbins;
+    const int hyp_idx = record && ch == 0 ? it[1] : -1;
     sc += ch;
-    red_rtr += (int64_t)ch * nrtr;
-    cnt += (int64_t)ch * nb * nwg;
-    var += (int64_t)ch * nb * nwg;
     if (ch != 0) hyp = nullptr;
     for (int b = 0; b < nb; b++) {
         double c = 0.0, v = 0.0;
@@ -1342,8 +1405,13 @@ static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd
         });
     }
     k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);
-    k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
-    k_hyper<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, h->d_it, h->hyp);
+    if (P.nbins <= 4)
+        k_bins_all<4><<<dim3(nwg_bins, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+    else if (P.nbins <= 16)
+        k_bins_all<16><<<dim3(nwg_bins, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+    else
+        k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+    k_hyper<<<C, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, h->d_it, h->hyp);
     if (P.naind > 0) {
         if (P.rng_mode == SB_RNG_REPLAY)
             k_ra_replay<<<1, 1, 0, st>>>(P, h->scal, h->ra);
