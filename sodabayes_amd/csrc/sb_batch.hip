@@ -836,9 +836,15 @@ __global__ __launch_bounds__(NT) void k_ramax(Par P, const double *ra, int nb, u
     const int c = blockIdx.x;
     const double *rc = ra + (int64_t)c * P.blocksize;
     unsigned long long m = 0;
-    for (int i = threadIdx.x; i < nb; i += NT) {
-        const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(rc[i]));
-        m = b > m ? b : m;
+    for (int i0 = 0; i0 < nb; i0 += 16 * NT) {   // 16 loads in flight per thread
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = rc[min(i0 + u * NT + (int)threadIdx.x, nb - 1)];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v[u]));
+            m = b > m ? b : m;
+        }
     }
     m = wg_max(m);
     if (threadIdx.x == 0) amax[c] = m;

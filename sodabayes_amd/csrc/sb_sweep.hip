@@ -50,13 +50,22 @@ __global__ __launch_bounds__(NT) void k_mu_pre(Par P, double *r, const Scal *sca
 
 // mu ~ N(sum(r)/n, vare/n) (models.f90:131) and the per-sweep constants
 // log_p, vare_gp (models.f90:330-338)
+// One workgroup per chain: the partial sums staged in LDS by coalesced loads,
+// then added by thread 0 in the order of a serial loop.
+constexpr int MU_LDS = 1024;   // doubles of staged partials
 __global__ void k_mu_draw(Par P, Scal *scal, const double *red, int nred, const int *it)
 {
+    __shared__ double stg[MU_LDS];
     const int rep = it[0];
-    const int c = threadIdx.x;
-    if (c >= P.nchains) return;
-    scal += c;
+    const int c = blockIdx.x;
     red += (int64_t)c * nred;
+    if (nred <= MU_LDS) {   // workgroup-uniform
+        for (int i = threadIdx.x; i < nred; i += blockDim.x) stg[i] = red[i];
+        __syncthreads();
+        red = stg;
+    }
+    if (threadIdx.x != 0) return;
+    scal += c;
     double s = 0.0;
     for (int i = 0; i < nred; i++) s += red[i];
     double nn = (double)P.n;
@@ -800,7 +809,10 @@ __global__ void k_accum(Par P, const int *it, const double *beta, const int32_t 
     gstore += c * P.p;
     vara_sstore += c * P.p;
     indiststore += c * P.p * P.nmix;
-    gstore[pos] += beta[pos];
+    // adding a zero effect leaves the sum's bits unchanged (the sum is never
+    // -0.0), so zero effects skip the read-modify-write
+    const double b = beta[pos];
+    if (b != 0.0) gstore[pos] += b;
     if (P.method != SB_BAYES_A) indiststore[(int64_t)(cls[pos] - 1) * P.p + pos] += 1.0;
     if (P.method == SB_BAYES_A || P.method == SB_BAYES_B) vara_sstore[pos] += vara_s[pos];
 }
@@ -1378,7 +1390,7 @@ static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd
     int rc = SB_OK;
     k_next_it<<<1, 1, 0, st>>>(h->d_it);
     k_mu_pre<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal, h->sc.red);
-    k_mu_draw<<<1, 64, 0, st>>>(P, h->scal, h->sc.red, nred, h->d_it);
+    k_mu_draw<<<C, 64, 0, st>>>(P, h->scal, h->sc.red, nred, h->d_it);
     k_mu_post<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->scal);
     h->rstat_nwg = 0;   // r changed: the batched path recomputes its statistics
     for (int b = 0; b < P.nblocks && !rc; b++) {
