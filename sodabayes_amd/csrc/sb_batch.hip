@@ -122,8 +122,17 @@ __device__ __forceinline__ void wave_sum_max(double &sm, uint64_t &mx)
 // Limb-major operand rows: limb k of chain c is row k * CP + c, CP = 16 *
 // ntc (chains padded to whole 16-column tiles), so NTILE = LIMBS * ntc
 // column tiles and the LIMBS limbs of a chain sit in the same lane of
-// column tiles tc, ntc + tc, 2 ntc + tc.
-constexpr int limb_row(int k, int c, int ntc) { return k * 16 * ntc + c; }
+// column tiles tc, ntc + tc, 2 ntc + tc.  With four chain tiles (49-64
+// chains) the rows form two chain groups of 32 (group g: chains 32 g ..
+// 32 g + 31, rows 96 g + 32 k + c mod 32), so that the products run as two
+// groups of six column tiles each.
+constexpr int CHAIN_GROUP = 32;
+constexpr int chain_groups(int ntc) { return ntc == 4 ? 2 : 1; }
+constexpr int limb_row(int k, int c, int ntc)
+{
+    return ntc == 4 ? (c / CHAIN_GROUP) * (LIMBS * CHAIN_GROUP) + k * CHAIN_GROUP + c % CHAIN_GROUP
+                    : k * 16 * ntc + c;
+}
 static_assert(LIMBS == 3, "the products combine three limbs per chain");
 
 // Row tiles per wave of the int8 products: four while the accumulators of
@@ -143,8 +152,11 @@ int with_tiles(int ntile, int ntc, F &&f)
     SB_TILE_CASE(1)
     SB_TILE_CASE(2)
     SB_TILE_CASE(3)
-    SB_TILE_CASE(4)
 #undef SB_TILE_CASE
+    if (ntc == 4 && ntile == LIMBS * 4) {   // two chain groups of two tiles
+        f(integral_constant<int, LIMBS * 2>{}, integral_constant<int, 2>{});
+        return 0;
+    }
     return SB_ERR_ARG;
 }
 
@@ -288,7 +300,8 @@ __global__ __launch_bounds__(NT) void k_quant(Par P, int ntile, const double *r,
 // A(it, S): the wave's genotype tile it of super-step S, one 1 KB load per
 // wave at abase + (toff[it] + S) * 64 (4 words of 16 2-bit genotypes per
 // lane); B(S, .): the super-step's limb slab of NTILE x 4 blocks of 1 KB at
-// B + S * SLAB * 64, shared by the four waves through LDS.
+// B + S * sstr * 64 (sstr = blocks per super-step of all chain groups),
+// shared by the four waves through LDS.
 // Pipelined so that the loads of a super-step are in flight while the
 // previous one computes: the genotype tiles of step S + 1 are loaded at the
 // start of step S, and each wave stages its quarter of slab S + 2 in
@@ -298,8 +311,8 @@ __global__ __launch_bounds__(NT) void k_quant(Par P, int ntile, const double *r,
 // with a barrier: the ring is free.
 template <int NTILE, int IT>
 __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, const int64_t (&toff)[IT],
-                                              const uint4 *__restrict__ B, int S0, int S1, uint4 *lds,
-                                              v4i (&out)[IT][NTILE])
+                                              const uint4 *__restrict__ B, int sstr, int S0, int S1,
+                                              uint4 *lds, v4i (&out)[IT][NTILE])
 {
     v4i acc[IT][NTILE];
 #pragma unroll
@@ -325,7 +338,7 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
     const v4i *Av = reinterpret_cast<const v4i *>(abase);
     v4i a[IT], an[IT], bs[NB];
 #pragma unroll
-    for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)S0 * SLAB + W * j) * 64];
+    for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)S0 * sstr + W * j) * 64];
 #pragma unroll
     for (int it = 0; it < IT; it++) a[it] = Av[(toff[it] + S0) * 64];
     v4i *Lw = reinterpret_cast<v4i *>(lds) + wave * 64 + lane;
@@ -334,7 +347,7 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
     {
         const int Sb = min(S0 + 1, Sl);
 #pragma unroll
-        for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)Sb * SLAB + W * j) * 64];
+        for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)Sb * sstr + W * j) * 64];
     }
     __syncthreads();
     // step S: A(S) in a, A(S + 1) loading into an; B(S) in LDS half
@@ -364,7 +377,7 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
 #pragma unroll
         for (int j = 0; j < NB; j++) Ln[W * j * 64] = bs[j];
 #pragma unroll
-        for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)Sb * SLAB + W * j) * 64];
+        for (int j = 0; j < NB; j++) bs[j] = Bw[((int64_t)Sb * sstr + W * j) * 64];
         __syncthreads();
 #pragma unroll
         for (int it = 0; it < IT; it++) a[it] = an[it];
@@ -444,7 +457,8 @@ constexpr int uq_lds(int ntile) { return UQ_GROUP * ntile * 64; }   // uint4 of 
 
 template <int MODE, int NTILE>
 __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
-                                        const uint4 *__restrict__ Bq, const double *__restrict__ ra,
+                                        const uint4 *__restrict__ Bq, int ntot, int cbase,
+                                        const double *__restrict__ ra,
                                         const unsigned long long *__restrict__ amax, const int *__restrict__ ue,
                                         const int *__restrict__ ce, double *__restrict__ part, int k,
                                         uint4 *lds)
@@ -480,10 +494,10 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
         if (MODE == 0) return ((uq_off0(Oc) + kk) * ULIMBS + ku) * 64 + lane;
         return (((uq_off1(Oc, nss) + (kk >> 2) - Oc / 16) * 4 + (kk & 3)) * ULIMBS + ku) * 64 + lane;
     };
-    // B block t of K step kk
+    // B block t of K step kk (mode 1: Dq, whose super-steps hold ntot tiles;
+    // Bq points at this chain group's first tile)
     auto bidx = [&](int kk, int t) -> int64_t {
-        if (MODE == 0) return ((int64_t)kk * NTILE + t) * 64 + lane;
-        return (((int64_t)(kk >> 2) * NTILE + t) * 4 + (kk & 3)) * 64 + lane;
+        return (((int64_t)(kk >> 2) * ntot + t) * 4 + (kk & 3)) * 64 + lane;
     };
     // The pair's K steps in rounds of four: each wave holds the A tiles of one
     // round in registers (loaded together); the B blocks of every K step of
@@ -501,7 +515,8 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
     const int C = P.nchains;
     int ea[NTC];   // mode 0: the exponent of chain 16 tc + l16's r_a limbs
 #pragma unroll
-    for (int tc = 0; tc < NTC; tc++) ea[tc] = MODE == 0 ? fixed_exponent(amax[min(16 * tc + l16, C - 1)], LIMBS) : 0;
+    for (int tc = 0; tc < NTC; tc++)
+        ea[tc] = MODE == 0 ? fixed_exponent(amax[min(cbase + 16 * tc + l16, C - 1)], LIMBS) : 0;
     if (MODE == 0) {
         // B from r_a itself, as LIMBS balanced limbs of rint(r_a 2^e) with e
         // from the chain's max |r_a| over the block's rows (zero past the
@@ -513,7 +528,7 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
         const int q = lane >> 4;
         for (int u = wave; u < nk * 4 * NTC; u += W) {   // wave-uniform
             const int tc = u % NTC, sg = u / NTC, s = sg >> 2, g4 = sg & 3;
-            const int c = 16 * tc + l16, i0 = 64 * (k0 + s) + 16 * g4 + 4 * q;
+            const int c = cbase + 16 * tc + l16, i0 = 64 * (k0 + s) + 16 * g4 + 4 * q;
             const double *rp = ra + (int64_t)min(c, C - 1) * P.blocksize + i0;
             double v[4];
             if (c < C && i0 + 4 <= nb && (P.blocksize & 1) == 0) {
@@ -603,7 +618,7 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
     // (Horner from the top weight), then scaled by 2^-(e_U + e_chain).
 #pragma unroll
     for (int tc = 0; tc < NTC; tc++) {
-        const int c = 16 * tc + l16;
+        const int c = cbase + 16 * tc + l16;
         if (c >= C) continue;
         const int ec = MODE == 0 ? ea[tc] : ce[c];
 #pragma unroll
@@ -641,8 +656,8 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
 template <int NTILE, int RW>
 __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_prod(Par P, const uint4 *__restrict__ gsw, int nbt, int nb,
                                              const uint4 *__restrict__ Q, int nch, long long *__restrict__ Tp,
-                                             int64_t tstride, int nrg, const uint4 *__restrict__ Ua,
-                                             const double *__restrict__ ra,
+                                             int64_t tstride, int nrg, int ngrp, int npair,
+                                             const uint4 *__restrict__ Ua, const double *__restrict__ ra,
                                              const unsigned long long *__restrict__ amax,
                                              unsigned long long *__restrict__ anext, const int *__restrict__ ue,
                                              double *__restrict__ upart)
@@ -650,16 +665,18 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_prod(Par P, co
     constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
     static_assert(2 * SLAB * 64 >= uq_lds(NTILE), "LDS of the U pairs");
     __shared__ uint4 lds[2 * SLAB * 64];
-    const int id = blockIdx.x;
+    const int id = blockIdx.x, nx = nrg * nch;
     if (id == 0 && (int)threadIdx.x < P.nchains) anext[threadIdx.x] = 0;
-    if (id >= nrg * nch) {
-        uq_pair<0, NTILE>(P, Ua, nbt, nb, (nb + 255) / 256, nullptr, ra, amax, ue, nullptr, upart, id - nrg * nch,
-                          lds);
+    if (id >= ngrp * nx) {   // U_s'r_a pairs, chain group by chain group
+        const int k = id - ngrp * nx, cg = k / npair;
+        uq_pair<0, NTILE>(P, Ua, nbt, nb, (nb + 255) / 256, nullptr, 0, cg * 16 * NTC, ra, amax, ue, nullptr,
+                          upart, k - cg * npair, lds);
         return;
     }
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int rg = id % nrg, ch = id / nrg;
+    const int cg = id / nx, rg = (id - cg * nx) % nrg, ch = (id - cg * nx) / nrg;   // chain group, row group, chunk
+    const int cbase = cg * 16 * NTC;
     const int nS = P.ldw / 16;
     const int S0 = (int)((int64_t)ch * nS / nch), S1 = (int)((int64_t)(ch + 1) * nS / nch);
     const int tile0 = (rg * W + wave) * RW;
@@ -667,7 +684,7 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_prod(Par P, co
 #pragma unroll
     for (int it = 0; it < RW; it++) toff[it] = (int64_t)min(tile0 + it, nbt - 1) * nS;
     v4i acc[RW][NTILE];
-    limb_mainloop<NTILE, RW>(gsw + lane, toff, Q, S0, S1, lds, acc);
+    limb_mainloop<NTILE, RW>(gsw + lane, toff, Q + (int64_t)cg * SLAB * 64, ngrp * SLAB, S0, S1, lds, acc);
     const int C = P.nchains;
     long long *out = Tp + (int64_t)ch * tstride;
 #pragma unroll
@@ -675,7 +692,7 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_prod(Par P, co
         const int jk0 = 16 * (tile0 + it) + 4 * g;
 #pragma unroll
         for (int tc = 0; tc < NTC; tc++) {
-            const int c = 16 * tc + l16;
+            const int c = cbase + 16 * tc + l16;
             if (c >= C) continue;
             long long *o = out + (int64_t)c * P.blocksize + jk0;
 #pragma unroll
@@ -1026,31 +1043,36 @@ template <int NTILE, int RW>
 __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, const uint4 *__restrict__ gt, int nss, int ntr,
                                             const uint4 *__restrict__ Dq, const int *__restrict__ expo,
                                             const double *__restrict__ kpart, int nkp, double *__restrict__ r,
-                                            uint64_t *__restrict__ rmax, int ng, const uint4 *__restrict__ Ua,
-                                            int nbt, int nb, const int *__restrict__ ue, double *__restrict__ upart)
+                                            uint64_t *__restrict__ rmax, int ng, int ngrp, int npair,
+                                            const uint4 *__restrict__ Ua, int nbt, int nb,
+                                            const int *__restrict__ ue, double *__restrict__ upart)
 {
     constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
     static_assert(2 * SLAB * 64 >= uq_lds(NTILE), "LDS of the U pairs");
     __shared__ uint4 lds[2 * SLAB * 64];   // limb slab ring, then the K-constant parts
     __shared__ double kc_s[64];
     __shared__ unsigned long long mx_s[W][16 * NTC];
-    if ((int)blockIdx.x >= ng) {
+    if ((int)blockIdx.x >= ngrp * ng) {
         // the workgroups after the update's: pairs of U_s diag(s) c~ (same
-        // limbs Dq and exponents), filling the CUs the update leaves idle
-        uq_pair<1, NTILE>(P, Ua, nbt, nb, nss, Dq, nullptr, nullptr, ue, expo, upart, (int)blockIdx.x - ng, lds);
+        // limbs Dq and exponents), chain group by chain group, filling the
+        // CUs the update leaves idle
+        const int k = (int)blockIdx.x - ngrp * ng, cg = k / npair;
+        uq_pair<1, NTILE>(P, Ua, nbt, nb, nss, Dq + (int64_t)cg * SLAB * 64, ngrp * NTILE, cg * 16 * NTC, nullptr,
+                          nullptr, ue, expo, upart, k - cg * npair, lds);
         return;
     }
     const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
-    const int wg = blockIdx.x;
+    const int cg = (int)blockIdx.x / ng, wg = (int)blockIdx.x - cg * ng;   // chain group, row workgroup
+    const int cbase = cg * 16 * NTC;
     const int t0 = (int)((int64_t)wg * ntr / ng), T = (int)((int64_t)(wg + 1) * ntr / ng) - t0;
     const int w0 = t0 + wave * T / W, nt = t0 + (wave + 1) * T / W - w0;   // nt <= RW
     int64_t toff[RW];
 #pragma unroll
     for (int it = 0; it < RW; it++) toff[it] = (int64_t)min(w0 + it, ntr - 1) * nss;
     v4i acc[RW][NTILE];
-    limb_mainloop<NTILE, RW>(gt + lane, toff, Dq, 0, nss, lds, acc);
+    limb_mainloop<NTILE, RW>(gt + lane, toff, Dq + (int64_t)cg * SLAB * 64, ngrp * SLAB, 0, nss, lds, acc);
     kconst_finish(kpre, kpart, nkp, P.nchains, reinterpret_cast<double *>(lds), kc_s);
     __syncthreads();
     const int C = P.nchains;
@@ -1058,7 +1080,7 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
     unsigned long long mx[NTC];
 #pragma unroll
     for (int tc = 0; tc < NTC; tc++) {
-        const int c = min(16 * tc + l16, C - 1);
+        const int c = min(cbase + 16 * tc + l16, C - 1);
         sc[tc] = ldexp(1.0, -expo[c]);   // S * sc == ldexp(S, -e) exactly
         kc[tc] = kc_s[c];
         mx[tc] = 0;
@@ -1071,13 +1093,13 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
 #pragma unroll
         for (int tc = 0; tc < NTC; tc++) {
             const double2 *rp = reinterpret_cast<const double2 *>(
-                r + (int64_t)min(16 * tc + l16, C - 1) * P.ldr + i0);
+                r + (int64_t)min(cbase + 16 * tc + l16, C - 1) * P.ldr + i0);
             rv[tc][0] = rp[0];
             rv[tc][1] = rp[1];
         }
 #pragma unroll
         for (int tc = 0; tc < NTC; tc++) {
-            const int c = 16 * tc + l16;
+            const int c = cbase + 16 * tc + l16;
             if (c >= C) continue;
             double *rp = r + (int64_t)c * P.ldr + i0;
 #pragma unroll
@@ -1104,11 +1126,11 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
         if (g == 0) mx_s[wave][16 * tc + l16] = mx[tc];
     }
     __syncthreads();
-    if ((int)threadIdx.x < C) {
+    if ((int)threadIdx.x < min(16 * NTC, C - cbase)) {
         unsigned long long m = mx_s[0][threadIdx.x];
 #pragma unroll
         for (int w = 1; w < W; w++) m = mx_s[w][threadIdx.x] > m ? mx_s[w][threadIdx.x] : m;
-        rmax[(int64_t)threadIdx.x * ng + wg] = m;
+        rmax[(int64_t)(cbase + threadIdx.x) * ng + wg] = m;
     }
 }
 
@@ -1275,14 +1297,15 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int nch, bool with_u)
     const uint4 *Q = reinterpret_cast<const uint4 *>(h->b_Q);
     const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
     const uint4 *gsw = reinterpret_cast<const uint4 *>(h->geno_sw) + h->gs_off[b];
+    const int ngrp = chain_groups((C + 15) / 16);
     KClock &kc = h->kc;
     return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
         constexpr int NTL = decltype(ntl)::value;
         with_rw<NTL>(rw_sel, [&](auto rw) {
             constexpr int RW = decltype(rw)::value;
             const int nrg = (nbt + 4 * RW - 1) / (4 * RW);
-            launch(kc, k_prod<NTL, RW>, dim3(nrg * nch + (unsigned)nup), dim3(NT), 0, st, P, gsw, nbt, nb, Q, nch,
-                   reinterpret_cast<long long *>(h->b_T), tstride, nrg,
+            launch(kc, k_prod<NTL, RW>, dim3(ngrp * (nrg * nch + (unsigned)nup)), dim3(NT), 0, st, P, gsw, nbt, nb,
+                   Q, nch, reinterpret_cast<long long *>(h->b_T), tstride, nrg, ngrp, (int)nup,
                    reinterpret_cast<const uint4 *>(h->Uq0) + h->uq0_off[b], h->ra, amax, anext,
                    h->ue0 + (int64_t)b * P.blocksize, h->b_rapart);
         });
@@ -1302,6 +1325,7 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out, bool wit
     const int nsw = batch_words(nb), ntr = (P.n + 15) / 16;
     const uint4 *Dq = reinterpret_cast<const uint4 *>(h->b_Dq);
     const uint4 *gt = reinterpret_cast<const uint4 *>(h->geno_t) + h->gt_off[b];
+    const int ngrp = chain_groups((C + 15) / 16);
     KClock &kc = h->kc;
     return with_tiles(h->b_ntile, (C + 15) / 16, [&](auto ntl, auto) {
         constexpr int NTL = decltype(ntl)::value;
@@ -1309,8 +1333,8 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out, bool wit
             constexpr int RW = decltype(rw)::value;
             const int ng = upd_groups(h, RW);
             *ng_out = ng;
-            launch(kc, k_upd<NTL, RW>, dim3(ng + (unsigned)nup), dim3(NT), 0, st, P, gt, nsw / 16, ntr, Dq,
-                   h->b_expo2, h->b_kpart, (nb + NT - 1) / NT, r, h->b_rmax, ng,
+            launch(kc, k_upd<NTL, RW>, dim3(ngrp * (ng + (unsigned)nup)), dim3(NT), 0, st, P, gt, nsw / 16, ntr,
+                   Dq, h->b_expo2, h->b_kpart, (nb + NT - 1) / NT, r, h->b_rmax, ng, ngrp, (int)nup,
                    reinterpret_cast<const uint4 *>(h->Uq1) + h->uq1_off[b], nbt, nb, h->ue1 + (int64_t)b * P.blocksize,
                    h->b_uapart);
         });
