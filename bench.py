@@ -31,6 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
 
 # kernel timing slots (sodabayes_amd._lib.KT_SLOTS) -> kernel names
 KERNEL_NAMES = {
@@ -44,14 +45,14 @@ KERNEL_NAMES = {
 
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE (x2, the factor a 1 GiB
 # device copy shows on this GPU) + WRITE_SIZE (x1), per (path, kernel slot),
-# measured on 4,096-SNP blocks at n = 50,000 (32 chains for the batched path),
-# the per-launch shape of this workload
-# (profiles/soda_bayesr_n50k_bs4096_x32_pmc_traffic.md); None where not measured
-TRAFFIC_CHAINS = 32
+# measured on 4,096-SNP blocks at n = 50,000 (64 chains, two chain groups,
+# for the batched path), the per-launch shape of this workload
+# (profiles/soda_bayesr_n50k_bs4096_x64_pmc_traffic.md); None where not measured
+TRAFFIC_CHAINS = 64
 TRAFFIC = {
-    ("mixed", "xupd"): int((50580 * 2 + 17515) * 1024),  # k_upd<6, 2> with the U_s diag(s) c pairs
-    ("mixed", "xq"): int((65067 * 2 + 12814) * 1024),    # k_prod<6, 2> with the U_s'r_a pairs
-    ("mixed", "drawb"): int((7918 * 2 + 3728) * 1024),   # k_draw_b<4>
+    ("mixed", "xupd"): int((95416 * 2 + 35102) * 1024),  # k_upd<6, 2> with the U_s diag(s) c pairs
+    ("mixed", "xq"): int((107201 * 2 + 25926) * 1024),   # k_prod<6, 2> with the U_s'r_a pairs
+    ("mixed", "drawb"): int((15678 * 2 + 7456) * 1024),  # k_draw_b<4>
     ("fp64", "rhs"): int((60338 * 2 + 919) * 1024),      # k_rhs
     ("fp64", "update"): int((50644 * 2 + 604) * 1024),   # k_update (k_ra_sum not counted)
 }
@@ -73,8 +74,8 @@ def parse_args():
     ap.add_argument("--cpu-iters", type=int, default=100, help="iterations of the CPU baseline's sample")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 accuracy report")
     ap.add_argument("--no-roofline", action="store_true")
-    ap.add_argument("--chains", type=int, default=32,
-                    help="chains batched on the mixed path (32: the best of 16, 24, 32, 40)")
+    ap.add_argument("--chains", type=int, default=64,
+                    help="chains batched on the mixed path (64: the best of 16, 32, 48, 64)")
     ap.add_argument("--chains-only", action="store_true", help="skip the FP64 single-chain pass")
     ap.add_argument("--fp64-only", action="store_true", help="only the FP64 single-chain pass (no JSON line)")
     ap.add_argument("--no-folds", action="store_true", help="skip the cross-validation folds pass")
@@ -230,12 +231,31 @@ def run_path(soda, torch, a, why, freq, bed_ptr, precision: str, chains: int, be
         dom = max(kernels, key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
         d = kernels[dom]
         out["kernels"] = kernels
-        out["roofline"] = {
-            "bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": TRAFFIC.get((precision, dom)) if (chains == (TRAFFIC_CHAINS if precision == "mixed" else 1)
-                                                         and a.nind == 50000 and a.blocksize == 4096) else None,
-            "kernel": d["name"], "avg_launch_us": d["avg_us"], "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
+        # The int8 product launches are bound by whichever of HBM and MFMA
+        # needs more time at peak for their algorithmic work: bytes as above,
+        # int8 operations 2 n n_b (3 C) for the genotype half and
+        # 2 (n_b (n_b + 1) / 2) (4 x 3 C) for the U half (four U limbs, three
+        # chain limbs)
+        ops = None
+        if precision == "mixed" and dom in ("xq", "xupd"):
+            nb = a.blocksize
+            ops = 2.0 * a.nind * nb * 3 * chains + 2.0 * (nb * (nb + 1) / 2) * 12 * chains
+        traffic = TRAFFIC.get((precision, dom)) if (chains == (TRAFFIC_CHAINS if precision == "mixed" else 1)
+                                                    and a.nind == 50000 and a.blocksize == 4096) else None
+        if ops is not None and ops / (INT8_PEAK_TOPS * 1e12) > d["algorithmic_bytes"] / (HBM_PEAK_GBS * 1e9):
+            tops = ops / (d["avg_us"] * 1e-6) / 1e12
+            out["roofline"] = {
+                "bound": "mfma", "achieved": round(tops, 1), "peak": INT8_PEAK_TOPS, "unit": "TOP/s (int8)",
+                "frac": round(tops / INT8_PEAK_TOPS, 4), "traffic": traffic, "kernel": d["name"],
+                "avg_launch_us": d["avg_us"], "algorithmic_ops_per_launch": int(ops),
+                "algorithmic_bytes_per_launch": d["algorithmic_bytes"],
+                "hbm_gbs": d["gbs"], "hbm_frac": round(d["gbs"] / HBM_PEAK_GBS, 4)}
+        else:
+            out["roofline"] = {
+                "bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": d["name"], "avg_launch_us": d["avg_us"],
+                "algorithmic_bytes_per_launch": d["algorithmic_bytes"]}
         for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["share"]):
             log(f"[{precision} x{chains}]   {v['name']:<34s} {v['avg_us']:8.2f} us/launch  {v['gbs']:7.0f} GB/s"
                 f"  {v['share']:6.1%} of kernel time")
@@ -448,7 +468,7 @@ def main():
         "metric": "SNP-effect updates/s (BayesR n=50k,p=500k): mixed+batched vs FP64 GPU, same acc.",
         "value": mixed["value"], "unit": "SNP-effect updates/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": mixed["ms"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int8 MFMA limbs (int32 sums) + f64 (U stored f32)", "data": "synthetic",
+        "vs_baseline": None, "dtype": "int8 MFMA limbs (int32 sums) + f64", "data": "synthetic",
         "config": {"workload": f"SODA BayesR n={n} p={p} n_s={bs}, 4-class mixture, {a.chains} chains "
                                "batched, counter-based RNG, mixed-precision products",
                    "nind": n, "nloci": p, "blocksize": bs, "chains": a.chains, "limbs": 3,
