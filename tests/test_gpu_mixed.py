@@ -59,12 +59,15 @@ def sampler(method, precision, chains, seed, blocksize=100, n=1001, p=300):
 
 
 # chain counts cover every (limb tile, chain tile) kernel configuration:
-# 1-16 chains (3 limb tiles, 1 chain tile), 20 (6, 2), 40 (9, 3), 64 (12, 4)
+# 1-16 chains (3 limb tiles, 1 chain tile), 20 (6, 2), 40 (9, 3), and 49-64
+# as two chain groups of (6, 2): 64 (both full) and 50 (the second group
+# holding 18 chains)
 @pytest.mark.parametrize("method,precision,chains,bar", [
     ("BayesR", "mixed", 4, 1e-6), ("BayesR", "mixed", 1, 1e-6), ("BayesR", "fp64", 1, 1e-12),
     ("BayesA", "mixed", 4, 1e-6), ("BayesB", "mixed", 3, 1e-6), ("BayesC", "mixed", 5, 1e-6),
     ("BayesCpi", "mixed", 6, 1e-6), ("BayesRc", "mixed", 4, 1e-6),
-    ("BayesR", "mixed", 20, 1e-6), ("BayesR", "mixed", 40, 1e-6), ("BayesR", "mixed", 64, 1e-6)])
+    ("BayesR", "mixed", 20, 1e-6), ("BayesR", "mixed", 40, 1e-6), ("BayesR", "mixed", 64, 1e-6),
+    ("BayesR", "mixed", 50, 1e-6)])
 def test_block_products_against_fp64(method, precision, chains, bar):
     s, bed, why = sampler(method, precision, chains, 5)
     s.setup()
@@ -215,7 +218,7 @@ def test_counter_mode_bound_matches_reference_d(case):
 @pytest.mark.parametrize("method,precision,chains,bar", [
     ("BayesR", "fp64", 1, 1e-11), ("BayesR", "mixed", 8, 1e-5), ("BayesA", "mixed", 4, 1e-5),
     ("BayesB", "mixed", 4, 1e-5), ("BayesRc", "mixed", 4, 1e-5), ("BayesR", "mixed", 64, 1e-5),
-    ("BayesR", "mixed", 40, 1e-5)])
+    ("BayesR", "mixed", 40, 1e-5), ("BayesR", "mixed", 50, 1e-5)])
 def test_residuals_stay_consistent(method, precision, chains, bar):
     """After the sweeps' incremental updates r_c = y - X beta_c - mu_c still
     holds for every chain (before the every-1000-iterations resync)."""
@@ -267,7 +270,7 @@ def _chain_states(s, chains):
     return [(s.get("G", p, c), s.get("CLASS", p, c), s.get("YADJ", n, c)) for c in range(chains)]
 
 
-@pytest.mark.parametrize("chains", [4, 40])
+@pytest.mark.parametrize("chains", [4, 40, 64])
 def test_tile_shapes_give_identical_chains(monkeypatch, chains):
     """The X_s'Q chunk length and the row tiles per wave only regroup exact
     int32 limb sums, so every shape - fixed ones (four and two chunks of the
