@@ -319,10 +319,10 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         h->b_ldq = ldr;
         const int nS = P.ldw / 16;
         h->b_rw = h->b_ntile <= 6 ? 4 : 2;
-        h->b_nch = sb::prod_chunks(h, h->b_rw, 1);
+        h->b_nch = sb::prod_chunks(h, h->b_rw, 4);
         if (const char *e = std::getenv("SODABAYES_ROWS_PER_WAVE")) {
             h->b_rw = (std::atoi(e) == 4 && h->b_ntile <= 6) ? 4 : 2;
-            h->b_nch = sb::prod_chunks(h, h->b_rw, 1);
+            h->b_nch = sb::prod_chunks(h, h->b_rw, 4);
             h->tuned = true;
         }
         if (const char *e = std::getenv("SODABAYES_KSS")) {

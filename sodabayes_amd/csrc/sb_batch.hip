@@ -1266,12 +1266,12 @@ int prod_groups(const sb_handle *h, int rw)
     return std::max(1, (nbt + 4 * rw - 1) / (4 * rw));
 }
 
-// X_s'Q chunks for about m workgroups per CU (row groups x chunks), each
-// chunk at least two super-steps when the individuals allow
-int prod_chunks(const sb_handle *h, int rw, int m)
+// X_s'Q chunks for about q / 4 workgroups per CU (chain groups x row groups x
+// chunks), each chunk at least two super-steps when the individuals allow
+int prod_chunks(const sb_handle *h, int rw, int q)
 {
-    const int nS = h->par.ldw / 16, nrg = prod_groups(h, rw);
-    const int want = std::max(1, (m * h->ncu + nrg / 2) / nrg);
+    const int nS = h->par.ldw / 16, nrg = prod_groups(h, rw) * chain_groups((h->par.nchains + 15) / 16);
+    const int want = std::max(1, (q * h->ncu / 4 + nrg / 2) / nrg);
     return std::max(1, std::min(want, std::max(1, nS / 2)));
 }
 
@@ -1421,25 +1421,30 @@ int batch_update(sb_handle *h, int b)
     return hip_check(hipGetLastError(), "batch update");
 }
 
-// Tile shape of the batched products: the X_s'Q chunk count (about 1, 2 or
-// 3 workgroups per CU) and the row tiles per wave of the int8 products (2
-// or 4).  Every shape gives bit-identical results (the limb products are
-// exact int32 sums and the chunk partials are added as 64-bit integers), so
-// the choice affects speed only.  Picked once per handle, before its first
-// sweep, by timing block 0's products (k_prod) and update (k_upd, on a
-// scratch copy of the residuals) for every candidate; SODABAYES_KSS (super-
-// steps per chunk) / SODABAYES_ROWS_PER_WAVE fix the shape instead.
+// Tile shape of the batched products: the X_s'Q chunk count (0.75 to 2
+// workgroups per CU) and the row tiles per wave of the int8 products (2 or
+// 4).  Every shape gives bit-identical results (the limb products are exact
+// int32 sums and the chunk partials are added as 64-bit integers), so the
+// choice affects speed only.  Picked once per handle, before its first
+// sweep, by timing the products (k_prod) and update (k_upd, on a scratch
+// copy of the residuals) of the first six blocks for every candidate, in
+// three interleaved rounds; SODABAYES_KSS (super-steps per chunk) /
+// SODABAYES_ROWS_PER_WAVE fix the shape instead.
 void tune_rws(const sb_handle *h, std::vector<int> &rws)
 {
     rws.assign({2});
     if (prod_rw(h->b_ntile) == 4) rws.push_back(4);
 }
+// chunk candidates: 0.75 to 2 workgroups per CU (in quarters), at most DCH
+// chunks (the draw sums more than DCH chunk partials in a serial loop, a cost
+// the trial launches do not see)
+constexpr int TUNE_Q[] = {3, 4, 5, 6, 8};
 void tune_chunks(const sb_handle *h, int rw, std::vector<int> &nchs)
 {
     nchs.clear();
-    for (int m = 1; m <= 3; m++) {
-        const int c = prod_chunks(h, rw, m);
-        if (std::find(nchs.begin(), nchs.end(), c) == nchs.end()) nchs.push_back(c);
+    for (int q : TUNE_Q) {
+        const int c = prod_chunks(h, rw, q);
+        if ((c <= DCH || nchs.empty()) && std::find(nchs.begin(), nchs.end(), c) == nchs.end()) nchs.push_back(c);
     }
 }
 
@@ -1448,7 +1453,7 @@ int batch_max_chunks(const sb_handle *h)
 {
     int mx = 1;
     for (int rw : {2, 4})
-        for (int m = 1; m <= 3; m++) mx = std::max(mx, prod_chunks(h, rw, m));
+        for (int q : TUNE_Q) mx = std::max(mx, prod_chunks(h, rw, q));
     return std::max(mx, h->b_nch);
 }
 
@@ -1465,33 +1470,49 @@ int batch_autotune(sb_handle *h)
     hipEventCreate(&e1);
     const bool timing = h->kc.on;
     h->kc.on = false;   // the trial launches are not kernel-clock spans
-    constexpr int REPS = 3;
+    // each candidate over several different blocks, so that its operands come
+    // from HBM as in a sweep (repeating one block would time it from the
+    // caches)
+    const int nbt = std::min(6, P.nblocks);
     auto time_it = [&](auto &&fn) {
-        fn();   // warm-up
+        fn(0);   // warm-up
         hipEventRecord(e0, st);
-        for (int k = 0; k < REPS; k++) fn();
+        for (int b = 0; b < nbt; b++) fn(b);
         hipEventRecord(e1, st);
         hipEventSynchronize(e1);
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
-        return ms / REPS;
+        return ms / nbt;
     };
-    double best = 1e30;
     int brw = h->b_rw, bnch = h->b_nch, rc = 0, ng = 0;
     std::vector<int> rws, nchs;
+    std::vector<std::pair<int, int>> cand;   // (rows per wave, chunks)
     tune_rws(h, rws);
     for (int rw : rws) {
         tune_chunks(h, rw, nchs);
-        const float tu = time_it([&] { rc |= upd_launch(h, 0, rw, h->tune_r, &ng, true); });
-        for (int nch : nchs) {
-            const float tp = time_it([&] { rc |= prod_launch(h, 0, rw, nch, true); });
-            if (tp + tu < best) {
-                best = tp + tu;
-                brw = rw;
-                bnch = nch;
-            }
-        }
+        for (int nch : nchs) cand.emplace_back(rw, nch);
     }
+    // products and update of each block in turn, as a sweep runs them; the
+    // candidates in rounds, each keeping its fastest round, so that a slow
+    // moment of the GPU does not decide
+    constexpr int ROUNDS = 3;
+    std::vector<double> tbest(cand.size(), 1e30);
+    for (int rd = 0; rd < ROUNDS; rd++)
+        for (size_t i = 0; i < cand.size(); i++) {
+            const int rw = cand[i].first, nch = cand[i].second;
+            const float t = time_it([&](int b) {
+                rc |= prod_launch(h, b, rw, nch, true);
+                rc |= upd_launch(h, b, rw, h->tune_r, &ng, true);
+            });
+            tbest[i] = std::min(tbest[i], (double)t);
+        }
+    double best = 1e30;
+    for (size_t i = 0; i < cand.size(); i++)
+        if (tbest[i] < best) {
+            best = tbest[i];
+            brw = cand[i].first;
+            bnch = cand[i].second;
+        }
     h->kc.on = timing;
     hipEventDestroy(e0);
     hipEventDestroy(e1);

@@ -353,7 +353,7 @@ int layout_stream(sb_handle *h, double *out);
 int batch_transpose(sb_handle *h);
 int batch_words(int nb);
 int64_t batch_uq_elems(int nb, int mode);
-int prod_chunks(const sb_handle *h, int rw, int m);
+int prod_chunks(const sb_handle *h, int rw, int q);
 int batch_max_chunks(const sb_handle *h);
 int batch_autotune(sb_handle *h);
 int init_xpx(sb_handle *h);
