@@ -1435,16 +1435,14 @@ void tune_rws(const sb_handle *h, std::vector<int> &rws)
     rws.assign({2});
     if (prod_rw(h->b_ntile) == 4) rws.push_back(4);
 }
-// chunk candidates: 0.75 to 2 workgroups per CU (in quarters), at most DCH
-// chunks (the draw sums more than DCH chunk partials in a serial loop, a cost
-// the trial launches do not see)
-constexpr int TUNE_Q[] = {3, 4, 5, 6, 8};
+// chunk candidates: 0.75 to 3 workgroups per CU (in quarters)
+constexpr int TUNE_Q[] = {3, 4, 5, 6, 8, 12};
 void tune_chunks(const sb_handle *h, int rw, std::vector<int> &nchs)
 {
     nchs.clear();
     for (int q : TUNE_Q) {
         const int c = prod_chunks(h, rw, q);
-        if ((c <= DCH || nchs.empty()) && std::find(nchs.begin(), nchs.end(), c) == nchs.end()) nchs.push_back(c);
+        if (std::find(nchs.begin(), nchs.end(), c) == nchs.end()) nchs.push_back(c);
     }
 }
 
@@ -1506,13 +1504,17 @@ int batch_autotune(sb_handle *h)
             });
             tbest[i] = std::min(tbest[i], (double)t);
         }
+    // each chunk adds nchains x blocksize int64 partials that the draw reads
+    // back (not in the trial launches): charged at 2.5 TB/s
     double best = 1e30;
-    for (size_t i = 0; i < cand.size(); i++)
-        if (tbest[i] < best) {
-            best = tbest[i];
+    for (size_t i = 0; i < cand.size(); i++) {
+        const double t = tbest[i] + cand[i].second * 8.0 * P.nchains * P.blocksize / 2.5e12 * 1e3;
+        if (t < best) {
+            best = t;
             brw = cand[i].first;
             bnch = cand[i].second;
         }
+    }
     h->kc.on = timing;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
