@@ -33,7 +33,7 @@ def run(monkeypatch, graph, rng, precision, chains):
 
 
 @pytest.mark.parametrize("rng,precision,chains", [("replay", "fp64", 1), ("counter", "fp64", 1),
-                                                  ("counter", "mixed", 6)])
+                                                  ("counter", "mixed", 6), ("counter", "mixed", 64)])
 def test_graph_replays_equal_eager_launches(monkeypatch, rng, precision, chains):
     he, re_, be = run(monkeypatch, False, rng, precision, chains)
     hg, rg, bg = run(monkeypatch, True, rng, precision, chains)
