@@ -206,86 +206,104 @@ __device__ __forceinline__ int fixed_exponent(unsigned long long m, int L)
     return e;
 }
 
-// One thread per (16-individual word, chain): balanced radix-256 digits of
-// q = rint(r * 2^e) in the limb-major rows limb_row(k, c), written as MFMA
-// B-operand blocks: Q[((S * NTILE + t) * 4 + s) * 64 + lane] (16 bytes
-// each) holds, for lane = l16 + 16 g, the 16 slots (order of unpack_word) of
-// row 16 t + l16 at word 16 S + 4 g + s.  Every workgroup reduces the nwg
-// partial maxima of |r| of its chain to e; workgroup 0 stores e.  Each
-// workgroup also leaves the exact integer sum of its q (qsum), so that the
-// centring term uses sum_i q_i 2^-e, the sum of the very values the
-// products see (qsum[c * nq + bx], nq workgroups per chain).
+// Balanced radix-256 digits of q = rint(r * 2^e) in the limb-major rows
+// limb_row(k, c), written as MFMA B-operand blocks: Q[((S * NTILE + t) * 4 +
+// s) * 64 + lane] (16 bytes each) holds, for lane = l16 + 16 g, the 16 slots
+// (order of unpack_word) of row 16 t + l16 at word 16 S + 4 g + s.  One
+// workgroup per (super-step S, chain tile ct): thread (wave s, lane l16 +
+// 16 g) takes word 16 S + 4 g + s of chain 16 ct + l16, so each wave writes
+// whole 1 KB blocks (one per limb); chains past nchains get zero digits.
+// Each workgroup reduces the nwg partial maxima of |r| of its 16 chains to
+// their exponents e (stored by the workgroups of S = 0) and leaves the exact
+// integer sum of each chain's q over its 256 individuals (qsum[c * nq + S],
+// nq = super-steps), so that the centring term uses sum_i q_i 2^-e, the sum
+// of the very values the products see.
 __device__ __forceinline__ void quant_wg(const Par &P, int ntile, const double *r, const uint64_t *rmax, int nwg,
-                                         uint4 *Q, int *expo, long long *qsum, int nq, int bx, int c)
+                                         uint4 *Q, int *expo, long long *qsum, int nq, int S, int ct)
 {
-    __shared__ int e_s;
-    __shared__ unsigned long long ml[NT / WAVE];
-    __shared__ long long ql[NT / WAVE];
+    constexpr int W = NT / WAVE;
+    static_assert(W == 4, "one wave per sub-step s of a super-step");
+    __shared__ unsigned long long ml[W][16];
+    __shared__ int e_s[16];
+    __shared__ long long ql[W][16];
+    const int C = P.nchains, ntc = ntile / LIMBS;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+    const int l16 = threadIdx.x & 15, j = threadIdx.x >> 4;   // j = 0..15
+    const int c = 16 * ct + l16, cc = min(c, C - 1);
+    // chain l16's maximum over its partials: thread (j, l16) takes j, j + 16, ...
     unsigned long long m = 0;
-    for (int w0 = 0; w0 < nwg; w0 += 4 * NT) {
+    for (int w0 = j; w0 < nwg; w0 += 16 * 4) {
         unsigned long long v[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            const int w = w0 + u * NT + threadIdx.x;
-            v[u] = w < nwg ? rmax[(int64_t)c * nwg + w] : 0ull;
+            const int w = w0 + 16 * u;
+            v[u] = w < nwg ? rmax[(int64_t)cc * nwg + w] : 0ull;
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) m = v[u] > m ? v[u] : m;
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(m, off, WAVE);
+    for (int off = 16; off < WAVE; off <<= 1) {
+        const unsigned long long o = __shfl_xor(m, off, WAVE);
         m = o > m ? o : m;
     }
-    if ((threadIdx.x & 63) == 0) ml[threadIdx.x >> 6] = m;
+    if (lane < 16) ml[wave][lane] = m;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < NT / WAVE; w++) m = ml[w] > m ? ml[w] : m;
-        e_s = fixed_exponent(m, LIMBS);
-        if (bx == 0) expo[c] = e_s;
+    if (threadIdx.x < 16) {
+        unsigned long long mm = ml[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < W; w++) mm = ml[w][threadIdx.x] > mm ? ml[w][threadIdx.x] : mm;
+        const int e = fixed_exponent(mm, LIMBS);
+        e_s[threadIdx.x] = e;
+        if (S == 0 && 16 * ct + (int)threadIdx.x < C) expo[16 * ct + threadIdx.x] = e;
     }
     __syncthreads();
-    const int e = e_s;
-    const int w = bx * NT + threadIdx.x;
+    const int e = e_s[l16];
+    const int g = lane >> 4, s = wave;
+    const int w = 16 * S + 4 * g + s;   // < ldw: ldw is a whole number of super-steps
+    int32_t q[16];
     long long qs = 0;
-    if (w < P.ldw) {
+    if (c < C) {
         const double *rc = r + (int64_t)c * P.ldr + (int64_t)w * 16;
-        int32_t q[16];
 #pragma unroll
         for (int t = 0; t < 16; t++) {
             q[t] = (int32_t)rint(ldexp(rc[t], e));
             qs += q[t];
         }
-        const int S = w >> 4, g = (w >> 2) & 3, s = w & 3;
-        const int ntc = ntile / LIMBS;
-        for (int k = 0; k < LIMBS; k++) {
-            uint32_t out[4];
+    } else {
 #pragma unroll
-            for (int m4 = 0; m4 < 4; m4++) {
-                uint32_t v = 0;
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    int t = 4 * b + m4;                    // individual within the word
-                    int8_t d = (int8_t)(q[t] & 0xFF);      // balanced digit in [-128, 127]
-                    v |= ((uint32_t)(uint8_t)d) << (8 * b);
-                }
-                out[m4] = v;
-            }
-#pragma unroll
-            for (int t = 0; t < 16; t++) q[t] = (q[t] - (int32_t)(int8_t)(q[t] & 0xFF)) >> 8;
-            const int row = limb_row(k, c, ntc);
-            Q[(((int64_t)S * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
-                make_uint4(out[0], out[1], out[2], out[3]);
-        }
+        for (int t = 0; t < 16; t++) q[t] = 0;
     }
-    // exact: |q| < 2^23 and at most ldw * 16 terms
+    for (int k = 0; k < LIMBS; k++) {
+        uint32_t out[4];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) qs += __shfl_xor(qs, off, WAVE);
-    if ((threadIdx.x & 63) == 0) ql[threadIdx.x >> 6] = qs;
+        for (int m4 = 0; m4 < 4; m4++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int t = 4 * b + m4;                    // individual within the word
+                const int8_t d = (int8_t)(q[t] & 0xFF);      // balanced digit in [-128, 127]
+                v |= ((uint32_t)(uint8_t)d) << (8 * b);
+            }
+            out[m4] = v;
+        }
+#pragma unroll
+        for (int t = 0; t < 16; t++) q[t] = (q[t] - (int32_t)(int8_t)(q[t] & 0xFF)) >> 8;
+        const int row = limb_row(k, c, ntc);
+        Q[(((int64_t)S * ntile + (row >> 4)) * 4 + s) * 64 + (row & 15) + 16 * g] =
+            make_uint4(out[0], out[1], out[2], out[3]);
+    }
+    // exact: |q| < 2^23 and at most ldw * 16 terms; the 16 threads of a
+    // chain (four lanes g per wave, four waves)
+#pragma unroll
+    for (int off = 16; off < WAVE; off <<= 1) qs += __shfl_xor(qs, off, WAVE);
+    if (lane < 16) ql[wave][lane] = qs;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int wv = 1; wv < NT / WAVE; wv++) qs += ql[wv];
-        qsum[(int64_t)c * nq + bx] = qs;
+    if (threadIdx.x < 16 && 16 * ct + (int)threadIdx.x < C) {
+        long long t = ql[0][threadIdx.x];
+#pragma unroll
+        for (int wv = 1; wv < W; wv++) t += ql[wv][threadIdx.x];
+        qsum[(int64_t)(16 * ct + threadIdx.x) * nq + S] = t;
     }
 }
 
@@ -868,7 +886,7 @@ __global__ __launch_bounds__(NT) void k_ramax(Par P, const double *ra, int nb, u
 }
 
 // The r_a reduction of this block beside the r limbs of the next block (both
-// need only the update just done): workgroups [0, nq * C) quantise r,
+// need only the update just done): workgroups [0, nq * chain tiles) quantise r,
 // the rest (nra per chain, covering max(nb, nbn) rows) reduce r_a and
 // collect the next block's max |r_a| (nbn rows) into amax.
 __global__ __launch_bounds__(NT) void k_rared_quant(Par P, const double *part, int spant, int ngroups, int nb,
@@ -876,11 +894,11 @@ __global__ __launch_bounds__(NT) void k_rared_quant(Par P, const double *part, i
                                                     const uint64_t *rmax, int nwg, uint4 *Q, int *expo,
                                                     long long *qsum, int nq, int nbn, unsigned long long *amax)
 {
-    const int id = blockIdx.x, C = P.nchains;
-    if (id < nq * C)
+    const int id = blockIdx.x, nqt = nq * (ntile / LIMBS);
+    if (id < nqt)
         quant_wg(P, ntile, r, rmax, nwg, Q, expo, qsum, nq, id % nq, id / nq);
     else
-        ra_reduce_wg(P, part, spant, ngroups, nb, ra, (id - nq * C) % nra, (id - nq * C) / nra, nbn, amax);
+        ra_reduce_wg(P, part, spant, ngroups, nb, ra, (id - nqt) % nra, (id - nqt) / nra, nbn, amax);
 }
 
 // SNP-major genotype tiles of one block for k_xq: element ((tile * nS + S) *
@@ -1193,7 +1211,17 @@ __global__ __launch_bounds__(NT) void k_draw_b(Par P, const Scal *sc, const long
         // quantisation workgroups' sums of q, formed by every wave (one part
         // per lane, then a butterfly; integer, so in any order)
         long long Sq = 0;
-        for (int u = threadIdx.x & (WAVE - 1); u < nq; u += WAVE) Sq += qsum[(int64_t)c * nq + u];
+        for (int u0 = 0; u0 < nq; u0 += 4 * WAVE) {   // four loads in flight per lane
+            long long v[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int u = u0 + i * WAVE + (threadIdx.x & (WAVE - 1));
+                v[i] = qsum[(int64_t)c * nq + min(u, nq - 1)];
+                v[i] = u < nq ? v[i] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) Sq += v[i];
+        }
         if (mixm && !rhs_out) mix_const(P, sc + c, zz, mc);   // workgroup-uniform
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) Sq += __shfl_xor(Sq, off, WAVE);   // every lane, before any exits
@@ -1362,8 +1390,8 @@ int batch_block(sb_handle *h, int b, double zz, int rep)
             h->rstat_nwg = nwg_r;
         }
         kc.last(KS_RQUANT, 16.0 * C * n + cl * n);
-        launch(kc, k_quant, dim3(h->b_nq, 1, C), dim3(NT), 0, st, P, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q,
-               h->b_expo, h->b_qsum);
+        launch(kc, k_quant, dim3(h->b_nq, 1, h->b_ntile / LIMBS), dim3(NT), 0, st, P, h->b_ntile, h->r, h->b_rmax,
+               h->rstat_nwg, Q, h->b_expo, h->b_qsum);
     }
     h->q_block = -1;
     if (h->ra_block != b) {
@@ -1408,7 +1436,8 @@ int batch_update(sb_handle *h, int b)
         const int nra = (std::max(nb, nbn) + NT - 1) / NT;
         uint4 *Q = reinterpret_cast<uint4 *>(h->b_Q);
         kc.last(KS_RARED, 8.0 * C * nb * (ncg + 3.0) + 8.0 * C * n + cl * n);
-        launch(kc, k_rared_quant, dim3((nq + nra) * C), dim3(NT), 0, st, P, h->b_uapart, spant, ncg, nb, h->ra,
+        launch(kc, k_rared_quant, dim3(nq * (h->b_ntile / LIMBS) + nra * C), dim3(NT), 0, st, P, h->b_uapart, spant,
+               ncg, nb, h->ra,
                nra, h->b_ntile, h->r, h->b_rmax, h->rstat_nwg, Q, h->b_expo, h->b_qsum, nq, nbn,
                h->b_ramax + (int64_t)((b + 1) & 1) * C);
         h->q_block = b + 1;

@@ -266,7 +266,7 @@ struct sb_handle {
     int64_t b_ldq = 0;
     int *b_expo = nullptr;        // fixed-point exponent per chain
     long long *b_qsum = nullptr;  // nchains x b_nq exact sums of the quantised residuals
-    int b_nq = 1;                 // quantisation workgroups per chain
+    int b_nq = 1;                 // super-steps: quantisation workgroups per chain tile
     int kss = 16;                 // super-steps (16 genotype words) per X_s'Q chunk, about nS / b_nch
     int b_nch = 1;                // X_s'Q chunks (of the individuals) per block
     int b_rw = 4;                 // row tiles per wave of the int8 products (2 or 4)
