@@ -10,8 +10,11 @@
 // 8L-1 (or, at the top of a binade, 8L-2) bits of its largest entry.  The
 // residual update r_c += Z_s delta_c is the same int8 product the other way
 // round (individual-major genotypes times the limbs of delta / s), with each
-// SNP's rounding remainder carried into its next update.  The augmented parts U_s' R_a and U_s delta
-// stay FP64 (v_mfma_f64_16x16x4f64 over all chains).
+// SNP's rounding remainder carried into its next update.  The augmented
+// parts U_s' R_a and U_s diag(s) c~ run on the same int8 MFMA, with U as four
+// balanced limbs per entry (31 bits of its column's or row's largest entry),
+// as trailing workgroups of the genotype launches.  Chains run in tiles of
+// 16; 49-64 chains run as two chain groups of 32 in each launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,18 +30,8 @@ namespace sb {
 namespace {
 constexpr int NT = 256;
 typedef int v4i __attribute__((ext_vector_type(4)));
-typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr uint32_t M2 = 0x03030303u;
 constexpr int LIMBS = 3;  // int8 limbs per fixed-point value (23 bits)
-
-// Exact value of a chain's limb sums t_0 .. t_{LIMBS-1} (limb k weighs 256^k)
-__device__ __forceinline__ long long limb_sum(const int *t)
-{
-    long long S = 0;
-#pragma unroll
-    for (int k = LIMBS - 1; k >= 0; k--) S = S * 256 + (long long)t[k];
-    return S;
-}
 
 // One 32-bit word of 2-bit counts as a 16-byte MFMA operand: dword m holds
 // the counts of positions m, m+4, m+8, m+12 (bytes 0..3).  Hence the limb
@@ -531,7 +524,7 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
 #pragma unroll
         for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(k0 + min(s, nk - 1), u)];
     const int C = P.nchains;
-    int ea[NTC];   // mode 0: the exponent of chain 16 tc + l16's r_a limbs
+    int ea[NTC];   // mode 0: the exponent of chain cbase + 16 tc + l16's r_a limbs
 #pragma unroll
     for (int tc = 0; tc < NTC; tc++)
         ea[tc] = MODE == 0 ? fixed_exponent(amax[min(cbase + 16 * tc + l16, C - 1)], LIMBS) : 0;
