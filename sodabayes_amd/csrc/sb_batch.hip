@@ -1146,8 +1146,8 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
 }
 
 // One thread per (SNP, chain): exact limb combination of the chunk
-// partials, rhs, draw (Philox, counter (iteration, SNP id, tag)); delta
-// also into the swizzled panel for U delta.  The residual update applies
+// partials and the U_s'r_a group partials, rhs, draw (Philox, counter
+// (iteration, SNP id, tag)).  The residual update applies
 // c_j = delta_j / s_j + carry_j, where carry_j is what the last
 // quantisation of SNP j's update left unapplied (error feedback, so the
 // rounding of the fixed-point update never accumulates in r); c_j goes to

@@ -219,15 +219,4 @@ __device__ double block_sum(double v, double *lds)
     return t;  // valid in thread 0
 }
 
-// Swizzled FP32 panel of a per-chain block vector (rows i of a block,
-// chains c): the operand layout of v_mfma_f64_16x16x4f64 over 16 rows x 16
-// chains, so that a wave loads a 16-row tile as one contiguous 1 KB float4
-// read.  Lane l16 + 16 lg holds rows 4 lg .. 4 lg + 3 of chain 16 t + l16 of
-// row tile K; ntc = chain tiles.
-__device__ __forceinline__ int64_t rsw_index(int i, int c, int ntc)
-{
-    const int K = i >> 4, lg = (i >> 2) & 3, s = i & 3, t = c >> 4, lane = (c & 15) + 16 * lg;
-    return (((int64_t)K * ntc + t) * 64 + lane) * 4 + s;
-}
-
 }  // namespace sb
