@@ -263,7 +263,7 @@ int sb_state_load(sb_handle *h, const void *buf, int64_t size);
  * block (1 KB per wave load, one launch per block as the sweep launches its
  * kernels) over every block; out[3k .. 3k+2] = bytes per block,
  * microseconds per block, GB/s, for k = 0..3: the batched path's SNP-major
- * and individual-major genotype tiles and its two FP32 U tile layouts, or
+ * and individual-major genotype tiles and its two int8-limb U tile layouts, or
  * the FP64 path's genotype words and packed U (k = 2, 3 zero).  12 doubles. */
 int sb_layout_stream(sb_handle *h, double *out);
 
