@@ -460,11 +460,12 @@ __host__ __device__ __forceinline__ int64_t uq_off1(int I, int nss)
 //           K steps I4 in [G gi, min(G gi + G, bx + 1));
 //   mode 1: quad bx = output rows I = 4 bx + w, column group gi: K steps
 //           kk = 4 ss + s in [max(G gi, 4 (bx / 4)), min(G gi + G, 4 nss))
-// (G = UQ_GROUP).  The B blocks of all the pair's K steps go to LDS before
-// one barrier; each wave then takes the K steps in rounds of four, its
-// ULIMBS A tiles of a round loaded together.  Waves past the block still
-// load and meet the barrier, but do no MFMA and store nothing.
-constexpr int uq_lds(int ntile) { return UQ_GROUP * ntile * 64; }   // uint4 of LDS per pair
+// (G = UQ_GROUP).  The K steps go in rounds of four: the B blocks of a
+// round go to LDS between two barriers, and each wave holds its ULIMBS A
+// tiles of the round in registers (loaded together, the next round's
+// before the barrier).  Waves past the block still load and meet the
+// barriers, but do no MFMA and store nothing.
+constexpr int uq_lds(int ntile) { return 4 * ntile * 64; }   // uint4 of LDS per pair: one round of B
 
 template <int MODE, int NTILE>
 __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
@@ -511,9 +512,9 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
         return (((int64_t)(kk >> 2) * ntot + t) * 4 + (kk & 3)) * 64 + lane;
     };
     // The pair's K steps in rounds of four: each wave holds the A tiles of one
-    // round in registers (loaded together); the B blocks of every K step of
-    // the pair go to LDS before the first barrier (each wave loads every
-    // fourth block, all loads of a round before its stores).
+    // round in registers (loaded together); the round's B blocks go to LDS
+    // between two barriers (each wave fills every fourth block, all of its
+    // loads before its stores).
     const v4i *Av = reinterpret_cast<const v4i *>(Ua);
     const v4i *Bv = reinterpret_cast<const v4i *>(Bq);
     v4i *L = reinterpret_cast<v4i *>(lds);
@@ -528,73 +529,77 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
 #pragma unroll
     for (int tc = 0; tc < NTC; tc++)
         ea[tc] = MODE == 0 ? fixed_exponent(amax[min(cbase + 16 * tc + l16, C - 1)], LIMBS) : 0;
-    if (MODE == 0) {
-        // B from r_a itself, as LIMBS balanced limbs of rint(r_a 2^e) with e
-        // from the chain's max |r_a| over the block's rows (zero past the
-        // block's rows and the chains).  Unit (s, g4, tc): lane l16 + 16 q
-        // takes rows 64 (k0 + s) + 16 g4 + 4 q .. + 3 of chain 16 tc + l16
-        // (32 contiguous bytes) and writes dword q of its limbs' B entries
-        // (lane l16 + 16 g4 of blocks (s, kk NTC + tc)).
-        uint32_t *Lw = reinterpret_cast<uint32_t *>(lds);
-        const int q = lane >> 4;
-        for (int u = wave; u < nk * 4 * NTC; u += W) {   // wave-uniform
-            const int tc = u % NTC, sg = u / NTC, s = sg >> 2, g4 = sg & 3;
-            const int c = cbase + 16 * tc + l16, i0 = 64 * (k0 + s) + 16 * g4 + 4 * q;
-            const double *rp = ra + (int64_t)min(c, C - 1) * P.blocksize + i0;
-            double v[4];
-            if (c < C && i0 + 4 <= nb && (P.blocksize & 1) == 0) {
-                const double2 x = reinterpret_cast<const double2 *>(rp)[0], y = reinterpret_cast<const double2 *>(rp)[1];
-                v[0] = x.x;
-                v[1] = x.y;
-                v[2] = y.x;
-                v[3] = y.y;
-            } else {
+    // the B blocks of round r (K steps k0 + KR r + s, s < KR) into LDS block s
+    auto fill = [&](int r) {
+        const int kb = k0 + KR * r, nkr = min(KR, k1 - kb);
+        if (MODE == 0) {
+            // B from r_a itself, as LIMBS balanced limbs of rint(r_a 2^e) with
+            // e from the chain's max |r_a| over the block's rows (zero past
+            // the block's rows and the chains).  Unit (s, g4, tc): lane l16 +
+            // 16 q takes rows 64 (kb + s) + 16 g4 + 4 q .. + 3 of chain
+            // cbase + 16 tc + l16 (32 contiguous bytes) and writes dword q of
+            // its limbs' B entries (lane l16 + 16 g4 of blocks (s, kk NTC + tc)).
+            uint32_t *Lw = reinterpret_cast<uint32_t *>(lds);
+            const int q = lane >> 4;
+            for (int u = wave; u < nkr * 4 * NTC; u += W) {   // wave-uniform
+                const int tc = u % NTC, sg = u / NTC, s = sg >> 2, g4 = sg & 3;
+                const int c = cbase + 16 * tc + l16, i0 = 64 * (kb + s) + 16 * g4 + 4 * q;
+                const double *rp = ra + (int64_t)min(c, C - 1) * P.blocksize + i0;
+                double v[4];
+                if (c < C && i0 + 4 <= nb && (P.blocksize & 1) == 0) {
+                    const double2 x = reinterpret_cast<const double2 *>(rp)[0];
+                    const double2 y = reinterpret_cast<const double2 *>(rp)[1];
+                    v[0] = x.x;
+                    v[1] = x.y;
+                    v[2] = y.x;
+                    v[3] = y.y;
+                } else {
 #pragma unroll
-                for (int b = 0; b < 4; b++) v[b] = (c < C && i0 + b < nb) ? rp[b] : 0.0;
-            }
-            int e = ea[0];
-#pragma unroll
-            for (int t = 1; t < NTC; t++) e = tc == t ? ea[t] : e;
-            uint32_t w[LIMBS] = {};
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                int32_t qv = (int32_t)rint(ldexp(v[b], e));
-#pragma unroll
-                for (int kk = 0; kk < LIMBS; kk++) {
-                    const int32_t d = (int32_t)(int8_t)(qv & 0xFF);
-                    w[kk] |= (uint32_t)(uint8_t)d << (8 * b);
-                    qv = (qv - d) >> 8;
+                    for (int b = 0; b < 4; b++) v[b] = (c < C && i0 + b < nb) ? rp[b] : 0.0;
                 }
+                int e = ea[0];
+#pragma unroll
+                for (int t = 1; t < NTC; t++) e = tc == t ? ea[t] : e;
+                uint32_t w[LIMBS] = {};
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    int32_t qv = (int32_t)rint(ldexp(v[b], e));
+#pragma unroll
+                    for (int kk = 0; kk < LIMBS; kk++) {
+                        const int32_t d = (int32_t)(int8_t)(qv & 0xFF);
+                        w[kk] |= (uint32_t)(uint8_t)d << (8 * b);
+                        qv = (qv - d) >> 8;
+                    }
+                }
+#pragma unroll
+                for (int kk = 0; kk < LIMBS; kk++)
+                    Lw[((s * NTILE + kk * NTC + tc) * 64 + l16 + 16 * g4) * 4 + q] = w[kk];
             }
-#pragma unroll
-            for (int kk = 0; kk < LIMBS; kk++) Lw[((s * NTILE + kk * NTC + tc) * 64 + l16 + 16 * g4) * 4 + q] = w[kk];
-        }
-    } else {
-        constexpr int NBQ = (KR * NTILE + W - 1) / W;   // B blocks per wave and round
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
+        } else {
+            constexpr int NBQ = (KR * NTILE + W - 1) / W;   // B blocks per wave
             v4i bq[NBQ];
 #pragma unroll
             for (int j = 0; j < NBQ; j++) {
-                const int q = KR * NTILE * r + wave + W * j, s = min(q / NTILE, nk - 1), t = q % NTILE;
-                bq[j] = Bv[bidx(k0 + s, t)];
+                const int q = wave + W * j, s = min(q / NTILE, nkr - 1), t = q % NTILE;
+                bq[j] = Bv[bidx(kb + s, t)];
             }
 #pragma unroll
             for (int j = 0; j < NBQ; j++) {
-                const int q = KR * NTILE * r + wave + W * j;
-                if (wave + W * j < KR * NTILE && q < nk * NTILE) L[q * 64 + lane] = bq[j];
+                const int q = wave + W * j;
+                if (q < nkr * NTILE) L[q * 64 + lane] = bq[j];
             }
         }
-    }
+    };
     // one accumulator per chain tile and weight 256^(ku + kr): the limb
     // products of equal weight are summed exactly in int32 (at most three
-    // products of 64 terms per K step, UQ_GROUP K steps)
+    // products of 64 terms of at most 2^14 per K step, UQ_GROUP K steps)
     constexpr int NWT = ULIMBS + LIMBS - 1;
     v4i acc[NWT][NTC];
 #pragma unroll
     for (int w = 0; w < NWT; w++)
 #pragma unroll
         for (int tc = 0; tc < NTC; tc++) acc[w][tc] = v4i{0, 0, 0, 0};
+    fill(0);
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -604,12 +609,15 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
             for (int s = 0; s < KR; s++)
 #pragma unroll
                 for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(k0 + min(KR * r + s, nk - 1), u)];
+            __syncthreads();   // every wave is done with the previous round's B
+            fill(r);
+            __syncthreads();
         }
         if (live) {
 #pragma unroll
             for (int s = 0; s < KR; s++) {
                 if (KR * r + s >= nk) break;
-                const v4i *Lb = L + (KR * r + s) * NTILE * 64 + lane;
+                const v4i *Lb = L + s * NTILE * 64 + lane;
 #pragma unroll
                 for (int kr = 0; kr < LIMBS; kr++)
 #pragma unroll

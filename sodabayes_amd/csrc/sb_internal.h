@@ -18,7 +18,7 @@ constexpr int CHUNK_WORDS = 128;    // 2048 individuals per genotype chunk (16 p
 constexpr int CHUNK_IND = CHUNK_WORDS * 16;
 // K steps (64 rows or columns each) per (quad, group) pair of the int8 U
 // products: the row split of their partial sums
-constexpr int UQ_GROUP = 8;
+constexpr int UQ_GROUP = 16;
 
 // Read-only run parameters, passed by value to kernels.
 struct Par {
