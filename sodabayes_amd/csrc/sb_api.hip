@@ -335,7 +335,7 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         const int nch = sb::batch_max_chunks(h);
         const int nwg_d = (int)((bs + 255) / 256);
         int nrg = (int)((bs + h->rows_per_group - 1) / h->rows_per_group);
-        const int64_t nwg_stat = std::max<int64_t>(h->nwg_r, ldr / 16);  // k_rstat or update workgroups
+        const int64_t nwg_stat = std::max<int64_t>(h->nwg_r, sb::RSLOT);  // k_rstat partials or update slots
         h->b_nq = (int)(P.ldw / 16);   // quantisation workgroups per chain tile: one per super-step
         A((void **)&h->b_rmax, sizeof(uint64_t) * C * nwg_stat);
         A((void **)&h->b_qsum, sizeof(long long) * C * h->b_nq);

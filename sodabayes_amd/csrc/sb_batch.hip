@@ -959,13 +959,16 @@ __global__ void k_transpose(const uint32_t *__restrict__ geno, int ldw, int p0, 
 // back to carry_j for the SNP's next update, and each workgroup leaves its
 // part of the centring constant K = sum_j 2 f_j c~_j (kpart), formed from
 // the same c~_j, so that r moves by exactly sum_j (x_ij - 2 f_j) c~_j.
+// It also clears the RSLOT max |r| slots of each chain that the update
+// launched next folds its maxima into.
 __global__ __launch_bounds__(NT) void k_dquant(Par P, int ntile, const double *cjv, const double *f2,
                                                int p0, int nb, const uint64_t *dmax, int nwg_d, uint4 *Dq,
-                                               int *expo, double *carry, double *kpart)
+                                               int *expo, double *carry, double *kpart, uint64_t *rslot)
 {
     __shared__ double lds[NT / WAVE];
     const int c = blockIdx.y;
     const int jk = blockIdx.x * NT + threadIdx.x;   // the grid covers whole words
+    if (blockIdx.x == 0 && threadIdx.x < RSLOT) rslot[(int64_t)c * RSLOT + threadIdx.x] = 0;   // for the update
     unsigned long long m = 0;
     for (int w0 = 0; w0 < nwg_d; w0 += 8) {
         unsigned long long v[8];
@@ -1056,8 +1059,9 @@ __device__ __forceinline__ void kconst_finish(const KPre &k, const double *__res
 // launch (workgroups ng and up, uq_pair mode 1).  Each lane holds the three
 // limbs of its chains in one accumulator position and updates four
 // consecutive residuals per (tile, chain tile) directly from registers.
-// Leaves per (chain, workgroup) the maximum |r| of the new values
-// (rmax[c * ng + wg]) for the next block's quantisation.
+// Folds the maximum |r| of its new values into slot wg mod RSLOT of each of
+// its chains (rmax[c * RSLOT + slot], atomic max; k_dquant clears the slots
+// before the launch) for the next block's quantisation.
 template <int NTILE, int RW>
 __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, const uint4 *__restrict__ gt, int nss, int ntr,
                                             const uint4 *__restrict__ Dq, const int *__restrict__ expo,
@@ -1149,7 +1153,8 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
         unsigned long long m = mx_s[0][threadIdx.x];
 #pragma unroll
         for (int w = 1; w < W; w++) m = mx_s[w][threadIdx.x] > m ? mx_s[w][threadIdx.x] : m;
-        rmax[(int64_t)(cbase + threadIdx.x) * ng + wg] = m;
+        atomicMax(reinterpret_cast<unsigned long long *>(rmax) + (int64_t)(cbase + threadIdx.x) * RSLOT + wg % RSLOT,
+                  (unsigned long long)m);
     }
 }
 
@@ -1422,14 +1427,15 @@ int batch_update(sb_handle *h, int b)
     kc.begin(st);
     kc.last(KS_DQUANT, 16.0 * C * nb + cl * nb);
     launch(kc, k_dquant, dim3(nsw * 16 / NT, C), dim3(NT), 0, st, P, h->b_ntile, h->b_cj, h->f2, p0, nb,
-           h->b_dmax, nwg_d, Dq, h->b_expo2, h->b_carry, h->b_kpart);
+           h->b_dmax, nwg_d, Dq, h->b_expo2, h->b_carry, h->b_kpart, h->b_rmax);
     const int spant = h->rows_per_group / 16, ncg = (nbt + spant - 1) / spant;
     kc.begin(st);
     kc.last(KS_XUPD, gbytes + cl * nb + 16.0 * C * n + tbytes + 12.0 * C * nb);
     int ng = 0;
     const int rc = upd_launch(h, b, h->b_rw, h->r, &ng, true);   // with U_s diag(s) c~
     if (rc) return fail(rc, "unsupported chain count");
-    h->rstat_nwg = ng;   // max |r| partials of the new r, per update workgroup
+    (void)ng;
+    h->rstat_nwg = RSLOT;   // max |r| of the new r, in the update's slots
     kc.begin(st);
     if (b + 1 < P.nblocks) {
         // the next block's r limbs and r_a maxima in the same launch

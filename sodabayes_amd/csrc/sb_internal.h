@@ -19,6 +19,9 @@ constexpr int CHUNK_IND = CHUNK_WORDS * 16;
 // K steps (64 rows or columns each) per (quad, group) pair of the int8 U
 // products: the row split of their partial sums
 constexpr int UQ_GROUP = 16;
+// slots per chain into which the residual update folds its workgroups' max |r|
+// (atomic max into slot workgroup mod RSLOT)
+constexpr int RSLOT = 16;
 
 // Read-only run parameters, passed by value to kernels.
 struct Par {
