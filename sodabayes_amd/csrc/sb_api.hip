@@ -534,6 +534,10 @@ int sb_get(sb_handle *h, int32_t what, int32_t chain, double *out, int64_t count
             for (int64_t k = 0; k < p; k++) out[c * p + h->permvec[k]] = tmp[c * p + k];
         return SB_OK;
     };
+    if (what == SB_Q_INDISTSTORE) {
+        const int rc = class1_counts(h);
+        if (rc) return rc;
+    }
     hipStreamSynchronize(st);
     Scal sc;
     bool need_scal = what == SB_Q_P || what == SB_Q_SCALARS || what == SB_Q_STORES || what == SB_Q_SNPSTORE ||
@@ -691,6 +695,7 @@ int sb_state_save(sb_handle *h, void *buf, int64_t size)
     if (!h->initialised) return fail(SB_ERR_STATE, "sb_state_save before sb_mcmc_init");
     const int64_t need = sb::state_bytes(h, h->use_heldout);
     if (size < need) return fail(SB_ERR_ARG, "sb_state_save: buffer smaller than sb_state_size");
+    if (int rc = sb::class1_counts(h)) return rc;   // the checkpoint holds every class's counts
     if (hip_check(hipStreamSynchronize(h->stream), "state save")) return SB_ERR_HIP;
     const std::vector<sb::Section> secs = sb::state_sections(h, h->use_heldout);
     sb::StateHeader hd;
@@ -761,6 +766,7 @@ int sb_state_load(sb_handle *h, const void *buf, int64_t size)
     h->ra_block = -1;
     int rc = sb::init_xpx(h);
     if (rc) return rc;
+    if ((rc = sb::samples_from_counts(h))) return rc;
     if (hip_check(hipStreamSynchronize(h->stream), "state load")) return SB_ERR_HIP;
     h->initialised = true;
     h->graph_ok = false;   // a captured sweep may name another state (held-out rows)

@@ -345,6 +345,8 @@ int setup_gram_block(sb_handle *h, int block, double *G_host);
 int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy = 0,
               const uint8_t *heldout = nullptr, int64_t ldh = 0);
 int mcmc_run(sb_handle *h, int first, int count, double *hyp);
+int class1_counts(sb_handle *h);
+int samples_from_counts(sb_handle *h);
 int counter_uniforms(int64_t seed, int chain, const uint32_t *ctr, int n, int per, double *out);
 
 // chain-batched mixed-precision block products and draws (sb_batch.hip)

@@ -794,13 +794,21 @@ __global__ void k_ra_counter(Par P, const Scal *sc, double *ra, const int *it, i
     ra[(int64_t)c * P.blocksize + i] = v;
 }
 
-// posterior sums per SNP (models.f90:528-544)
-__global__ void k_accum(Par P, const int *it, const double *beta, const int32_t *cls, const double *vara_s,
+// posterior sums per SNP (models.f90:528-544).  A class-1 draw sets the
+// effect to exactly 0.0 (sb_draw.h draw_core) and every sample adds 1.0 to
+// exactly one class of each SNP, so class 1's count is the number of samples
+// minus the other classes' counts: only classes above 1 are read, modified
+// and written here, the samples are counted in it[2], and k_class1 writes
+// class 1's counts where the sums are read out (the same integers in FP64).
+// Adding a zero effect leaves the effect sum's bits unchanged (the sum is
+// never -0.0), so zero effects skip it too.
+__global__ void k_accum(Par P, int *it, const double *beta, const int32_t *cls, const double *vara_s,
                         double *gstore, double *indiststore, double *vara_sstore)
 {
     const int rep = it[0];
     if (!(rep > P.burnin && rep % P.thin == 0)) return;   // models.f90:526
     int pos = blockIdx.x * NT + threadIdx.x;
+    if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) it[2] += 1;   // samples in the sums
     if (pos >= P.p) return;
     const int64_t c = blockIdx.z;
     beta += c * P.p;
@@ -809,12 +817,39 @@ __global__ void k_accum(Par P, const int *it, const double *beta, const int32_t 
     gstore += c * P.p;
     vara_sstore += c * P.p;
     indiststore += c * P.p * P.nmix;
-    // adding a zero effect leaves the sum's bits unchanged (the sum is never
-    // -0.0), so zero effects skip the read-modify-write
-    const double b = beta[pos];
-    if (b != 0.0) gstore[pos] += b;
-    if (P.method != SB_BAYES_A) indiststore[(int64_t)(cls[pos] - 1) * P.p + pos] += 1.0;
+    if (P.method == SB_BAYES_A) {
+        const double b = beta[pos];
+        if (b != 0.0) gstore[pos] += b;
+    } else {
+        const int k = cls[pos];
+        if (k > 1) {
+            const double b = beta[pos];
+            if (b != 0.0) gstore[pos] += b;
+            indiststore[(int64_t)(k - 1) * P.p + pos] += 1.0;
+        }
+    }
     if (P.method == SB_BAYES_A || P.method == SB_BAYES_B) vara_sstore[pos] += vara_s[pos];
+}
+
+// Class 1's posterior counts: the samples (it[2]) minus the other classes'
+// counts, per (SNP, chain); not used by BayesA, which keeps no class counts.
+__global__ void k_class1(Par P, const int *it, double *indiststore)
+{
+    const int pos = blockIdx.x * NT + threadIdx.x;
+    if (pos >= P.p) return;
+    double *d = indiststore + (int64_t)blockIdx.z * P.p * P.nmix;
+    double o = 0.0;
+    for (int k = 1; k < P.nmix; k++) o += d[(int64_t)k * P.p + pos];
+    d[pos] = (double)it[2] - o;
+}
+
+// it[2] = the samples in the posterior sums of a loaded state: the sum of
+// SNP 0's class counts of chain 0 (every sample counted one class of it)
+__global__ void k_samples_from_counts(Par P, const double *indiststore, int *it)
+{
+    double s = 0.0;
+    for (int k = 0; k < P.nmix; k++) s += indiststore[(int64_t)k * P.p];
+    it[2] = P.method == SB_BAYES_A ? 0 : (int)s;
 }
 
 // ------------------------------------------------------------- resync --
@@ -1049,7 +1084,8 @@ __global__ void k_rhs_out(Par P, const double *part, const double *partA, const 
 
 // dense factor of block b (pivoted-Cholesky fallback), or nullptr
 // The sweep's device iteration state: it[0] = the iteration being run,
-// it[1] = the index of its .hyp record.  Set at the start of each
+// it[1] = the index of its .hyp record (it[2]: the samples in the posterior
+// sums, counted by k_accum, cleared by mcmc_init).  Set at the start of each
 // sb_mcmc_run call; every iteration begins by advancing it[0], so that one
 // captured iteration replays for any iteration number.
 __global__ void k_set_it(int *it, int rep, int rec)
@@ -1229,6 +1265,7 @@ int mcmc_init(sb_handle *h, const sb_priors *pr, const double *why, int64_t ldy,
     hipMemsetAsync(h->gstore, 0, sizeof(double) * p * Cs, h->stream);
     hipMemsetAsync(h->vara_sstore, 0, sizeof(double) * p * Cs, h->stream);
     hipMemsetAsync(h->indiststore, 0, sizeof(double) * p * nmix * Cs, h->stream);
+    hipMemsetAsync(h->d_it + 2, 0, sizeof(int), h->stream);   // samples in the posterior sums
     std::vector<int32_t> two(p * Cs, 2);
     hipMemcpyAsync(h->cls, two.data(), sizeof(int32_t) * p * Cs, hipMemcpyHostToDevice, h->stream);
     std::vector<double> vs(p * Cs);
@@ -1467,6 +1504,22 @@ static bool graph_wanted(const sb_handle *h)
     const char *e = std::getenv("SODABAYES_GRAPH");
     if (!e || e[0] != '1') return false;
     return !h->kc.on && !h->par.regular && h->nskipped == 0 && h->par.nblocks > 0;
+}
+
+// class 1's posterior counts into indiststore (before it is read out)
+int class1_counts(sb_handle *h)
+{
+    const Par &P = h->par;
+    if (P.method == SB_BAYES_A) return SB_OK;
+    k_class1<<<dim3((P.p + NT - 1) / NT, 1, P.nchains), NT, 0, h->stream>>>(P, h->d_it, h->indiststore);
+    return hip_check(hipGetLastError(), "class-1 counts");
+}
+
+// the sample count of the posterior sums after a state load
+int samples_from_counts(sb_handle *h)
+{
+    k_samples_from_counts<<<1, 1, 0, h->stream>>>(h->par, h->indiststore, h->d_it);
+    return hip_check(hipGetLastError(), "sample count");
 }
 
 int mcmc_run(sb_handle *h, int first, int count, double *hyp_host)

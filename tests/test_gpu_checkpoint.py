@@ -15,7 +15,7 @@ from sodabayes_amd import soda, synth
 pytestmark = pytest.mark.gpu
 
 R4 = "0.0,0.0001,0.001,0.01"
-QUERIES = ["G", "CLASS", "YADJ", "YAADJ", "SCALARS", "GSTORE", "STORES", "P", "WHY"]
+QUERIES = ["G", "CLASS", "YADJ", "YAADJ", "SCALARS", "GSTORE", "STORES", "P", "WHY", "INDISTSTORE"]
 
 
 def make(opts, n, p, bed, why, **kw):
@@ -26,7 +26,8 @@ def make(opts, n, p, bed, why, **kw):
 
 def snapshot(s, chains):
     sizes = {"G": s.nloci, "CLASS": s.nloci, "YADJ": s.nind, "YAADJ": s.blocksize, "SCALARS": 4,
-             "GSTORE": s.nloci, "STORES": 5, "P": s.ngroup * s.nmix, "WHY": s.nind}
+             "GSTORE": s.nloci, "STORES": 5, "P": s.ngroup * s.nmix, "WHY": s.nind,
+             "INDISTSTORE": s.nloci * s.nmix}
     return [{q: s.get(q, sizes[q], c) for q in QUERIES} for c in range(chains)]
 
 
@@ -48,6 +49,11 @@ def check_resume(opts, n, p, bed, why, chains, k1, k2, **kw):
     for c in range(chains):
         for q in QUERIES:
             assert np.array_equal(want[c][q], got[c][q]), (c, q)
+        # every sample after burn-in counts one class of each SNP (the counts
+        # were read out once mid-run, by the save, and again at the end)
+        counts = want[c]["INDISTSTORE"].reshape(-1, p)
+        assert np.all(counts >= 0) and np.all(counts == np.round(counts)), c
+        assert np.all(counts.sum(axis=0) == k1 + k2 - opts.burnin), c
     return blob
 
 
