@@ -608,17 +608,49 @@ __global__ __launch_bounds__(NT) void k_bins(Par P, const double *beta, const in
     }
 }
 
+// Posterior sums of one (SNP, chain) sample, from its effect b and class k
+// (models.f90:528-544); the chain's arrays.  A class-1 draw sets the effect
+// to exactly 0.0 (sb_draw.h draw_core) and every sample adds 1.0 to exactly
+// one class of each SNP, so class 1's count is the number of samples minus
+// the other classes' counts: only classes above 1 are read, modified and
+// written, the samples are counted in it[2], and k_class1 writes class 1's
+// counts where the sums are read out (the same integers in FP64).  Adding a
+// zero effect leaves the effect sum's bits unchanged (the sum is never
+// -0.0), so zero effects skip it too.
+__device__ __forceinline__ void accum_one(const Par &P, int pos, double b, int k, const double *vara_s,
+                                          double *gstore, double *indiststore, double *vara_sstore)
+{
+    if (P.method == SB_BAYES_A || k > 1) {
+        if (b != 0.0) gstore[pos] += b;
+        if (P.method != SB_BAYES_A) indiststore[(int64_t)(k - 1) * P.p + pos] += 1.0;
+    }
+    if (P.method == SB_BAYES_A || P.method == SB_BAYES_B) vara_sstore[pos] += vara_s[pos];
+}
+
 // The same, every bin in one pass over the chunk (nbins <= NBM): each thread
 // adds to its bins' sums in the order of the per-bin pass, so the sums are
-// bit-identical, and beta / cls / grp are read once instead of once per bin.
+// bit-identical, and beta / cls / grp are read once instead of once per bin
+// (four positions' loads in flight).  When the iteration's sample goes into
+// the posterior sums (it[0] past burn-in and on the thinning grid), the same
+// pass adds it (accum_one): nothing between this launch and the end of the
+// iteration changes an effect, class or per-SNP variance.
 template <int NBM>
 __global__ __launch_bounds__(NT) void k_bins_all(Par P, const double *beta, const int32_t *cls,
-                                                 const int32_t *grp, int chunk, double *cnt_out, double *var_out)
+                                                 const int32_t *grp, int chunk, double *cnt_out, double *var_out,
+                                                 int *it, const double *vara_s, double *gstore,
+                                                 double *indiststore, double *vara_sstore)
 {
     __shared__ double lds[NT / WAVE];
     const int wg = blockIdx.x, ch = blockIdx.z;
+    const int rep = it[0];
+    const bool acc = rep > P.burnin && rep % P.thin == 0;   // models.f90:526
+    if (acc && wg == 0 && ch == 0 && threadIdx.x == 0) it[2] += 1;   // samples in the sums
     beta += (int64_t)ch * P.p;
     cls += (int64_t)ch * P.p;
+    vara_s += (int64_t)ch * P.p;
+    gstore += (int64_t)ch * P.p;
+    vara_sstore += (int64_t)ch * P.p;
+    indiststore += (int64_t)ch * P.p * P.nmix;
     cnt_out += (int64_t)ch * P.nbins * gridDim.x;
     var_out += (int64_t)ch * P.nbins * gridDim.x;
     const bool all = P.method == SB_BAYES_A;
@@ -626,15 +658,31 @@ __global__ __launch_bounds__(NT) void k_bins_all(Par P, const double *beta, cons
     double c[NBM], v[NBM];
 #pragma unroll
     for (int b = 0; b < NBM; b++) c[b] = v[b] = 0.0;
-    for (int pos = a + threadIdx.x; pos < e; pos += NT) {
-        const int gq = grp[pos], kq = cls[pos] - 1;   // bin (class kq, group gq), as k_bins matches it
-        const int bp = (gq >= 0 && gq < P.ngroup && kq >= 0 && kq < P.nmix) ? kq * P.ngroup + gq : -1;
-        const double x = beta[pos] * beta[pos];
+    constexpr int U = 4;
+    for (int p0 = a + threadIdx.x; p0 < e; p0 += U * NT) {
+        int gq[U], kc[U];
+        double bt[U];
 #pragma unroll
-        for (int b = 0; b < NBM; b++) {
-            const bool in = all || b == bp;
-            c[b] += in ? 1.0 : 0.0;
-            v[b] += in ? x : 0.0;
+        for (int u = 0; u < U; u++) {
+            const int pos = min(p0 + u * NT, e - 1);
+            gq[u] = grp[pos];
+            kc[u] = cls[pos];
+            bt[u] = beta[pos];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int pos = p0 + u * NT;
+            if (pos >= e) break;
+            const int kq = kc[u] - 1;   // bin (class kq, group gq), as k_bins matches it
+            const int bp = (gq[u] >= 0 && gq[u] < P.ngroup && kq >= 0 && kq < P.nmix) ? kq * P.ngroup + gq[u] : -1;
+            const double x = bt[u] * bt[u];
+#pragma unroll
+            for (int b = 0; b < NBM; b++) {
+                const bool in = all || b == bp;
+                c[b] += in ? 1.0 : 0.0;
+                v[b] += in ? x : 0.0;
+            }
+            if (acc) accum_one(P, pos, bt[u], kc[u], vara_s, gstore, indiststore, vara_sstore);
         }
     }
     for (int b = 0; b < P.nbins; b++) {   // workgroup-uniform
@@ -794,14 +842,8 @@ __global__ void k_ra_counter(Par P, const Scal *sc, double *ra, const int *it, i
     ra[(int64_t)c * P.blocksize + i] = v;
 }
 
-// posterior sums per SNP (models.f90:528-544).  A class-1 draw sets the
-// effect to exactly 0.0 (sb_draw.h draw_core) and every sample adds 1.0 to
-// exactly one class of each SNP, so class 1's count is the number of samples
-// minus the other classes' counts: only classes above 1 are read, modified
-// and written here, the samples are counted in it[2], and k_class1 writes
-// class 1's counts where the sums are read out (the same integers in FP64).
-// Adding a zero effect leaves the effect sum's bits unchanged (the sum is
-// never -0.0), so zero effects skip it too.
+// posterior sums per SNP (models.f90:528-544), for the sweeps whose bin
+// sums take one launch per bin (k_bins): accum_one per (SNP, chain)
 __global__ void k_accum(Par P, int *it, const double *beta, const int32_t *cls, const double *vara_s,
                         double *gstore, double *indiststore, double *vara_sstore)
 {
@@ -811,24 +853,8 @@ __global__ void k_accum(Par P, int *it, const double *beta, const int32_t *cls, 
     if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) it[2] += 1;   // samples in the sums
     if (pos >= P.p) return;
     const int64_t c = blockIdx.z;
-    beta += c * P.p;
-    cls += c * P.p;
-    vara_s += c * P.p;
-    gstore += c * P.p;
-    vara_sstore += c * P.p;
-    indiststore += c * P.p * P.nmix;
-    if (P.method == SB_BAYES_A) {
-        const double b = beta[pos];
-        if (b != 0.0) gstore[pos] += b;
-    } else {
-        const int k = cls[pos];
-        if (k > 1) {
-            const double b = beta[pos];
-            if (b != 0.0) gstore[pos] += b;
-            indiststore[(int64_t)(k - 1) * P.p + pos] += 1.0;
-        }
-    }
-    if (P.method == SB_BAYES_A || P.method == SB_BAYES_B) vara_sstore[pos] += vara_s[pos];
+    accum_one(P, pos, beta[c * P.p + pos], P.method == SB_BAYES_A ? 2 : cls[c * P.p + pos], vara_s + c * P.p,
+              gstore + c * P.p, indiststore + c * P.p * P.nmix, vara_sstore + c * P.p);
 }
 
 // Class 1's posterior counts: the samples (it[2]) minus the other classes'
@@ -1085,7 +1111,7 @@ __global__ void k_rhs_out(Par P, const double *part, const double *partA, const 
 // dense factor of block b (pivoted-Cholesky fallback), or nullptr
 // The sweep's device iteration state: it[0] = the iteration being run,
 // it[1] = the index of its .hyp record (it[2]: the samples in the posterior
-// sums, counted by k_accum, cleared by mcmc_init).  Set at the start of each
+// sums, counted by the pass that adds them, cleared by mcmc_init).  Set at the start of each
 // sb_mcmc_run call; every iteration begins by advancing it[0], so that one
 // captured iteration replays for any iteration number.
 __global__ void k_set_it(int *it, int rep, int rec)
@@ -1454,10 +1480,16 @@ static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd
         });
     }
     k_rtr<<<dim3(nred, 1, C), NT, 0, st>>>(P, h->r, h->sc.red);
+    // one-pass bins also add the posterior sums (k_accum otherwise, below)
+    const bool bins1 = P.nbins <= 16;
     if (P.nbins <= 4)
-        k_bins_all<4><<<dim3(nwg_bins, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+        k_bins_all<4><<<dim3(nwg_bins, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar,
+                                                           h->d_it, h->vara_s, h->gstore, h->indiststore,
+                                                           h->vara_sstore);
     else if (P.nbins <= 16)
-        k_bins_all<16><<<dim3(nwg_bins, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
+        k_bins_all<16><<<dim3(nwg_bins, 1, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar,
+                                                            h->d_it, h->vara_s, h->gstore, h->indiststore,
+                                                            h->vara_sstore);
     else
         k_bins<<<dim3(nwg_bins, P.nbins, C), NT, 0, st>>>(P, h->beta, h->cls, h->grp, bin_chunk, bcnt, bvar);
     k_hyper<<<C, 64, 0, st>>>(P, h->scal, h->sc.red, nred, bcnt, bvar, nwg_bins, h->d_it, h->hyp);
@@ -1471,8 +1503,9 @@ static int sweep_iteration(sb_handle *h, int rep, const std::vector<double> &xxd
         k_impute<<<dim3((P.n + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->scal, h->heldout, h->why, h->r, h->d_it);
         h->rstat_nwg = 0;
     }
-    k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->d_it, h->beta, h->cls, h->vara_s, h->gstore,
-                                                           h->indiststore, h->vara_sstore);
+    if (!bins1)
+        k_accum<<<dim3((P.p + NT - 1) / NT, 1, C), NT, 0, st>>>(P, h->d_it, h->beta, h->cls, h->vara_s,
+                                                               h->gstore, h->indiststore, h->vara_sstore);
     return hip_check(hipGetLastError(), "sweep launch");
 }
 
