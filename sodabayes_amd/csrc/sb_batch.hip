@@ -1471,8 +1471,11 @@ void tune_rws(const sb_handle *h, std::vector<int> &rws)
     rws.assign({2});
     if (prod_rw(h->b_ntile) == 4) rws.push_back(4);
 }
-// chunk candidates: 0.75 to 3 workgroups per CU (in quarters)
-constexpr int TUNE_Q[] = {3, 4, 5, 6, 8, 12};
+// chunk candidates: 0.75 to 3 workgroups per CU (in quarters).  The best
+// count is often the one whose X_s'Q workgroups and U'r_a pairs together just
+// fill the CUs' workgroup slots (at 64 chains and n = 50,000: 7 chunks, 448 +
+// 320 workgroups = three per CU), so the steps between 1.5 and 2 are fine.
+constexpr int TUNE_Q[] = {3, 4, 5, 6, 7, 8, 10, 12};
 void tune_chunks(const sb_handle *h, int rw, std::vector<int> &nchs)
 {
     nchs.clear();
