@@ -50,11 +50,11 @@ KERNEL_NAMES = {
 # (profiles/soda_bayesr_n50k_bs4096_x64_pmc_traffic.md); None where not measured
 TRAFFIC_CHAINS = 64
 TRAFFIC = {
-    ("mixed", "xupd"): int((87127 * 2 + 30922) * 1024),  # k_upd<6, 2> with the U_s diag(s) c pairs
-    ("mixed", "xq"): int((97652 * 2 + 15391) * 1024),    # k_prod<6, 2> with the U_s'r_a pairs
-    ("mixed", "drawb"): int((10934 * 2 + 7456) * 1024),  # k_draw_b<4>
-    ("fp64", "rhs"): int((60332 * 2 + 920) * 1024),      # k_rhs
-    ("fp64", "update"): int((50640 * 2 + 604) * 1024),   # k_update (k_ra_sum not counted)
+    ("mixed", "xupd"): int((87088 * 2 + 30922) * 1024),  # k_upd<6, 2> with the U_s diag(s) c pairs
+    ("mixed", "xq"): int((100614 * 2 + 19565) * 1024),    # k_prod<6, 2> with the U_s'r_a pairs
+    ("mixed", "drawb"): int((12982 * 2 + 7456) * 1024),  # k_draw_b<4>
+    ("fp64", "rhs"): int((60338 * 2 + 920) * 1024),      # k_rhs
+    ("fp64", "update"): int((50642 * 2 + 604) * 1024),   # k_update (k_ra_sum not counted)
 }
 
 
