@@ -332,6 +332,8 @@ int sb_create(const sb_dims *dims, const uint8_t *bed, const double *freq, const
         }
         h->kss = (nS + h->b_nch - 1) / h->b_nch;
         // the chunk partials are sized for the most chunks the tuner tries
+        h->uq_group = sb::batch_uq_group(C);   // before the chunk candidates, which count the U pairs
+        h->rows_per_group = 64 * h->uq_group;
         const int nch = sb::batch_max_chunks(h);
         const int nwg_d = (int)((bs + 255) / 256);
         int nrg = (int)((bs + h->rows_per_group - 1) / h->rows_per_group);

@@ -410,7 +410,7 @@ __device__ __forceinline__ void limb_mainloop(const uint4 *__restrict__ abase, c
 // chain, from the chain's max |r_a| over the block's rows (each pair
 // quantises the rows it needs); mode 1 multiplies with the limbs of
 // c~ = delta / s + carry that the genotype update applies (k_dquant), which
-// is why its U columns carry s_j.  A group is 64 UQ_GROUP rows (mode 0) or
+// is why its U columns carry s_j.  A group is 64 G rows (mode 0) or
 // columns (mode 1), the row split of the partial sums that the draw and the
 // r_a reduction add up.  The pairs run as the trailing workgroups of the
 // genotype launches (mode 0 after X_s'Q, mode 1 after the residual update),
@@ -432,22 +432,22 @@ __host__ __device__ __forceinline__ int64_t uq_off0(int J)
     const int64_t a = J >> 2, r = J & 3;   // sum over J' < J of (J' / 4 + 1)
     return (a + 1) * (2 * a + r);
 }
-// pairs before quad bx: mode 0, bx / UQ_GROUP + 1 row groups per quad
-__host__ __device__ __forceinline__ int64_t uq_pairs0(int bx)
+// pairs before quad bx (G K steps per group): mode 0, bx / G + 1 row groups per quad
+__host__ __device__ __forceinline__ int64_t uq_pairs0(int bx, int G)
 {
-    const int64_t a = bx / UQ_GROUP, r = bx % UQ_GROUP;   // sum over x < bx of (x / G + 1)
-    return (a + 1) * ((int64_t)UQ_GROUP * a / 2 + r);
+    const int64_t a = bx / G, r = bx % G;   // sum over x < bx of (x / G + 1)
+    return (a + 1) * ((int64_t)G * a / 2 + r);
 }
-// mode 1: ng - bx / UQ_GROUP column groups per quad (ng groups in all)
-__host__ __device__ __forceinline__ int64_t uq_pairs1(int bx, int ng)
+// mode 1: ng - bx / G column groups per quad (ng groups in all)
+__host__ __device__ __forceinline__ int64_t uq_pairs1(int bx, int ng, int G)
 {
-    return (int64_t)bx * (ng + 1) - uq_pairs0(bx);
+    return (int64_t)bx * (ng + 1) - uq_pairs0(bx, G);
 }
 // (quad, group) pairs of a block of nbt tiles, nss super-steps
-__host__ __device__ __forceinline__ int64_t uq_pairs(int nbt, int nss, int mode)
+__host__ __device__ __forceinline__ int64_t uq_pairs(int nbt, int nss, int mode, int G)
 {
     const int nq4 = (nbt + 3) / 4;
-    return mode == 0 ? uq_pairs0(nq4) : uq_pairs1(nq4, (nss * 4 + UQ_GROUP - 1) / UQ_GROUP);
+    return mode == 0 ? uq_pairs0(nq4, G) : uq_pairs1(nq4, (nss * 4 + G - 1) / G, G);
 }
 __host__ __device__ __forceinline__ int64_t uq_off1(int I, int nss)
 {
@@ -460,12 +460,13 @@ __host__ __device__ __forceinline__ int64_t uq_off1(int I, int nss)
 //           K steps I4 in [G gi, min(G gi + G, bx + 1));
 //   mode 1: quad bx = output rows I = 4 bx + w, column group gi: K steps
 //           kk = 4 ss + s in [max(G gi, 4 (bx / 4)), min(G gi + G, 4 nss))
-// (G = UQ_GROUP).  The K steps go in rounds of four: the B blocks of a
-// round go to LDS between two barriers, and each wave holds its ULIMBS A
-// tiles of the round in registers (loaded together, the next round's
-// before the barrier).  Waves past the block still load and meet the
-// barriers, but do no MFMA and store nothing.
-constexpr int uq_lds(int ntile) { return 4 * ntile * 64; }   // uint4 of LDS per pair: one round of B
+// (G = K steps per group: 8 or 16).  The B blocks go to LDS eight K steps
+// at a time (so a pair of 8 K steps fills LDS once, between its loads and
+// one barrier), and each wave holds the ULIMBS A tiles of four K steps in
+// registers (loaded together, the next four's before any barrier).  Waves
+// past the block still load and meet the barriers, but do no MFMA and store
+// nothing.
+constexpr int uq_lds(int ntile) { return 8 * ntile * 64; }   // uint4 of LDS per pair: eight K steps of B
 
 template <int MODE, int NTILE>
 __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ Ua, int nbt, int nb, int nss,
@@ -473,30 +474,30 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
                                         const double *__restrict__ ra,
                                         const unsigned long long *__restrict__ amax, const int *__restrict__ ue,
                                         const int *__restrict__ ce, double *__restrict__ part, int k,
-                                        uint4 *lds)
+                                        int G, uint4 *lds)
 {
     constexpr int W = NT / WAVE, NTC = NTILE / LIMBS;
-    constexpr int KR = 4, NR = UQ_GROUP / KR;   // K steps per round, rounds per pair
+    constexpr int KR = 4, KB = 8;   // K steps per round of A tiles (registers), of B blocks (LDS)
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int g = lane >> 4, l16 = lane & 15;
     int bx, gi, k0, k1;
     if (MODE == 0) {
-        // pairs of quad bx: bx / UQ_GROUP + 1 groups; uq_pairs0(bx) pairs before it
-        bx = (int)sqrtf(2.0f * UQ_GROUP * (float)k);
-        while (bx > 0 && uq_pairs0(bx) > k) bx--;
-        while (uq_pairs0(bx + 1) <= k) bx++;
-        gi = (int)(k - uq_pairs0(bx));
-        k0 = UQ_GROUP * gi;
-        k1 = min(UQ_GROUP * gi + UQ_GROUP, bx + 1);
+        // pairs of quad bx: bx / G + 1 groups; uq_pairs0(bx, G) pairs before it
+        bx = (int)sqrtf(2.0f * G * (float)k);
+        while (bx > 0 && uq_pairs0(bx, G) > k) bx--;
+        while (uq_pairs0(bx + 1, G) <= k) bx++;
+        gi = (int)(k - uq_pairs0(bx, G));
+        k0 = G * gi;
+        k1 = min(G * gi + G, bx + 1);
     } else {
-        // pairs of quad bx: the column groups (UQ_GROUP / 4 super-steps each)
+        // pairs of quad bx: the column groups (G / 4 super-steps each)
         // from the one holding super-step bx / 4
-        const int ng = (nss * 4 + UQ_GROUP - 1) / UQ_GROUP;
+        const int ng = (nss * 4 + G - 1) / G;
         bx = 0;
-        while (uq_pairs1(bx + 1, ng) <= k) bx++;
-        gi = bx / UQ_GROUP + (int)(k - uq_pairs1(bx, ng));
-        k0 = max(UQ_GROUP * gi, 4 * (bx / 4));
-        k1 = min(UQ_GROUP * gi + UQ_GROUP, 4 * nss);
+        while (uq_pairs1(bx + 1, ng, G) <= k) bx++;
+        gi = bx / G + (int)(k - uq_pairs1(bx, ng, G));
+        k0 = max(G * gi, 4 * (bx / 4));
+        k1 = min(G * gi + G, 4 * nss);
     }
     const int O = 4 * bx + wave;      // the wave's output tile (J or I)
     const bool live = O < nbt;
@@ -512,9 +513,9 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
         return (((int64_t)(kk >> 2) * ntot + t) * 4 + (kk & 3)) * 64 + lane;
     };
     // The pair's K steps in rounds of four: each wave holds the A tiles of one
-    // round in registers (loaded together); the round's B blocks go to LDS
-    // between two barriers (each wave fills every fourth block, all of its
-    // loads before its stores).
+    // round in registers (loaded together); the B blocks of eight K steps go
+    // to LDS at a time (each wave fills every fourth block, its loads of four
+    // K steps before their stores).
     const v4i *Av = reinterpret_cast<const v4i *>(Ua);
     const v4i *Bv = reinterpret_cast<const v4i *>(Bq);
     v4i *L = reinterpret_cast<v4i *>(lds);
@@ -529,9 +530,9 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
 #pragma unroll
     for (int tc = 0; tc < NTC; tc++)
         ea[tc] = MODE == 0 ? fixed_exponent(amax[min(cbase + 16 * tc + l16, C - 1)], LIMBS) : 0;
-    // the B blocks of round r (K steps k0 + KR r + s, s < KR) into LDS block s
-    auto fill = [&](int r) {
-        const int kb = k0 + KR * r, nkr = min(KR, k1 - kb);
+    // the B blocks of K steps k0 + KB rb + s, s < KB, into LDS block s
+    auto fill = [&](int rb) {
+        const int kb = k0 + KB * rb, nkr = min(KB, k1 - kb);
         if (MODE == 0) {
             // B from r_a itself, as LIMBS balanced limbs of rint(r_a 2^e) with
             // e from the chain's max |r_a| over the block's rows (zero past
@@ -576,23 +577,27 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
                     Lw[((s * NTILE + kk * NTC + tc) * 64 + l16 + 16 * g4) * 4 + q] = w[kk];
             }
         } else {
-            constexpr int NBQ = (KR * NTILE + W - 1) / W;   // B blocks per wave
-            v4i bq[NBQ];
+            constexpr int NBQ = (KR * NTILE + W - 1) / W;   // B blocks per wave and four K steps
 #pragma unroll
-            for (int j = 0; j < NBQ; j++) {
-                const int q = wave + W * j, s = min(q / NTILE, nkr - 1), t = q % NTILE;
-                bq[j] = Bv[bidx(kb + s, t)];
-            }
+            for (int h = 0; h < KB / KR; h++) {
+                if (KR * h >= nkr) break;   // workgroup-uniform
+                v4i bq[NBQ];
 #pragma unroll
-            for (int j = 0; j < NBQ; j++) {
-                const int q = wave + W * j;
-                if (q < nkr * NTILE) L[q * 64 + lane] = bq[j];
+                for (int j = 0; j < NBQ; j++) {
+                    const int q = KR * NTILE * h + wave + W * j, s = min(q / NTILE, nkr - 1), t = q % NTILE;
+                    bq[j] = Bv[bidx(kb + s, t)];
+                }
+#pragma unroll
+                for (int j = 0; j < NBQ; j++) {
+                    const int q = KR * NTILE * h + wave + W * j;
+                    if (wave + W * j < KR * NTILE && q < nkr * NTILE) L[q * 64 + lane] = bq[j];
+                }
             }
         }
     };
     // one accumulator per chain tile and weight 256^(ku + kr): the limb
     // products of equal weight are summed exactly in int32 (at most three
-    // products of 64 terms of at most 2^14 per K step, UQ_GROUP K steps)
+    // products of 64 terms of at most 2^14 per K step, G <= 16 K steps)
     constexpr int NWT = ULIMBS + LIMBS - 1;
     v4i acc[NWT][NTC];
 #pragma unroll
@@ -601,23 +606,24 @@ __device__ __forceinline__ void uq_pair(const Par &P, const uint4 *__restrict__ 
         for (int tc = 0; tc < NTC; tc++) acc[w][tc] = v4i{0, 0, 0, 0};
     fill(0);
     __syncthreads();
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
+    for (int r = 0; KR * r < nk; r++) {   // workgroup-uniform
         if (r > 0) {
-            if (KR * r >= nk) break;   // workgroup-uniform
 #pragma unroll
             for (int s = 0; s < KR; s++)
 #pragma unroll
                 for (int u = 0; u < ULIMBS; u++) a[s][u] = Av[aidx(k0 + min(KR * r + s, nk - 1), u)];
-            __syncthreads();   // every wave is done with the previous round's B
-            fill(r);
-            __syncthreads();
+            if ((KR * r) % KB == 0) {
+                __syncthreads();   // every wave is done with the previous B blocks
+                fill(KR * r / KB);
+                __syncthreads();
+            }
         }
         if (live) {
+            const int sb = (KR * r) % KB;   // LDS block of the round's first K step
 #pragma unroll
             for (int s = 0; s < KR; s++) {
                 if (KR * r + s >= nk) break;
-                const v4i *Lb = L + s * NTILE * 64 + lane;
+                const v4i *Lb = L + (sb + s) * NTILE * 64 + lane;
 #pragma unroll
                 for (int kr = 0; kr < LIMBS; kr++)
 #pragma unroll
@@ -679,7 +685,7 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_prod(Par P, co
                                              const uint4 *__restrict__ Ua, const double *__restrict__ ra,
                                              const unsigned long long *__restrict__ amax,
                                              unsigned long long *__restrict__ anext, const int *__restrict__ ue,
-                                             double *__restrict__ upart)
+                                             double *__restrict__ upart, int ug)
 {
     constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
     static_assert(2 * SLAB * 64 >= uq_lds(NTILE), "LDS of the U pairs");
@@ -689,7 +695,7 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_prod(Par P, co
     if (id >= ngrp * nx) {   // U_s'r_a pairs, chain group by chain group
         const int k = id - ngrp * nx, cg = k / npair;
         uq_pair<0, NTILE>(P, Ua, nbt, nb, (nb + 255) / 256, nullptr, 0, cg * 16 * NTC, ra, amax, ue, nullptr,
-                          upart, k - cg * npair, lds);
+                          upart, k - cg * npair, ug, lds);
         return;
     }
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -1068,7 +1074,7 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
                                             const double *__restrict__ kpart, int nkp, double *__restrict__ r,
                                             uint64_t *__restrict__ rmax, int ng, int ngrp, int npair,
                                             const uint4 *__restrict__ Ua, int nbt, int nb,
-                                            const int *__restrict__ ue, double *__restrict__ upart)
+                                            const int *__restrict__ ue, double *__restrict__ upart, int ug)
 {
     constexpr int W = NT / WAVE, SLAB = NTILE * 4, NTC = NTILE / LIMBS;
     static_assert(2 * SLAB * 64 >= uq_lds(NTILE), "LDS of the U pairs");
@@ -1081,7 +1087,7 @@ __global__ __launch_bounds__(NT, RW * NTILE <= 12 ? 3 : 2) void k_upd(Par P, con
         // CUs the update leaves idle
         const int k = (int)blockIdx.x - ngrp * ng, cg = k / npair;
         uq_pair<1, NTILE>(P, Ua, nbt, nb, nss, Dq + (int64_t)cg * SLAB * 64, ngrp * NTILE, cg * 16 * NTC, nullptr,
-                          nullptr, ue, expo, upart, k - cg * npair, lds);
+                          nullptr, ue, expo, upart, k - cg * npair, ug, lds);
         return;
     }
     const KPre kpre = kconst_prefetch(kpart, nkp, P.nchains);
@@ -1300,6 +1306,12 @@ int prod_groups(const sb_handle *h, int rw)
     return std::max(1, (nbt + 4 * rw - 1) / (4 * rw));
 }
 
+// K steps (64 rows or columns each) per group of the U pairs: 16 with two
+// chain groups, whose genotype workgroups keep the CUs busy while the
+// longer pairs run (half the partial sums), else 8, so that no pair
+// outlasts the genotype workgroups of its launch
+int batch_uq_group(int nchains) { return chain_groups((nchains + 15) / 16) >= 2 ? 16 : 8; }
+
 // X_s'Q chunks for about q / 4 workgroups per CU (chain groups x row groups x
 // chunks), each chunk at least two super-steps when the individuals allow
 int prod_chunks(const sb_handle *h, int rw, int q)
@@ -1326,7 +1338,7 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int nch, bool with_u)
     hipStream_t st = h->stream;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
     const int C = P.nchains;
-    const int64_t nup = with_u ? uq_pairs(nbt, batch_words(nb) / 16, 0) : 0;   // U_s'r_a pairs
+    const int64_t nup = with_u ? uq_pairs(nbt, batch_words(nb) / 16, 0, h->uq_group) : 0;   // U_s'r_a pairs
     unsigned long long *amax = h->b_ramax + (int64_t)(b & 1) * C, *anext = h->b_ramax + (int64_t)((b + 1) & 1) * C;
     const uint4 *Q = reinterpret_cast<const uint4 *>(h->b_Q);
     const int64_t tstride = (int64_t)P.blocksize * P.nchains;   // int64 chunk partials per chunk
@@ -1341,7 +1353,7 @@ int prod_launch(sb_handle *h, int b, int rw_sel, int nch, bool with_u)
             launch(kc, k_prod<NTL, RW>, dim3(ngrp * (nrg * nch + (unsigned)nup)), dim3(NT), 0, st, P, gsw, nbt, nb,
                    Q, nch, reinterpret_cast<long long *>(h->b_T), tstride, nrg, ngrp, (int)nup,
                    reinterpret_cast<const uint4 *>(h->Uq0) + h->uq0_off[b], h->ra, amax, anext,
-                   h->ue0 + (int64_t)b * P.blocksize, h->b_rapart);
+                   h->ue0 + (int64_t)b * P.blocksize, h->b_rapart, h->uq_group);
         });
     });
 }
@@ -1354,7 +1366,7 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out, bool wit
     const Par &P = h->par;
     hipStream_t st = h->stream;
     const int p0 = h->ls[b], nb = h->ls[b + 1] - p0, nbt = (nb + 15) / 16;
-    const int64_t nup = with_u ? uq_pairs(nbt, batch_words(nb) / 16, 1) : 0;   // U_s diag(s) c~ pairs
+    const int64_t nup = with_u ? uq_pairs(nbt, batch_words(nb) / 16, 1, h->uq_group) : 0;   // U_s diag(s) c~ pairs
     const int C = P.nchains;
     const int nsw = batch_words(nb), ntr = (P.n + 15) / 16;
     const uint4 *Dq = reinterpret_cast<const uint4 *>(h->b_Dq);
@@ -1370,7 +1382,7 @@ int upd_launch(sb_handle *h, int b, int rw_sel, double *r, int *ng_out, bool wit
             launch(kc, k_upd<NTL, RW>, dim3(ngrp * (ng + (unsigned)nup)), dim3(NT), 0, st, P, gt, nsw / 16, ntr,
                    Dq, h->b_expo2, h->b_kpart, (nb + NT - 1) / NT, r, h->b_rmax, ng, ngrp, (int)nup,
                    reinterpret_cast<const uint4 *>(h->Uq1) + h->uq1_off[b], nbt, nb, h->ue1 + (int64_t)b * P.blocksize,
-                   h->b_uapart);
+                   h->b_uapart, h->uq_group);
         });
     });
 }
@@ -1471,17 +1483,27 @@ void tune_rws(const sb_handle *h, std::vector<int> &rws)
     rws.assign({2});
     if (prod_rw(h->b_ntile) == 4) rws.push_back(4);
 }
-// chunk candidates: 0.75 to 3 workgroups per CU (in quarters).  The best
-// count is often the one whose X_s'Q workgroups and U'r_a pairs together just
-// fill the CUs' workgroup slots (at 64 chains and n = 50,000: 7 chunks, 448 +
-// 320 workgroups = three per CU), so the steps between 1.5 and 2 are fine.
+// chunk candidates: 0.75 to 3 workgroups per CU (in quarters), and the
+// counts whose X_s'Q workgroups and U'r_a pairs together come closest to one,
+// two or three workgroups per CU without passing it.  Those often win: at
+// n = 50,000, 7 chunks both at 64 chains (448 + 320 workgroups, three per
+// CU) and at 32 chains (224 + 288, two per CU), which the quarter steps miss
+// at 32 chains.
 constexpr int TUNE_Q[] = {3, 4, 5, 6, 7, 8, 10, 12};
 void tune_chunks(const sb_handle *h, int rw, std::vector<int> &nchs)
 {
     nchs.clear();
-    for (int q : TUNE_Q) {
-        const int c = prod_chunks(h, rw, q);
+    auto add = [&](int c) {
+        c = std::max(1, std::min(c, std::max(1, h->par.ldw / 32)));
         if (std::find(nchs.begin(), nchs.end(), c) == nchs.end()) nchs.push_back(c);
+    };
+    for (int q : TUNE_Q) add(prod_chunks(h, rw, q));
+    const int ngrp = chain_groups((h->par.nchains + 15) / 16), nrgt = prod_groups(h, rw) * ngrp;
+    const int nbt = (h->par.blocksize + 15) / 16;
+    const int64_t npair = uq_pairs(nbt, batch_words(h->par.blocksize) / 16, 0, h->uq_group) * ngrp;
+    for (int k = 1; k <= 3; k++) {
+        const int64_t c = ((int64_t)k * h->ncu - npair) / nrgt;
+        if (c >= 1) add((int)c);
     }
 }
 
@@ -1489,8 +1511,11 @@ void tune_chunks(const sb_handle *h, int rw, std::vector<int> &nchs)
 int batch_max_chunks(const sb_handle *h)
 {
     int mx = 1;
-    for (int rw : {2, 4})
-        for (int q : TUNE_Q) mx = std::max(mx, prod_chunks(h, rw, q));
+    std::vector<int> nchs;
+    for (int rw : {2, 4}) {
+        tune_chunks(h, rw, nchs);
+        for (int c : nchs) mx = std::max(mx, c);
+    }
     return std::max(mx, h->b_nch);
 }
 

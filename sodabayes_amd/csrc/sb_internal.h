@@ -16,9 +16,6 @@ constexpr int MAXGRP = 100;         // globals.f90:34 maxgrp
 constexpr int MAXBIN = MAXMIX * MAXGRP;
 constexpr int CHUNK_WORDS = 128;    // 2048 individuals per genotype chunk (16 per word)
 constexpr int CHUNK_IND = CHUNK_WORDS * 16;
-// K steps (64 rows or columns each) per (quad, group) pair of the int8 U
-// products: the row split of their partial sums
-constexpr int UQ_GROUP = 16;
 // slots per chain into which the residual update folds its workgroups' max |r|
 // (atomic max into slot workgroup mod RSLOT)
 constexpr int RSLOT = 16;
@@ -278,7 +275,8 @@ struct sb_handle {
     double *tune_r = nullptr;     // scratch residuals of the tuning updates
     int b_ntile = 3;              // 16-column MFMA tiles: limbs x ceil(nchains / 16)
     int *b_T = nullptr;           // chunks x nchains x blocksize int64 partials of X_s'Q
-    int rows_per_group = 64 * sb::UQ_GROUP;   // row split of the augmented product
+    int uq_group = 8;             // K steps per group of the int8 U products (batch_uq_group)
+    int rows_per_group = 64 * 8;  // row split of the augmented product: 64 uq_group
     double *b_rapart = nullptr;   // (blocksize / rows_per_group) x blocksize x nchains
     double *b_uapart = nullptr;   // same shape: partial U_s delta of the update
     int8_t *b_Dq = nullptr;       // (16 * b_ntile) x b_ldd int8 limbs of delta / s
@@ -359,6 +357,7 @@ int batch_transpose(sb_handle *h);
 int batch_words(int nb);
 int64_t batch_uq_elems(int nb, int mode);
 int prod_chunks(const sb_handle *h, int rw, int q);
+int batch_uq_group(int nchains);
 int batch_max_chunks(const sb_handle *h);
 int batch_autotune(sb_handle *h);
 int init_xpx(sb_handle *h);
